@@ -1,0 +1,199 @@
+"""Benchmark of the LBM-IB hot path on one MI355X.
+
+Workload (default): 256^3 D3Q19 BGK with inlet/outlet x, periodic y, wall z,
+tau 0.6, u_in 0.01, one 104x104 fiber sheet (spacing 1, jitter 0.1) centred
+at (64, 128, 128), Ks = Kb = 1e-3, populations stored as FP32 deviations.
+A "step" is one full LBM-IB time step (fiber forces, spreading, fused
+collide-stream, band moments, interpolation and move) over the whole grid.
+
+Prints one JSON line: MLUPS over all nx*ny*nz cells, the HBM roofline of the
+fused collide-stream kernel (algorithmic 2*19*4 = 152 B per cell, timed with
+HIP events on the engine's stream), and the reference's own pthread code
+timed on this machine's host cores on a bounded sample of the same grid.
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "distributed-lb-ib_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS + % HBM roofline, 256³ D3Q19+sheet; mixed/surrogate vs plain at stated err"
+
+
+def parse_args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=50)
+    p.add_argument("--precision", default="f32", choices=["f32", "f64", "f16", "bf16"])
+    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-steps", default="2,6", help="N1,N2 steps of the differenced CPU timing")
+    return p.parse_args()
+
+
+def traffic_from_profiles(workload, precision):
+    """HBM bytes per launch of the fused kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    kernel calibration), if one exists for this workload."""
+    path = os.path.join(REPO, "profiles", f"collide_stream_{workload}_{precision}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*hbm_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def cpu_reference_baseline(n, rows, steps_pair):
+    """cube_pthread/lbm_ib.c built by oracle/Makefile with its own flags, run
+    on this host's cores at its own constants on the same grid and sheet
+    size; T_step from two step counts (cancels init and the step-1 dumps)."""
+    exe = os.path.join(REPO, "oracle", "_ref", "scalable_IB_v4.1")
+    if not os.path.exists(exe):
+        return None
+    ncpu = os.cpu_count() or 1
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cubes = n // 4
+    threads = 1
+    while threads * 2 <= min(ncpu, 64) and cubes % (threads * 2) == 0:
+        threads *= 2
+    # P x Q x R with P*Q*R = threads, split over axes (all cube counts are
+    # powers of two here)
+    pqr = [1, 1, 1]
+    k = 0
+    t = threads
+    while t > 1:
+        pqr[k % 3] *= 2
+        t //= 2
+        k += 1
+    if threads % 2:
+        return None
+    width = float(rows - 1)
+    y0 = n / 2.0 - width / 2.0
+    times = {}
+    tmp = tempfile.mkdtemp(prefix="lbmib_cpu_", dir="/tmp")
+    try:
+        for steps in steps_pair:
+            args = [exe, "-steps", str(steps), "-dump", str(10 ** 6), "-fluid_grid_xyz", str(n), str(n), str(n),
+                    "-cube_size", "4", "-thread_per_task_xyz", str(pqr[0]), str(pqr[1]), str(pqr[2]),
+                    "-num_fibersht", "1", "-fibersht_width_height", str(width), str(width),
+                    "-fibersht_row_clmn", str(rows), str(rows), "-fibersht_xyz_0", str(n / 4.0), str(y0), str(y0)]
+            t0 = time.time()
+            out = subprocess.run(args, cwd=tmp, capture_output=True, text=True, timeout=900)
+            wall = time.time() - t0
+            m = re.search(r"TOTAL TIME TAKEN IN Seconds:([0-9.]+)", out.stdout)
+            if out.returncode != 0 or not m:
+                return None
+            times[steps] = float(m.group(1))
+            print(f"[cpu baseline] {steps} steps: {times[steps]:.2f} s (wall {wall:.1f} s)", file=sys.stderr, flush=True)
+            for f in os.listdir(tmp):
+                os.remove(os.path.join(tmp, f))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    n1, n2 = steps_pair
+    t_step = (times[n2] - times[n1]) / (n2 - n1)
+    if t_step <= 0:
+        return None
+    return {"value": n ** 3 / t_step / 1e6, "unit": "MLUPS", "cores": threads, "kind": "reference",
+            "sample": f"cube_pthread/lbm_ib.c (gcc -O3), {n}^3 grid, {rows}x{rows} sheet, reference constants, "
+                      f"{pqr[0]}x{pqr[1]}x{pqr[2]} threads on {ncpu} host cpus, "
+                      f"T_step = (T({n2}) - T({n1}))/{n2 - n1} = {t_step:.3f} s"}
+
+
+def main():
+    a = parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import lbmib
+    import lbmib_configs
+
+    cfg_fn = getattr(lbmib_configs, a.workload)
+    kw, nominal_steps = cfg_fn(precision=a.precision)
+    n = kw["nx"] * kw["ny"] * kw["nz"]
+    nodes = sum(s.rows * s.cols for s in kw["sheets"])
+    print(f"[bench] {a.workload} {kw['nx']}x{kw['ny']}x{kw['nz']} {a.precision} nodes={nodes} "
+          f"warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        ms_total, ms_main = e.time_steps(a.warmup, a.steps)
+        status = e.status()
+        mass = e.totals()[0]
+        bpc = e.bytes_per_cell()
+    if status != 0:
+        print(json.dumps({"error": f"device status {lbmib.ERR_NAMES.get(status, status)}"}))
+        sys.exit(1)
+    ms_step = ms_total / a.steps
+    mlups = n / (ms_step * 1e-3) / 1e6
+    alg_bytes = bpc * n
+    achieved = alg_bytes / (ms_main * 1e-3) / 1e9
+    traffic = traffic_from_profiles(a.workload, a.precision)
+    print(f"[bench] {ms_step:.4f} ms/step, {mlups:.1f} MLUPS, fused kernel {ms_main:.4f} ms "
+          f"= {achieved:.1f} GB/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu_baseline and a.workload == "c2":
+        n1, n2 = (int(x) for x in a.cpu_steps.split(","))
+        try:
+            cpu = cpu_reference_baseline(kw["nx"], kw["sheets"][0].rows, (n1, n2))
+        except Exception as ex:  # the baseline is a report, not the measurement
+            print(f"[cpu baseline] failed: {ex}", file=sys.stderr)
+            cpu = None
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.precision,
+        "data": "synthetic",
+        "config": {
+            "workload": f"{kw['nx']}x{kw['ny']}x{kw['nz']} D3Q19 BGK, inlet/outlet x, periodic y, walls z, "
+                        f"tau {kw['tau']}, {len(kw['sheets'])} sheet(s) of "
+                        f"{kw['sheets'][0].rows}x{kw['sheets'][0].cols} nodes (Ks=Kb=1e-3, jitter 0.1)",
+            "grid": [kw["nx"], kw["ny"], kw["nz"]],
+            "fiber_nodes": nodes,
+            "storage": a.precision + (" deviations f-w" if a.precision != "f64" else ""),
+            "bytes_per_cell_update": bpc,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "whole_step_hbm_frac": round(alg_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fused_kernel_ms": round(ms_main, 5),
+        "fiber_node_updates_per_s": round(nodes / (ms_step * 1e-3), 1),
+        "mass_after": mass,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

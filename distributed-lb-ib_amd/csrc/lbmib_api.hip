@@ -1,0 +1,627 @@
+// lbmib_api.hip -- the C-ABI of liblbmib.so (declared in include/lbmib.h).
+//
+// One context owns one device, one HIP stream (default priority) and all
+// device buffers of a simulation:
+//   f[2]      population ping-pong, SoA, 19 planes of `pitch` elements
+//   F[2]      force field ping-pong (step parity), dense [3][n], valid only
+//             inside that step's support boxes
+//   U         band velocity [3][n] (FP64), valid inside the current boxes
+//   node arrays (FP64 positions/forces, support starts, delta factors)
+//   bins      (y,z) 4x4-column bins of fiber node ids for the gather
+// Per-step entry points only enqueue; downloads synchronise.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "lbmib_internal.h"
+
+using namespace lbmib;
+
+struct lbmib_ctx {
+  lbmib_fluid_params fp;
+  int nsheets = 0;
+  lbmib_sheet_params sp[kMaxSheets];
+  SheetDesc sd[kMaxSheets];
+  int nnodes = 0;
+  Geometry g;
+  FluidConsts<double> c;
+  size_t esz = 0, fsz = 0;  // population / force element sizes
+  void *f[2] = {nullptr, nullptr};
+  int cur = 0;
+  int own = 1;       // f[cur] holds the initial f (not a post-collision state)
+  int pending = 0;   // collide done, new state in f[1-cur], swap pending
+  int band_done = 0; // band moments of the pending state computed
+  int ib_clean = 1;  // bins empty and this step's box slot reset
+  int spread_slot = -1;  // slot of the last completed gather
+  void *F[2] = {nullptr, nullptr};
+  IBState ib;
+  SheetDesc *d_sheets = nullptr;
+  int *d_err = nullptr;
+  double *d_partials = nullptr;
+  int nparts = 0;
+  double *d_tot = nullptr;
+  double *d_mat = nullptr;   // materialisation scratch (lazy)
+  size_t mat_bytes = 0;
+  hipStream_t stream = nullptr;
+  int time = 1;
+  int init_bits = 0;  // 1: DF0, 2: df1, 4: inout
+  int latched = LBMIB_OK;
+  std::vector<hipEvent_t> evs;
+};
+
+#define HIPCHK(x)                               \
+  do {                                          \
+    if ((x) != hipSuccess) return LBMIB_ERR_HIP; \
+  } while (0)
+#define RCHK(x)             \
+  do {                      \
+    int _r = (x);           \
+    if (_r) return _r;      \
+  } while (0)
+
+static double host_feq(int q, double rho, double vx, double vy, double vz) {
+  // the expression of init_eqlbrmdistrfuncDF0, lbm_ib.c:1561-1594
+  double usq = vx * vx + vy * vy + vz * vz;
+  if (q == 0) return 1.0 / 3.0 * rho * (1.0 - 1.5 * usq);
+  double cu = kCX[q] * vx + kCY[q] * vy + kCZ[q] * vz;
+  double w = (q <= 6) ? 1.0 / 18.0 : 1.0 / 36.0;
+  return w * rho * (1.0 + 3.0 * cu + 4.5 * (pow(cu, 2.0)) - 1.5 * usq);
+}
+
+static void axis_ranges(int bc, int n, int *lo, int *hi, int *mlo, int *mhi) {
+  if (bc == LBMIB_BC_PERIODIC) { *lo = 0; *hi = n - 1; *mlo = 0; *mhi = n - 1; }
+  else if (bc == LBMIB_BC_INOUT) { *lo = 2; *hi = n - 3; *mlo = 3; *mhi = n - 4; }
+  else { *lo = 2; *hi = n - 3; *mlo = 2; *mhi = n - 3; }
+}
+
+static int map_device_err(int bits) {
+  if (bits & DERR_FIBER_OOB) return LBMIB_ERR_FIBER_OOB;
+  if (bits & DERR_BIN_OVERFLOW) return LBMIB_ERR_BIN_OVERFLOW;
+  return LBMIB_OK;
+}
+
+static size_t elem_size(int prec) {
+  switch (prec) {
+    case LBMIB_F64: return 8;
+    case LBMIB_F32: return 4;
+    case LBMIB_F16:
+    case LBMIB_BF16: return 2;
+  }
+  return 0;
+}
+
+extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
+  if (!ctx) return LBMIB_ERR_ARG;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto e : ctx->evs) (void)hipEventDestroy(e);
+  void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
+                  ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.U, ctx->d_sheets,
+                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const lbmib_sheet_params *sheets,
+                            lbmib_ctx **out) {
+  if (!fluid || !out || nsheets < 0 || nsheets > kMaxSheets || (nsheets > 0 && !sheets)) return LBMIB_ERR_ARG;
+  *out = nullptr;
+  const lbmib_fluid_params &p = *fluid;
+  if (p.nx < 8 || p.ny < 8 || p.nz < 8 || p.nx > 4096 || p.ny > 65535 || p.nz > 65535) return LBMIB_ERR_ARG;
+  if (p.bc_x < 0 || p.bc_x > 2 || p.bc_y < 1 || p.bc_y > 2 || p.bc_z < 1 || p.bc_z > 2) return LBMIB_ERR_ARG;
+  if (!(p.tau > 0.5) || !(p.rho0 > 0)) return LBMIB_ERR_ARG;
+  if (elem_size(p.precision) == 0) return LBMIB_ERR_ARG;
+  const long n = (long)p.nx * p.ny * p.nz;
+  if (n > (1L << 31) - 1) return LBMIB_ERR_ARG;
+
+  lbmib_ctx *ctx = new lbmib_ctx();
+  ctx->fp = p;
+  ctx->nsheets = nsheets;
+  int off = 0;
+  for (int s = 0; s < nsheets; ++s) {
+    const lbmib_sheet_params &q = sheets[s];
+    if (q.rows < 4 || q.cols < 4 || !(q.width > 0) || !(q.height > 0)) { delete ctx; return LBMIB_ERR_ARG; }
+    ctx->sp[s] = q;
+    SheetDesc &d = ctx->sd[s];
+    d.rows = q.rows; d.cols = q.cols; d.off = off; d.pad = 0;
+    d.ds1 = q.width / (q.cols - 1);   // lbm_ib.c:994
+    d.ds2 = q.height / (q.rows - 1);  // lbm_ib.c:995
+    d.bend_c = q.kb / pow(d.ds2, 3.0);
+    d.str_row_c = q.ks / (d.ds1 * d.ds1);
+    d.str_col_c = q.ks / (d.ds2 * d.ds2);
+    off += q.rows * q.cols;
+  }
+  ctx->nnodes = off;
+
+  Geometry &g = ctx->g;
+  g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
+  g.bc[0] = p.bc_x; g.bc[1] = p.bc_y; g.bc[2] = p.bc_z;
+  const int dims[3] = {p.nx, p.ny, p.nz};
+  for (int a = 0; a < 3; ++a) axis_ranges(g.bc[a], dims[a], &g.lo[a], &g.hi[a], &g.mlo[a], &g.mhi[a]);
+  g.n = n;
+  long pad = 256;
+  if (const char *e = getenv("LBMIB_PLANE_PAD")) pad = atol(e);
+  if (pad < 0) pad = 0;
+  pad = (pad + 127) / 128 * 128;
+  g.pitch = (n + 127) / 128 * 128 + pad;
+
+  // BGK/Guo constants as init_gv and compute_eqlbrmdistrfuncDF1 form them
+  FluidConsts<double> &c = ctx->c;
+  const double tau = p.tau;
+  c.omc = 1.0 - 1.0 / tau;
+  c.om = 1.0 / tau;
+  for (int q = 0; q < 19; ++q) {
+    c.pref[q] = 1 * (1.0 - 1.0 / (2.0 * tau)) / (double)kWDEN[q];
+    c.pw[q] = (1.0 / kWDEN[q]) * (1.0 - 1.0 / (2.0 * tau));
+  }
+  const double cs = 1.0 / (sqrt(3.0));  // lbm_ib.c:765
+  c.cs2 = cs * cs;
+  c.cs4 = pow(cs, 4.0);
+  c.g = p.g;
+  c.rho0 = p.rho0; c.u0x = p.u0[0]; c.u0y = p.u0[1]; c.u0z = p.u0[2];
+  const bool dev = p.precision != LBMIB_F64;
+  for (int q = 0; q < 19; ++q) {
+    const double feq = host_feq(q, p.rho0, p.u0[0], p.u0[1], p.u0[2]);
+    c.tab_in[q] = c.tab_out[q] = dev ? feq - 1.0 / kWDEN[q] : feq;
+  }
+
+  ctx->esz = elem_size(p.precision);
+  ctx->fsz = p.precision == LBMIB_F64 ? 8 : 4;
+
+  int rc = LBMIB_OK;
+  auto alloc = [&](void **ptr, size_t bytes) {
+    if (rc) return;
+    if (hipMalloc(ptr, bytes) != hipSuccess) { *ptr = nullptr; rc = LBMIB_ERR_ALLOC; }
+  };
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
+  const size_t fbytes = (size_t)19 * g.pitch * ctx->esz;
+  alloc(&ctx->f[0], fbytes);
+  alloc(&ctx->f[1], fbytes);
+  alloc((void **)&ctx->d_err, sizeof(int));
+  ctx->nparts = totals_blocks(g);
+  alloc((void **)&ctx->d_partials, sizeof(double) * 4 * (size_t)ctx->nparts);
+  alloc((void **)&ctx->d_tot, sizeof(double) * 4);
+  alloc((void **)&ctx->ib.box, sizeof(int) * 2 * kMaxSheets * 6);
+
+  IBState &ib = ctx->ib;
+  ib.nsheets = nsheets;
+  ib.nnodes = ctx->nnodes;
+  ib.quirks = p.quirks;
+  ib.nby = (p.ny + 3) / 4;
+  ib.nbz = (p.nz + 3) / 4;
+  // bin capacity: a 4x4 column holds about (4/ds1+2)(4/ds2+2) nodes per
+  // sheet; keep a factor two of headroom for deformation.
+  double dens = 0;
+  for (int s = 0; s < nsheets; ++s) dens += (4.0 / ctx->sd[s].ds1 + 2.0) * (4.0 / ctx->sd[s].ds2 + 2.0);
+  int cap = 32;
+  while (cap < 2 * dens && cap < 4096) cap <<= 1;
+  ib.cap = cap;
+  if (ctx->nnodes > 0) {
+    const size_t M = ctx->nnodes;
+    alloc(&ctx->F[0], (size_t)3 * n * ctx->fsz);
+    alloc(&ctx->F[1], (size_t)3 * n * ctx->fsz);
+    alloc((void **)&ib.U, sizeof(double) * 3 * (size_t)n);
+    alloc((void **)&ib.pos, sizeof(double) * 3 * M);
+    alloc((void **)&ib.frc, sizeof(double) * NF_COUNT * M);
+    alloc((void **)&ib.start, sizeof(int) * 3 * M);
+    alloc((void **)&ib.wgt, sizeof(double) * 12 * M);
+    alloc((void **)&ib.bin_count, sizeof(int) * (size_t)ib.nby * ib.nbz);
+    alloc((void **)&ib.bin_list, sizeof(int) * (size_t)ib.nby * ib.nbz * cap);
+    alloc((void **)&ctx->d_sheets, sizeof(SheetDesc) * kMaxSheets);
+  }
+  ib.sheets = ctx->d_sheets;
+  ib.err = ctx->d_err;
+  if (!rc && ctx->nnodes > 0) {
+    if (hipMemcpy(ctx->d_sheets, ctx->sd, sizeof(SheetDesc) * nsheets, hipMemcpyHostToDevice) != hipSuccess)
+      rc = LBMIB_ERR_HIP;
+    if (!rc && hipMemset(ib.pos, 0, sizeof(double) * 3 * ctx->nnodes) != hipSuccess) rc = LBMIB_ERR_HIP;
+    if (!rc && hipMemset(ib.frc, 0, sizeof(double) * NF_COUNT * ctx->nnodes) != hipSuccess) rc = LBMIB_ERR_HIP;
+  }
+  if (!rc && hipMemset(ctx->d_err, 0, sizeof(int)) != hipSuccess) rc = LBMIB_ERR_HIP;
+  if (!rc) rc = launch_ib_reset(ib, ctx->nnodes > 0, 3, ctx->stream);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = LBMIB_ERR_HIP;
+  if (rc) {
+    lbmib_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_set_fiber_positions(lbmib_ctx *ctx, int sheet, const double *xyz) {
+  if (!ctx || !xyz || sheet < 0 || sheet >= ctx->nsheets) return LBMIB_ERR_ARG;
+  const SheetDesc &d = ctx->sd[sheet];
+  const int m = d.rows * d.cols, M = ctx->nnodes;
+  std::vector<double> soa((size_t)3 * m);
+  for (int k = 0; k < m; ++k)
+    for (int a = 0; a < 3; ++a) soa[(size_t)a * m + k] = xyz[3 * (size_t)k + a];
+  for (int a = 0; a < 3; ++a)
+    HIPCHK(hipMemcpyAsync(ctx->ib.pos + (size_t)a * M + d.off, soa.data() + (size_t)a * m, sizeof(double) * m,
+                          hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return LBMIB_OK;
+}
+
+// ------------------------------------------------------------------ init
+
+extern "C" int lbmib_init_eqlbrmdistrfuncDF0(lbmib_ctx *ctx) {
+  if (!ctx) return LBMIB_ERR_ARG;
+  // rho = rho0 and u = u0 everywhere (lbm_ib.c:837-849); the equilibrium is
+  // uniform, so it is held as the 19-entry table in the context.
+  ctx->init_bits |= 1;
+  ctx->time = 1;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_init_df1(lbmib_ctx *ctx) {
+  if (!ctx) return LBMIB_ERR_ARG;
+  if (!(ctx->init_bits & 1)) return LBMIB_ERR_STATE;
+  RCHK(launch_fill_init(ctx->fp.precision, ctx->g, ctx->f[0], ctx->c, ctx->stream));
+  ctx->cur = 0;
+  ctx->own = 1;
+  ctx->pending = 0;
+  ctx->band_done = 0;
+  ctx->spread_slot = -1;
+  RCHK(launch_ib_reset(ctx->ib, ctx->nnodes > 0, 3, ctx->stream));
+  ctx->ib_clean = 1;
+  ctx->init_bits |= 2;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_init_df_inout(lbmib_ctx *ctx) {
+  if (!ctx) return LBMIB_ERR_ARG;
+  if (!(ctx->init_bits & 1)) return LBMIB_ERR_STATE;
+  // the inlet/outlet tables (feq(rho0,u0) sampled at x = 2 and x = N-1,
+  // lbm_ib.c:1632-1668) are the context's tab_in/tab_out
+  ctx->init_bits |= 4;
+  return LBMIB_OK;
+}
+
+static int ready(lbmib_ctx *ctx) {
+  if (!ctx) return LBMIB_ERR_ARG;
+  if (ctx->init_bits != 7) return LBMIB_ERR_STATE;
+  return LBMIB_OK;
+}
+
+// ----------------------------------------------------------- IB pieces
+
+static int slot_now(const lbmib_ctx *ctx) { return ctx->time & 1; }
+
+static int ensure_ib_clean(lbmib_ctx *ctx) {
+  if (!ctx->ib_clean) {
+    RCHK(launch_ib_reset(ctx->ib, 1, 1 << slot_now(ctx), ctx->stream));
+    ctx->ib_clean = 1;
+  }
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_compute_bendingforce(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  return launch_fiber(ctx->ib, ctx->g, FK_BEND, slot_now(ctx), ctx->stream);
+}
+extern "C" int lbmib_compute_stretchingforce(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  return launch_fiber(ctx->ib, ctx->g, FK_STRETCH, slot_now(ctx), ctx->stream);
+}
+extern "C" int lbmib_compute_elasticforce(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  return launch_fiber(ctx->ib, ctx->g, FK_ELASTIC, slot_now(ctx), ctx->stream);
+}
+extern "C" int lbmib_compute_fiber_forces(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  return launch_fiber(ctx->ib, ctx->g, FK_BEND | FK_STRETCH | FK_ELASTIC, slot_now(ctx), ctx->stream);
+}
+
+extern "C" int lbmib_fiber_SpreadForce(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  RCHK(ensure_ib_clean(ctx));
+  ctx->ib_clean = 0;
+  return launch_fiber(ctx->ib, ctx->g, FK_PREP, slot_now(ctx), ctx->stream);
+}
+
+extern "C" int lbmib_fluid_get_SpreadForce(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  if (ctx->nnodes == 0) return LBMIB_OK;
+  const int s = slot_now(ctx);
+  RCHK(launch_gather(ctx->fp.precision, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream));
+  ctx->spread_slot = s;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_get_influentialdomain_fluid_grid_and_SpreadForce(lbmib_ctx *ctx) {
+  RCHK(lbmib_fiber_SpreadForce(ctx));
+  return lbmib_fluid_get_SpreadForce(ctx);
+}
+
+// ------------------------------------------------------------ fluid step
+
+extern "C" int lbmib_compute_eqlbrmdistrfuncDF1(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  if (ctx->pending) return LBMIB_ERR_STATE;
+  const int s = slot_now(ctx);
+  const int *box = ctx->ib.box;
+  RCHK(launch_collide_stream(ctx->fp.precision, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c,
+                             ctx->nsheets, box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s],
+                             ctx->F[1 - s], ctx->stream));
+  ctx->pending = 1;
+  ctx->band_done = 0;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_stream_distrfunc(lbmib_ctx *ctx) { return ready(ctx); }
+extern "C" int lbmib_bounceback_rigidwalls(lbmib_ctx *ctx) { return ready(ctx); }
+extern "C" int lbmib_copy_inout_to_df2(lbmib_ctx *ctx) { return ready(ctx); }
+extern "C" int lbmib_periodicBC(lbmib_ctx *ctx) { return ready(ctx); }
+
+extern "C" int lbmib_compute_rho_and_u(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  if (!ctx->pending) return LBMIB_ERR_STATE;
+  if (ctx->nnodes == 0 || ctx->spread_slot != slot_now(ctx)) return LBMIB_OK;
+  const int s = slot_now(ctx);
+  RCHK(launch_band_moments(ctx->fp.precision, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], s, ctx->F[s], ctx->c,
+                           ctx->stream));
+  ctx->band_done = 1;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_fluid_SpreadVelocity(lbmib_ctx *ctx) { return lbmib_compute_rho_and_u(ctx); }
+
+extern "C" int lbmib_fiber_get_SpreadVelocity(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  if (!ctx->pending) return LBMIB_ERR_STATE;
+  if (ctx->nnodes == 0 || ctx->spread_slot != slot_now(ctx)) return LBMIB_OK;
+  if (!ctx->band_done) RCHK(lbmib_compute_rho_and_u(ctx));
+  RCHK(launch_move(ctx->ib, ctx->g, 1 - slot_now(ctx), ctx->stream));
+  ctx->ib_clean = 1;  // k_move clears the bins and the next step's box slot
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_moveFiberSheet(lbmib_ctx *ctx) { return lbmib_fiber_get_SpreadVelocity(ctx); }
+
+extern "C" int lbmib_replace_old_DF(lbmib_ctx *ctx) {
+  RCHK(ready(ctx));
+  if (!ctx->pending) return LBMIB_ERR_STATE;
+  ctx->cur = 1 - ctx->cur;
+  ctx->own = 0;
+  ctx->pending = 0;
+  ctx->band_done = 0;
+  if (!ctx->ib_clean) {
+    // a spread without a move: clear before the next step's binning
+    RCHK(launch_ib_reset(ctx->ib, 1, 1 << ((ctx->time + 1) & 1), ctx->stream));
+    ctx->ib_clean = 1;
+  }
+  ctx->time += 1;
+  return LBMIB_OK;
+}
+
+static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
+  const int s = slot_now(ctx);
+  const int prec = ctx->fp.precision;
+  if (ctx->nnodes > 0) {
+    RCHK(ensure_ib_clean(ctx));
+    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, ctx->stream));
+    RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream));
+    ctx->spread_slot = s;
+  }
+  const int *box = ctx->ib.box;
+  if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
+  RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
+                             box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
+                             ctx->stream));
+  if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
+  if (ctx->nnodes > 0) {
+    RCHK(launch_band_moments(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], s, ctx->F[s], ctx->c, ctx->stream));
+    RCHK(launch_move(ctx->ib, ctx->g, 1 - s, ctx->stream));
+    ctx->ib_clean = 1;
+  }
+  ctx->cur = 1 - ctx->cur;
+  ctx->own = 0;
+  ctx->pending = 0;
+  ctx->band_done = 0;
+  ctx->time += 1;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_step(lbmib_ctx *ctx, int nsteps) {
+  RCHK(ready(ctx));
+  if (ctx->pending || nsteps < 0) return LBMIB_ERR_STATE;
+  for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, nullptr, nullptr));
+  return LBMIB_OK;
+}
+
+// --------------------------------------------------------------- queries
+
+static int poll_status(lbmib_ctx *ctx) {
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int bits = 0;
+  HIPCHK(hipMemcpy(&bits, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  const int e = map_device_err(bits);
+  if (e && !ctx->latched) ctx->latched = e;
+  return ctx->latched;
+}
+
+extern "C" int lbmib_sync(lbmib_ctx *ctx) {
+  if (!ctx) return LBMIB_ERR_ARG;
+  return poll_status(ctx);
+}
+
+extern "C" int lbmib_status(lbmib_ctx *ctx) {
+  if (!ctx) return LBMIB_ERR_ARG;
+  return poll_status(ctx);
+}
+
+extern "C" int lbmib_get_time(lbmib_ctx *ctx, int *t) {
+  if (!ctx || !t) return LBMIB_ERR_ARG;
+  *t = ctx->time;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_get_fibers(lbmib_ctx *ctx, int sheet, double *out) {
+  if (!ctx || !out || sheet < 0 || sheet >= ctx->nsheets) return LBMIB_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const SheetDesc &d = ctx->sd[sheet];
+  const int m = d.rows * d.cols, M = ctx->nnodes;
+  std::vector<double> pos((size_t)3 * m), frc((size_t)NF_COUNT * m);
+  for (int a = 0; a < 3; ++a)
+    HIPCHK(hipMemcpy(pos.data() + (size_t)a * m, ctx->ib.pos + (size_t)a * M + d.off, sizeof(double) * m,
+                     hipMemcpyDeviceToHost));
+  for (int a = 0; a < NF_COUNT; ++a)
+    HIPCHK(hipMemcpy(frc.data() + (size_t)a * m, ctx->ib.frc + (size_t)a * M + d.off, sizeof(double) * m,
+                     hipMemcpyDeviceToHost));
+  for (int k = 0; k < m; ++k) {
+    double *r = out + (size_t)12 * k;
+    for (int a = 0; a < 3; ++a) r[a] = pos[(size_t)a * m + k];
+    for (int a = 0; a < NF_COUNT; ++a) r[3 + a] = frc[(size_t)a * m + k];
+  }
+  return LBMIB_OK;
+}
+
+static int ensure_mat(lbmib_ctx *ctx, size_t bytes) {
+  if (ctx->mat_bytes >= bytes) return LBMIB_OK;
+  if (ctx->d_mat) (void)hipFree(ctx->d_mat);
+  ctx->d_mat = nullptr;
+  ctx->mat_bytes = 0;
+  if (hipMalloc((void **)&ctx->d_mat, bytes) != hipSuccess) return LBMIB_ERR_ALLOC;
+  ctx->mat_bytes = bytes;
+  return LBMIB_OK;
+}
+
+// slot/box/F describing the force of the last completed step
+static void last_force(lbmib_ctx *ctx, const int **box, const void **F, int *nsh) {
+  const int s = ctx->spread_slot;
+  if (s < 0 || ctx->nnodes == 0) {
+    *box = ctx->ib.box;  // both slots empty before the first spread
+    *F = nullptr;
+    *nsh = 0;
+    return;
+  }
+  *box = ctx->ib.box + s * kMaxSheets * 6;
+  *F = ctx->F[s];
+  *nsh = ctx->nsheets;
+}
+
+extern "C" int lbmib_get_macroscopic(lbmib_ctx *ctx, double *rho, double *vel) {
+  RCHK(ready(ctx));
+  if (ctx->pending || (!rho && !vel)) return LBMIB_ERR_STATE;
+  const long n = ctx->g.n;
+  RCHK(ensure_mat(ctx, sizeof(double) * 4 * (size_t)n));
+  const int *box;
+  const void *F;
+  int nsh;
+  last_force(ctx, &box, &F, &nsh);
+  RCHK(launch_materialize(ctx->fp.precision, ctx->g, ctx->f[ctx->cur], ctx->own, ctx->c, nsh, box, F, nullptr,
+                          ctx->d_mat, ctx->d_mat + n, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (rho) HIPCHK(hipMemcpy(rho, ctx->d_mat, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (vel) HIPCHK(hipMemcpy(vel, ctx->d_mat + n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_get_df(lbmib_ctx *ctx, double *df) {
+  RCHK(ready(ctx));
+  if (ctx->pending || !df) return LBMIB_ERR_STATE;
+  const long n = ctx->g.n;
+  RCHK(ensure_mat(ctx, sizeof(double) * 19 * (size_t)n));
+  RCHK(launch_materialize(ctx->fp.precision, ctx->g, ctx->f[ctx->cur], ctx->own, ctx->c, 0, ctx->ib.box, nullptr,
+                          ctx->d_mat, nullptr, nullptr, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(df, ctx->d_mat, sizeof(double) * 19 * n, hipMemcpyDeviceToHost));
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_get_force(lbmib_ctx *ctx, double *force) {
+  RCHK(ready(ctx));
+  if (!force) return LBMIB_ERR_ARG;
+  const long n = ctx->g.n;
+  memset(force, 0, sizeof(double) * 3 * n);
+  if (ctx->nnodes == 0 || ctx->spread_slot < 0) return LBMIB_OK;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int s = ctx->spread_slot;
+  int box[kMaxSheets * 6];
+  HIPCHK(hipMemcpy(box, ctx->ib.box + s * kMaxSheets * 6, sizeof(int) * 6 * ctx->nsheets, hipMemcpyDeviceToHost));
+  std::vector<char> raw((size_t)3 * n * ctx->fsz);
+  HIPCHK(hipMemcpy(raw.data(), ctx->F[s], raw.size(), hipMemcpyDeviceToHost));
+  for (int sh = 0; sh < ctx->nsheets; ++sh) {
+    const int *b = box + 6 * sh;
+    if (b[0] > b[1]) continue;
+    for (int z = b[4]; z <= b[5]; ++z)
+      for (int y = b[2]; y <= b[3]; ++y)
+        for (int x = b[0]; x <= b[1]; ++x) {
+          const long c0 = (long)x + (long)ctx->g.nx * ((long)y + (long)ctx->g.ny * z);
+          for (int a = 0; a < 3; ++a) {
+            const size_t k = (size_t)a * n + c0;
+            force[k] = ctx->fsz == 8 ? ((const double *)raw.data())[k] : (double)((const float *)raw.data())[k];
+          }
+        }
+  }
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_get_totals(lbmib_ctx *ctx, double *out4) {
+  RCHK(ready(ctx));
+  if (ctx->pending || !out4) return LBMIB_ERR_STATE;
+  RCHK(launch_totals(ctx->fp.precision, ctx->g, ctx->f[ctx->cur], ctx->own, ctx->c, ctx->d_partials, ctx->nparts,
+                     ctx->d_tot, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(out4, ctx->d_tot, sizeof(double) * 4, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; ++k)
+    if (!isfinite(out4[k])) {
+      if (!ctx->latched) ctx->latched = LBMIB_ERR_NONFINITE;
+      return LBMIB_ERR_NONFINITE;
+    }
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_get_band_box(lbmib_ctx *ctx, int sheet, int *box6) {
+  if (!ctx || !box6 || sheet < 0 || sheet >= ctx->nsheets) return LBMIB_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int s = ctx->spread_slot < 0 ? 0 : ctx->spread_slot;
+  HIPCHK(hipMemcpy(box6, ctx->ib.box + (s * kMaxSheets + sheet) * 6, sizeof(int) * 6, hipMemcpyDeviceToHost));
+  return LBMIB_OK;
+}
+
+// ----------------------------------------------------------- measurement
+
+extern "C" int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *ms_total, double *ms_main_mean) {
+  RCHK(ready(ctx));
+  if (ctx->pending || nsteps <= 0 || nwarm < 0) return LBMIB_ERR_STATE;
+  const size_t need = 2 + 2 * (size_t)nsteps;
+  while (ctx->evs.size() < need) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    ctx->evs.push_back(e);
+  }
+  for (int k = 0; k < nwarm; ++k) RCHK(one_step(ctx, nullptr, nullptr));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipEventRecord(ctx->evs[0], ctx->stream));
+  for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, nullptr, nullptr));
+  HIPCHK(hipEventRecord(ctx->evs[1], ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->evs[1]));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->evs[0], ctx->evs[1]));
+  if (ms_total) *ms_total = ms;
+  if (ms_main_mean) {
+    for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, ctx->evs[2 + 2 * k], ctx->evs[3 + 2 * k]));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double acc = 0;
+    for (int k = 0; k < nsteps; ++k) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, ctx->evs[2 + 2 * k], ctx->evs[3 + 2 * k]));
+      acc += t;
+    }
+    *ms_main_mean = acc / nsteps;
+  }
+  return poll_status(ctx);
+}
+
+extern "C" int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes) {
+  if (!ctx || !bytes) return LBMIB_ERR_ARG;
+  *bytes = 2.0 * 19.0 * (double)ctx->esz;
+  return LBMIB_OK;
+}

@@ -1,0 +1,107 @@
+// lbmib_internal.h -- device-side types shared by the HIP translation units of
+// liblbmib.so.  Nothing here is part of the public C-ABI (include/lbmib.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lbmib.h"
+
+namespace lbmib {
+
+// D3Q19 lattice in the reference's direction order (lbm_ib.c:777-833).
+constexpr int kCX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int kCY[19] = {0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, 1, -1, -1, 1, 1, -1};
+constexpr int kCZ[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0};
+// pairs (2k-1, 2k) are opposite (Appendix of the layout: lbm_ib.c:2310-2423)
+constexpr int kOPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+// 1/w_q denominators (lbm_ib.c:1742, 1773, 1804)
+constexpr int kWDEN[19] = {3, 18, 18, 18, 18, 18, 18, 36, 36, 36,
+                                                36, 36, 36, 36, 36, 36, 36, 36, 36};
+
+// The reference's truncated pi (lbm_ib.c:33).
+constexpr double kRefPi = 3.14159265358979;
+
+constexpr int kMaxSheets = 16;
+constexpr int kDevScale16 = 256;  // 16-bit deviation storage holds (f - w) * 2^8
+
+// Fibernode-equivalent SoA offsets inside the node force array.
+enum { NF_BX = 0, NF_BY, NF_BZ, NF_SX, NF_SY, NF_SZ, NF_EX, NF_EY, NF_EZ, NF_COUNT };
+
+// Fiber kernel task bits.
+enum { FK_BEND = 1, FK_STRETCH = 2, FK_ELASTIC = 4, FK_PREP = 8, FK_ALL = 15 };
+
+// Device error bits (combined with atomicOr, mapped to LBMIB_ERR_* on the host).
+enum { DERR_FIBER_OOB = 1, DERR_BIN_OVERFLOW = 2 };
+
+struct SheetDesc {
+  int rows, cols, off;   // node range [off, off + rows*cols)
+  int pad;
+  double ds1, ds2;       // spacings along cols (z) and rows (y), lbm_ib.c:994-995
+  double bend_c;         // Kb / ds2^3 (both passes, lbm_ib.c:996, :1065)
+  double str_row_c;      // Ks / ds1^2 (lbm_ib.c:1171)
+  double str_col_c;      // Ks / ds2^2 (lbm_ib.c:1235)
+};
+
+// Everything the fused fluid kernels need, passed by value.
+template <class T>
+struct FluidConsts {
+  T omc, om;          // 1 - 1/tau, 1/tau
+  T pref[19];         // dt (1 - 1/(2 tau)) / wden_q      (F64 exact form)
+  T pw[19];           // w_q (1 - 1/(2 tau))               (deviation form)
+  T cs2, cs4;         // cs^2, cs^4 as the reference computes them
+  T g;
+  T rho0, u0x, u0y, u0z;
+  T tab_in[19], tab_out[19];  // f (F64) or f - w (deviation forms)
+};
+
+struct Geometry {
+  int nx, ny, nz;
+  int bc[3];
+  int lo[3], hi[3];    // active box per axis
+  int mlo[3], mhi[3];  // moment box per axis
+  long n;              // cells
+  long pitch;          // elements between population planes
+};
+
+// IB device state pointers, passed by value to the IB kernels.
+struct IBState {
+  int nsheets, nnodes;
+  const SheetDesc *sheets;
+  double *pos;       // [3][M]
+  double *frc;       // [9][M]
+  int *start;        // [3][M]; start[0] < 0 marks an invalid node
+  double *wgt;       // [12][M]: 1+cos for x(4), y(4), z(4)
+  int *bin_count;    // [nbins]
+  int *bin_list;     // [nbins][cap]
+  int nby, nbz, cap;
+  int *box;          // [2][kMaxSheets][6]
+  double *U;         // band velocity [3][n]
+  int *err;
+  int quirks;
+};
+
+// host-side launchers (defined in lbmib_fluid.hip / lbmib_ib.hip)
+struct Launch {
+  hipStream_t stream;
+};
+
+int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
+int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
+                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                          const void *Fcur, const void *Fprev, hipStream_t s);
+int launch_materialize(int prec, const Geometry &g, const void *f, int own_mode, const FluidConsts<double> &c,
+                       int nsheets, const int *box, const void *F, double *df, double *rho, double *vel,
+                       hipStream_t s);
+int launch_totals(int prec, const Geometry &g, const void *f, int own_mode, const FluidConsts<double> &c,
+                  double *partials, int nblocks_max, double *out4, hipStream_t s);
+int totals_blocks(const Geometry &g);
+
+int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s);
+int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s);
+int launch_band_moments(int prec, const IBState &ib, const Geometry &g, const void *f, int slot, const void *F,
+                        const FluidConsts<double> &c, hipStream_t s);
+int launch_move(const IBState &ib, const Geometry &g, int reset_slot, hipStream_t s);
+int launch_ib_reset(const IBState &ib, int reset_bins, int reset_slot_mask, hipStream_t s);
+
+}  // namespace lbmib

@@ -1,0 +1,285 @@
+"""Python binding of the MI355X LBM-IB engine (ctypes over liblbmib.so).
+
+The engine's product surface is the C-ABI in include/lbmib.h; this module
+mirrors it for tests and benchmarks.  Arrays cross the boundary as host
+numpy buffers; all simulation state stays on the device.
+
+There is no CPU fallback: if liblbmib.so is missing, or no GPU is present,
+constructing an Engine raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbmib.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_FIBER_OOB, ERR_BIN_OVERFLOW, ERR_NONFINITE, ERR_STATE = range(8)
+ERR_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_ALLOC", 4: "ERR_FIBER_OOB",
+             5: "ERR_BIN_OVERFLOW", 6: "ERR_NONFINITE", 7: "ERR_STATE"}
+BC_INOUT, BC_WALL, BC_PERIODIC = 0, 1, 2
+PRECISIONS = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3}
+STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}
+
+# Entry points of include/lbmib.h that take only the context.
+CTX_CALLS = (
+    "init_eqlbrmdistrfuncDF0", "init_df1", "init_df_inout",
+    "compute_bendingforce", "compute_stretchingforce", "compute_elasticforce", "compute_fiber_forces",
+    "fiber_SpreadForce", "fluid_get_SpreadForce", "get_influentialdomain_fluid_grid_and_SpreadForce",
+    "compute_eqlbrmdistrfuncDF1", "stream_distrfunc", "bounceback_rigidwalls", "copy_inout_to_df2",
+    "periodicBC", "compute_rho_and_u", "fluid_SpreadVelocity", "fiber_get_SpreadVelocity",
+    "moveFiberSheet", "replace_old_DF", "sync", "status",
+)
+# Every symbol the header declares (checked by the CPU test suite).
+ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lbmib_step",
+               "lbmib_get_time", "lbmib_get_fibers", "lbmib_get_macroscopic", "lbmib_get_df",
+               "lbmib_get_force", "lbmib_get_totals", "lbmib_get_band_box", "lbmib_time_steps",
+               "lbmib_bytes_per_cell") + tuple("lbmib_" + n for n in CTX_CALLS)
+
+
+class LbmibError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where}: {ERR_NAMES.get(code, code)}")
+        self.code = code
+
+
+class FluidParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("bc_x", ctypes.c_int), ("bc_y", ctypes.c_int), ("bc_z", ctypes.c_int),
+                ("precision", ctypes.c_int), ("quirks", ctypes.c_int),
+                ("tau", ctypes.c_double), ("rho0", ctypes.c_double),
+                ("u0", ctypes.c_double * 3), ("g", ctypes.c_double)]
+
+
+class SheetParams(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int), ("cols", ctypes.c_int),
+                ("width", ctypes.c_double), ("height", ctypes.c_double),
+                ("ks", ctypes.c_double), ("kb", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads liblbmib.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run __graft_entry__.build() or make -C distributed-lb-ib_amd)")
+    L = ctypes.CDLL(path)
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.lbmib_create.argtypes = [ctypes.POINTER(FluidParams), i, ctypes.POINTER(SheetParams), ctypes.POINTER(vp)]
+    L.lbmib_destroy.argtypes = [vp]
+    L.lbmib_set_fiber_positions.argtypes = [vp, i, dp]
+    L.lbmib_step.argtypes = [vp, i]
+    L.lbmib_get_time.argtypes = [vp, ip]
+    L.lbmib_get_fibers.argtypes = [vp, i, dp]
+    L.lbmib_get_macroscopic.argtypes = [vp, dp, dp]
+    L.lbmib_get_df.argtypes = [vp, dp]
+    L.lbmib_get_force.argtypes = [vp, dp]
+    L.lbmib_get_totals.argtypes = [vp, dp]
+    L.lbmib_get_band_box.argtypes = [vp, i, ip]
+    L.lbmib_time_steps.argtypes = [vp, i, i, dp, dp]
+    L.lbmib_bytes_per_cell.argtypes = [vp, dp]
+    for n in CTX_CALLS:
+        getattr(L, "lbmib_" + n).argtypes = [vp]
+    for n in ALL_SYMBOLS:
+        getattr(L, n).restype = i
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ------------------------------------------------------------ sheet geometry
+
+def splitmix64(seed):
+    """SplitMix64 generator (Steele, Lea, Flood 2014); yields uint64 ints."""
+    state = seed & 0xFFFFFFFFFFFFFFFF
+    mask = 0xFFFFFFFFFFFFFFFF
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & mask
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        yield z ^ (z >> 31)
+
+
+def uniform_stream(seed):
+    """Doubles in [0, 1) from the top 53 bits of SplitMix64."""
+    for z in splitmix64(seed):
+        yield (z >> 11) * (1.0 / 9007199254740992.0)
+
+
+def sheet_positions(rows, cols, width, height, x0, y0, z0, jitter=0.0, seed=0):
+    """Node coordinates of gen_fiber_shape (lbm_ib.c:899-906): the sheet is
+    normal to x, rows run along y (spacing height/(rows-1)) and columns along
+    z (spacing width/(cols-1)).  Optional jitter adds U(-jitter, jitter) to
+    each coordinate, drawn in (row, col, xyz) order from SplitMix64(seed).
+    Returns float64 [rows, cols, 3]."""
+    zdist = width / (cols - 1)
+    ydist = height / (rows - 1)
+    xyz = np.empty((rows, cols, 3), dtype=np.float64)
+    for r in range(rows):
+        for c in range(cols):
+            xyz[r, c, 0] = x0
+            xyz[r, c, 1] = ydist * r + y0
+            xyz[r, c, 2] = zdist * c + z0
+    if jitter:
+        u = uniform_stream(seed)
+        flat = xyz.reshape(-1)
+        for k in range(flat.size):
+            flat[k] += jitter * (2.0 * next(u) - 1.0)
+    return xyz
+
+
+class Sheet:
+    """One fiber sheet: parameters plus initial positions [rows, cols, 3]."""
+
+    def __init__(self, rows, cols, width, height, ks, kb, xyz):
+        self.rows, self.cols = rows, cols
+        self.width, self.height = width, height
+        self.ks, self.kb = ks, kb
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(rows, cols, 3)
+
+    @classmethod
+    def centred(cls, rows, cols, spacing, centre, ks, kb, jitter=0.0, seed=0):
+        width = spacing * (cols - 1)
+        height = spacing * (rows - 1)
+        xc, yc, zc = centre
+        xyz = sheet_positions(rows, cols, width, height, xc, yc - height / 2.0, zc - width / 2.0, jitter, seed)
+        return cls(rows, cols, width, height, ks, kb, xyz)
+
+
+# ------------------------------------------------------------------ engine
+
+class Engine:
+    """One simulation on the GPU behind the C-ABI.
+
+    ``precision``: "f64", "f32", "f16" or "bf16".  ``bc``: per-axis boundary
+    kinds (x may be BC_INOUT).  Field arrays returned by the getters are
+    indexed [..., z, y, x].
+    """
+
+    def __init__(self, nx, ny, nz, *, bc=(BC_INOUT, BC_WALL, BC_WALL), precision="f32", tau=0.6,
+                 rho0=1.0, u0=(0.01, 0.0, 0.0), quirks=True, sheets=(), g=0.0, preshift=True,
+                 preshift_all=True):
+        self.L = load_library()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (nz, ny, nx)
+        self.precision = precision
+        self.sheets = list(sheets)
+        fp = FluidParams(nx, ny, nz, bc[0], bc[1], bc[2], PRECISIONS[precision], int(bool(quirks)),
+                         tau, rho0, (ctypes.c_double * 3)(*u0), g)
+        sp = (SheetParams * max(1, len(self.sheets)))()
+        for k, s in enumerate(self.sheets):
+            sp[k] = SheetParams(s.rows, s.cols, s.width, s.height, s.ks, s.kb)
+        h = ctypes.c_void_p()
+        self._check(self.L.lbmib_create(ctypes.byref(fp), len(self.sheets), sp, ctypes.byref(h)), "lbmib_create")
+        self.h = h
+        for k, s in enumerate(self.sheets):
+            xyz = s.xyz.copy()
+            # init_gv shifts the sheet by u_l*dt before the first step
+            # (lbm_ib.c:857-862); the reference does it for sheets[0] only.
+            if preshift and (preshift_all or k == 0):
+                xyz[..., 0] += u0[0] * 1
+            self.set_positions(k, xyz)
+
+    @staticmethod
+    def _check(rc, where):
+        if rc != OK:
+            raise LbmibError(rc, where)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lbmib_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def call(self, name):
+        """Calls one context-only entry point, e.g. ``call("compute_rho_and_u")``."""
+        self._check(getattr(self.L, "lbmib_" + name)(self.h), "lbmib_" + name)
+
+    def set_positions(self, sheet, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self._check(self.L.lbmib_set_fiber_positions(self.h, sheet, _dptr(xyz)), "set_fiber_positions")
+
+    def init(self):
+        for n in ("init_eqlbrmdistrfuncDF0", "init_df1", "init_df_inout"):
+            self.call(n)
+        return self
+
+    def step(self, n=1):
+        self._check(self.L.lbmib_step(self.h, n), "lbmib_step")
+
+    def sync(self):
+        self._check(self.L.lbmib_sync(self.h), "lbmib_sync")
+
+    def status(self):
+        return self.L.lbmib_status(self.h)
+
+    @property
+    def time(self):
+        t = ctypes.c_int()
+        self._check(self.L.lbmib_get_time(self.h, ctypes.byref(t)), "get_time")
+        return t.value
+
+    def fibers(self, sheet=0):
+        s = self.sheets[sheet]
+        out = np.empty((s.rows, s.cols, 12), dtype=np.float64)
+        self._check(self.L.lbmib_get_fibers(self.h, sheet, _dptr(out)), "get_fibers")
+        return out
+
+    def macroscopic(self):
+        rho = np.empty(self.shape, dtype=np.float64)
+        vel = np.empty((3,) + self.shape, dtype=np.float64)
+        self._check(self.L.lbmib_get_macroscopic(self.h, _dptr(rho), _dptr(vel)), "get_macroscopic")
+        return rho, vel
+
+    def df(self):
+        out = np.empty((19,) + self.shape, dtype=np.float64)
+        self._check(self.L.lbmib_get_df(self.h, _dptr(out)), "get_df")
+        return out
+
+    def force(self):
+        out = np.empty((3,) + self.shape, dtype=np.float64)
+        self._check(self.L.lbmib_get_force(self.h, _dptr(out)), "get_force")
+        return out
+
+    def totals(self):
+        out = np.empty(4, dtype=np.float64)
+        self._check(self.L.lbmib_get_totals(self.h, _dptr(out)), "get_totals")
+        return out
+
+    def band_box(self, sheet=0):
+        b = (ctypes.c_int * 6)()
+        self._check(self.L.lbmib_get_band_box(self.h, sheet, b), "get_band_box")
+        return tuple(b)
+
+    def time_steps(self, nwarm, nsteps, main_kernel=True):
+        tot = ctypes.c_double()
+        main = ctypes.c_double()
+        self._check(self.L.lbmib_time_steps(self.h, nwarm, nsteps, ctypes.byref(tot),
+                                            ctypes.byref(main) if main_kernel else None), "time_steps")
+        return tot.value, (main.value if main_kernel else None)
+
+    def bytes_per_cell(self):
+        b = ctypes.c_double()
+        self._check(self.L.lbmib_bytes_per_cell(self.h, ctypes.byref(b)), "bytes_per_cell")
+        return b.value

@@ -1,0 +1,150 @@
+/*
+ * lbmib.h -- C-ABI of the MI355X LBM-IB engine (liblbmib.so).
+ *
+ * Plain C types only: an opaque context, ints, doubles and host pointers.
+ * Every per-step entry point enqueues HIP kernels on the context's stream and
+ * returns immediately; results become visible through the get/sync calls.
+ *
+ * Each declaration names the reference routine it replaces
+ * (cube_pthread/lbm_ib.c unless a path is given).  The reference passes an
+ * `LV = {int tid; GV gv}` to every per-step routine (lbm_ib.c:164-167) and
+ * calls it from all P*Q*R threads; here one host thread calls each entry
+ * point once with the context that replaces GV.  The host layer in
+ * include/lbmib_host.h keeps the reference's own names and LV signature on
+ * top of these calls.
+ *
+ * Errors: the reference calls exit(1) when a fiber's support leaves the grid
+ * (lbm_ib.c:1458-1466, 2624-2632) and abort() on bad flags.  Here every entry
+ * point returns an LBMIB_* status; device-side conditions are latched in a
+ * device flag and reported by lbmib_status() and by every synchronising call.
+ */
+#ifndef LBMIB_H
+#define LBMIB_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define LBMIB_OK 0
+#define LBMIB_ERR_ARG 1          /* invalid argument / configuration */
+#define LBMIB_ERR_HIP 2          /* HIP runtime call failed */
+#define LBMIB_ERR_ALLOC 3        /* device or host allocation failed */
+#define LBMIB_ERR_FIBER_OOB 4    /* fiber support left the grid (lbm_ib.c:1458-1466) */
+#define LBMIB_ERR_BIN_OVERFLOW 5 /* more fiber nodes in one spread bin than its capacity */
+#define LBMIB_ERR_NONFINITE 6    /* NaN/Inf in the mass/momentum watchdog */
+#define LBMIB_ERR_STATE 7        /* call out of order (e.g. step before init) */
+
+/* ---- boundary kinds per axis ---- */
+#define LBMIB_BC_INOUT 0    /* x only: fixed-equilibrium planes x=1,2 and N-3,N-2 (lbm_ib.c:2769-2907) */
+#define LBMIB_BC_WALL 1     /* half-way bounce-back at planes 2 and N-3 (lbm_ib.c:2257-2437) */
+#define LBMIB_BC_PERIODIC 2 /* wrap over the whole axis */
+
+/* ---- population storage ---- */
+#define LBMIB_F64 0  /* f in FP64, FP64 arithmetic in the reference's operation order */
+#define LBMIB_F32 1  /* f - w_i in FP32, FP32 collision */
+#define LBMIB_F16 2  /* (f - w_i) * 2^8 in FP16, FP32 collision, FP64 reductions */
+#define LBMIB_BF16 3 /* (f - w_i) * 2^8 in BF16, FP32 collision, FP64 reductions */
+
+typedef struct lbmib_ctx lbmib_ctx;
+
+/* Fluid configuration.  The reference hard-codes these in init_gv
+ * (lbm_ib.c:750-773): tau = 0.5004, rho_l = 1, u_l = (0.001, 0, 0), g = 0. */
+typedef struct {
+  int nx, ny, nz;      /* grid; x is the inlet->outlet axis and the sheet normal */
+  int bc_x, bc_y, bc_z;
+  int precision;       /* LBMIB_F64 / F32 / F16 / BF16 */
+  int quirks;          /* 1: reproduce the reference's two fiber-force slips */
+  double tau;
+  double rho0;
+  double u0[3];        /* initial velocity; inlet/outlet tables are feq(rho0, u0) */
+  double g;            /* gravity along z (lbm_ib.c:769 sets 0) */
+} lbmib_fluid_params;
+
+/* One fiber sheet (Fibersheet, lbm_ib.c:68-76): rows along y, cols along z.
+ * Spacings are ds1 = width/(cols-1), ds2 = height/(rows-1)
+ * (lbm_ib.c:994-995). ks/kb are Ks_l and Kb_l. */
+typedef struct {
+  int rows, cols;
+  double width, height;
+  double ks, kb;
+} lbmib_sheet_params;
+
+/* ---- construction (gen_fluid_grid lbm_ib.c:924, gen_fiber_shape :866, init_gv :705) ---- */
+int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const lbmib_sheet_params *sheets,
+                 lbmib_ctx **out);
+int lbmib_destroy(lbmib_ctx *ctx);
+/* Uploads sheet positions, rows*cols*3 doubles in (row, col, xyz) order.
+ * Replaces the coordinate fill of gen_fiber_shape (lbm_ib.c:899-906). */
+int lbmib_set_fiber_positions(lbmib_ctx *ctx, int sheet, const double *xyz);
+
+/* ---- initialisation ---- */
+int lbmib_init_eqlbrmdistrfuncDF0(lbmib_ctx *ctx); /* lbm_ib.c:1528-1606 (rho/u of init_gv :837-849) */
+int lbmib_init_df1(lbmib_ctx *ctx);                 /* lbm_ib.c:2725-2767 */
+int lbmib_init_df_inout(lbmib_ctx *ctx);            /* lbm_ib.c:1610-1670 */
+
+/* ---- fiber forces (one kernel per sheet) ---- */
+int lbmib_compute_bendingforce(lbmib_ctx *ctx);    /* lbm_ib.c:961-1152 */
+int lbmib_compute_stretchingforce(lbmib_ctx *ctx); /* lbm_ib.c:1154-1282 */
+int lbmib_compute_elasticforce(lbmib_ctx *ctx);    /* lbm_ib.c:1285-1318 */
+int lbmib_compute_fiber_forces(lbmib_ctx *ctx);    /* the three above in one launch */
+
+/* ---- force spreading: deterministic cell-owned gather ---- */
+int lbmib_fiber_SpreadForce(lbmib_ctx *ctx);      /* mpi/fiber_SpreadForce.c:64-283: bin nodes, support boxes */
+int lbmib_fluid_get_SpreadForce(lbmib_ctx *ctx);  /* mpi/fluid_get_SpreadForce.c:28-189: cell-owned apply */
+int lbmib_get_influentialdomain_fluid_grid_and_SpreadForce(lbmib_ctx *ctx); /* lbm_ib.c:1323-1525 */
+
+/* ---- fluid step ----
+ * compute_eqlbrmdistrfuncDF1 launches the fused kernel: it pulls the previous
+ * step's post-collision populations (streaming, bounce-back and inlet/outlet
+ * of the previous step), then collides with the lagged rho/u and the current
+ * force.  The separate stream/bounce-back/inout/periodic calls are therefore
+ * no-ops kept for drop-in compatibility. */
+int lbmib_compute_eqlbrmdistrfuncDF1(lbmib_ctx *ctx); /* lbm_ib.c:1673-1825 (+ :1827, :2257, :2769) */
+int lbmib_stream_distrfunc(lbmib_ctx *ctx);           /* lbm_ib.c:1827-2254: fused, no-op */
+int lbmib_bounceback_rigidwalls(lbmib_ctx *ctx);      /* lbm_ib.c:2257-2437: fused, no-op */
+int lbmib_copy_inout_to_df2(lbmib_ctx *ctx);          /* lbm_ib.c:2769-2907: fused, no-op */
+int lbmib_periodicBC(lbmib_ctx *ctx);                 /* lbm_ib.c:2970-3083: wrap is fused; no-op */
+int lbmib_compute_rho_and_u(lbmib_ctx *ctx);          /* lbm_ib.c:2440-2538 over the sheets' support band */
+int lbmib_fluid_SpreadVelocity(lbmib_ctx *ctx);       /* mpi/fluid_SpreadVelocity.c:25-163 (= band rho/u) */
+int lbmib_fiber_get_SpreadVelocity(lbmib_ctx *ctx);   /* mpi/fiber_get_SpreadVelocity.c:26-276: interpolate + move */
+int lbmib_moveFiberSheet(lbmib_ctx *ctx);             /* lbm_ib.c:2540-2723 (uses the band rho/u) */
+int lbmib_replace_old_DF(lbmib_ctx *ctx);             /* lbm_ib.c:2909-2968: buffer swap, time += dt */
+
+/* ---- whole steps: the do_thread loop body, lbm_ib.c:3115-3563 ---- */
+int lbmib_step(lbmib_ctx *ctx, int nsteps);
+
+/* ---- queries (synchronising) ---- */
+int lbmib_sync(lbmib_ctx *ctx);
+int lbmib_status(lbmib_ctx *ctx);
+int lbmib_get_time(lbmib_ctx *ctx, int *time);
+/* Fibernode records (lbm_ib.c:54-59): rows*cols*12 doubles
+ * {x,y,z, bend xyz, stretch xyz, elastic xyz}. */
+int lbmib_get_fibers(lbmib_ctx *ctx, int sheet, double *out);
+/* Full-field rho (n) and velocity (3n) after the last step, with the
+ * reference's frozen planes (x = 2, N-3 keep their initial values). */
+int lbmib_get_macroscopic(lbmib_ctx *ctx, double *rho, double *vel);
+/* Post-stream populations df2 (19n, = df1 after replace_old_DF) as f,
+ * on active cells; zero on dead/ghost cells. */
+int lbmib_get_df(lbmib_ctx *ctx, double *df);
+/* Force field of the last spread (3n), zero outside the sheets' band. */
+int lbmib_get_force(lbmib_ctx *ctx, double *force);
+/* FP64 deterministic totals over active cells of the post-stream state:
+ * out = {mass, momentum x, y, z}.  Non-finite totals latch LBMIB_ERR_NONFINITE. */
+int lbmib_get_totals(lbmib_ctx *ctx, double *out4);
+/* Support box of the last spread for one sheet: {x0,x1,y0,y1,z0,z1}. */
+int lbmib_get_band_box(lbmib_ctx *ctx, int sheet, int *box6);
+
+/* ---- measurement ----
+ * Runs nsteps full steps after nwarm untimed ones and reports, from HIP
+ * events on the context's stream: total milliseconds, and the mean duration
+ * of the fused collide-stream kernel (timed in a separate pass of the same
+ * length with one event pair per launch). */
+int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *ms_total, double *ms_main_mean);
+/* Bytes per cell update of the population storage (2 * 19 * sizeof). */
+int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBMIB_H */

@@ -1,0 +1,156 @@
+/*
+ * lbmib_host.h -- the reference's per-step C API on top of the MI355X engine
+ * (liblbmib_host.so).
+ *
+ * Same names, argument types and call order as cube_pthread/lbm_ib.c: a
+ * caller written against the reference (GV/LV structs, do_thread-style loop
+ * with a barrier after each call) links this library instead.  Every
+ * per-step routine takes the reference's LV; when several threads call it,
+ * only tid 0 enqueues device work and the others return at once, so the
+ * caller's barriers keep working.  The fluid state lives on the GPU; the
+ * fiber arrays and, for dumps, the fluid mirror are refreshed on the host by
+ * lbmib_sync_to_host().
+ *
+ * Errors: the reference exit(1)s on a fiber leaving the grid
+ * (lbm_ib.c:1458-1466); here the first device error is stored in
+ * gv->last_error and returned by lbmib_host_status().
+ */
+#ifndef LBMIB_HOST_H
+#define LBMIB_HOST_H
+
+#include "lbmib.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fibernode, lbm_ib.c:54-59 (96 B, same field order) */
+typedef struct fiber_node_t {
+  double x, y, z;
+  double bend_force_x, bend_force_y, bend_force_z;
+  double stretch_force_x, stretch_force_y, stretch_force_z;
+  double elastic_force_x, elastic_force_y, elastic_force_z;
+} Fibernode;
+
+/* Fiber, Fibersheet, Fibershape, lbm_ib.c:62-83 */
+typedef struct fiber_t {
+  Fibernode *nodes;
+  int num_nodes;
+} Fiber;
+
+typedef struct fiber_sheet_t {
+  Fiber *fibers;
+  double width, height;
+  int num_cols, num_rows;
+  double x_orig, y_orig, z_orig;
+} Fibersheet;
+
+typedef struct fiber_shape_t {
+  Fibersheet *sheets;
+  int num_sheets;
+} Fibershape;
+
+/* Fluidgrid, lbm_ib.c:124-132.  The populations live on the device; the
+ * host keeps the dimensions and, after lbmib_sync_to_host(), a mirror of the
+ * post-stream state in x-fastest arrays (index x + x_dim*(y + y_dim*z)). */
+typedef struct fluid_grid_t {
+  int x_dim, y_dim, z_dim;
+  double *df;     /* [19][n]: df1 == df2 after replace_old_DF */
+  double *dfeq;   /* [19][n]: equilibrium of the last collision */
+  double *rho;    /* [n] */
+  double *vel;    /* [3][n] */
+  double *force;  /* [3][n] elastic_force_{x,y,z} */
+} Fluidgrid;
+
+/* GV, lbm_ib.c:135-161, plus the device context and the physics switches
+ * that the reference hard-codes. */
+typedef struct gv_t {
+  Fibershape *fiber_shape;
+  Fluidgrid *fluid_grid;
+  double tau, nu_l, u_l, rho_l, L_l, M_l, g_l, Ks_l, Kb_l, Kst_l;
+  double Re, cs_l, Ma, Kn, Kshat, Ksthat, Kbhat, Mhat, Fr;
+  int dt, time, TIME_STOP, dump;
+  double c[19][3];
+  int ib, ie, je, jb, ke, kb;
+  int total_threads;
+  int cube_size;
+  int num_cubes_x, num_cubes_y, num_cubes_z;
+  int P, Q, R;
+  /* -- MI355X engine -- */
+  lbmib_ctx *dev;
+  int custom_physics; /* 1: init_gv keeps tau,u_l,rho_l,Ks_l,Kb_l and the fields below */
+  int bc_x, bc_y, bc_z;
+  int precision;      /* LBMIB_F64 / F32 / F16 / BF16 */
+  int quirks;
+  int shift_all_sheets;
+  int last_error;
+  int quiet;
+  int dump_mask;      /* do_thread dumps: 1 binary fiber, 2 fluid cubes */
+} *GV;
+
+/* LV, lbm_ib.c:164-167 */
+typedef struct lv_t {
+  int tid;
+  GV gv;
+} *LV;
+
+/* construction: lbm_ib.c:705, :866, :924 */
+void init_gv(GV gv, Fibershape *fiber_shape, Fluidgrid *fluid_grid, int cube_size);
+Fibershape *gen_fiber_shape(GV gv, double width, double height, int num_cols, int num_rows, double x0, double y0,
+                            double z0);
+Fluidgrid *gen_fluid_grid(GV gv, int elem_x, int elem_y, int elem_z, int sub_fluid_grid_dim_k);
+/* Places sheet s with its own geometry and U(-jitter, jitter) node jitter
+ * drawn from SplitMix64(seed) in (row, col, xyz) order.  Not in the
+ * reference (its gen_fiber_shape gives every sheet the same geometry). */
+void gen_fiber_sheet_jittered(GV gv, int s, double width, double height, int num_cols, int num_rows, double x0,
+                              double y0, double z0, double jitter, unsigned long long seed);
+
+/* mappings, lbm_ib.c:3572-3602 (kept for callers that partition host work) */
+int cube2thread(int BI, int BJ, int BK, int num_cubes_x, int num_cubes_y, int num_cubes_z, int P, int Q, int R);
+int fiber2thread(int fiber_row, int num_fibers, int num_threads);
+
+/* IB routines, lbm_ib.c:213-216 and the mpi/ twins */
+void compute_bendingforce(LV lv);
+void compute_stretchingforce(LV lv);
+void compute_elasticforce(LV lv);
+void get_influentialdomain_fluid_grid_and_SpreadForce(LV lv);
+void fiber_SpreadForce(LV lv);
+void fluid_get_SpreadForce(LV lv);
+void fluid_SpreadVelocity(LV lv);
+void fiber_get_SpreadVelocity(LV lv);
+
+/* LBM routines, lbm_ib.c:219-228 */
+void init_eqlbrmdistrfuncDF0(GV gv);
+void init_df_inout(GV gv);
+void init_df1(GV gv);
+void compute_eqlbrmdistrfuncDF1(LV lv);
+void stream_distrfunc(LV lv);
+void bounceback_rigidwalls(LV lv);
+void compute_rho_and_u(LV lv);
+void moveFiberSheet(LV lv);
+void copy_inout_to_df2(LV lv);
+void periodicBC(LV lv);
+void replace_old_DF(LV lv);
+
+/* the step loop, lbm_ib.c:3087-3570 (single caller; dumps at time==1 and
+ * every gv->dump steps when dumps are enabled) */
+void *do_thread(void *v);
+
+/* dump writers, lbm_ib.c:332-392, :560-646 */
+void save_fiber_sub_grid(GV gv, int start_y, int start_z, int end_y, int end_z, char fName[]);
+void save_fiber_sub_grid_binary(GV gv, int start_y, int start_z, int end_y, int end_z, char fName[]);
+void save_fluid_sub_grid(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y, int end_z,
+                         char fName[]);
+double get_cur_time(void);
+
+/* host mirror and status */
+int lbmib_sync_to_host(GV gv, int with_fluid);
+int lbmib_host_status(GV gv);
+void lbmib_host_free(GV gv);
+/* dump switches for do_thread: 1 = binary fiber dump, 2 = fluid cube dump */
+void lbmib_host_set_dumps(GV gv, int mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBMIB_HOST_H */

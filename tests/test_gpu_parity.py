@@ -1,0 +1,208 @@
+"""HIP path against the FP64 oracle and the reference's own fixtures.
+
+Everything here runs the kernels of liblbmib.so through its C-ABI.
+Tolerances:
+  F64            <= 1e-12 relative (positions, fields) -- same operation order
+                 as the oracle; only cos() and the order of FMA-free sums on
+                 the device can differ by an ulp.
+  F32            velocity <= 1e-5 relative to max|u| against the FP64 oracle.
+  F16 / BF16     velocity <= 1e-3 / 1e-2 relative to max|u|.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, read_fiber_dump, reference_sheet_xyz
+from oracle.oracle import BC_INOUT, BC_PERIODIC, BC_WALL, Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_for(kw, preshift=True, preshift_all=True):
+    o = Oracle(kw["nx"], kw["ny"], kw["nz"], kw["bc"], kw["tau"], kw["rho0"], kw["u0"], kw.get("quirks", True))
+    for s in kw.get("sheets", []):
+        o.add_sheet(s.rows, s.cols, s.width, s.height, s.ks, s.kb, s.xyz)
+    o.init()
+    if preshift and kw.get("sheets"):
+        o.preshift(all_sheets=preshift_all)
+    return o
+
+
+def active_slices(kw):
+    sl = []
+    for a, n in ((2, kw["nz"]), (1, kw["ny"]), (0, kw["nx"])):
+        if kw["bc"][a] == BC_PERIODIC:
+            sl.append(slice(0, n))
+        else:
+            sl.append(slice(2, n - 2))
+    return tuple(sl)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def small_ib_config(lbmib, precision, bc=(BC_INOUT, BC_PERIODIC, BC_WALL), quirks=True, n=(40, 32, 36)):
+    nx, ny, nz = n
+    sheet = lbmib.Sheet.centred(10, 12, 1.0, (14.0, ny / 2.0, nz / 2.0), 2e-3, 2e-3, jitter=0.08, seed=11)
+    return dict(nx=nx, ny=ny, nz=nz, bc=bc, precision=precision, tau=0.7, rho0=1.0,
+                u0=(0.01, 0.0, 0.0), quirks=quirks, sheets=[sheet])
+
+
+# ------------------------------------------------------------ fixtures replay
+
+def test_f64_replays_reference_fixture_refA(lbmib, configs, fixtures_meta):
+    """The compiled reference's own dumps (64^3, 26x26 sheet, 20 steps) from
+    the HIP FP64 path."""
+    meta = fixtures_meta["refA"]
+    kw = configs.reference_mode(meta["grid"], meta["rows"], meta["cols"], meta["width"], meta["height"],
+                                tuple(meta["origin"]), precision="f64")
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        done = 0
+        for step in meta["fiber_steps"]:
+            e.step(step - done)
+            done = step
+            assert e.status() == 0
+            _, ref = read_fiber_dump(os.path.join(GOLDEN, f"refA_fiber_step{step}.bin"))
+            got = e.fibers(0).reshape(-1, 12)
+            print(f"step {step}: pos {np.abs(got[:, :3] - ref[:, :3]).max():.3e}  "
+                  f"force {np.abs(got[:, 3:] - ref[:, 3:]).max():.3e} (max |F| {np.abs(ref[:, 3:]).max():.3e})")
+            assert np.abs(got[:, :3] - ref[:, :3]).max() <= 1e-12
+            assert np.abs(got[:, 3:] - ref[:, 3:]).max() <= 1e-9 * max(np.abs(ref[:, 3:]).max(), 1e-12) + 1e-20
+        fx = np.load(os.path.join(GOLDEN, f"refA_fluid_step{meta['fluid_steps'][-1]}.npz"))
+        X, Y, Z = fx["xyz"][:, 0], fx["xyz"][:, 1], fx["xyz"][:, 2]
+        rho, vel = e.macroscopic()
+        df = e.df()
+        frc = e.force()
+        assert rel(vel[:, Z, Y, X].T, fx["vel"]) < 1e-11
+        assert np.abs(rho[Z, Y, X] - fx["rho"]).max() < 1e-13
+        assert np.abs(df[:, Z, Y, X].T - fx["df2"]).max() < 1e-14
+        assert np.abs(frc[:, Z, Y, X].T - fx["force"]).max() <= 1e-9 * np.abs(fx["force"]).max() + 1e-22
+
+
+# ---------------------------------------------------------- F64 vs oracle
+
+BC_CASES = [
+    (BC_INOUT, BC_WALL, BC_WALL),
+    (BC_INOUT, BC_PERIODIC, BC_WALL),
+    (BC_PERIODIC, BC_PERIODIC, BC_PERIODIC),
+    (BC_WALL, BC_WALL, BC_WALL),
+    (BC_WALL, BC_PERIODIC, BC_PERIODIC),
+]
+
+
+@pytest.mark.parametrize("bc", BC_CASES)
+def test_f64_fluid_matches_oracle(lbmib, bc):
+    kw = dict(nx=20, ny=17, nz=14, bc=bc, precision="f64", tau=0.8, rho0=1.0, u0=(0.02, 0.003, -0.004),
+              quirks=True, sheets=[])
+    o = oracle_for(kw)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        # before any step: df1 = feq on the active box
+        sl = (slice(None),) + active_slices(kw)
+        assert np.array_equal(e.df()[sl], o.df1()[sl])
+        for k in range(3):
+            e.step(2)
+            o.step(2)
+            assert rel(e.df()[sl], o.df1()[sl]) < 1e-14
+            rho, vel = e.macroscopic()
+            assert rel(rho, o.rho()) < 1e-14
+            assert rel(vel, o.vel()) < 1e-12
+
+
+@pytest.mark.parametrize("bc", [(BC_INOUT, BC_PERIODIC, BC_WALL), (BC_INOUT, BC_WALL, BC_WALL),
+                                (BC_PERIODIC, BC_PERIODIC, BC_WALL)])
+@pytest.mark.parametrize("quirks", [True, False])
+def test_f64_ib_matches_oracle(lbmib, bc, quirks):
+    kw = small_ib_config(lbmib, "f64", bc=bc, quirks=quirks)
+    o = oracle_for(kw)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        for chunk in (1, 4, 10):
+            e.step(chunk)
+            assert o.step(chunk) == 0
+            assert e.status() == 0
+            got, ref = e.fibers(0), o.nodes(0)
+            assert np.abs(got[..., :3] - ref[..., :3]).max() < 1e-12
+            assert rel(got[..., 3:], ref[..., 3:]) < 1e-9
+            assert rel(e.force(), o.force()) < 1e-9
+            rho, vel = e.macroscopic()
+            assert rel(vel, o.vel()) < 1e-11
+            assert rel(rho, o.rho()) < 1e-14
+
+
+def test_named_entry_points_match_step(lbmib):
+    """The reference's per-step sequence called one entry point at a time
+    (do_thread order, lbm_ib.c:3124-3460) gives exactly lbmib_step."""
+    kw = small_ib_config(lbmib, "f32")
+    seq = ["compute_bendingforce", "compute_stretchingforce", "compute_elasticforce",
+           "get_influentialdomain_fluid_grid_and_SpreadForce", "compute_eqlbrmdistrfuncDF1",
+           "stream_distrfunc", "bounceback_rigidwalls", "compute_rho_and_u", "moveFiberSheet",
+           "copy_inout_to_df2", "replace_old_DF", "periodicBC"]
+    with lbmib.Engine(**kw) as a, lbmib.Engine(**kw) as b:
+        a.init()
+        b.init()
+        for _ in range(6):
+            for name in seq:
+                a.call(name)
+        b.step(6)
+        assert a.time == b.time == 7
+        assert np.array_equal(a.fibers(0), b.fibers(0))
+        assert np.array_equal(a.df(), b.df())
+
+
+# ----------------------------------------------------- reduced precisions
+
+@pytest.mark.parametrize("precision,tol", [("f32", 1e-5), ("f16", 1e-3), ("bf16", 1e-2)])
+def test_reduced_precision_error_vs_fp64_oracle(lbmib, precision, tol):
+    kw = small_ib_config(lbmib, precision)
+    o = oracle_for(kw)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(40)
+        assert o.step(40) == 0
+        _, vel = e.macroscopic()
+        err_u = rel(vel, o.vel())
+        err_x = np.abs(e.fibers(0)[..., :3] - o.nodes(0)[..., :3]).max()
+        print(f"{precision}: velocity rel err {err_u:.3e}, position abs err {err_x:.3e}")
+        assert err_u < tol
+        assert err_x < 10 * tol
+
+
+# ------------------------------------------------------------ invariants
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32", 1e-6)])
+@pytest.mark.parametrize("bc", [(BC_PERIODIC, BC_PERIODIC, BC_PERIODIC), (BC_WALL, BC_WALL, BC_WALL)])
+def test_closed_box_conserves_mass(lbmib, precision, tol, bc):
+    kw = small_ib_config(lbmib, precision, bc=bc, quirks=False, n=(32, 32, 32))
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        m0 = e.totals()[0]
+        e.step(60)
+        m1 = e.totals()[0]
+        print(f"{precision} {bc}: mass drift {abs(m1 - m0) / m0:.3e}")
+        assert abs(m1 - m0) / m0 < tol
+
+
+def test_runs_are_bitwise_deterministic(lbmib):
+    kw = small_ib_config(lbmib, "f32")
+    outs = []
+    for _ in range(2):
+        with lbmib.Engine(**kw) as e:
+            e.init()
+            e.step(15)
+            outs.append((e.fibers(0), e.df(), e.force()))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_fiber_leaving_grid_reports_error(lbmib):
+    kw = small_ib_config(lbmib, "f32")
+    s = kw["sheets"][0]
+    s.xyz[..., 0] = 0.5  # support would start at x = -1 (lbm_ib.c:1458 exits here)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(1)
+        assert e.status() == lbmib.ERR_FIBER_OOB
