@@ -39,6 +39,8 @@ def parse_args():
     p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", default="2,6", help="N1,N2 steps of the differenced CPU timing")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run only a plain 1 GiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -115,8 +117,26 @@ def cpu_reference_baseline(n, rows, steps_pair):
                       f"T_step = (T({n2}) - T({n1}))/{n2 - n1} = {t_step:.3f} s"}
 
 
+def calibrate_copy():
+    """A plain copy of 2^28 float32 (1 GiB read + 1 GiB written per launch),
+    five launches: the counters it shows per dispatch calibrate the
+    FETCH_SIZE / WRITE_SIZE units of the fused kernel's profile."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                      "launches": 5}))
+
+
 def main():
     a = parse_args()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": a.gpus}))
@@ -144,6 +164,7 @@ def main():
     alg_bytes = bpc * n
     achieved = alg_bytes / (ms_main * 1e-3) / 1e9
     traffic = traffic_from_profiles(a.workload, a.precision)
+    copy_gbs = lbmib.probe_copy_bandwidth(4e9, 5)
     print(f"[bench] {ms_step:.4f} ms/step, {mlups:.1f} MLUPS, fused kernel {ms_main:.4f} ms "
           f"= {achieved:.1f} GB/s", file=sys.stderr, flush=True)
 
@@ -187,6 +208,8 @@ def main():
             "traffic": traffic,
         },
         "whole_step_hbm_frac": round(alg_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "copy_probe_gbs": round(copy_gbs, 1),
+        "fused_vs_copy": round(achieved / copy_gbs, 4),
         "fused_kernel_ms": round(ms_main, 5),
         "fiber_node_updates_per_s": round(nodes / (ms_step * 1e-3), 1),
         "mass_after": mass,

@@ -5,7 +5,6 @@
 //   f[2]      population ping-pong, SoA, 19 planes of `pitch` elements
 //   F[2]      force field ping-pong (step parity), dense [3][n], valid only
 //             inside that step's support boxes
-//   U         band velocity [3][n] (FP64), valid inside the current boxes
 //   node arrays (FP64 positions/forces, support starts, delta factors)
 //   bins      (y,z) 4x4-column bins of fiber node ids for the gather
 // Per-step entry points only enqueue; downloads synchronise.
@@ -99,7 +98,7 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->evs) (void)hipEventDestroy(e);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
-                  ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.U, ctx->d_sheets,
+                  ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->d_sheets,
                   ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -200,13 +199,12 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   double dens = 0;
   for (int s = 0; s < nsheets; ++s) dens += (4.0 / ctx->sd[s].ds1 + 2.0) * (4.0 / ctx->sd[s].ds2 + 2.0);
   int cap = 32;
-  while (cap < 2 * dens && cap < 4096) cap <<= 1;
+  while (cap < 2 * dens && cap < 1024) cap <<= 1;
   ib.cap = cap;
   if (ctx->nnodes > 0) {
     const size_t M = ctx->nnodes;
     alloc(&ctx->F[0], (size_t)3 * n * ctx->fsz);
     alloc(&ctx->F[1], (size_t)3 * n * ctx->fsz);
-    alloc((void **)&ib.U, sizeof(double) * 3 * (size_t)n);
     alloc((void **)&ib.pos, sizeof(double) * 3 * M);
     alloc((void **)&ib.frc, sizeof(double) * NF_COUNT * M);
     alloc((void **)&ib.start, sizeof(int) * 3 * M);
@@ -359,14 +357,12 @@ extern "C" int lbmib_bounceback_rigidwalls(lbmib_ctx *ctx) { return ready(ctx); 
 extern "C" int lbmib_copy_inout_to_df2(lbmib_ctx *ctx) { return ready(ctx); }
 extern "C" int lbmib_periodicBC(lbmib_ctx *ctx) { return ready(ctx); }
 
+// rho/u are evaluated inside the interpolation kernel on exactly the cells
+// it reads (the fiber supports); full fields come from
+// lbmib_get_macroscopic.  Kept for drop-in compatibility.
 extern "C" int lbmib_compute_rho_and_u(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
   if (!ctx->pending) return LBMIB_ERR_STATE;
-  if (ctx->nnodes == 0 || ctx->spread_slot != slot_now(ctx)) return LBMIB_OK;
-  const int s = slot_now(ctx);
-  RCHK(launch_band_moments(ctx->fp.precision, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], s, ctx->F[s], ctx->c,
-                           ctx->stream));
-  ctx->band_done = 1;
   return LBMIB_OK;
 }
 
@@ -376,8 +372,8 @@ extern "C" int lbmib_fiber_get_SpreadVelocity(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
   if (!ctx->pending) return LBMIB_ERR_STATE;
   if (ctx->nnodes == 0 || ctx->spread_slot != slot_now(ctx)) return LBMIB_OK;
-  if (!ctx->band_done) RCHK(lbmib_compute_rho_and_u(ctx));
-  RCHK(launch_move(ctx->ib, ctx->g, 1 - slot_now(ctx), ctx->stream));
+  const int s = slot_now(ctx);
+  RCHK(launch_move(ctx->fp.precision, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream));
   ctx->ib_clean = 1;  // k_move clears the bins and the next step's box slot
   return LBMIB_OK;
 }
@@ -416,8 +412,7 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
                              ctx->stream));
   if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
   if (ctx->nnodes > 0) {
-    RCHK(launch_band_moments(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], s, ctx->F[s], ctx->c, ctx->stream));
-    RCHK(launch_move(ctx->ib, ctx->g, 1 - s, ctx->stream));
+    RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream));
     ctx->ib_clean = 1;
   }
   ctx->cur = 1 - ctx->cur;
@@ -624,4 +619,49 @@ extern "C" int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes) {
   if (!ctx || !bytes) return LBMIB_ERR_ARG;
   *bytes = 2.0 * 19.0 * (double)ctx->esz;
   return LBMIB_OK;
+}
+
+// ------------------------------------------------------- copy bandwidth
+
+typedef unsigned int probe_u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_copy16(const probe_u32x4 *__restrict__ a, probe_u32x4 *__restrict__ b,
+                                                long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(a[i], b + i);
+}
+
+extern "C" int lbmib_probe_copy_bandwidth(double bytes, int reps, double *gbps) {
+  if (!gbps || bytes < 1e6 || reps < 1) return LBMIB_ERR_ARG;
+  const long n = (long)(bytes / 2 / sizeof(probe_u32x4));
+  probe_u32x4 *a = nullptr, *b = nullptr;
+  if (hipMalloc((void **)&a, n * sizeof(probe_u32x4)) != hipSuccess) return LBMIB_ERR_ALLOC;
+  if (hipMalloc((void **)&b, n * sizeof(probe_u32x4)) != hipSuccess) { (void)hipFree(a); return LBMIB_ERR_ALLOC; }
+  int rc = LBMIB_OK;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = LBMIB_ERR_HIP;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (!rc && (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)) rc = LBMIB_ERR_HIP;
+  double best = 0;
+  if (!rc) {
+    (void)hipMemset(a, 1, n * sizeof(probe_u32x4));
+    const int blocks = prop.multiProcessorCount * 16;
+    for (int r = 0; r < reps + 1 && !rc; ++r) {
+      (void)hipEventRecord(e0, 0);
+      k_copy16<<<blocks, 256, 0, 0>>>(a, b, n);
+      (void)hipEventRecord(e1, 0);
+      if (hipEventSynchronize(e1) != hipSuccess) { rc = LBMIB_ERR_HIP; break; }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      const double g = 2.0 * n * sizeof(probe_u32x4) / (ms * 1e-3) / 1e9;
+      if (r > 0 && g > best) best = g;  // the first launch warms up
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  *gbps = best;
+  return rc;
 }

@@ -22,63 +22,11 @@
 //             the O(0.3) rest population.
 //   F16/BF16  g * 2^8 in 16 bits; collision in FP32.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
-#include "lbmib_internal.h"
+#include "lbmib_device.h"
 
 namespace lbmib {
-
-typedef _Float16 half_t;
-
-template <int P>
-struct Traits;
-template <>
-struct Traits<LBMIB_F64> {
-  using S = double;
-  using T = double;
-  using FT = double;
-  static constexpr bool dev = false;
-};
-template <>
-struct Traits<LBMIB_F32> {
-  using S = float;
-  using T = float;
-  using FT = float;
-  static constexpr bool dev = true;
-};
-template <>
-struct Traits<LBMIB_F16> {
-  using S = half_t;
-  using T = float;
-  using FT = float;
-  static constexpr bool dev = true;
-};
-template <>
-struct Traits<LBMIB_BF16> {
-  using S = uint16_t;
-  using T = float;
-  using FT = float;
-  static constexpr bool dev = true;
-};
-
-__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (values are finite)
-  return (uint16_t)(u >> 16);
-}
-
-template <int P>
-__device__ __forceinline__ typename Traits<P>::T ld(const typename Traits<P>::S *p) {
-  if constexpr (P == LBMIB_F16) return (float)(*p) * (1.0f / kDevScale16);
-  else if constexpr (P == LBMIB_BF16) return bf16_to_f32(*p) * (1.0f / kDevScale16);
-  else return *p;
-}
-template <int P>
-__device__ __forceinline__ void st(typename Traits<P>::S *p, typename Traits<P>::T v) {
-  if constexpr (P == LBMIB_F16) *p = (half_t)(v * (float)kDevScale16);
-  else if constexpr (P == LBMIB_BF16) *p = f32_to_bf16(v * (float)kDevScale16);
-  else *p = v;
-}
 
 // Host double constants -> kernel compute type.
 template <class T>
@@ -102,49 +50,6 @@ static FluidConsts<T> cast_consts(const FluidConsts<double> &d) {
   return c;
 }
 
-__device__ __forceinline__ bool in_box6(const int *b, int x, int y, int z) {
-  return x >= b[0] && x <= b[1] && y >= b[2] && y <= b[3] && z >= b[4] && z <= b[5];
-}
-
-__device__ __forceinline__ bool in_any_box(const int *boxes, int nsheets, int x, int y, int z) {
-  bool in = false;
-  for (int s = 0; s < nsheets; ++s) in |= in_box6(boxes + 6 * s, x, y, z);
-  return in;
-}
-
-// Pulls the 19 populations arriving at (x,y,z) from the post-collision buffer.
-// Wall axes: a source outside the active box is replaced by the opposite
-// population of the same node (half-way bounce-back, lbm_ib.c:2310-2423).
-// Periodic axes wrap.  Interior cells take the branch-free path.
-template <int P>
-__device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>::S *__restrict__ f, int x, int y,
-                                     int z, long c, typename Traits<P>::T v[19]) {
-  const bool interior = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1] && z > g.lo[2] && z < g.hi[2];
-  const long sx = 1, sy = g.nx, sz = (long)g.nx * g.ny;
-  if (interior) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const long off = kCX[q] * sx + kCY[q] * sy + kCZ[q] * sz;
-      v[q] = ld<P>(f + q * g.pitch + c - off);
-    }
-    return;
-  }
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    int px = x - kCX[q], py = y - kCY[q], pz = z - kCZ[q];
-    bool bb = false;
-    if (g.bc[0] == LBMIB_BC_PERIODIC) px = px < 0 ? px + g.nx : (px >= g.nx ? px - g.nx : px);
-    else bb |= (px < g.lo[0] || px > g.hi[0]);
-    if (g.bc[1] == LBMIB_BC_PERIODIC) py = py < 0 ? py + g.ny : (py >= g.ny ? py - g.ny : py);
-    else bb |= (py < g.lo[1] || py > g.hi[1]);
-    if (g.bc[2] == LBMIB_BC_PERIODIC) pz = pz < 0 ? pz + g.nz : (pz >= g.nz ? pz - g.nz : pz);
-    else bb |= (pz < g.lo[2] || pz > g.hi[2]);
-    const long src = bb ? c + (long)kOPP[q] * g.pitch - (long)q * g.pitch
-                        : (long)px + sy * py + sz * pz;
-    v[q] = ld<P>(f + q * g.pitch + src);
-  }
-}
-
 // Equilibrium + BGK + Guo forcing in the reference's operation order
 // (lbm_ib.c:1731-1818).  Used by the F64 form.
 __device__ __forceinline__ double collide_exact(int q, double fq, double rho, double vx, double vy, double vz,
@@ -162,81 +67,57 @@ __device__ __forceinline__ double collide_exact(int q, double fq, double rho, do
   return fq * c.omc + c.om * eq + c.pref[q] * (a / c.cs2 + cu / c.cs4 * cf);
 }
 
+// Lagged rho/u of compute_rho_and_u (lbm_ib.c:2498-2525) from the pulled
+// populations and the previous step's force.
 template <int P>
-__global__ void __launch_bounds__(1024)
-    k_collide_stream(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
-                     int own, FluidConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box_cur,
-                     const int *__restrict__ box_prev, const typename Traits<P>::FT *__restrict__ Fcur,
-                     const typename Traits<P>::FT *__restrict__ Fprev) {
+__device__ __forceinline__ void lagged_moments(const typename Traits<P>::T v[19], typename Traits<P>::T fpx,
+                                               typename Traits<P>::T fpy, typename Traits<P>::T fpz,
+                                               const FluidConsts<typename Traits<P>::T> &c,
+                                               typename Traits<P>::T &rho, typename Traits<P>::T &ux,
+                                               typename Traits<P>::T &uy, typename Traits<P>::T &uz) {
   using T = typename Traits<P>::T;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x < g.lo[0] || x > g.hi[0] || y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) return;
-  const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
-
-  T v[19];
-  T rho, ux, uy, uz;
-  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
-  if (frozen) {
-    // planes x = 2 and N-3 hold the inlet/outlet tables after
-    // copy_inout_to_df2, and their rho/vel are never updated (the moment box
-    // starts at x = 3, lbm_ib.c:2482-2483).
-    const T *tab = (x == g.lo[0]) ? c.tab_in : c.tab_out;
+  if constexpr (!Traits<P>::dev) {
+    // rho = Sum df2, u = (Sum c df2 + dt/2 F)/rho in the reference's order
+    double s1 = v[0], s2 = 0.0, s3 = 0.0, s4 = 0.0;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) v[q] = tab[q];
-    rho = c.rho0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
-  } else {
-    if (own) {
-#pragma unroll
-      for (int q = 0; q < 19; ++q) v[q] = ld<P>(fin + q * g.pitch + c0);
-      rho = c.rho0; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // init_gv values, lbm_ib.c:837-849
-    } else {
-      pull<P>(g, fin, x, y, z, c0, v);
-      T fpx = 0, fpy = 0, fpz = 0;
-      if (in_any_box(box_prev, nsheets, x, y, z)) {
-        fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
-      }
-      if constexpr (!Traits<P>::dev) {
-        // rho = Sum df2, u = (Sum c df2 + dt/2 F)/rho (lbm_ib.c:2498-2525)
-        double s1 = v[0], s2 = 0.0, s3 = 0.0, s4 = 0.0;
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-          s1 += v[q];
-          if (kCX[q]) s2 = kCX[q] > 0 ? s2 + v[q] : s2 - v[q];
-          if (kCY[q]) s3 = kCY[q] > 0 ? s3 + v[q] : s3 - v[q];
-          if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + v[q] : s4 - v[q];
-        }
-        rho = s1;
-        ux = (s2 + 0.5 * fpx) / s1;
-        uy = (s3 + 0.5 * fpy) / s1;
-        uz = (s4 + 0.5 * (fpz + rho * c.g)) / s1;
-      } else {
-        T d = 0, mx = 0, my = 0, mz = 0;
-#pragma unroll
-        for (int q = 0; q < 19; ++q) {
-          d += v[q];
-          if (kCX[q]) mx = kCX[q] > 0 ? mx + v[q] : mx - v[q];
-          if (kCY[q]) my = kCY[q] > 0 ? my + v[q] : my - v[q];
-          if (kCZ[q]) mz = kCZ[q] > 0 ? mz + v[q] : mz - v[q];
-        }
-        rho = (T)1 + d;
-        const T ir = (T)1 / rho;
-        ux = (mx + (T)0.5 * fpx) * ir;
-        uy = (my + (T)0.5 * fpy) * ir;
-        uz = (mz + (T)0.5 * (fpz + rho * c.g)) * ir;
-      }
+    for (int q = 1; q < 19; ++q) {
+      s1 += v[q];
+      if (kCX[q]) s2 = kCX[q] > 0 ? s2 + v[q] : s2 - v[q];
+      if (kCY[q]) s3 = kCY[q] > 0 ? s3 + v[q] : s3 - v[q];
+      if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + v[q] : s4 - v[q];
     }
+    rho = s1;
+    ux = (s2 + 0.5 * fpx) / s1;
+    uy = (s3 + 0.5 * fpy) / s1;
+    uz = (s4 + 0.5 * (fpz + rho * c.g)) / s1;
+  } else {
+    T d = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      d += v[q];
+      if (kCX[q]) mx = kCX[q] > 0 ? mx + v[q] : mx - v[q];
+      if (kCY[q]) my = kCY[q] > 0 ? my + v[q] : my - v[q];
+      if (kCZ[q]) mz = kCZ[q] > 0 ? mz + v[q] : mz - v[q];
+    }
+    rho = (T)1 + d;
+    const T ir = (T)1 / rho;
+    ux = (mx + (T)0.5 * fpx) * ir;
+    uy = (my + (T)0.5 * fpy) * ir;
+    uz = (mz + (T)0.5 * (fpz + rho * c.g)) * ir;
   }
+}
 
-  T fx = 0, fy = 0, fz = 0;
-  if (in_any_box(box_cur, nsheets, x, y, z)) {
-    fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
-  }
-
+// BGK + Guo collision of one cell, in place.
+template <int P>
+__device__ __forceinline__ void collide_cell(typename Traits<P>::T v[19], typename Traits<P>::T rho,
+                                             typename Traits<P>::T ux, typename Traits<P>::T uy,
+                                             typename Traits<P>::T uz, typename Traits<P>::T fx,
+                                             typename Traits<P>::T fy, typename Traits<P>::T fz,
+                                             const FluidConsts<typename Traits<P>::T> &c) {
+  using T = typename Traits<P>::T;
   if constexpr (!Traits<P>::dev) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q)
-      st<P>(fout + q * g.pitch + c0, collide_exact(q, v[q], rho, ux, uy, uz, fx, fy, fz, c));
+    for (int q = 0; q < 19; ++q) v[q] = collide_exact(q, v[q], rho, ux, uy, uz, fx, fy, fz, c);
   } else {
     // Deviation form: with g_q = f_q - w_q and rho = 1 + Sum g,
     //   feq_q - w_q = w_q (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2))
@@ -253,8 +134,160 @@ __global__ void __launch_bounds__(1024)
       const T w = (T)1 / (T)kWDEN[q];
       const T geq = w * (drho + rho * ((T)3 * cu + (T)4.5 * cu * cu - usq15));
       const T S = c.pw[q] * ((T)3 * (cF - uF) + (T)9 * cu * cF);
-      st<P>(fout + q * g.pitch + c0, c.omc * v[q] + c.om * geq + S);
+      v[q] = c.omc * v[q] + c.om * geq + S;
     }
+  }
+}
+
+// The whole update of one active cell: arriving populations (inlet/outlet
+// table on the frozen planes, own populations on the first step, pull
+// otherwise), lagged moments, collision with the current force, store.
+template <int P, bool NT>
+__device__ __forceinline__ void update_cell(const Geometry &g, const typename Traits<P>::S *__restrict__ fin,
+                                            typename Traits<P>::S *__restrict__ fout, int own,
+                                            const FluidConsts<typename Traits<P>::T> &c, int nsheets,
+                                            const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                                            const typename Traits<P>::FT *__restrict__ Fcur,
+                                            const typename Traits<P>::FT *__restrict__ Fprev, int x, int y, int z) {
+  using T = typename Traits<P>::T;
+  if (x < g.lo[0] || x > g.hi[0] || y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) return;
+  const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
+  T v[19];
+  T rho, ux, uy, uz;
+  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+  if (frozen) {
+    // planes x = 2 and N-3 hold the inlet/outlet tables after
+    // copy_inout_to_df2, and their rho/vel are never updated (the moment box
+    // starts at x = 3, lbm_ib.c:2482-2483).
+    const T *tab = (x == g.lo[0]) ? c.tab_in : c.tab_out;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) v[q] = tab[q];
+    rho = c.rho0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
+  } else if (own) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) v[q] = ld<P>(fin + q * g.pitch + c0);
+    rho = c.rho0; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // init_gv values, lbm_ib.c:837-849
+  } else {
+    pull<P>(g, fin, x, y, z, c0, v);
+    T fpx = 0, fpy = 0, fpz = 0;
+    if (in_any_box(box_prev, nsheets, x, y, z)) {
+      fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
+    }
+    lagged_moments<P>(v, fpx, fpy, fpz, c, rho, ux, uy, uz);
+  }
+  T fx = 0, fy = 0, fz = 0;
+  if (in_any_box(box_cur, nsheets, x, y, z)) {
+    fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
+  }
+  collide_cell<P>(v, rho, ux, uy, uz, fx, fy, fz, c);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) st<P, NT>(fout + q * g.pitch + c0, v[q]);
+}
+
+// One cell per thread, row-aligned blocks (any grid).
+template <int P, bool NT>
+__global__ void __launch_bounds__(1024)
+    k_collide_stream(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
+                     int own, FluidConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box_cur,
+                     const int *__restrict__ box_prev, const typename Traits<P>::FT *__restrict__ Fcur,
+                     const typename Traits<P>::FT *__restrict__ Fprev) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  update_cell<P, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, blockIdx.y, blockIdx.z);
+}
+
+// V consecutive cells of one x-row per thread: the 9 populations with
+// c_x = 0 are read with aligned V-wide loads, the 10 x-shifted ones with
+// one-element-offset V-wide loads (dword aligned), and all 19 results are
+// written with aligned V-wide stores, so each wave moves 64*V*sizeof(S)
+// contiguous bytes per memory instruction.  Threads whose cells touch a
+// boundary (frozen planes, walls, periodic wrap, the first step) fall back
+// to the per-cell path.  Requires nx % V == 0.
+template <class S, int V>
+struct alignas(sizeof(S) * V) VecT {
+  S a[V];
+};
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Aligned store of a VecT as native 16- or 8-byte vectors (nontemporal when
+// NT: the populations are not re-read before the next step).
+template <bool NT, class VT>
+__device__ __forceinline__ void store_vec(VT *dst, const VT &t) {
+  constexpr int B = sizeof(VT);
+  if constexpr (B % 16 == 0) {
+    u32x4 tmp[B / 16];
+    __builtin_memcpy(tmp, &t, B);
+    u32x4 *d = reinterpret_cast<u32x4 *>(dst);
+#pragma unroll
+    for (int i = 0; i < B / 16; ++i) {
+      if constexpr (NT) __builtin_nontemporal_store(tmp[i], d + i);
+      else d[i] = tmp[i];
+    }
+  } else if constexpr (B == 8) {
+    u32x2 tmp;
+    __builtin_memcpy(&tmp, &t, 8);
+    if constexpr (NT) __builtin_nontemporal_store(tmp, reinterpret_cast<u32x2 *>(dst));
+    else *reinterpret_cast<u32x2 *>(dst) = tmp;
+  } else {
+    *dst = t;
+  }
+}
+
+template <int P, int V, bool NT>
+__global__ void __launch_bounds__(256)
+    k_collide_stream_vec(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                         typename Traits<P>::S *__restrict__ fout, int own, FluidConsts<typename Traits<P>::T> c,
+                         int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                         const typename Traits<P>::FT *__restrict__ Fcur,
+                         const typename Traits<P>::FT *__restrict__ Fprev) {
+  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
+  const int x0 = threadIdx.x * V;
+  const int y = blockIdx.x * blockDim.y + threadIdx.y, z = blockIdx.y;
+  if (y >= g.ny || x0 >= g.nx) return;
+  const bool fast = !own && x0 - 1 >= g.lo[0] && x0 + V <= g.hi[0] && y > g.lo[1] && y < g.hi[1] && z > g.lo[2] &&
+                    z < g.hi[2];
+  if (!fast) {
+#pragma unroll 1
+    for (int k = 0; k < V; ++k)
+      update_cell<P, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x0 + k, y, z);
+    return;
+  }
+  const long c0 = (long)x0 + (long)g.nx * ((long)y + (long)g.ny * z);
+  const long sy = g.nx, sz = (long)g.nx * g.ny;
+  T v[V][19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const S *p = fin + q * g.pitch + c0 - (kCX[q] + kCY[q] * sy + kCZ[q] * sz);
+    VecT<S, V> t;
+    if (kCX[q] == 0) t = *reinterpret_cast<const VecT<S, V> *>(p);
+    else __builtin_memcpy(&t, p, sizeof(t));
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k][q] = ld<P>(&t.a[k]);
+  }
+  const bool inprev = nsheets > 0 && any_box_overlaps(box_prev, nsheets, x0, x0 + V - 1, y, z);
+  const bool incur = nsheets > 0 && any_box_overlaps(box_cur, nsheets, x0, x0 + V - 1, y, z);
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const int x = x0 + k;
+    T fpx = 0, fpy = 0, fpz = 0, fx = 0, fy = 0, fz = 0;
+    if (inprev && in_any_box(box_prev, nsheets, x, y, z)) {
+      fpx = (T)Fprev[c0 + k]; fpy = (T)Fprev[g.n + c0 + k]; fpz = (T)Fprev[2 * g.n + c0 + k];
+    }
+    if (incur && in_any_box(box_cur, nsheets, x, y, z)) {
+      fx = (T)Fcur[c0 + k]; fy = (T)Fcur[g.n + c0 + k]; fz = (T)Fcur[2 * g.n + c0 + k];
+    }
+    T rho, ux, uy, uz;
+    lagged_moments<P>(v[k], fpx, fpy, fpz, c, rho, ux, uy, uz);
+    collide_cell<P>(v[k], rho, ux, uy, uz, fx, fy, fz, c);
+  }
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    VecT<S, V> t;
+#pragma unroll
+    for (int k = 0; k < V; ++k) st<P, false>(&t.a[k], v[k][q]);
+    store_vec<NT>(reinterpret_cast<VecT<S, V> *>(fout + q * g.pitch + c0), t);
   }
 }
 
@@ -401,6 +434,13 @@ __global__ void __launch_bounds__(kTotThreads) k_totals_final(const double *__re
 // cache lines.
 static inline dim3 row_block(const Geometry &g) {
   int bx;
+  static int env_bx = -1;
+  if (env_bx < 0) {
+    const char *e = getenv("LBMIB_BLOCK_X");
+    env_bx = e ? atoi(e) : 0;
+    if (env_bx % 64 != 0 || env_bx > 1024) env_bx = 0;
+  }
+  if (env_bx > 0) return dim3(env_bx, 1, 1);
   if (g.nx % 64 == 0 && g.nx >= 256 && g.nx <= 1024) bx = g.nx;
   else if (g.nx > 1024) bx = 256;
   else bx = ((g.nx + 63) / 64) * 64;
@@ -426,16 +466,81 @@ int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<dou
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
+// Cells per thread of the vector kernel: 16-byte accesses per lane by
+// default (F64: 2, F32: 4, 16-bit: 8); LBMIB_VEC=1 selects the one-cell
+// kernel, LBMIB_VEC=2/4/8 another width.  Falls back to one cell per thread
+// when nx is not a multiple of the width or a row needs more than 256
+// threads.
+static int vec_width(int prec, int nx) {
+  static int env = -2;
+  if (env == -2) {
+    const char *e = getenv("LBMIB_VEC");
+    env = e ? atoi(e) : -1;
+  }
+  int v = env > 0 ? env : (prec == LBMIB_F64 ? 2 : (prec == LBMIB_F32 ? 4 : 8));
+  if (prec == LBMIB_F64 && v > 4) v = 4;
+  if (v != 2 && v != 4 && v != 8) return 1;
+  if (prec != LBMIB_F32 && prec != LBMIB_F64 && v == 2) v = 4;
+  if (nx % v != 0 || nx / v > 256) return 1;
+  return v;
+}
+
+template <int P, int V, bool NT>
+static void launch_vec(const Geometry &g, const void *fin, void *fout, int own, const FluidConsts<double> &c,
+                       int nsheets, const int *box_cur, const int *box_prev, const void *Fcur, const void *Fprev,
+                       hipStream_t s) {
+  using TT = typename Traits<P>::T;
+  using FT = typename Traits<P>::FT;
+  using S = typename Traits<P>::S;
+  const int bx = g.nx / V;
+  const int by = bx >= 256 ? 1 : 256 / bx;
+  dim3 b(bx, by, 1), gr((g.ny + by - 1) / by, g.nz, 1);
+  k_collide_stream_vec<P, V, NT><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, cast_consts<TT>(c), nsheets,
+                                                  box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev);
+}
+
+template <int P, bool NT>
+static void launch_scalar(const Geometry &g, const void *fin, void *fout, int own, const FluidConsts<double> &c,
+                          int nsheets, const int *box_cur, const int *box_prev, const void *Fcur, const void *Fprev,
+                          hipStream_t s) {
+  using TT = typename Traits<P>::T;
+  using FT = typename Traits<P>::FT;
+  using S = typename Traits<P>::S;
+  dim3 b = row_block(g), gr = row_grid(g, b);
+  k_collide_stream<P, NT><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, cast_consts<TT>(c), nsheets, box_cur,
+                                           box_prev, (const FT *)Fcur, (const FT *)Fprev);
+}
+
+template <int P, bool NT>
+static void launch_any(int v, const Geometry &g, const void *fin, void *fout, int own, const FluidConsts<double> &c,
+                       int nsheets, const int *box_cur, const int *box_prev, const void *Fcur, const void *Fprev,
+                       hipStream_t s) {
+  if constexpr (P == LBMIB_F64) {
+    if (v == 2) return launch_vec<P, 2, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+    if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+  } else if constexpr (P == LBMIB_F32) {
+    if (v == 2) return launch_vec<P, 2, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+    if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+    if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+  } else {
+    if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+    if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+  }
+  launch_scalar<P, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+}
+
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, hipStream_t s) {
-  dim3 b = row_block(g), gr = row_grid(g, b);
+  static int nt = -1;
+  if (nt < 0) {
+    const char *e = getenv("LBMIB_NT");
+    nt = (e && atoi(e) == 0) ? 0 : 1;  // nontemporal stores unless LBMIB_NT=0
+  }
+  const int v = vec_width(prec, g.nx);
   LBMIB_DISPATCH(prec, ({
-    using TT = typename Traits<P>::T;
-    using FT = typename Traits<P>::FT;
-    k_collide_stream<P><<<gr, b, 0, s>>>(g, (const typename Traits<P>::S *)fin, (typename Traits<P>::S *)fout,
-                                          own_mode, cast_consts<TT>(c), nsheets, box_cur, box_prev,
-                                          (const FT *)Fcur, (const FT *)Fprev);
+    if (nt) launch_any<P, true>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+    else launch_any<P, false>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   }));
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }

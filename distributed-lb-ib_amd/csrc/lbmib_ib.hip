@@ -6,27 +6,26 @@
 //                  floor(X-2)+1 and its twelve 1+cos(pi r/2) factors, its
 //                  (y,z) bin and the sheet's support box.  One launch covers
 //                  every sheet.
-//   k_gather       cell-owned force spreading: each workgroup owns a 4x4 (y,z)
-//                  column tile of a sheet's support box, loads the candidate
-//                  node ids of the 2x2 bins that can reach it into LDS, sorts
-//                  them (bitonic), and every cell sums F_node * delta over the
-//                  candidates in ascending (sheet, row, col) order -- the
-//                  order of the sequential spread (lbm_ib.c:1420-1495).  No
-//                  float atomics; the result is run-to-run identical.
-//   k_band_moments rho/u of the new populations on the support boxes only
-//                  (compute_rho_and_u, lbm_ib.c:2440-2538, restricted to the
-//                  cells the interpolation reads).
-//   k_move         one thread per node: u(X) = Sum u delta (lbm_ib.c:2670-2673),
-//                  X += dt u(X) in FP64; also clears the bins and the box slot
-//                  that the next step will fill.
+//   k_gather       cell-owned force spreading: each workgroup owns an 8x8
+//                  (y,z) column tile of a sheet's support box, loads the
+//                  candidate node ids of the 3x3 bins that can reach it into
+//                  LDS, rank-sorts them, and every cell sums F_node * delta
+//                  over the candidates in ascending (sheet, row, col) order --
+//                  the order of the sequential spread (lbm_ib.c:1420-1495).
+//                  No float atomics; the result is run-to-run identical.
+//   k_move         one wave per node: rho/u of the new populations on the
+//                  node's 64 support cells (compute_rho_and_u,
+//                  lbm_ib.c:2440-2538, evaluated only where the
+//                  interpolation reads), u(X) = Sum u delta summed in the
+//                  reference's order (lbm_ib.c:2670-2673), X += dt u(X) in
+//                  FP64; also clears the bins and the box slot that the next
+//                  step will fill.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
-#include "lbmib_internal.h"
+#include "lbmib_device.h"
 
 namespace lbmib {
-
-typedef _Float16 half_t;
 
 __device__ __forceinline__ int find_sheet(const IBState &ib, int n) {
   int s = 0;
@@ -79,78 +78,116 @@ __device__ __forceinline__ void stretch_1d(const double *X0, const double *X1, c
   }
 }
 
+// Warp-level min/max so that only one lane per wave touches the shared
+// support box (same-address atomics from every node serialise in L2).
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask, int slot) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= ib.nnodes) return;
-  const int s = find_sheet(ib, n);
-  const SheetDesc sd = ib.sheets[s];
-  const int k = n - sd.off, i = k / sd.cols, j = k % sd.cols;
+  const bool live = n < ib.nnodes;
   const int M = ib.nnodes;
-  const double *px = ib.pos, *py = ib.pos + M, *pz = ib.pos + 2 * M;
-  const double *lines[3] = {px, py, pz};
-  const bool quirk = ib.quirks != 0;
+  int s = 0;
+  bool ok = false;
+  int st[3] = {INT_MAX, INT_MAX, INT_MAX};
+  if (live) {
+    s = find_sheet(ib, n);
+    const SheetDesc sd = ib.sheets[s];
+    const int k = n - sd.off, i = k / sd.cols, j = k % sd.cols;
+    const double *px = ib.pos, *py = ib.pos + M, *pz = ib.pos + 2 * M;
+    const double *lines[3] = {px, py, pz};
+    const bool quirk = ib.quirks != 0;
 
-  if (mask & FK_BEND) {
-    for (int d = 0; d < 3; ++d) {
-      const double *row = lines[d] + sd.off + (long)i * sd.cols;  // along cols, stride 1
-      const double *col = lines[d] + sd.off + j;                  // along rows, stride cols
-      const double r = bend_1d(row, 1, j, sd.cols, sd.bend_c, quirk && d == 0);
-      const double c = bend_1d(col, sd.cols, i, sd.rows, sd.bend_c, false);
-      ib.frc[(NF_BX + d) * (long)M + n] = r + c;  // row pass '=', column pass '+=' (lbm_ib.c:1070)
-    }
-  }
-  if (mask & FK_STRETCH) {
-    double r[3], c[3];
-    const long ro = sd.off + (long)i * sd.cols, co = sd.off + j;
-    stretch_1d(px + ro, py + ro, pz + ro, 1, j, sd.cols, sd.ds1, sd.str_row_c, quirk, r);
-    stretch_1d(px + co, py + co, pz + co, sd.cols, i, sd.rows, sd.ds2, sd.str_col_c, quirk, c);
-    for (int d = 0; d < 3; ++d) ib.frc[(NF_SX + d) * (long)M + n] = r[d] + c[d];
-  }
-  if (mask & FK_ELASTIC) {
-    for (int d = 0; d < 3; ++d)
-      ib.frc[(NF_EX + d) * (long)M + n] = ib.frc[(NF_SX + d) * (long)M + n] + ib.frc[(NF_BX + d) * (long)M + n];
-  }
-  if (mask & FK_PREP) {
-    const double X[3] = {px[n], py[n], pz[n]};
-    const int dims[3] = {g.nx, g.ny, g.nz};
-    int st[3];
-    bool ok = true;
-    for (int d = 0; d < 3; ++d) {
-      st[d] = (int)(floor(X[d] / 1.0 - 2) + 1);  // lbm_ib.c:1424-1429
-      ok &= st[d] >= 0 && st[d] + 3 < dims[d];
-    }
-    if (!ok) {
-      // the reference exits here (lbm_ib.c:1458-1466); latch the error and
-      // leave the node out of spreading and moving.
-      atomicOr(ib.err, DERR_FIBER_OOB);
-      ib.start[n] = -1;
-      return;
-    }
-    const double cpi = kRefPi / (2.0 * 1.0);
-    for (int d = 0; d < 3; ++d) {
-      ib.start[d * (long)M + n] = st[d];
-      for (int a = 0; a < 4; ++a) {
-        const double r = 1.0 * (st[d] + a) - X[d];
-        ib.wgt[(4 * d + a) * (long)M + n] = 1.0e0 + cos(cpi * r);
+    if (mask & FK_BEND) {
+      for (int d = 0; d < 3; ++d) {
+        const double *row = lines[d] + sd.off + (long)i * sd.cols;  // along cols, stride 1
+        const double *col = lines[d] + sd.off + j;                  // along rows, stride cols
+        const double r = bend_1d(row, 1, j, sd.cols, sd.bend_c, quirk && d == 0);
+        const double c = bend_1d(col, sd.cols, i, sd.rows, sd.bend_c, false);
+        ib.frc[(NF_BX + d) * (long)M + n] = r + c;  // row pass '=', column pass '+=' (lbm_ib.c:1070)
       }
     }
-    const int b = (st[1] >> 2) * ib.nbz + (st[2] >> 2);
-    const int slot_in_bin = atomicAdd(&ib.bin_count[b], 1);
-    if (slot_in_bin < ib.cap) ib.bin_list[(long)b * ib.cap + slot_in_bin] = n;
-    else atomicOr(ib.err, DERR_BIN_OVERFLOW);
+    if (mask & FK_STRETCH) {
+      double r[3], c[3];
+      const long ro = sd.off + (long)i * sd.cols, co = sd.off + j;
+      stretch_1d(px + ro, py + ro, pz + ro, 1, j, sd.cols, sd.ds1, sd.str_row_c, quirk, r);
+      stretch_1d(px + co, py + co, pz + co, sd.cols, i, sd.rows, sd.ds2, sd.str_col_c, quirk, c);
+      for (int d = 0; d < 3; ++d) ib.frc[(NF_SX + d) * (long)M + n] = r[d] + c[d];
+    }
+    if (mask & FK_ELASTIC) {
+      for (int d = 0; d < 3; ++d)
+        ib.frc[(NF_EX + d) * (long)M + n] = ib.frc[(NF_SX + d) * (long)M + n] + ib.frc[(NF_BX + d) * (long)M + n];
+    }
+    if (mask & FK_PREP) {
+      const double X[3] = {px[n], py[n], pz[n]};
+      const int dims[3] = {g.nx, g.ny, g.nz};
+      ok = true;
+      for (int d = 0; d < 3; ++d) {
+        st[d] = (int)(floor(X[d] / 1.0 - 2) + 1);  // lbm_ib.c:1424-1429
+        ok &= st[d] >= 0 && st[d] + 3 < dims[d];
+      }
+      if (!ok) {
+        // the reference exits here (lbm_ib.c:1458-1466); latch the error and
+        // leave the node out of spreading and moving.
+        atomicOr(ib.err, DERR_FIBER_OOB);
+        ib.start[n] = -1;
+      } else {
+        const double cpi = kRefPi / (2.0 * 1.0);
+        for (int d = 0; d < 3; ++d) {
+          ib.start[d * (long)M + n] = st[d];
+          for (int a = 0; a < 4; ++a) {
+            const double r = 1.0 * (st[d] + a) - X[d];
+            ib.wgt[(4 * d + a) * (long)M + n] = 1.0e0 + cos(cpi * r);
+          }
+        }
+        const int b = (st[1] >> 2) * ib.nbz + (st[2] >> 2);
+        const int slot_in_bin = atomicAdd(&ib.bin_count[b], 1);
+        if (slot_in_bin < ib.cap) ib.bin_list[(long)b * ib.cap + slot_in_bin] = n;
+        else atomicOr(ib.err, DERR_BIN_OVERFLOW);
+      }
+    }
+  }
+  if (!(mask & FK_PREP)) return;  // uniform over the grid
+  // support box of each sheet: reduce per wave, one lane updates the box
+  const int s_lo = __shfl(s, 0, 64);
+  const bool uniform = __all(!live || s == s_lo);
+  const int lo[3] = {ok ? st[0] : INT_MAX, ok ? st[1] : INT_MAX, ok ? st[2] : INT_MAX};
+  const int hi[3] = {ok ? st[0] + 3 : INT_MIN, ok ? st[1] + 3 : INT_MIN, ok ? st[2] + 3 : INT_MIN};
+  if (uniform) {
+    int rlo[3], rhi[3];
+    for (int d = 0; d < 3; ++d) {
+      rlo[d] = wave_min(lo[d]);
+      rhi[d] = wave_max(hi[d]);
+    }
+    if ((threadIdx.x & 63) == 0 && rlo[0] != INT_MAX) {
+      int *box = ib.box + (slot * kMaxSheets + s_lo) * 6;
+      for (int d = 0; d < 3; ++d) {
+        atomicMin(box + 2 * d, rlo[d]);
+        atomicMax(box + 2 * d + 1, rhi[d]);
+      }
+    }
+  } else if (ok) {
+    // a wave spanning two sheets: per-node updates (only at sheet seams)
     int *box = ib.box + (slot * kMaxSheets + s) * 6;
-    atomicMin(box + 0, st[0]);
-    atomicMax(box + 1, st[0] + 3);
-    atomicMin(box + 2, st[1]);
-    atomicMax(box + 3, st[1] + 3);
-    atomicMin(box + 4, st[2]);
-    atomicMax(box + 5, st[2] + 3);
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(box + 2 * d, lo[d]);
+      atomicMax(box + 2 * d + 1, hi[d]);
+    }
   }
 }
 
 // -------------------------------------------------------------- k_gather
 
 constexpr int kGatherThreads = 256;
+constexpr int kTileBins = 2;              // tile side in 4-cell bins
+constexpr int kTile = 4 * kTileBins;      // tile side in cells (y and z)
+constexpr int kCandBins = kTileBins + 1;  // candidate bins per axis
 
 // Enumerates (sheet, tile) work items of the support boxes in a fixed order.
 // Returns false when w is past the last item.
@@ -158,7 +195,7 @@ __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int 
   for (s = 0; s < ib.nsheets; ++s) {
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
     if (b[0] > b[1]) continue;  // empty (all nodes invalid)
-    const int ty0 = b[2] >> 2, ty1 = b[3] >> 2, tz0 = b[4] >> 2, tz1 = b[5] >> 2;
+    const int ty0 = b[2] / kTile, ty1 = b[3] / kTile, tz0 = b[4] / kTile, tz1 = b[5] / kTile;
     const int nt = (ty1 - ty0 + 1) * (tz1 - tz0 + 1);
     if (w < nt) {
       ty = ty0 + w / (tz1 - tz0 + 1);
@@ -170,68 +207,68 @@ __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int 
   return false;
 }
 
+constexpr int kGatherMaxCand = 3072;  // 5 ints each: 60 KB of LDS
+
 template <class FT>
-__global__ void __launch_bounds__(kGatherThreads) k_gather(IBState ib, Geometry g, int slot, FT *__restrict__ F) {
-  extern __shared__ int ids[];  // [m2] candidate node ids
+__global__ void __launch_bounds__(kGatherThreads)
+    k_gather(IBState ib, Geometry g, int slot, FT *__restrict__ F, int mcap) {
+  // LDS: raw candidate ids, then the sorted ids and their support starts
+  extern __shared__ int smem[];
+  int *raw = smem;
+  int *ids = raw + mcap;
+  int *sx = ids + mcap, *sy = sx + mcap, *sz = sy + mcap;
   const int M = ib.nnodes;
   const double cf = 1.0 / (64.0 * 1.0 * 1.0 * 1.0);  // lbm_ib.c:1357
   for (int w = blockIdx.x;; w += gridDim.x) {
     int s, ty, tz;
     if (!tile_of(ib, slot, w, s, ty, tz)) break;
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
-    // candidate bins (ty-1..ty) x (tz-1..tz): a node with support start j0
-    // reaches rows j0..j0+3, so tile rows 4ty..4ty+3 need j0 in [4ty-3, 4ty+3].
-    __syncthreads();
-    int cnt[4], base[4];
+    // a node with support start j0 reaches rows j0..j0+3, so tile rows
+    // kTile*ty .. kTile*ty+kTile-1 need j0 in [kTile*ty-3, kTile*ty+kTile-1]:
+    // bins kTileBins*ty-1 .. kTileBins*ty+kTileBins-1.
+    const int by0 = kTileBins * ty - 1, bz0 = kTileBins * tz - 1;
     int tot = 0;
-    for (int k = 0; k < 4; ++k) {
-      const int by = ty - 1 + (k >> 1), bz = tz - 1 + (k & 1);
-      int c = 0;
-      if (by >= 0 && bz >= 0 && by < ib.nby && bz < ib.nbz) c = min(ib.bin_count[by * ib.nbz + bz], ib.cap);
-      cnt[k] = c;
-      base[k] = tot;
+    for (int k = 0; k < kCandBins * kCandBins; ++k) {
+      const int by = by0 + k / kCandBins, bz = bz0 + k % kCandBins;
+      if (by < 0 || bz < 0 || by >= ib.nby || bz >= ib.nbz) continue;
+      const int bin = by * ib.nbz + bz;
+      int c = min(ib.bin_count[bin], ib.cap);
+      if (tot + c > mcap) {
+        if (threadIdx.x == 0) atomicOr(ib.err, DERR_BIN_OVERFLOW);
+        c = mcap - tot;
+      }
+      for (int e = threadIdx.x; e < c; e += blockDim.x) raw[tot + e] = ib.bin_list[(long)bin * ib.cap + e];
       tot += c;
     }
-    int m2 = 1;
-    while (m2 < tot) m2 <<= 1;
-    for (int k = 0; k < 4; ++k) {
-      const int by = ty - 1 + (k >> 1), bz = tz - 1 + (k & 1);
-      for (int e = threadIdx.x; e < cnt[k]; e += blockDim.x)
-        ids[base[k] + e] = ib.bin_list[(long)(by * ib.nbz + bz) * ib.cap + e];
-    }
-    for (int e = tot + threadIdx.x; e < m2; e += blockDim.x) ids[e] = INT_MAX;
     __syncthreads();
-    // bitonic sort, ascending
-    for (int kk = 2; kk <= m2; kk <<= 1) {
-      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        for (int e = threadIdx.x; e < m2; e += blockDim.x) {
-          const int p = e ^ jj;
-          if (p > e) {
-            const int a = ids[e], c = ids[p];
-            const bool up = (e & kk) == 0;
-            if ((a > c) == up) { ids[e] = c; ids[p] = a; }
-          }
-        }
-        __syncthreads();
-      }
+    // rank sort (node ids are distinct): ascending (sheet, row, col) order
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      const int v = raw[e];
+      int r = 0;
+      for (int k = 0; k < tot; ++k) r += raw[k] < v;
+      ids[r] = v;
+      sx[r] = ib.start[v];
+      sy[r] = ib.start[M + v];
+      sz[r] = ib.start[2 * M + v];
     }
-    // cells of this tile inside the sheet's support box
+    __syncthreads();
     const int x0 = b[0], x1 = b[1];
-    const int y0 = max(4 * ty, b[2]), y1 = min(4 * ty + 3, b[3]);
-    const int z0 = max(4 * tz, b[4]), z1 = min(4 * tz + 3, b[5]);
+    const int y0 = max(kTile * ty, b[2]), y1 = min(kTile * ty + kTile - 1, b[3]);
+    const int z0 = max(kTile * tz, b[4]), z1 = min(kTile * tz + kTile - 1, b[5]);
     const int nxs = x1 - x0 + 1, nys = y1 - y0 + 1, nzs = z1 - z0 + 1;
     const int ncell = nxs * nys * nzs;
     for (int e = threadIdx.x; e < ncell; e += blockDim.x) {
       const int x = x0 + e % nxs, y = y0 + (e / nxs) % nys, z = z0 + e / (nxs * nys);
       double ax = 0.0, ay = 0.0, az = 0.0;
       for (int k = 0; k < tot; ++k) {
-        const int n = ids[k];
-        const int ix = x - ib.start[n], iy = y - ib.start[M + n], iz = z - ib.start[2 * M + n];
+        const int ix = x - sx[k], iy = y - sy[k], iz = z - sz[k];
         if ((unsigned)ix > 3u || (unsigned)iy > 3u || (unsigned)iz > 3u) continue;
-        const double w = cf * ib.wgt[ix * (long)M + n] * ib.wgt[(4 + iy) * (long)M + n] * ib.wgt[(8 + iz) * (long)M + n];
-        ax += ib.frc[NF_EX * (long)M + n] * w;
-        ay += ib.frc[NF_EY * (long)M + n] * w;
-        az += ib.frc[NF_EZ * (long)M + n] * w;
+        const int n = ids[k];
+        const double wgt = cf * ib.wgt[ix * (long)M + n] * ib.wgt[(4 + iy) * (long)M + n] *
+                           ib.wgt[(8 + iz) * (long)M + n];
+        ax += ib.frc[NF_EX * (long)M + n] * wgt;
+        ay += ib.frc[NF_EY * (long)M + n] * wgt;
+        az += ib.frc[NF_EZ * (long)M + n] * wgt;
       }
       const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
       F[c0] = (FT)ax;
@@ -242,120 +279,63 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(IBState ib, Geometry 
   }
 }
 
-// -------------------------------------------------------- k_band_moments
+// ----------------------------------------------------------------- k_move
+//
+// One wave per fiber node, one lane per cell of its 4x4x4 support
+// (lane = 16 i2 + 4 j2 + k2, the loop order of lbm_ib.c:2614-2682).  Each
+// lane computes the cell's rho/u from the new populations exactly as
+// compute_rho_and_u does (lbm_ib.c:2498-2525; cells outside the moment box
+// keep init_gv's velocity), weights it, and lane 0 sums the 64 products in
+// lane order -- the reference's summation order -- before X += dt u(X).
+
+constexpr int kMoveThreads = 256;
 
 template <int P>
-struct BandTraits;
-template <>
-struct BandTraits<LBMIB_F64> { using S = double; using FT = double; static constexpr bool dev = false; };
-template <>
-struct BandTraits<LBMIB_F32> { using S = float; using FT = float; static constexpr bool dev = true; };
-template <>
-struct BandTraits<LBMIB_F16> { using S = half_t; using FT = float; static constexpr bool dev = true; };
-template <>
-struct BandTraits<LBMIB_BF16> { using S = uint16_t; using FT = float; static constexpr bool dev = true; };
-
-template <int P>
-__device__ __forceinline__ double bld(const typename BandTraits<P>::S *p) {
-  if constexpr (P == LBMIB_F16) return (double)(float)(*p) * (1.0 / kDevScale16);
-  else if constexpr (P == LBMIB_BF16) return (double)__uint_as_float((uint32_t)(*p) << 16) * (1.0 / kDevScale16);
-  else return (double)*p;
-}
-
-template <int P>
-__global__ void __launch_bounds__(256)
-    k_band_moments(IBState ib, Geometry g, const typename BandTraits<P>::S *__restrict__ f, int slot,
-                   const typename BandTraits<P>::FT *__restrict__ F, double rho0, double u0x, double u0y, double u0z,
-                   double gz) {
-  // total cells over all sheets' boxes, enumerated sheet by sheet
-  long total = 0;
-  for (int s = 0; s < ib.nsheets; ++s) {
-    const int *b = ib.box + (slot * kMaxSheets + s) * 6;
-    if (b[0] <= b[1]) total += (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
-  }
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    long r = e;
-    int s = 0;
-    const int *b = nullptr;
-    for (s = 0; s < ib.nsheets; ++s) {
-      b = ib.box + (slot * kMaxSheets + s) * 6;
-      if (b[0] > b[1]) continue;
-      const long v = (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
-      if (r < v) break;
-      r -= v;
-    }
-    const int nxs = b[1] - b[0] + 1, nys = b[3] - b[2] + 1;
-    const int x = b[0] + (int)(r % nxs), y = b[2] + (int)((r / nxs) % nys), z = b[4] + (int)(r / ((long)nxs * nys));
+__global__ void __launch_bounds__(kMoveThreads)
+    k_move(IBState ib, Geometry g, const typename Traits<P>::S *__restrict__ f, const typename Traits<P>::FT *__restrict__ F,
+           double u0x, double u0y, double u0z, double gz, int reset_slot) {
+  __shared__ double prod[kMoveThreads / 64][3][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = blockIdx.x * (kMoveThreads / 64) + wv;
+  const int M = ib.nnodes;
+  const bool live = n < M && ib.start[n] >= 0;
+  if (live) {
+    const int i2 = lane >> 4, j2 = (lane >> 2) & 3, k2 = lane & 3;
+    const int x = ib.start[n] + i2, y = ib.start[M + n] + j2, z = ib.start[2 * M + n] + k2;
     const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
     const bool mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] && z <= g.mhi[2];
-    double ux = u0x, uy = u0y, uz = u0z;  // cells outside the moment box keep init_gv's velocity
+    double ux = u0x, uy = u0y, uz = u0z;
     if (mom) {
-      double v[19];
-      const bool interior = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1] && z > g.lo[2] && z < g.hi[2];
-      for (int q = 0; q < 19; ++q) {
-        int px = x - kCX[q], py = y - kCY[q], pz = z - kCZ[q];
-        bool bb = false;
-        if (!interior) {
-          if (g.bc[0] == LBMIB_BC_PERIODIC) px = (px + g.nx) % g.nx;
-          else bb |= (px < g.lo[0] || px > g.hi[0]);
-          if (g.bc[1] == LBMIB_BC_PERIODIC) py = (py + g.ny) % g.ny;
-          else bb |= (py < g.lo[1] || py > g.hi[1]);
-          if (g.bc[2] == LBMIB_BC_PERIODIC) pz = (pz + g.nz) % g.nz;
-          else bb |= (pz < g.lo[2] || pz > g.hi[2]);
-        }
-        const long src = bb ? (long)kOPP[q] * g.pitch + c0
-                            : (long)q * g.pitch + px + (long)g.nx * (py + (long)g.ny * pz);
-        double fq = bld<P>(f + src);
-        if (BandTraits<P>::dev) fq += 1.0 / kWDEN[q];
-        v[q] = fq;
-      }
-      // the F64 sum is the reference's order (lbm_ib.c:2498-2525)
-      double s1 = v[0], s2 = 0, s3 = 0, s4 = 0;
+      typename Traits<P>::T v[19];
+      pull<P>(g, f, x, y, z, c0, v);
+      double fv[19];
+      for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + 1.0 / kWDEN[q] : (double)v[q];
+      double s1 = fv[0], s2 = 0, s3 = 0, s4 = 0;
       for (int q = 1; q < 19; ++q) {
-        s1 += v[q];
-        if (kCX[q]) s2 = kCX[q] > 0 ? s2 + v[q] : s2 - v[q];
-        if (kCY[q]) s3 = kCY[q] > 0 ? s3 + v[q] : s3 - v[q];
-        if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + v[q] : s4 - v[q];
+        s1 += fv[q];
+        if (kCX[q]) s2 = kCX[q] > 0 ? s2 + fv[q] : s2 - fv[q];
+        if (kCY[q]) s3 = kCY[q] > 0 ? s3 + fv[q] : s3 - fv[q];
+        if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + fv[q] : s4 - fv[q];
       }
       const double fx = F[c0], fy = F[g.n + c0], fz = F[2 * g.n + c0];
       ux = (s2 + 0.5 * fx) / s1;
       uy = (s3 + 0.5 * fy) / s1;
       uz = (s4 + 0.5 * (fz + s1 * gz)) / s1;
     }
-    ib.U[c0] = ux;
-    ib.U[g.n + c0] = uy;
-    ib.U[2 * g.n + c0] = uz;
-  }
-}
-
-// ----------------------------------------------------------------- k_move
-
-__global__ void __launch_bounds__(256) k_move(IBState ib, Geometry g, int reset_slot) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int M = ib.nnodes;
-  if (tid < M && ib.start[tid] >= 0) {
-    const int n = tid;
     const double c_64 = 1.0 / (6.4e1);
-    const int i0 = ib.start[n], j0 = ib.start[M + n], k0 = ib.start[2 * M + n];
-    double s1 = 0, s2 = 0, s3 = 0;
-    for (int i2 = 0; i2 <= 3; i2++) {
-      const double wx = ib.wgt[i2 * (long)M + n];
-      for (int j2 = 0; j2 <= 3; j2++) {
-        const double wy = ib.wgt[(4 + j2) * (long)M + n];
-        for (int k2 = 0; k2 <= 3; k2++) {
-          const double wz = ib.wgt[(8 + k2) * (long)M + n];
-          const long c0 = (long)(i0 + i2) + (long)g.nx * ((long)(j0 + j2) + (long)g.ny * (k0 + k2));
-          s1 += ib.U[c0] * wx * wy * wz * c_64;
-          s2 += ib.U[g.n + c0] * wx * wy * wz * c_64;
-          s3 += ib.U[2 * g.n + c0] * wx * wy * wz * c_64;
-        }
-      }
-    }
-    ib.pos[n] += 1 * s1;  // X += dt * u(X), lbm_ib.c:2697-2699
-    ib.pos[M + n] += 1 * s2;
-    ib.pos[2 * M + n] += 1 * s3;
+    const double wx = ib.wgt[i2 * (long)M + n], wy = ib.wgt[(4 + j2) * (long)M + n], wz = ib.wgt[(8 + k2) * (long)M + n];
+    prod[wv][0][lane] = ux * wx * wy * wz * c_64;
+    prod[wv][1][lane] = uy * wx * wy * wz * c_64;
+    prod[wv][2][lane] = uz * wx * wy * wz * c_64;
+  }
+  __syncthreads();
+  if (live && lane < 3) {
+    double acc = 0;
+    for (int l = 0; l < 64; ++l) acc += prod[wv][lane][l];
+    ib.pos[(long)lane * M + n] += 1 * acc;  // X += dt * u(X), lbm_ib.c:2697-2699
   }
   // clear the bins for the next spread and the box slot it will fill
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nbins = ib.nby * ib.nbz;
   for (int b = tid; b < nbins; b += gridDim.x * blockDim.x) ib.bin_count[b] = 0;
   if (tid < ib.nsheets * 6) {
@@ -389,56 +369,41 @@ int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipSt
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
-static int gather_blocks(const IBState &ib) {
-  // enough workgroups to cover typical sheets in one pass; the kernel
-  // strides over any remaining tiles
-  long tiles = 0;
-  (void)tiles;
-  return 1024;
-}
-
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
-  int m2 = 1;
-  while (m2 < 4 * ib.cap) m2 <<= 1;
-  const size_t shm = sizeof(int) * (size_t)m2;
-  if (prec == LBMIB_F64) k_gather<double><<<gather_blocks(ib), kGatherThreads, shm, s>>>(ib, g, slot, (double *)F);
-  else k_gather<float><<<gather_blocks(ib), kGatherThreads, shm, s>>>(ib, g, slot, (float *)F);
+  const int mcap = min(kCandBins * kCandBins * ib.cap, kGatherMaxCand);
+  const size_t shm = sizeof(int) * 5 * (size_t)mcap;
+  const int blocks = 512;  // the kernel strides over any further tiles
+  if (prec == LBMIB_F64) k_gather<double><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (double *)F, mcap);
+  else k_gather<float><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (float *)F, mcap);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
-int launch_band_moments(int prec, const IBState &ib, const Geometry &g, const void *f, int slot, const void *F,
-                        const FluidConsts<double> &c, hipStream_t s) {
+int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
+                const FluidConsts<double> &c, int reset_slot, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
-  const int nb = 1024;
+  const int nodes_per_block = kMoveThreads / 64;
+  const int blocks = grid_for(ib.nnodes, nodes_per_block);
   switch (prec) {
     case LBMIB_F64:
-      k_band_moments<LBMIB_F64><<<nb, 256, 0, s>>>(ib, g, (const double *)f, slot, (const double *)F, c.rho0, c.u0x,
-                                                    c.u0y, c.u0z, c.g);
+      k_move<LBMIB_F64><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const double *)f, (const double *)F, c.u0x, c.u0y,
+                                                         c.u0z, c.g, reset_slot);
       break;
     case LBMIB_F32:
-      k_band_moments<LBMIB_F32><<<nb, 256, 0, s>>>(ib, g, (const float *)f, slot, (const float *)F, c.rho0, c.u0x,
-                                                    c.u0y, c.u0z, c.g);
+      k_move<LBMIB_F32><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const float *)f, (const float *)F, c.u0x, c.u0y,
+                                                         c.u0z, c.g, reset_slot);
       break;
     case LBMIB_F16:
-      k_band_moments<LBMIB_F16><<<nb, 256, 0, s>>>(ib, g, (const half_t *)f, slot, (const float *)F, c.rho0, c.u0x,
-                                                    c.u0y, c.u0z, c.g);
+      k_move<LBMIB_F16><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const half_t *)f, (const float *)F, c.u0x, c.u0y,
+                                                         c.u0z, c.g, reset_slot);
       break;
     case LBMIB_BF16:
-      k_band_moments<LBMIB_BF16><<<nb, 256, 0, s>>>(ib, g, (const uint16_t *)f, slot, (const float *)F, c.rho0,
-                                                     c.u0x, c.u0y, c.u0z, c.g);
+      k_move<LBMIB_BF16><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const uint16_t *)f, (const float *)F, c.u0x,
+                                                          c.u0y, c.u0z, c.g, reset_slot);
       break;
     default:
       return LBMIB_ERR_ARG;
   }
-  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
-}
-
-int launch_move(const IBState &ib, const Geometry &g, int reset_slot, hipStream_t s) {
-  if (ib.nnodes <= 0) return LBMIB_OK;
-  const int nbins = ib.nby * ib.nbz;
-  const long work = ib.nnodes > nbins ? ib.nnodes : nbins;
-  k_move<<<grid_for(work > kMaxSheets * 6 ? work : kMaxSheets * 6, 256), 256, 0, s>>>(ib, g, reset_slot);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 

@@ -76,15 +76,11 @@ struct IBState {
   int *bin_list;     // [nbins][cap]
   int nby, nbz, cap;
   int *box;          // [2][kMaxSheets][6]
-  double *U;         // band velocity [3][n]
   int *err;
   int quirks;
 };
 
 // host-side launchers (defined in lbmib_fluid.hip / lbmib_ib.hip)
-struct Launch {
-  hipStream_t stream;
-};
 
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
@@ -99,9 +95,8 @@ int totals_blocks(const Geometry &g);
 
 int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s);
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s);
-int launch_band_moments(int prec, const IBState &ib, const Geometry &g, const void *f, int slot, const void *F,
-                        const FluidConsts<double> &c, hipStream_t s);
-int launch_move(const IBState &ib, const Geometry &g, int reset_slot, hipStream_t s);
+int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
+                const FluidConsts<double> &c, int reset_slot, hipStream_t s);
 int launch_ib_reset(const IBState &ib, int reset_bins, int reset_slot_mask, hipStream_t s);
 
 }  // namespace lbmib

@@ -35,7 +35,7 @@ CTX_CALLS = (
 ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lbmib_step",
                "lbmib_get_time", "lbmib_get_fibers", "lbmib_get_macroscopic", "lbmib_get_df",
                "lbmib_get_force", "lbmib_get_totals", "lbmib_get_band_box", "lbmib_time_steps",
-               "lbmib_bytes_per_cell") + tuple("lbmib_" + n for n in CTX_CALLS)
+               "lbmib_bytes_per_cell", "lbmib_probe_copy_bandwidth") + tuple("lbmib_" + n for n in CTX_CALLS)
 
 
 class LbmibError(RuntimeError):
@@ -84,6 +84,7 @@ def load_library(path=LIB_PATH):
     L.lbmib_get_band_box.argtypes = [vp, i, ip]
     L.lbmib_time_steps.argtypes = [vp, i, i, dp, dp]
     L.lbmib_bytes_per_cell.argtypes = [vp, dp]
+    L.lbmib_probe_copy_bandwidth.argtypes = [d, i, dp]
     for n in CTX_CALLS:
         getattr(L, "lbmib_" + n).argtypes = [vp]
     for n in ALL_SYMBOLS:
@@ -283,3 +284,14 @@ class Engine:
         b = ctypes.c_double()
         self._check(self.L.lbmib_bytes_per_cell(self.h, ctypes.byref(b)), "bytes_per_cell")
         return b.value
+
+
+def probe_copy_bandwidth(nbytes=4e9, reps=5):
+    """Best-of-reps GB/s of a plain 16-byte-per-lane streaming copy of
+    nbytes (read + write), the measured ceiling for HBM-bound kernels."""
+    L = load_library()
+    out = ctypes.c_double()
+    rc = L.lbmib_probe_copy_bandwidth(float(nbytes), int(reps), ctypes.byref(out))
+    if rc != OK:
+        raise LbmibError(rc, "probe_copy_bandwidth")
+    return out.value

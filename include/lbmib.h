@@ -143,6 +143,11 @@ int lbmib_get_band_box(lbmib_ctx *ctx, int sheet, int *box6);
 int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *ms_total, double *ms_main_mean);
 /* Bytes per cell update of the population storage (2 * 19 * sizeof). */
 int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes);
+/* Device copy bandwidth on the current device: a 16-byte-per-lane streaming
+ * copy of `bytes` (read + write counted), best of `reps` launches timed with
+ * HIP events.  Not in the reference: the measured ceiling that the fused
+ * kernel's achieved bandwidth is compared against. */
+int lbmib_probe_copy_bandwidth(double bytes, int reps, double *gbps);
 
 #ifdef __cplusplus
 }
