@@ -49,6 +49,8 @@ struct lbmib_ctx {
   int time = 1;
   int init_bits = 0;  // 1: DF0, 2: df1, 4: inout
   int latched = LBMIB_OK;
+  int vec_req = 0;   // fused-kernel cells per thread (0 = automatic)
+  int nt_req = -1;   // nontemporal population stores (-1 = automatic)
   std::vector<hipEvent_t> evs;
 };
 
@@ -346,7 +348,7 @@ extern "C" int lbmib_compute_eqlbrmdistrfuncDF1(lbmib_ctx *ctx) {
   const int *box = ctx->ib.box;
   RCHK(launch_collide_stream(ctx->fp.precision, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c,
                              ctx->nsheets, box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s],
-                             ctx->F[1 - s], ctx->stream));
+                             ctx->F[1 - s], ctx->vec_req, ctx->nt_req, ctx->stream));
   ctx->pending = 1;
   ctx->band_done = 0;
   return LBMIB_OK;
@@ -409,7 +411,7 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
   if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
   RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
                              box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
-                             ctx->stream));
+                             ctx->vec_req, ctx->nt_req, ctx->stream));
   if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
   if (ctx->nnodes > 0) {
     RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream));
@@ -613,6 +615,15 @@ extern "C" int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *m
     *ms_main_mean = acc / nsteps;
   }
   return poll_status(ctx);
+}
+
+extern "C" int lbmib_set_kernel_variant(lbmib_ctx *ctx, int cells_per_thread, int nontemporal) {
+  if (!ctx || cells_per_thread < 0 || cells_per_thread > 8 || nontemporal < -1 || nontemporal > 1) return LBMIB_ERR_ARG;
+  if (cells_per_thread > 1 && collide_vec_width(ctx->fp.precision, ctx->g.nx, cells_per_thread) != cells_per_thread)
+    return LBMIB_ERR_ARG;  // not available for this precision / grid
+  ctx->vec_req = cells_per_thread;
+  ctx->nt_req = nontemporal;
+  return LBMIB_OK;
 }
 
 extern "C" int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes) {

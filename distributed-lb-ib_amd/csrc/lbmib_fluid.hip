@@ -434,13 +434,7 @@ __global__ void __launch_bounds__(kTotThreads) k_totals_final(const double *__re
 // cache lines.
 static inline dim3 row_block(const Geometry &g) {
   int bx;
-  static int env_bx = -1;
-  if (env_bx < 0) {
-    const char *e = getenv("LBMIB_BLOCK_X");
-    env_bx = e ? atoi(e) : 0;
-    if (env_bx % 64 != 0 || env_bx > 1024) env_bx = 0;
-  }
-  if (env_bx > 0) return dim3(env_bx, 1, 1);
+
   if (g.nx % 64 == 0 && g.nx >= 256 && g.nx <= 1024) bx = g.nx;
   else if (g.nx > 1024) bx = 256;
   else bx = ((g.nx + 63) / 64) * 64;
@@ -466,21 +460,16 @@ int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<dou
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
-// Cells per thread of the vector kernel: 16-byte accesses per lane by
-// default (F64: 2, F32: 4, 16-bit: 8); LBMIB_VEC=1 selects the one-cell
-// kernel, LBMIB_VEC=2/4/8 another width.  Falls back to one cell per thread
-// when nx is not a multiple of the width or a row needs more than 256
-// threads.
-static int vec_width(int prec, int nx) {
-  static int env = -2;
-  if (env == -2) {
-    const char *e = getenv("LBMIB_VEC");
-    env = e ? atoi(e) : -1;
-  }
-  int v = env > 0 ? env : (prec == LBMIB_F64 ? 2 : (prec == LBMIB_F32 ? 4 : 8));
-  if (prec == LBMIB_F64 && v > 4) v = 4;
-  if (v != 2 && v != 4 && v != 8) return 1;
-  if (prec != LBMIB_F32 && prec != LBMIB_F64 && v == 2) v = 4;
+// Cells per thread of the fused kernel: at most 16 bytes per lane (wider
+// vectors push the kernel past 256 VGPRs).  Falls back to one cell per
+// thread when nx is not a multiple of the width or a row would need more
+// than 256 threads.
+int collide_vec_width(int prec, int nx, int vec_req) {
+  int v = vec_req;
+  if (v <= 0) v = (prec == LBMIB_F16 || prec == LBMIB_BF16) ? 8 : 1;
+  const int max_v = prec == LBMIB_F64 ? 2 : (prec == LBMIB_F32 ? 4 : 8);
+  const int min_v = (prec == LBMIB_F16 || prec == LBMIB_BF16) ? 4 : 2;
+  if (v == 1 || v > max_v || v < min_v || (v & (v - 1))) return 1;
   if (nx % v != 0 || nx / v > 256) return 1;
   return v;
 }
@@ -517,11 +506,9 @@ static void launch_any(int v, const Geometry &g, const void *fin, void *fout, in
                        hipStream_t s) {
   if constexpr (P == LBMIB_F64) {
     if (v == 2) return launch_vec<P, 2, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
-    if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   } else if constexpr (P == LBMIB_F32) {
     if (v == 2) return launch_vec<P, 2, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
-    if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   } else {
     if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
@@ -531,13 +518,9 @@ static void launch_any(int v, const Geometry &g, const void *fin, void *fout, in
 
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                          const void *Fcur, const void *Fprev, hipStream_t s) {
-  static int nt = -1;
-  if (nt < 0) {
-    const char *e = getenv("LBMIB_NT");
-    nt = (e && atoi(e) == 0) ? 0 : 1;  // nontemporal stores unless LBMIB_NT=0
-  }
-  const int v = vec_width(prec, g.nx);
+                          const void *Fcur, const void *Fprev, int vec_req, int nt_req, hipStream_t s) {
+  const int nt = nt_req < 0 ? 1 : (nt_req ? 1 : 0);  // nontemporal stores by default
+  const int v = collide_vec_width(prec, g.nx, vec_req);
   LBMIB_DISPATCH(prec, ({
     if (nt) launch_any<P, true>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     else launch_any<P, false>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);

@@ -85,7 +85,8 @@ struct IBState {
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                          const void *Fcur, const void *Fprev, hipStream_t s);
+                          const void *Fcur, const void *Fprev, int vec_req, int nt_req, hipStream_t s);
+int collide_vec_width(int prec, int nx, int vec_req);
 int launch_materialize(int prec, const Geometry &g, const void *f, int own_mode, const FluidConsts<double> &c,
                        int nsheets, const int *box, const void *F, double *df, double *rho, double *vel,
                        hipStream_t s);

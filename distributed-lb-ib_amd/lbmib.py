@@ -35,7 +35,8 @@ CTX_CALLS = (
 ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lbmib_step",
                "lbmib_get_time", "lbmib_get_fibers", "lbmib_get_macroscopic", "lbmib_get_df",
                "lbmib_get_force", "lbmib_get_totals", "lbmib_get_band_box", "lbmib_time_steps",
-               "lbmib_bytes_per_cell", "lbmib_probe_copy_bandwidth") + tuple("lbmib_" + n for n in CTX_CALLS)
+               "lbmib_bytes_per_cell", "lbmib_probe_copy_bandwidth",
+               "lbmib_set_kernel_variant") + tuple("lbmib_" + n for n in CTX_CALLS)
 
 
 class LbmibError(RuntimeError):
@@ -85,6 +86,7 @@ def load_library(path=LIB_PATH):
     L.lbmib_time_steps.argtypes = [vp, i, i, dp, dp]
     L.lbmib_bytes_per_cell.argtypes = [vp, dp]
     L.lbmib_probe_copy_bandwidth.argtypes = [d, i, dp]
+    L.lbmib_set_kernel_variant.argtypes = [vp, i, i]
     for n in CTX_CALLS:
         getattr(L, "lbmib_" + n).argtypes = [vp]
     for n in ALL_SYMBOLS:
@@ -279,6 +281,11 @@ class Engine:
         self._check(self.L.lbmib_time_steps(self.h, nwarm, nsteps, ctypes.byref(tot),
                                             ctypes.byref(main) if main_kernel else None), "time_steps")
         return tot.value, (main.value if main_kernel else None)
+
+    def set_kernel_variant(self, cells_per_thread=0, nontemporal=-1):
+        """Fused-kernel variant (see lbmib_set_kernel_variant); all variants
+        give identical results."""
+        self._check(self.L.lbmib_set_kernel_variant(self.h, cells_per_thread, nontemporal), "set_kernel_variant")
 
     def bytes_per_cell(self):
         b = ctypes.c_double()

@@ -206,3 +206,32 @@ def test_fiber_leaving_grid_reports_error(lbmib):
         e.init()
         e.step(1)
         assert e.status() == lbmib.ERR_FIBER_OOB
+
+
+VARIANTS = {
+    "f64": [(1, 0), (1, 1), (2, 0), (2, 1)],
+    "f32": [(1, 0), (1, 1), (2, 1), (4, 0), (4, 1)],
+    "f16": [(1, 1), (4, 1), (8, 0), (8, 1)],
+    "bf16": [(1, 1), (4, 1), (8, 1)],
+}
+
+
+@pytest.mark.parametrize("precision", sorted(VARIANTS))
+def test_kernel_variants_are_bitwise_identical(lbmib, precision):
+    """Every cells-per-thread / store variant of the fused kernel computes
+    the same bits (same per-cell arithmetic, only the memory access width
+    differs)."""
+    kw = small_ib_config(lbmib, precision, n=(64, 32, 36))
+    ref = None
+    for v, nt in VARIANTS[precision]:
+        with lbmib.Engine(**kw) as e:
+            e.set_kernel_variant(v, nt)
+            e.init()
+            e.step(12)
+            assert e.status() == 0
+            out = (e.df(), e.fibers(0))
+        if ref is None:
+            ref = out
+        else:
+            for a, b in zip(out, ref):
+                assert np.array_equal(a, b), (precision, v, nt)
