@@ -39,8 +39,8 @@ def parse_args():
     p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", default="2,6", help="N1,N2 steps of the differenced CPU timing")
-    p.add_argument("--vec", type=int, default=-1, help="fused-kernel cells per thread (-1: autotune, 0: default)")
-    p.add_argument("--nt", type=int, default=-2, help="nontemporal stores 0/1 (-2: autotune)")
+    p.add_argument("--variant", default="auto",
+                   help="fused-kernel launch shape 'cells,ntmask,blocks_per_cu' or 'auto' (autotune)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -119,33 +119,33 @@ def cpu_reference_baseline(n, rows, steps_pair):
                       f"T_step = (T({n2}) - T({n1}))/{n2 - n1} = {t_step:.3f} s"}
 
 
-VARIANTS = {"f64": [(1, 1), (1, 0), (2, 1)], "f32": [(1, 1), (1, 0), (2, 1), (4, 1)],
-            "f16": [(8, 1), (4, 1), (1, 1)], "bf16": [(8, 1), (4, 1), (1, 1)]}
+# (cells per thread, nontemporal mask: 1 stores / 2 loads, blocks per CU; 0 = one block per row)
+VARIANTS = {"f64": [(1, 1, 0), (1, 3, 0), (1, 3, 4), (1, 3, 8), (1, 1, 4), (2, 1, 0)],
+            "f32": [(1, 1, 0), (1, 3, 0), (1, 3, 2), (1, 3, 4), (1, 3, 8), (1, 1, 4), (4, 1, 0)],
+            "f16": [(8, 1, 0), (4, 1, 0), (1, 1, 0), (1, 3, 0), (1, 3, 4), (1, 3, 8)],
+            "bf16": [(8, 1, 0), (4, 1, 0), (1, 1, 0), (1, 3, 0), (1, 3, 4), (1, 3, 8)]}
 
 
 def autotune(e, a, lbmib):
     """Picks the fastest fused-kernel variant (cells per thread, nontemporal
     stores) by timing a few steps of each on the live engine; all variants
     compute identical bits."""
-    if a.vec >= 0 and a.nt >= -1:
-        e.set_kernel_variant(a.vec, a.nt)
-        return (a.vec, a.nt)
+    if a.variant != "auto":
+        v = tuple(int(x) for x in a.variant.split(","))
+        e.set_kernel_variant(*v)
+        return v
     best = None
-    for v, nt in VARIANTS[a.precision]:
-        if a.vec >= 0 and v != a.vec:
-            continue
-        if a.nt >= 0 and nt != a.nt:
-            continue
+    for v in VARIANTS[a.precision]:
         try:
-            e.set_kernel_variant(v, nt)
+            e.set_kernel_variant(*v)
         except lbmib.LbmibError:
             continue
         _, ms = e.time_steps(5, 30)
-        print(f"[autotune] cells/thread={v} nt={nt}: fused kernel {ms:.4f} ms", file=sys.stderr, flush=True)
+        print(f"[autotune] variant {v}: fused kernel {ms:.4f} ms", file=sys.stderr, flush=True)
         if best is None or ms < best[0]:
-            best = (ms, v, nt)
-    e.set_kernel_variant(best[1], best[2])
-    return (best[1], best[2])
+            best = (ms, v)
+    e.set_kernel_variant(*best[1])
+    return best[1]
 
 
 def calibrate_copy():
@@ -230,7 +230,8 @@ def main():
             "fiber_nodes": nodes,
             "storage": a.precision + (" deviations f-w" if a.precision != "f64" else ""),
             "bytes_per_cell_update": bpc,
-            "kernel_variant": {"cells_per_thread": variant[0], "nontemporal_stores": variant[1]},
+            "kernel_variant": {"cells_per_thread": variant[0], "nontemporal_mask": variant[1],
+                               "blocks_per_cu": variant[2]},
         },
         "roofline": {
             "bound": "hbm",

@@ -49,8 +49,8 @@ struct lbmib_ctx {
   int time = 1;
   int init_bits = 0;  // 1: DF0, 2: df1, 4: inout
   int latched = LBMIB_OK;
-  int vec_req = 0;   // fused-kernel cells per thread (0 = automatic)
-  int nt_req = -1;   // nontemporal population stores (-1 = automatic)
+  KernelVariant kv = {0, -1, 0};  // fused-kernel launch shape
+  int ncu = 0;                    // compute units of the device
   std::vector<hipEvent_t> evs;
 };
 
@@ -100,7 +100,7 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->evs) (void)hipEventDestroy(e);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
-                  ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->d_sheets,
+                  ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.box_part, ctx->ib.done, ctx->d_sheets,
                   ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -181,6 +181,12 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
     if (hipMalloc(ptr, bytes) != hipSuccess) { *ptr = nullptr; rc = LBMIB_ERR_ALLOC; }
   };
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
+  {
+    int dev = 0;
+    if (!rc && (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess))
+      rc = LBMIB_ERR_HIP;
+  }
   const size_t fbytes = (size_t)19 * g.pitch * ctx->esz;
   alloc(&ctx->f[0], fbytes);
   alloc(&ctx->f[1], fbytes);
@@ -214,6 +220,8 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
     alloc((void **)&ib.bin_count, sizeof(int) * (size_t)ib.nby * ib.nbz);
     alloc((void **)&ib.bin_list, sizeof(int) * (size_t)ib.nby * ib.nbz * cap);
     alloc((void **)&ctx->d_sheets, sizeof(SheetDesc) * kMaxSheets);
+    alloc((void **)&ib.box_part, sizeof(int) * kMaxSheets * 6 * (size_t)((M + 255) / 256));
+    alloc((void **)&ib.done, sizeof(unsigned));
   }
   ib.sheets = ctx->d_sheets;
   ib.err = ctx->d_err;
@@ -222,6 +230,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
       rc = LBMIB_ERR_HIP;
     if (!rc && hipMemset(ib.pos, 0, sizeof(double) * 3 * ctx->nnodes) != hipSuccess) rc = LBMIB_ERR_HIP;
     if (!rc && hipMemset(ib.frc, 0, sizeof(double) * NF_COUNT * ctx->nnodes) != hipSuccess) rc = LBMIB_ERR_HIP;
+    if (!rc && hipMemset(ib.done, 0, sizeof(unsigned)) != hipSuccess) rc = LBMIB_ERR_HIP;
   }
   if (!rc && hipMemset(ctx->d_err, 0, sizeof(int)) != hipSuccess) rc = LBMIB_ERR_HIP;
   if (!rc) rc = launch_ib_reset(ib, ctx->nnodes > 0, 3, ctx->stream);
@@ -348,7 +357,7 @@ extern "C" int lbmib_compute_eqlbrmdistrfuncDF1(lbmib_ctx *ctx) {
   const int *box = ctx->ib.box;
   RCHK(launch_collide_stream(ctx->fp.precision, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c,
                              ctx->nsheets, box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s],
-                             ctx->F[1 - s], ctx->vec_req, ctx->nt_req, ctx->stream));
+                             ctx->F[1 - s], ctx->kv, ctx->stream));
   ctx->pending = 1;
   ctx->band_done = 0;
   return LBMIB_OK;
@@ -411,7 +420,7 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
   if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
   RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
                              box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
-                             ctx->vec_req, ctx->nt_req, ctx->stream));
+                             ctx->kv, ctx->stream));
   if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
   if (ctx->nnodes > 0) {
     RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream));
@@ -617,12 +626,16 @@ extern "C" int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *m
   return poll_status(ctx);
 }
 
-extern "C" int lbmib_set_kernel_variant(lbmib_ctx *ctx, int cells_per_thread, int nontemporal) {
-  if (!ctx || cells_per_thread < 0 || cells_per_thread > 8 || nontemporal < -1 || nontemporal > 1) return LBMIB_ERR_ARG;
+extern "C" int lbmib_set_kernel_variant(lbmib_ctx *ctx, int cells_per_thread, int nontemporal, int blocks_per_cu) {
+  if (!ctx || cells_per_thread < 0 || cells_per_thread > 8 || nontemporal < -1 || nontemporal > 3 ||
+      blocks_per_cu < 0 || blocks_per_cu > 64)
+    return LBMIB_ERR_ARG;
   if (cells_per_thread > 1 && collide_vec_width(ctx->fp.precision, ctx->g.nx, cells_per_thread) != cells_per_thread)
     return LBMIB_ERR_ARG;  // not available for this precision / grid
-  ctx->vec_req = cells_per_thread;
-  ctx->nt_req = nontemporal;
+  if (cells_per_thread > 1 && (blocks_per_cu > 0 || (nontemporal > 0 && (nontemporal & 2)))) return LBMIB_ERR_ARG;
+  ctx->kv.cells_per_thread = cells_per_thread;
+  ctx->kv.nt_mask = nontemporal;
+  ctx->kv.blocks = blocks_per_cu * ctx->ncu;
   return LBMIB_OK;
 }
 

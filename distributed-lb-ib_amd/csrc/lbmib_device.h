@@ -50,11 +50,14 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
   return (uint16_t)(u >> 16);
 }
 
-template <int P>
+template <int P, bool NTL = false>
 __device__ __forceinline__ typename Traits<P>::T ld(const typename Traits<P>::S *p) {
-  if constexpr (P == LBMIB_F16) return (float)(*p) * (1.0f / kDevScale16);
-  else if constexpr (P == LBMIB_BF16) return bf16_to_f32(*p) * (1.0f / kDevScale16);
-  else return *p;
+  typename Traits<P>::S s;
+  if constexpr (NTL) s = __builtin_nontemporal_load(p);
+  else s = *p;
+  if constexpr (P == LBMIB_F16) return (float)s * (1.0f / kDevScale16);
+  else if constexpr (P == LBMIB_BF16) return bf16_to_f32(s) * (1.0f / kDevScale16);
+  else return s;
 }
 template <int P, bool NT = false>
 __device__ __forceinline__ void st(typename Traits<P>::S *p, typename Traits<P>::T v) {
@@ -90,7 +93,7 @@ __device__ __forceinline__ bool in_any_box(const int *boxes, int nsheets, int x,
 // Wall axes: a source outside the active box is replaced by the opposite
 // population of the same node (half-way bounce-back, lbm_ib.c:2310-2423).
 // Periodic axes wrap.  Interior cells take the branch-free path.
-template <int P>
+template <int P, bool NTL = false>
 __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>::S *__restrict__ f, int x, int y,
                                      int z, long c, typename Traits<P>::T v[19]) {
   const bool interior = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1] && z > g.lo[2] && z < g.hi[2];
@@ -99,7 +102,7 @@ __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       const long off = kCX[q] * sx + kCY[q] * sy + kCZ[q] * sz;
-      v[q] = ld<P>(f + q * g.pitch + c - off);
+      v[q] = ld<P, NTL>(f + q * g.pitch + c - off);
     }
     return;
   }
@@ -115,7 +118,7 @@ __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>
     else bb |= (pz < g.lo[2] || pz > g.hi[2]);
     const long src = bb ? c + (long)kOPP[q] * g.pitch - (long)q * g.pitch
                         : (long)px + sy * py + sz * pz;
-    v[q] = ld<P>(f + q * g.pitch + src);
+    v[q] = ld<P, NTL>(f + q * g.pitch + src);
   }
 }
 

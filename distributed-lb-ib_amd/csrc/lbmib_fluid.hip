@@ -142,7 +142,7 @@ __device__ __forceinline__ void collide_cell(typename Traits<P>::T v[19], typena
 // The whole update of one active cell: arriving populations (inlet/outlet
 // table on the frozen planes, own populations on the first step, pull
 // otherwise), lagged moments, collision with the current force, store.
-template <int P, bool NT>
+template <int P, bool NT, bool NTL = false>
 __device__ __forceinline__ void update_cell(const Geometry &g, const typename Traits<P>::S *__restrict__ fin,
                                             typename Traits<P>::S *__restrict__ fout, int own,
                                             const FluidConsts<typename Traits<P>::T> &c, int nsheets,
@@ -168,7 +168,7 @@ __device__ __forceinline__ void update_cell(const Geometry &g, const typename Tr
     for (int q = 0; q < 19; ++q) v[q] = ld<P>(fin + q * g.pitch + c0);
     rho = c.rho0; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // init_gv values, lbm_ib.c:837-849
   } else {
-    pull<P>(g, fin, x, y, z, c0, v);
+    pull<P, NTL>(g, fin, x, y, z, c0, v);
     T fpx = 0, fpy = 0, fpz = 0;
     if (in_any_box(box_prev, nsheets, x, y, z)) {
       fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
@@ -193,6 +193,23 @@ __global__ void __launch_bounds__(1024)
                      const typename Traits<P>::FT *__restrict__ Fprev) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   update_cell<P, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, blockIdx.y, blockIdx.z);
+}
+
+// Persistent form: a fixed grid of a few blocks per CU walks the (y, z) rows
+// with a stride, each block updating whole rows; optional nontemporal loads.
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(1024)
+    k_collide_stream_rows(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                          typename Traits<P>::S *__restrict__ fout, int own, FluidConsts<typename Traits<P>::T> c,
+                          int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                          const typename Traits<P>::FT *__restrict__ Fcur,
+                          const typename Traits<P>::FT *__restrict__ Fprev) {
+  const int nrows = g.ny * g.nz;
+  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int y = row % g.ny, z = row / g.ny;
+    for (int x = threadIdx.x; x < g.nx; x += blockDim.x)
+      update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
+  }
 }
 
 // V consecutive cells of one x-row per thread: the 9 populations with
@@ -500,6 +517,20 @@ static void launch_scalar(const Geometry &g, const void *fin, void *fout, int ow
                                            box_prev, (const FT *)Fcur, (const FT *)Fprev);
 }
 
+template <int P, bool NT, bool NTL>
+static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
+                        const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                        const void *Fcur, const void *Fprev, hipStream_t s) {
+  using TT = typename Traits<P>::T;
+  using FT = typename Traits<P>::FT;
+  using S = typename Traits<P>::S;
+  const dim3 b = row_block(g);
+  const int nrows = g.ny * g.nz;
+  k_collide_stream_rows<P, NT, NTL><<<dim3(nblocks < nrows ? nblocks : nrows), b, 0, s>>>(
+      g, (const S *)fin, (S *)fout, own, cast_consts<TT>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
+      (const FT *)Fprev);
+}
+
 template <int P, bool NT>
 static void launch_any(int v, const Geometry &g, const void *fin, void *fout, int own, const FluidConsts<double> &c,
                        int nsheets, const int *box_cur, const int *box_prev, const void *Fcur, const void *Fprev,
@@ -518,9 +549,20 @@ static void launch_any(int v, const Geometry &g, const void *fin, void *fout, in
 
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                          const void *Fcur, const void *Fprev, int vec_req, int nt_req, hipStream_t s) {
-  const int nt = nt_req < 0 ? 1 : (nt_req ? 1 : 0);  // nontemporal stores by default
-  const int v = collide_vec_width(prec, g.nx, vec_req);
+                          const void *Fcur, const void *Fprev, const KernelVariant &kv, hipStream_t s) {
+  const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);  // nontemporal stores by default
+  const int ntl = kv.nt_mask < 0 ? 0 : ((kv.nt_mask >> 1) & 1);
+  const int v = collide_vec_width(prec, g.nx, kv.cells_per_thread);
+  if (v == 1 && (kv.blocks > 0 || ntl)) {
+    const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
+    LBMIB_DISPATCH(prec, ({
+      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+    }));
+    return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+  }
   LBMIB_DISPATCH(prec, ({
     if (nt) launch_any<P, true>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     else launch_any<P, false>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);

@@ -154,7 +154,14 @@ __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask,
     }
   }
   if (!(mask & FK_PREP)) return;  // uniform over the grid
-  // support box of each sheet: reduce per wave, one lane updates the box
+  // Support box of each sheet without contended atomics: wave min/max, LDS
+  // combine per block, one partial per block in global memory, and the last
+  // block to finish reduces the partials (min/max are order-independent, so
+  // the box is deterministic).
+  __shared__ int sbox[kMaxSheets * 6];
+  __shared__ int s_last;
+  if (threadIdx.x < kMaxSheets * 6) sbox[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
+  __syncthreads();
   const int s_lo = __shfl(s, 0, 64);
   const bool uniform = __all(!live || s == s_lo);
   const int lo[3] = {ok ? st[0] : INT_MAX, ok ? st[1] : INT_MAX, ok ? st[2] : INT_MAX};
@@ -165,21 +172,35 @@ __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask,
       rlo[d] = wave_min(lo[d]);
       rhi[d] = wave_max(hi[d]);
     }
-    if ((threadIdx.x & 63) == 0 && rlo[0] != INT_MAX) {
-      int *box = ib.box + (slot * kMaxSheets + s_lo) * 6;
+    if ((threadIdx.x & 63) == 0 && rlo[0] != INT_MAX)
       for (int d = 0; d < 3; ++d) {
-        atomicMin(box + 2 * d, rlo[d]);
-        atomicMax(box + 2 * d + 1, rhi[d]);
+        atomicMin(&sbox[s_lo * 6 + 2 * d], rlo[d]);
+        atomicMax(&sbox[s_lo * 6 + 2 * d + 1], rhi[d]);
       }
-    }
   } else if (ok) {
-    // a wave spanning two sheets: per-node updates (only at sheet seams)
-    int *box = ib.box + (slot * kMaxSheets + s) * 6;
-    for (int d = 0; d < 3; ++d) {
-      atomicMin(box + 2 * d, lo[d]);
-      atomicMax(box + 2 * d + 1, hi[d]);
+    for (int d = 0; d < 3; ++d) {  // a wave across a sheet seam
+      atomicMin(&sbox[s * 6 + 2 * d], lo[d]);
+      atomicMax(&sbox[s * 6 + 2 * d + 1], hi[d]);
     }
   }
+  __syncthreads();
+  if (threadIdx.x < kMaxSheets * 6) ib.box_part[blockIdx.x * kMaxSheets * 6 + threadIdx.x] = sbox[threadIdx.x];
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (threadIdx.x < ib.nsheets * 6) {
+    const int k = threadIdx.x;
+    int acc = (k & 1) ? INT_MIN : INT_MAX;
+    for (unsigned b = 0; b < gridDim.x; ++b) {
+      const int v = __hip_atomic_load(ib.box_part + b * kMaxSheets * 6 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      acc = (k & 1) ? max(acc, v) : min(acc, v);
+    }
+    ib.box[slot * kMaxSheets * 6 + k] = acc;
+  }
+  if (threadIdx.x == 0) atomicExch(ib.done, 0u);
 }
 
 // -------------------------------------------------------------- k_gather
@@ -207,14 +228,17 @@ __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int 
   return false;
 }
 
-constexpr int kGatherMaxCand = 3072;  // 5 ints each: 60 KB of LDS
+constexpr int kGatherMaxCand = 1536;   // ids, sorted ids, 3 starts: 30 KB of LDS
+constexpr int kGatherStage = 256;      // candidates whose delta factors / forces are staged: 30 KB
 
 template <class FT>
 __global__ void __launch_bounds__(kGatherThreads)
     k_gather(IBState ib, Geometry g, int slot, FT *__restrict__ F, int mcap) {
   // LDS: raw candidate ids, then the sorted ids and their support starts
-  extern __shared__ int smem[];
-  int *raw = smem;
+  extern __shared__ double smem_d[];
+  double *sw = smem_d;                      // [kGatherStage][12] delta factors
+  double *se = sw + kGatherStage * 12;      // [kGatherStage][3] elastic force
+  int *raw = reinterpret_cast<int *>(se + kGatherStage * 3);
   int *ids = raw + mcap;
   int *sx = ids + mcap, *sy = sx + mcap, *sz = sy + mcap;
   const int M = ib.nnodes;
@@ -242,6 +266,7 @@ __global__ void __launch_bounds__(kGatherThreads)
     }
     __syncthreads();
     // rank sort (node ids are distinct): ascending (sheet, row, col) order
+    const bool staged = tot <= kGatherStage;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
       const int v = raw[e];
       int r = 0;
@@ -250,6 +275,10 @@ __global__ void __launch_bounds__(kGatherThreads)
       sx[r] = ib.start[v];
       sy[r] = ib.start[M + v];
       sz[r] = ib.start[2 * M + v];
+      if (staged) {
+        for (int a = 0; a < 12; ++a) sw[r * 12 + a] = ib.wgt[a * (long)M + v];
+        for (int a = 0; a < 3; ++a) se[r * 3 + a] = ib.frc[(NF_EX + a) * (long)M + v];
+      }
     }
     __syncthreads();
     const int x0 = b[0], x1 = b[1];
@@ -263,12 +292,18 @@ __global__ void __launch_bounds__(kGatherThreads)
       for (int k = 0; k < tot; ++k) {
         const int ix = x - sx[k], iy = y - sy[k], iz = z - sz[k];
         if ((unsigned)ix > 3u || (unsigned)iy > 3u || (unsigned)iz > 3u) continue;
-        const int n = ids[k];
-        const double wgt = cf * ib.wgt[ix * (long)M + n] * ib.wgt[(4 + iy) * (long)M + n] *
-                           ib.wgt[(8 + iz) * (long)M + n];
-        ax += ib.frc[NF_EX * (long)M + n] * wgt;
-        ay += ib.frc[NF_EY * (long)M + n] * wgt;
-        az += ib.frc[NF_EZ * (long)M + n] * wgt;
+        double wgt, ex, ey, ez;
+        if (staged) {
+          wgt = cf * sw[k * 12 + ix] * sw[k * 12 + 4 + iy] * sw[k * 12 + 8 + iz];
+          ex = se[k * 3]; ey = se[k * 3 + 1]; ez = se[k * 3 + 2];
+        } else {
+          const int n = ids[k];
+          wgt = cf * ib.wgt[ix * (long)M + n] * ib.wgt[(4 + iy) * (long)M + n] * ib.wgt[(8 + iz) * (long)M + n];
+          ex = ib.frc[NF_EX * (long)M + n]; ey = ib.frc[NF_EY * (long)M + n]; ez = ib.frc[NF_EZ * (long)M + n];
+        }
+        ax += ex * wgt;
+        ay += ey * wgt;
+        az += ez * wgt;
       }
       const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
       F[c0] = (FT)ax;
@@ -281,8 +316,9 @@ __global__ void __launch_bounds__(kGatherThreads)
 
 // ----------------------------------------------------------------- k_move
 //
-// One wave per fiber node, one lane per cell of its 4x4x4 support
-// (lane = 16 i2 + 4 j2 + k2, the loop order of lbm_ib.c:2614-2682).  Each
+// One wave per fiber node, one lane per cell of its 4x4x4 support (lanes
+// ordered along x for coalescing; products kept in the order i2, j2, k2 of
+// lbm_ib.c:2614-2682).  Each
 // lane computes the cell's rho/u from the new populations exactly as
 // compute_rho_and_u does (lbm_ib.c:2498-2525; cells outside the moment box
 // keep init_gv's velocity), weights it, and lane 0 sums the 64 products in
@@ -300,7 +336,10 @@ __global__ void __launch_bounds__(kMoveThreads)
   const int M = ib.nnodes;
   const bool live = n < M && ib.start[n] >= 0;
   if (live) {
-    const int i2 = lane >> 4, j2 = (lane >> 2) & 3, k2 = lane & 3;
+    // lanes run along x (consecutive addresses); the product goes to its
+    // slot in the reference's (i2, j2, k2) loop order
+    const int i2 = lane & 3, j2 = (lane >> 2) & 3, k2 = lane >> 4;
+    const int slot_ref = i2 * 16 + j2 * 4 + k2;
     const int x = ib.start[n] + i2, y = ib.start[M + n] + j2, z = ib.start[2 * M + n] + k2;
     const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
     const bool mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] && z <= g.mhi[2];
@@ -324,9 +363,9 @@ __global__ void __launch_bounds__(kMoveThreads)
     }
     const double c_64 = 1.0 / (6.4e1);
     const double wx = ib.wgt[i2 * (long)M + n], wy = ib.wgt[(4 + j2) * (long)M + n], wz = ib.wgt[(8 + k2) * (long)M + n];
-    prod[wv][0][lane] = ux * wx * wy * wz * c_64;
-    prod[wv][1][lane] = uy * wx * wy * wz * c_64;
-    prod[wv][2][lane] = uz * wx * wy * wz * c_64;
+    prod[wv][0][slot_ref] = ux * wx * wy * wz * c_64;
+    prod[wv][1][slot_ref] = uy * wx * wy * wz * c_64;
+    prod[wv][2][slot_ref] = uz * wx * wy * wz * c_64;
   }
   __syncthreads();
   if (live && lane < 3) {
@@ -372,7 +411,7 @@ int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipSt
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
   const int mcap = min(kCandBins * kCandBins * ib.cap, kGatherMaxCand);
-  const size_t shm = sizeof(int) * 5 * (size_t)mcap;
+  const size_t shm = sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * 5 * (size_t)mcap;
   const int blocks = 512;  // the kernel strides over any further tiles
   if (prec == LBMIB_F64) k_gather<double><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (double *)F, mcap);
   else k_gather<float><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (float *)F, mcap);

@@ -76,6 +76,8 @@ struct IBState {
   int *bin_list;     // [nbins][cap]
   int nby, nbz, cap;
   int *box;          // [2][kMaxSheets][6]
+  int *box_part;     // [fiber blocks][kMaxSheets][6] per-block partial boxes
+  unsigned *done;    // blocks of k_fiber finished (reset by the last one)
   int *err;
   int quirks;
 };
@@ -83,9 +85,16 @@ struct IBState {
 // host-side launchers (defined in lbmib_fluid.hip / lbmib_ib.hip)
 
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
+// Launch shape of the fused kernel (all forms compute identical bits).
+struct KernelVariant {
+  int cells_per_thread;  // 0 automatic, 1, 2, 4, 8
+  int nt_mask;           // -1 automatic; bit 0 nontemporal stores, bit 1 nontemporal loads
+  int blocks;            // > 0: persistent grid of this many blocks walking rows (one cell per thread)
+};
+
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                          const void *Fcur, const void *Fprev, int vec_req, int nt_req, hipStream_t s);
+                          const void *Fcur, const void *Fprev, const KernelVariant &kv, hipStream_t s);
 int collide_vec_width(int prec, int nx, int vec_req);
 int launch_materialize(int prec, const Geometry &g, const void *f, int own_mode, const FluidConsts<double> &c,
                        int nsheets, const int *box, const void *F, double *df, double *rho, double *vel,

@@ -86,7 +86,7 @@ def load_library(path=LIB_PATH):
     L.lbmib_time_steps.argtypes = [vp, i, i, dp, dp]
     L.lbmib_bytes_per_cell.argtypes = [vp, dp]
     L.lbmib_probe_copy_bandwidth.argtypes = [d, i, dp]
-    L.lbmib_set_kernel_variant.argtypes = [vp, i, i]
+    L.lbmib_set_kernel_variant.argtypes = [vp, i, i, i]
     for n in CTX_CALLS:
         getattr(L, "lbmib_" + n).argtypes = [vp]
     for n in ALL_SYMBOLS:
@@ -282,10 +282,11 @@ class Engine:
                                             ctypes.byref(main) if main_kernel else None), "time_steps")
         return tot.value, (main.value if main_kernel else None)
 
-    def set_kernel_variant(self, cells_per_thread=0, nontemporal=-1):
-        """Fused-kernel variant (see lbmib_set_kernel_variant); all variants
-        give identical results."""
-        self._check(self.L.lbmib_set_kernel_variant(self.h, cells_per_thread, nontemporal), "set_kernel_variant")
+    def set_kernel_variant(self, cells_per_thread=0, nontemporal=-1, blocks_per_cu=0):
+        """Fused-kernel launch shape (see lbmib_set_kernel_variant); all
+        variants give identical results."""
+        self._check(self.L.lbmib_set_kernel_variant(self.h, cells_per_thread, nontemporal, blocks_per_cu),
+                    "set_kernel_variant")
 
     def bytes_per_cell(self):
         b = ctypes.c_double()

@@ -208,24 +208,24 @@ def test_fiber_leaving_grid_reports_error(lbmib):
         assert e.status() == lbmib.ERR_FIBER_OOB
 
 
-VARIANTS = {
-    "f64": [(1, 0), (1, 1), (2, 0), (2, 1)],
-    "f32": [(1, 0), (1, 1), (2, 1), (4, 0), (4, 1)],
-    "f16": [(1, 1), (4, 1), (8, 0), (8, 1)],
-    "bf16": [(1, 1), (4, 1), (8, 1)],
+VARIANTS = {  # (cells per thread, nontemporal mask, blocks per CU)
+    "f64": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (1, 3, 0), (1, 3, 4), (1, 1, 8)],
+    "f32": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (4, 1, 0), (1, 3, 0), (1, 3, 4), (1, 1, 8), (1, 2, 2)],
+    "f16": [(1, 1, 0), (4, 1, 0), (8, 1, 0), (8, 0, 0), (1, 3, 4)],
+    "bf16": [(1, 1, 0), (4, 1, 0), (8, 1, 0), (1, 3, 4)],
 }
 
 
 @pytest.mark.parametrize("precision", sorted(VARIANTS))
 def test_kernel_variants_are_bitwise_identical(lbmib, precision):
-    """Every cells-per-thread / store variant of the fused kernel computes
-    the same bits (same per-cell arithmetic, only the memory access width
-    differs)."""
+    """Every launch shape of the fused kernel computes the same bits (same
+    per-cell arithmetic; only the memory access width, cache hints and the
+    block-to-row mapping differ)."""
     kw = small_ib_config(lbmib, precision, n=(64, 32, 36))
     ref = None
-    for v, nt in VARIANTS[precision]:
+    for v, nt, bpc in VARIANTS[precision]:
         with lbmib.Engine(**kw) as e:
-            e.set_kernel_variant(v, nt)
+            e.set_kernel_variant(v, nt, bpc)
             e.init()
             e.step(12)
             assert e.status() == 0
@@ -234,4 +234,4 @@ def test_kernel_variants_are_bitwise_identical(lbmib, precision):
             ref = out
         else:
             for a, b in zip(out, ref):
-                assert np.array_equal(a, b), (precision, v, nt)
+                assert np.array_equal(a, b), (precision, v, nt, bpc)
