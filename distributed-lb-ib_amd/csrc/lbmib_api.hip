@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "lbmib_internal.h"
@@ -51,6 +52,19 @@ struct lbmib_ctx {
   int latched = LBMIB_OK;
   KernelVariant kv = {0, -1, 0};  // fused-kernel launch shape
   int ncu = 0;                    // compute units of the device
+  // split steps: bulk on `stream`, sheet band + IB chain on `stream_b`
+  int overlap_req = -1;           // -1 automatic (on when there are sheets), 0 off, 1 on
+  hipStream_t stream_b = nullptr;
+  hipEvent_t evA[2] = {nullptr, nullptr}, evB[2] = {nullptr, nullptr}, evJoin = nullptr;
+  int split_run = 0;              // steps taken in the current run of split steps
+  Slabs slabs = {0, {0}, {0}};
+  int slab_age = 0;               // steps since the slabs were planned
+  // ring of asynchronous support-box readbacks (pinned host memory)
+  static constexpr int kRing = 16;
+  int *h_box_ring = nullptr;      // [kRing][kMaxSheets * 6]
+  hipEvent_t ev_box[kRing] = {};
+  int ring_step[kRing];           // step whose box each entry holds (-1: none)
+  double hx[kMaxSheets][2];       // host x-extent of each sheet's nodes (from the uploaded positions)
   std::vector<hipEvent_t> evs;
 };
 
@@ -82,6 +96,7 @@ static void axis_ranges(int bc, int n, int *lo, int *hi, int *mlo, int *mhi) {
 static int map_device_err(int bits) {
   if (bits & DERR_FIBER_OOB) return LBMIB_ERR_FIBER_OOB;
   if (bits & DERR_BIN_OVERFLOW) return LBMIB_ERR_BIN_OVERFLOW;
+  if (bits & DERR_BAND) return LBMIB_ERR_BAND;
   return LBMIB_OK;
 }
 
@@ -99,6 +114,13 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (!ctx) return LBMIB_ERR_ARG;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->evs) (void)hipEventDestroy(e);
+  if (ctx->stream_b) (void)hipStreamSynchronize(ctx->stream_b);
+  for (hipEvent_t e : {ctx->evA[0], ctx->evA[1], ctx->evB[0], ctx->evB[1], ctx->evJoin})
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->ev_box)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->h_box_ring) (void)hipHostFree(ctx->h_box_ring);
+  if (ctx->stream_b) (void)hipStreamDestroy(ctx->stream_b);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
                   ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.box_part, ctx->ib.done, ctx->d_sheets,
                   ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat};
@@ -181,6 +203,16 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
     if (hipMalloc(ptr, bytes) != hipSuccess) { *ptr = nullptr; rc = LBMIB_ERR_ALLOC; }
   };
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
+  if (!rc && hipStreamCreateWithFlags(&ctx->stream_b, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
+  for (hipEvent_t *e : {&ctx->evA[0], &ctx->evA[1], &ctx->evB[0], &ctx->evB[1], &ctx->evJoin})
+    if (!rc && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = LBMIB_ERR_HIP;
+  for (int k = 0; k < lbmib_ctx::kRing; ++k) {
+    ctx->ring_step[k] = -1;
+    if (!rc && hipEventCreateWithFlags(&ctx->ev_box[k], hipEventDisableTiming) != hipSuccess) rc = LBMIB_ERR_HIP;
+  }
+  if (!rc && hipHostMalloc((void **)&ctx->h_box_ring, sizeof(int) * lbmib_ctx::kRing * kMaxSheets * 6,
+                           hipHostMallocDefault) != hipSuccess)
+    rc = LBMIB_ERR_ALLOC;
   {
     int dev = 0;
     if (!rc && (hipGetDevice(&dev) != hipSuccess ||
@@ -250,6 +282,14 @@ extern "C" int lbmib_set_fiber_positions(lbmib_ctx *ctx, int sheet, const double
   std::vector<double> soa((size_t)3 * m);
   for (int k = 0; k < m; ++k)
     for (int a = 0; a < 3; ++a) soa[(size_t)a * m + k] = xyz[3 * (size_t)k + a];
+  double lo = 1e300, hi = -1e300;
+  for (int k = 0; k < m; ++k) {
+    lo = fmin(lo, xyz[3 * (size_t)k]);
+    hi = fmax(hi, xyz[3 * (size_t)k]);
+  }
+  ctx->hx[sheet][0] = lo;
+  ctx->hx[sheet][1] = hi;
+  ctx->slabs.n = 0;  // re-plan the split at the next step
   for (int a = 0; a < 3; ++a)
     HIPCHK(hipMemcpyAsync(ctx->ib.pos + (size_t)a * M + d.off, soa.data() + (size_t)a * m, sizeof(double) * m,
                           hipMemcpyHostToDevice, ctx->stream));
@@ -312,26 +352,26 @@ static int ensure_ib_clean(lbmib_ctx *ctx) {
 
 extern "C" int lbmib_compute_bendingforce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_BEND, slot_now(ctx), ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_BEND, slot_now(ctx), nullptr, ctx->stream);
 }
 extern "C" int lbmib_compute_stretchingforce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_STRETCH, slot_now(ctx), ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_STRETCH, slot_now(ctx), nullptr, ctx->stream);
 }
 extern "C" int lbmib_compute_elasticforce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_ELASTIC, slot_now(ctx), ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_ELASTIC, slot_now(ctx), nullptr, ctx->stream);
 }
 extern "C" int lbmib_compute_fiber_forces(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_BEND | FK_STRETCH | FK_ELASTIC, slot_now(ctx), ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_BEND | FK_STRETCH | FK_ELASTIC, slot_now(ctx), nullptr, ctx->stream);
 }
 
 extern "C" int lbmib_fiber_SpreadForce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
   RCHK(ensure_ib_clean(ctx));
   ctx->ib_clean = 0;
-  return launch_fiber(ctx->ib, ctx->g, FK_PREP, slot_now(ctx), ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_PREP, slot_now(ctx), nullptr, ctx->stream);
 }
 
 extern "C" int lbmib_fluid_get_SpreadForce(lbmib_ctx *ctx) {
@@ -412,7 +452,7 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
   const int prec = ctx->fp.precision;
   if (ctx->nnodes > 0) {
     RCHK(ensure_ib_clean(ctx));
-    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, ctx->stream));
+    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, nullptr, ctx->stream));
     RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream));
     ctx->spread_slot = s;
   }
@@ -434,9 +474,190 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
   return LBMIB_OK;
 }
 
+// ------------------------------------------------------------ split steps
+//
+// With sheets present, a step is split so that the IB chain hides behind
+// the bulk of the fluid update:
+//   stream   : bulk(t)  -- cells outside the sheets' x-slabs, no force
+//   stream_b : fiber(t), gather(t) | band(t) -- slab cells, with force |
+//              move(t)
+// bulk(t) reads the previous step's populations of the whole grid, so it
+// waits for band(t-1); band(t) waits for bulk(t-1).  k_move(t) reads the
+// new populations only on support cells and their pull neighbours, which
+// the slabs contain with a margin of kSlabMargin cells.  Every split step
+// copies the support boxes to pinned host memory; the host checks the newest
+// completed copy (at most kRing steps old) and moves the slabs once a box
+// comes within kSlabMargin/2 of an edge.  A sheet moves less than one cell
+// per step, and k_fiber latches LBMIB_ERR_BAND if a support box ever gets
+// within one cell of a slab edge.
+static constexpr int kSlabMargin = 10;
+
+static bool split_enabled(const lbmib_ctx *ctx) {
+  if (ctx->nnodes == 0) return false;
+  return ctx->overlap_req != 0;
+}
+
+// Slabs around the given per-sheet support x-ranges.
+static void plan_slabs(lbmib_ctx *ctx, const int (*xr)[2]) {
+  int lo[kMaxSheets], hi[kMaxSheets], n = 0;
+  for (int s = 0; s < ctx->nsheets; ++s) {
+    if (xr[s][0] > xr[s][1]) continue;
+    lo[n] = std::max(0, xr[s][0] - kSlabMargin);
+    hi[n] = std::min(ctx->g.nx - 1, xr[s][1] + kSlabMargin);
+    ++n;
+  }
+  // sort and merge overlapping / touching ranges
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (lo[j] < lo[i]) { std::swap(lo[i], lo[j]); std::swap(hi[i], hi[j]); }
+  int m = 0;
+  for (int i = 0; i < n; ++i) {
+    if (m > 0 && lo[i] <= hi[m - 1] + 1) hi[m - 1] = std::max(hi[m - 1], hi[i]);
+    else { lo[m] = lo[i]; hi[m] = hi[i]; ++m; }
+  }
+  while (m > kMaxSlabs) {  // join the closest neighbours
+    int best = 0;
+    for (int i = 1; i + 1 < m; ++i)
+      if (lo[i + 1] - hi[i] < lo[best + 1] - hi[best]) best = i;
+    hi[best] = hi[best + 1];
+    for (int i = best + 1; i + 1 < m; ++i) { lo[i] = lo[i + 1]; hi[i] = hi[i + 1]; }
+    --m;
+  }
+  ctx->slabs.n = m;
+  for (int i = 0; i < m; ++i) { ctx->slabs.lo[i] = lo[i]; ctx->slabs.hi[i] = hi[i]; }
+  ctx->slab_age = 0;
+}
+
+// Slab check on the support boxes read back asynchronously (one ring entry
+// per split step): the newest completed entry is compared with the slabs;
+// the host blocks only if all entries of the last kRing steps are pending.
+static int check_slabs(lbmib_ctx *ctx) {
+  int xr[kMaxSheets][2];
+  if (ctx->slabs.n == 0) {
+    // first plan: from the uploaded positions (support start floor(X-1))
+    for (int s = 0; s < ctx->nsheets; ++s) {
+      xr[s][0] = (int)floor(ctx->hx[s][0] - 1.0);
+      xr[s][1] = (int)floor(ctx->hx[s][1] - 1.0) + 3;
+    }
+    plan_slabs(ctx, xr);
+    for (int k = 0; k < lbmib_ctx::kRing; ++k) ctx->ring_step[k] = -1;
+    return LBMIB_OK;
+  }
+  ++ctx->slab_age;
+  int newest = -1, oldest = -1;
+  for (int k = 0; k < lbmib_ctx::kRing; ++k) {
+    if (ctx->ring_step[k] < 0) continue;
+    if (oldest < 0 || ctx->ring_step[k] < ctx->ring_step[oldest]) oldest = k;
+    if (hipEventQuery(ctx->ev_box[k]) == hipSuccess && (newest < 0 || ctx->ring_step[k] > ctx->ring_step[newest]))
+      newest = k;
+  }
+  if (newest < 0) {
+    if (oldest < 0 || ctx->time - ctx->ring_step[oldest] < lbmib_ctx::kRing) return LBMIB_OK;
+    HIPCHK(hipEventSynchronize(ctx->ev_box[oldest]));
+    newest = oldest;
+  }
+  const int *box = ctx->h_box_ring + newest * kMaxSheets * 6;
+  bool replan = false;
+  for (int s = 0; s < ctx->nsheets; ++s) {
+    xr[s][0] = box[6 * s];
+    xr[s][1] = box[6 * s + 1];
+    if (xr[s][0] > xr[s][1]) continue;
+    bool ok = false;
+    for (int k = 0; k < ctx->slabs.n; ++k)
+      ok |= xr[s][0] >= ctx->slabs.lo[k] + kSlabMargin / 2 && xr[s][1] <= ctx->slabs.hi[k] - kSlabMargin / 2;
+    replan |= !ok;
+  }
+  // entries older than the one checked carry no newer information
+  const int checked = ctx->ring_step[newest];
+  for (int k = 0; k < lbmib_ctx::kRing; ++k)
+    if (ctx->ring_step[k] >= 0 && ctx->ring_step[k] <= checked) ctx->ring_step[k] = -1;
+  if (replan) plan_slabs(ctx, xr);
+  return LBMIB_OK;
+}
+
+static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
+  RCHK(check_slabs(ctx));
+  const int t = ctx->time, s = t & 1, prec = ctx->fp.precision;
+  const int *box = ctx->ib.box;
+  if (ctx->split_run == 0) {
+    // sB starts after everything already queued on the main stream
+    HIPCHK(hipEventRecord(ctx->evJoin, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evJoin, 0));
+  }
+  const bool prev = ctx->split_run > 0;
+  // main stream: bulk(t)
+  if (prev) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evB[1 - s], 0));
+  if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
+  RCHK(launch_collide_bulk(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->slabs,
+                           ctx->kv.nt_mask, ctx->stream));
+  if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->evA[s], ctx->stream));
+  // second stream: IB chain and the band
+  if (!ctx->ib_clean) {
+    RCHK(launch_ib_reset(ctx->ib, 1, 1 << s, ctx->stream_b));
+    ctx->ib_clean = 1;
+  }
+  RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, &ctx->slabs, ctx->stream_b));
+  {
+    const int r = t % lbmib_ctx::kRing;
+    HIPCHK(hipMemcpyAsync(ctx->h_box_ring + r * kMaxSheets * 6, box + s * kMaxSheets * 6,
+                          sizeof(int) * 6 * ctx->nsheets, hipMemcpyDeviceToHost, ctx->stream_b));
+    HIPCHK(hipEventRecord(ctx->ev_box[r], ctx->stream_b));
+    ctx->ring_step[r] = t;
+  }
+  RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream_b));
+  ctx->spread_slot = s;
+  if (prev) HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evA[1 - s], 0));
+  RCHK(launch_collide_band(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
+                           box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
+                           ctx->slabs, ctx->kv.nt_mask, ctx->stream_b));
+  HIPCHK(hipEventRecord(ctx->evB[s], ctx->stream_b));
+  RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream_b));
+  ctx->ib_clean = 1;
+  ctx->cur = 1 - ctx->cur;
+  ctx->own = 0;
+  ctx->pending = 0;
+  ctx->band_done = 0;
+  ctx->time += 1;
+  ctx->split_run += 1;
+  return LBMIB_OK;
+}
+
+// Joins the second stream back into the main one after a run of split steps.
+static int split_join(lbmib_ctx *ctx) {
+  if (ctx->split_run == 0) return LBMIB_OK;
+  HIPCHK(hipEventRecord(ctx->evJoin, ctx->stream_b));
+  HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
+  ctx->split_run = 0;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_set_overlap(lbmib_ctx *ctx, int mode) {
+  if (!ctx || mode < -1 || mode > 1) return LBMIB_ERR_ARG;
+  RCHK(split_join(ctx));
+  ctx->overlap_req = mode;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_timed_kernel_cells(lbmib_ctx *ctx, double *cells) {
+  if (!ctx || !cells) return LBMIB_ERR_ARG;
+  double n = (double)ctx->g.n;
+  if (split_enabled(ctx)) {
+    if (ctx->slabs.n == 0) RCHK(check_slabs(ctx));
+    for (int k = 0; k < ctx->slabs.n; ++k)
+      n -= (double)(ctx->slabs.hi[k] - ctx->slabs.lo[k] + 1) * ctx->g.ny * ctx->g.nz;
+  }
+  *cells = n;
+  return LBMIB_OK;
+}
+
 extern "C" int lbmib_step(lbmib_ctx *ctx, int nsteps) {
   RCHK(ready(ctx));
   if (ctx->pending || nsteps < 0) return LBMIB_ERR_STATE;
+  if (split_enabled(ctx)) {
+    for (int k = 0; k < nsteps; ++k) RCHK(split_step(ctx, nullptr, nullptr));
+    return split_join(ctx);
+  }
   for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, nullptr, nullptr));
   return LBMIB_OK;
 }
@@ -445,6 +666,7 @@ extern "C" int lbmib_step(lbmib_ctx *ctx, int nsteps) {
 
 static int poll_status(lbmib_ctx *ctx) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream_b));
   int bits = 0;
   HIPCHK(hipMemcpy(&bits, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost));
   const int e = map_device_err(bits);
@@ -603,17 +825,22 @@ extern "C" int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *m
     HIPCHK(hipEventCreate(&e));
     ctx->evs.push_back(e);
   }
-  for (int k = 0; k < nwarm; ++k) RCHK(one_step(ctx, nullptr, nullptr));
+  const bool split = split_enabled(ctx);
+  auto step_once = [&](hipEvent_t a, hipEvent_t b) { return split ? split_step(ctx, a, b) : one_step(ctx, a, b); };
+  for (int k = 0; k < nwarm; ++k) RCHK(step_once(nullptr, nullptr));
+  RCHK(split_join(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipEventRecord(ctx->evs[0], ctx->stream));
-  for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, nullptr, nullptr));
+  for (int k = 0; k < nsteps; ++k) RCHK(step_once(nullptr, nullptr));
+  RCHK(split_join(ctx));
   HIPCHK(hipEventRecord(ctx->evs[1], ctx->stream));
   HIPCHK(hipEventSynchronize(ctx->evs[1]));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ctx->evs[0], ctx->evs[1]));
   if (ms_total) *ms_total = ms;
   if (ms_main_mean) {
-    for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, ctx->evs[2 + 2 * k], ctx->evs[3 + 2 * k]));
+    for (int k = 0; k < nsteps; ++k) RCHK(step_once(ctx->evs[2 + 2 * k], ctx->evs[3 + 2 * k]));
+    RCHK(split_join(ctx));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     double acc = 0;
     for (int k = 0; k < nsteps; ++k) {

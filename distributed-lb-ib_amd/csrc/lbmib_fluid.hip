@@ -212,6 +212,44 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// Bulk of a split step: every cell outside the slabs, no force (the slabs
+// contain all cells where the force of this or the previous step is
+// non-zero).  Row-aligned blocks as k_collide_stream.
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(1024)
+    k_collide_bulk(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
+                   int own, FluidConsts<typename Traits<P>::T> c, Slabs slabs) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  bool skip = false;
+  for (int k = 0; k < slabs.n; ++k) skip |= x >= slabs.lo[k] && x <= slabs.hi[k];
+  if (skip) return;
+  update_cell<P, NT, NTL>(g, fin, fout, own, c, 0, nullptr, nullptr, nullptr, nullptr, x, blockIdx.y, blockIdx.z);
+}
+
+// Band of a split step: the cells of the slabs, enumerated flat (slab by
+// slab, x fastest), with the force.
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(256)
+    k_collide_band(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
+                   int own, FluidConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box_cur,
+                   const int *__restrict__ box_prev, const typename Traits<P>::FT *__restrict__ Fcur,
+                   const typename Traits<P>::FT *__restrict__ Fprev, Slabs slabs) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long plane = (long)g.ny * g.nz;
+  for (int k = 0; k < slabs.n; ++k) {
+    const int w = slabs.hi[k] - slabs.lo[k] + 1;
+    const long cnt = (long)w * plane;
+    if (e < cnt) {
+      const int x = slabs.lo[k] + (int)(e % w);
+      const long r = e / w;
+      update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, (int)(r % g.ny),
+                              (int)(r / g.ny));
+      return;
+    }
+    e -= cnt;
+  }
+}
+
 // V consecutive cells of one x-row per thread: the 9 populations with
 // c_x = 0 are read with aligned V-wide loads, the 10 x-shifted ones with
 // one-element-offset V-wide loads (dword aligned), and all 19 results are
@@ -566,6 +604,53 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
   LBMIB_DISPATCH(prec, ({
     if (nt) launch_any<P, true>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     else launch_any<P, false>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+  }));
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+
+int launch_collide_bulk(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
+                        const FluidConsts<double> &c, const Slabs &slabs, int nt_mask, hipStream_t s) {
+  const int nt = nt_mask < 0 ? 1 : (nt_mask & 1);
+  const int ntl = nt_mask < 0 ? 0 : ((nt_mask >> 1) & 1);
+  const dim3 b = row_block(g), gr = row_grid(g, b);
+  LBMIB_DISPATCH(prec, ({
+    using TT = typename Traits<P>::T;
+    using S = typename Traits<P>::S;
+    const FluidConsts<TT> cc = cast_consts<TT>(c);
+    if (nt && ntl) k_collide_bulk<P, true, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
+    else if (nt) k_collide_bulk<P, true, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
+    else if (ntl) k_collide_bulk<P, false, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
+    else k_collide_bulk<P, false, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
+  }));
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+
+int launch_collide_band(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
+                        const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                        const void *Fcur, const void *Fprev, const Slabs &slabs, int nt_mask, hipStream_t s) {
+  const int nt = nt_mask < 0 ? 1 : (nt_mask & 1);
+  const int ntl = nt_mask < 0 ? 0 : ((nt_mask >> 1) & 1);
+  long cells = 0;
+  for (int k = 0; k < slabs.n; ++k) cells += (long)(slabs.hi[k] - slabs.lo[k] + 1) * g.ny * g.nz;
+  if (cells == 0) return LBMIB_OK;
+  const int blocks = (int)((cells + 255) / 256);
+  LBMIB_DISPATCH(prec, ({
+    using TT = typename Traits<P>::T;
+    using FT = typename Traits<P>::FT;
+    using S = typename Traits<P>::S;
+    const FluidConsts<TT> cc = cast_consts<TT>(c);
+    if (nt && ntl)
+      k_collide_band<P, true, true><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
+                                                           box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
+    else if (nt)
+      k_collide_band<P, true, false><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
+                                                            box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
+    else if (ntl)
+      k_collide_band<P, false, true><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
+                                                            box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
+    else
+      k_collide_band<P, false, false><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
+                                                             box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
   }));
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }

@@ -89,7 +89,7 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-__global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask, int slot) {
+__global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask, int slot, Slabs guard) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = n < ib.nnodes;
   const int M = ib.nnodes;
@@ -199,6 +199,18 @@ __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask,
       acc = (k & 1) ? max(acc, v) : min(acc, v);
     }
     ib.box[slot * kMaxSheets * 6 + k] = acc;
+    sbox[k] = acc;
+  }
+  if (guard.n > 0) {
+    // a split step needs every support cell and its pull neighbours inside
+    // one slab; latch an error if the sheet got closer than that
+    __syncthreads();
+    if (threadIdx.x < ib.nsheets) {
+      const int x0 = sbox[threadIdx.x * 6], x1 = sbox[threadIdx.x * 6 + 1];
+      bool inside = x0 > x1;  // empty box
+      for (int k = 0; k < guard.n; ++k) inside |= x0 >= guard.lo[k] + 1 && x1 <= guard.hi[k] - 1;
+      if (!inside) atomicOr(ib.err, DERR_BAND);
+    }
   }
   if (threadIdx.x == 0) atomicExch(ib.done, 0u);
 }
@@ -402,9 +414,11 @@ __global__ void k_ib_reset(IBState ib, int reset_bins, int slot_mask) {
 
 static inline int grid_for(long n, int b) { return (int)((n + b - 1) / b); }
 
-int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s) {
+int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, const Slabs *guard, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
-  k_fiber<<<grid_for(ib.nnodes, 256), 256, 0, s>>>(ib, g, mask, slot);
+  Slabs none;
+  none.n = 0;
+  k_fiber<<<grid_for(ib.nnodes, 256), 256, 0, s>>>(ib, g, mask, slot, guard ? *guard : none);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 

@@ -32,7 +32,15 @@ enum { NF_BX = 0, NF_BY, NF_BZ, NF_SX, NF_SY, NF_SZ, NF_EX, NF_EY, NF_EZ, NF_COU
 enum { FK_BEND = 1, FK_STRETCH = 2, FK_ELASTIC = 4, FK_PREP = 8, FK_ALL = 15 };
 
 // Device error bits (combined with atomicOr, mapped to LBMIB_ERR_* on the host).
-enum { DERR_FIBER_OOB = 1, DERR_BIN_OVERFLOW = 2 };
+enum { DERR_FIBER_OOB = 1, DERR_BIN_OVERFLOW = 2, DERR_BAND = 4 };
+
+// x-slabs holding the sheets' support (with margin) when a step is split
+// into a force-free bulk update and a band update on a second stream.
+constexpr int kMaxSlabs = 4;
+struct Slabs {
+  int n;
+  int lo[kMaxSlabs], hi[kMaxSlabs];  // inclusive x ranges, disjoint, ascending
+};
 
 struct SheetDesc {
   int rows, cols, off;   // node range [off, off + rows*cols)
@@ -103,7 +111,14 @@ int launch_totals(int prec, const Geometry &g, const void *f, int own_mode, cons
                   double *partials, int nblocks_max, double *out4, hipStream_t s);
 int totals_blocks(const Geometry &g);
 
-int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s);
+int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, const Slabs *guard, hipStream_t s);
+// Split step: the bulk update (cells outside the slabs, no force) and the
+// band update (cells inside the slabs, with force).
+int launch_collide_bulk(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
+                        const FluidConsts<double> &c, const Slabs &slabs, int nt_mask, hipStream_t s);
+int launch_collide_band(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
+                        const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                        const void *Fcur, const void *Fprev, const Slabs &slabs, int nt_mask, hipStream_t s);
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s);
 int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
                 const FluidConsts<double> &c, int reset_slot, hipStream_t s);

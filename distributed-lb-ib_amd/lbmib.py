@@ -15,9 +15,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbmib.so")
 
-OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_FIBER_OOB, ERR_BIN_OVERFLOW, ERR_NONFINITE, ERR_STATE = range(8)
+OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_FIBER_OOB, ERR_BIN_OVERFLOW, ERR_NONFINITE, ERR_STATE, ERR_BAND = range(9)
 ERR_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_ALLOC", 4: "ERR_FIBER_OOB",
-             5: "ERR_BIN_OVERFLOW", 6: "ERR_NONFINITE", 7: "ERR_STATE"}
+             5: "ERR_BIN_OVERFLOW", 6: "ERR_NONFINITE", 7: "ERR_STATE", 8: "ERR_BAND"}
 BC_INOUT, BC_WALL, BC_PERIODIC = 0, 1, 2
 PRECISIONS = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3}
 STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}
@@ -36,7 +36,7 @@ ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lb
                "lbmib_get_time", "lbmib_get_fibers", "lbmib_get_macroscopic", "lbmib_get_df",
                "lbmib_get_force", "lbmib_get_totals", "lbmib_get_band_box", "lbmib_time_steps",
                "lbmib_bytes_per_cell", "lbmib_probe_copy_bandwidth",
-               "lbmib_set_kernel_variant") + tuple("lbmib_" + n for n in CTX_CALLS)
+               "lbmib_set_kernel_variant", "lbmib_set_overlap", "lbmib_timed_kernel_cells") + tuple("lbmib_" + n for n in CTX_CALLS)
 
 
 class LbmibError(RuntimeError):
@@ -87,6 +87,8 @@ def load_library(path=LIB_PATH):
     L.lbmib_bytes_per_cell.argtypes = [vp, dp]
     L.lbmib_probe_copy_bandwidth.argtypes = [d, i, dp]
     L.lbmib_set_kernel_variant.argtypes = [vp, i, i, i]
+    L.lbmib_set_overlap.argtypes = [vp, i]
+    L.lbmib_timed_kernel_cells.argtypes = [vp, dp]
     for n in CTX_CALLS:
         getattr(L, "lbmib_" + n).argtypes = [vp]
     for n in ALL_SYMBOLS:
@@ -287,6 +289,15 @@ class Engine:
         variants give identical results."""
         self._check(self.L.lbmib_set_kernel_variant(self.h, cells_per_thread, nontemporal, blocks_per_cu),
                     "set_kernel_variant")
+
+    def set_overlap(self, mode):
+        """-1 automatic, 0 never split steps, 1 always (see lbmib_set_overlap)."""
+        self._check(self.L.lbmib_set_overlap(self.h, mode), "set_overlap")
+
+    def timed_kernel_cells(self):
+        out = ctypes.c_double()
+        self._check(self.L.lbmib_timed_kernel_cells(self.h, ctypes.byref(out)), "timed_kernel_cells")
+        return out.value
 
     def bytes_per_cell(self):
         b = ctypes.c_double()

@@ -34,6 +34,7 @@ extern "C" {
 #define LBMIB_ERR_BIN_OVERFLOW 5 /* more fiber nodes in one spread bin than its capacity */
 #define LBMIB_ERR_NONFINITE 6    /* NaN/Inf in the mass/momentum watchdog */
 #define LBMIB_ERR_STATE 7        /* call out of order (e.g. step before init) */
+#define LBMIB_ERR_BAND 8         /* a sheet outran the x-slab of a split step (see lbmib_set_overlap) */
 
 /* ---- boundary kinds per axis ---- */
 #define LBMIB_BC_INOUT 0    /* x only: fixed-equilibrium planes x=1,2 and N-3,N-2 (lbm_ib.c:2769-2907) */
@@ -113,6 +114,16 @@ int lbmib_replace_old_DF(lbmib_ctx *ctx);             /* lbm_ib.c:2909-2968: buf
 
 /* ---- whole steps: the do_thread loop body, lbm_ib.c:3115-3563 ---- */
 int lbmib_step(lbmib_ctx *ctx, int nsteps);
+/* Split steps for lbmib_step / lbmib_time_steps (not in the reference): with
+ * sheets present, each step updates the cells outside x-slabs around the
+ * sheets on the main stream while a second stream runs the fiber forces,
+ * spreading, the slab cells and the move, so the IB work overlaps the bulk
+ * of the fluid update.  Results are bit-identical to unsplit steps.
+ * mode: -1 automatic (split when there are sheets), 0 never, 1 always. */
+int lbmib_set_overlap(lbmib_ctx *ctx, int mode);
+/* Cells updated by the kernel lbmib_time_steps times (all cells, or the
+ * cells outside the slabs when steps are split). */
+int lbmib_timed_kernel_cells(lbmib_ctx *ctx, double *cells);
 
 /* ---- queries (synchronising) ---- */
 int lbmib_sync(lbmib_ctx *ctx);
