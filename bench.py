@@ -188,13 +188,16 @@ def main():
         status = e.status()
         mass = e.totals()[0]
         bpc = e.bytes_per_cell()
+        kernel_cells = e.timed_kernel_cells()
     if status != 0:
         print(json.dumps({"error": f"device status {lbmib.ERR_NAMES.get(status, status)}"}))
         sys.exit(1)
     ms_step = ms_total / a.steps
     mlups = n / (ms_step * 1e-3) / 1e6
     alg_bytes = bpc * n
-    achieved = alg_bytes / (ms_main * 1e-3) / 1e9
+    # the timed kernel updates kernel_cells cells per launch (all cells,
+    # unless steps are split into a bulk and a sheet band)
+    achieved = bpc * kernel_cells / (ms_main * 1e-3) / 1e9
     traffic = traffic_from_profiles(a.workload, a.precision)
     copy_gbs = lbmib.probe_copy_bandwidth(4e9, 5)
     print(f"[bench] {ms_step:.4f} ms/step, {mlups:.1f} MLUPS, fused kernel {ms_main:.4f} ms "

@@ -119,7 +119,7 @@ int lbmib_step(lbmib_ctx *ctx, int nsteps);
  * sheets on the main stream while a second stream runs the fiber forces,
  * spreading, the slab cells and the move, so the IB work overlaps the bulk
  * of the fluid update.  Results are bit-identical to unsplit steps.
- * mode: -1 automatic (split when there are sheets), 0 never, 1 always. */
+ * mode: -1 automatic (the default; currently never splits), 0 never, 1 always. */
 int lbmib_set_overlap(lbmib_ctx *ctx, int mode);
 /* Cells updated by the kernel lbmib_time_steps times (all cells, or the
  * cells outside the slabs when steps are split). */
