@@ -45,6 +45,8 @@ struct lbmib_ctx {
   int nparts = 0;
   double *d_tot = nullptr;
   double *d_mat = nullptr;   // materialisation scratch (lazy)
+  void *d_tab = nullptr;     // frozen-plane populations in kernel units (19 values)
+  uint32_t *d_rowtab = nullptr;  // row source table of the buffer-addressed fused kernel
   size_t mat_bytes = 0;
   hipStream_t stream = nullptr;
   int time = 1;
@@ -123,7 +125,7 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (ctx->stream_b) (void)hipStreamDestroy(ctx->stream_b);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
                   ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.box_part, ctx->ib.done, ctx->d_sheets,
-                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat};
+                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -227,6 +229,30 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   alloc((void **)&ctx->d_partials, sizeof(double) * 4 * (size_t)ctx->nparts);
   alloc((void **)&ctx->d_tot, sizeof(double) * 4);
   alloc((void **)&ctx->ib.box, sizeof(int) * 2 * kMaxSheets * 6);
+  alloc(&ctx->d_tab, sizeof(double) * 19);
+  if (!rc) {
+    // kernel units: f (F64), f - w (F32), 2^8 (f - w) (16-bit storage)
+    double td[19];
+    float tf[19];
+    const double sc = (p.precision == LBMIB_F16 || p.precision == LBMIB_BF16) ? (double)kDevScale16 : 1.0;
+    for (int q = 0; q < 19; ++q) {
+      td[q] = c.tab_in[q];
+      tf[q] = (float)(sc * c.tab_in[q]);
+    }
+    if (hipMemcpy(ctx->d_tab, dev ? (const void *)tf : (const void *)td, dev ? sizeof(tf) : sizeof(td),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      rc = LBMIB_ERR_HIP;
+    c.dtab = ctx->d_tab;
+  }
+  if (!rc && row_path_buffered(g, ctx->esz)) {
+    std::vector<uint32_t> tab;
+    if (row_source_table(g, ctx->esz, tab) == LBMIB_OK) {
+      alloc((void **)&ctx->d_rowtab, sizeof(uint32_t) * tab.size());
+      if (!rc && hipMemcpy(ctx->d_rowtab, tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice) !=
+                     hipSuccess)
+        rc = LBMIB_ERR_HIP;
+    }
+  }
 
   IBState &ib = ctx->ib;
   ib.nsheets = nsheets;
@@ -397,7 +423,7 @@ extern "C" int lbmib_compute_eqlbrmdistrfuncDF1(lbmib_ctx *ctx) {
   const int *box = ctx->ib.box;
   RCHK(launch_collide_stream(ctx->fp.precision, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c,
                              ctx->nsheets, box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s],
-                             ctx->F[1 - s], ctx->kv, ctx->stream));
+                             ctx->F[1 - s], ctx->d_rowtab, ctx->kv, ctx->stream));
   ctx->pending = 1;
   ctx->band_done = 0;
   return LBMIB_OK;
@@ -460,7 +486,7 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
   if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
   RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
                              box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
-                             ctx->kv, ctx->stream));
+                             ctx->d_rowtab, ctx->kv, ctx->stream));
   if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
   if (ctx->nnodes > 0) {
     RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream));

@@ -14,12 +14,15 @@ typedef _Float16 half_t;
 
 template <int P>
 struct Traits;
+// S: stored type, T: arithmetic type, FT: force field type; dev: stored as
+// deviations f - w; scale: stored value = scale * (f - w).
 template <>
 struct Traits<LBMIB_F64> {
   using S = double;
   using T = double;
   using FT = double;
   static constexpr bool dev = false;
+  static constexpr float scale = 1.0f, inv_scale = 1.0f;
 };
 template <>
 struct Traits<LBMIB_F32> {
@@ -27,6 +30,7 @@ struct Traits<LBMIB_F32> {
   using T = float;
   using FT = float;
   static constexpr bool dev = true;
+  static constexpr float scale = 1.0f, inv_scale = 1.0f;
 };
 template <>
 struct Traits<LBMIB_F16> {
@@ -34,6 +38,7 @@ struct Traits<LBMIB_F16> {
   using T = float;
   using FT = float;
   static constexpr bool dev = true;
+  static constexpr float scale = (float)kDevScale16, inv_scale = 1.0f / kDevScale16;
 };
 template <>
 struct Traits<LBMIB_BF16> {
@@ -41,32 +46,78 @@ struct Traits<LBMIB_BF16> {
   using T = float;
   using FT = float;
   static constexpr bool dev = true;
+  static constexpr float scale = (float)kDevScale16, inv_scale = 1.0f / kDevScale16;
 };
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (values are finite)
-  return (uint16_t)(u >> 16);
+// round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// Stored value <-> kernel units (h = scale (f - w), or f for F64).
+template <int P>
+__device__ __forceinline__ typename Traits<P>::T from_bits(typename Traits<P>::S s) {
+  if constexpr (P == LBMIB_F16) return (float)s;
+  else if constexpr (P == LBMIB_BF16) return bf16_to_f32(s);
+  else return s;
+}
+template <int P>
+__device__ __forceinline__ typename Traits<P>::S to_bits(typename Traits<P>::T h) {
+  if constexpr (P == LBMIB_F16) return (half_t)h;
+  else if constexpr (P == LBMIB_BF16) return f32_to_bf16(h);
+  else return h;
+}
+template <int P, bool NTL = false>
+__device__ __forceinline__ typename Traits<P>::T ld_h(const typename Traits<P>::S *p) {
+  if constexpr (NTL) return from_bits<P>(__builtin_nontemporal_load(p));
+  else return from_bits<P>(*p);
+}
+template <int P, bool NT = false>
+__device__ __forceinline__ void st_h(typename Traits<P>::S *p, typename Traits<P>::T h) {
+  if constexpr (NT) __builtin_nontemporal_store(to_bits<P>(h), p);
+  else *p = to_bits<P>(h);
 }
 
+// Buffer-addressed forms: a 128-bit resource over the whole population array,
+// a per-lane byte offset (VGPR) and a wave-uniform byte offset (SGPR), so one
+// VGPR offset serves all 19 planes.  Offsets must stay below 2^32.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
+}
+template <int P, bool NTL>
+__device__ __forceinline__ typename Traits<P>::T bld_h(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  constexpr int aux = NTL ? 2 : 0;  // nt
+  if constexpr (P == LBMIB_F64) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, aux));
+  } else if constexpr (P == LBMIB_F32) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, aux));
+  } else {
+    const uint16_t b = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, aux);
+    return from_bits<P>(__builtin_bit_cast(typename Traits<P>::S, b));
+  }
+}
+template <int P, bool NT>
+__device__ __forceinline__ void bst_h(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                      typename Traits<P>::T h) {
+  constexpr int aux = NT ? 2 : 0;
+  const typename Traits<P>::S s = to_bits<P>(h);
+  if constexpr (P == LBMIB_F64) {
+    typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, s), r, voff, soff, aux);
+  } else if constexpr (P == LBMIB_F32) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, s), r, voff, soff, aux);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, s), r, voff, soff, aux);
+  }
+}
+
+// Deviation g = f - w (or f for F64) of a stored population.
 template <int P, bool NTL = false>
 __device__ __forceinline__ typename Traits<P>::T ld(const typename Traits<P>::S *p) {
-  typename Traits<P>::S s;
-  if constexpr (NTL) s = __builtin_nontemporal_load(p);
-  else s = *p;
-  if constexpr (P == LBMIB_F16) return (float)s * (1.0f / kDevScale16);
-  else if constexpr (P == LBMIB_BF16) return bf16_to_f32(s) * (1.0f / kDevScale16);
-  else return s;
+  return ld_h<P, NTL>(p) * Traits<P>::inv_scale;
 }
 template <int P, bool NT = false>
 __device__ __forceinline__ void st(typename Traits<P>::S *p, typename Traits<P>::T v) {
-  typename Traits<P>::S s;
-  if constexpr (P == LBMIB_F16) s = (half_t)(v * (float)kDevScale16);
-  else if constexpr (P == LBMIB_BF16) s = f32_to_bf16(v * (float)kDevScale16);
-  else s = v;
-  if constexpr (NT) __builtin_nontemporal_store(s, p);
-  else *p = s;
+  st_h<P, NT>(p, v * Traits<P>::scale);
 }
 
 __device__ __forceinline__ bool in_box6(const int *b, int x, int y, int z) {
@@ -93,7 +144,8 @@ __device__ __forceinline__ bool in_any_box(const int *boxes, int nsheets, int x,
 // Wall axes: a source outside the active box is replaced by the opposite
 // population of the same node (half-way bounce-back, lbm_ib.c:2310-2423).
 // Periodic axes wrap.  Interior cells take the branch-free path.
-template <int P, bool NTL = false>
+// RAW: values in kernel units (ld_h) instead of deviations (ld).
+template <int P, bool NTL = false, bool RAW = false>
 __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>::S *__restrict__ f, int x, int y,
                                      int z, long c, typename Traits<P>::T v[19]) {
   const bool interior = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1] && z > g.lo[2] && z < g.hi[2];
@@ -102,7 +154,7 @@ __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       const long off = kCX[q] * sx + kCY[q] * sy + kCZ[q] * sz;
-      v[q] = ld<P, NTL>(f + q * g.pitch + c - off);
+      v[q] = RAW ? ld_h<P, NTL>(f + q * g.pitch + c - off) : ld<P, NTL>(f + q * g.pitch + c - off);
     }
     return;
   }
@@ -118,7 +170,7 @@ __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>
     else bb |= (pz < g.lo[2] || pz > g.hi[2]);
     const long src = bb ? c + (long)kOPP[q] * g.pitch - (long)q * g.pitch
                         : (long)px + sy * py + sz * pz;
-    v[q] = ld<P, NTL>(f + q * g.pitch + src);
+    v[q] = RAW ? ld_h<P, NTL>(f + q * g.pitch + src) : ld<P, NTL>(f + q * g.pitch + src);
   }
 }
 

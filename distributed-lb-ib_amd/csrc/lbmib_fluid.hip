@@ -24,21 +24,25 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 
+#include <vector>
+
 #include "lbmib_device.h"
 
 namespace lbmib {
 
-// Host double constants -> kernel compute type.
-template <class T>
-static FluidConsts<T> cast_consts(const FluidConsts<double> &d) {
-  FluidConsts<T> c;
+// Host FP64 constants -> kernel constants of precision P (kernel units).
+template <int P>
+static KConsts<typename Traits<P>::T> kconsts(const FluidConsts<double> &d) {
+  using T = typename Traits<P>::T;
+  KConsts<T> c;
+  const double s = Traits<P>::scale;
   c.omc = (T)d.omc;
   c.om = (T)d.om;
-  for (int q = 0; q < 19; ++q) {
-    c.pref[q] = (T)d.pref[q];
-    c.pw[q] = (T)d.pw[q];
-    c.tab_in[q] = (T)d.tab_in[q];
-    c.tab_out[q] = (T)d.tab_out[q];
+  for (int k = 0; k < 3; ++k) {
+    const double w = 1.0 / kWDEN_CLS[k];
+    const int q = k == 0 ? 0 : (k == 1 ? 1 : 7);  // a direction of class k
+    c.wa[k] = Traits<P>::dev ? (T)(s * d.om * w) : (T)0;
+    c.wb[k] = Traits<P>::dev ? (T)(s * d.pw[q]) : (T)d.pref[q];
   }
   c.cs2 = (T)d.cs2;
   c.cs4 = (T)d.cs4;
@@ -47,13 +51,15 @@ static FluidConsts<T> cast_consts(const FluidConsts<double> &d) {
   c.u0x = (T)d.u0x;
   c.u0y = (T)d.u0y;
   c.u0z = (T)d.u0z;
+  c.tab = (const T *)d.dtab;
   return c;
 }
 
 // Equilibrium + BGK + Guo forcing in the reference's operation order
-// (lbm_ib.c:1731-1818).  Used by the F64 form.
+// (lbm_ib.c:1731-1818).  Used by the F64 form.  pref_q depends on the
+// weight class only.
 __device__ __forceinline__ double collide_exact(int q, double fq, double rho, double vx, double vy, double vz,
-                                                double fx, double fy, double fz, const FluidConsts<double> &c) {
+                                                double fx, double fy, double fz, const KConsts<double> &c) {
   const double cx = kCX[q], cy = kCY[q], cz = kCZ[q];
   const double usq = vx * vx + vy * vy + vz * vz;
   const double cu = cx * vx + cy * vy + cz * vz;
@@ -64,18 +70,28 @@ __device__ __forceinline__ double collide_exact(int q, double fq, double rho, do
   const double fzg = fz + rho * c.g;
   const double a = (cx - vx) * fx + (cy - vy) * fy + (cz - vz) * fzg;
   const double cf = cx * fx + cy * fy + cz * fzg;
-  return fq * c.omc + c.om * eq + c.pref[q] * (a / c.cs2 + cu / c.cs4 * cf);
+  return fq * c.omc + c.om * eq + c.wb[kCLS[q]] * (a / c.cs2 + cu / c.cs4 * cf);
+}
+
+// c_q . (a, b, c) with only the non-zero lattice components (q is a
+// compile-time constant after unrolling).
+__device__ __forceinline__ float cdot(int q, float a, float b, float c) {
+  float r = 0.0f;
+  bool first = true;
+  if (kCX[q]) { r = kCX[q] > 0 ? a : -a; first = false; }
+  if (kCY[q]) { const float t = kCY[q] > 0 ? b : -b; r = first ? t : r + t; first = false; }
+  if (kCZ[q]) { const float t = kCZ[q] > 0 ? c : -c; r = first ? t : r + t; }
+  return r;
 }
 
 // Lagged rho/u of compute_rho_and_u (lbm_ib.c:2498-2525) from the pulled
-// populations and the previous step's force.
+// populations (kernel units) and the previous step's force.  d = rho - 1.
 template <int P>
 __device__ __forceinline__ void lagged_moments(const typename Traits<P>::T v[19], typename Traits<P>::T fpx,
                                                typename Traits<P>::T fpy, typename Traits<P>::T fpz,
-                                               const FluidConsts<typename Traits<P>::T> &c,
+                                               const KConsts<typename Traits<P>::T> &c, typename Traits<P>::T &d,
                                                typename Traits<P>::T &rho, typename Traits<P>::T &ux,
                                                typename Traits<P>::T &uy, typename Traits<P>::T &uz) {
-  using T = typename Traits<P>::T;
   if constexpr (!Traits<P>::dev) {
     // rho = Sum df2, u = (Sum c df2 + dt/2 F)/rho in the reference's order
     double s1 = v[0], s2 = 0.0, s3 = 0.0, s4 = 0.0;
@@ -87,65 +103,114 @@ __device__ __forceinline__ void lagged_moments(const typename Traits<P>::T v[19]
       if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + v[q] : s4 - v[q];
     }
     rho = s1;
+    d = s1 - 1.0;
     ux = (s2 + 0.5 * fpx) / s1;
     uy = (s3 + 0.5 * fpy) / s1;
     uz = (s4 + 0.5 * (fpz + rho * c.g)) / s1;
   } else {
-    T d = 0, mx = 0, my = 0, mz = 0;
+    // sums of h = s (f - w); rho = 1 + Sum(h)/s
+    float D = v[0], MX = v[1], MY = v[6], MZ = v[3];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      d += v[q];
-      if (kCX[q]) mx = kCX[q] > 0 ? mx + v[q] : mx - v[q];
-      if (kCY[q]) my = kCY[q] > 0 ? my + v[q] : my - v[q];
-      if (kCZ[q]) mz = kCZ[q] > 0 ? mz + v[q] : mz - v[q];
+    for (int q = 1; q < 19; ++q) {
+      D += v[q];
+      if (kCX[q] && q != 1) MX = kCX[q] > 0 ? MX + v[q] : MX - v[q];
+      if (kCY[q] && q != 6) MY = kCY[q] > 0 ? MY + v[q] : MY - v[q];
+      if (kCZ[q] && q != 3) MZ = kCZ[q] > 0 ? MZ + v[q] : MZ - v[q];
     }
-    rho = (T)1 + d;
-    const T ir = (T)1 / rho;
-    ux = (mx + (T)0.5 * fpx) * ir;
-    uy = (my + (T)0.5 * fpy) * ir;
-    uz = (mz + (T)0.5 * (fpz + rho * c.g)) * ir;
+    constexpr float is = Traits<P>::inv_scale;
+    d = D * is;
+    rho = 1.0f + d;
+    const float ir = __builtin_amdgcn_rcpf(rho);
+    ux = __builtin_fmaf(MX, is, 0.5f * fpx) * ir;
+    uy = __builtin_fmaf(MY, is, 0.5f * fpy) * ir;
+    uz = __builtin_fmaf(MZ, is, 0.5f * __builtin_fmaf(rho, c.g, fpz)) * ir;
   }
 }
 
-// BGK + Guo collision of one cell, in place.
+// BGK + Guo collision of one cell, in place, in kernel units.
 template <int P>
-__device__ __forceinline__ void collide_cell(typename Traits<P>::T v[19], typename Traits<P>::T rho,
-                                             typename Traits<P>::T ux, typename Traits<P>::T uy,
-                                             typename Traits<P>::T uz, typename Traits<P>::T fx,
-                                             typename Traits<P>::T fy, typename Traits<P>::T fz,
-                                             const FluidConsts<typename Traits<P>::T> &c) {
-  using T = typename Traits<P>::T;
+__device__ __forceinline__ void collide_cell(typename Traits<P>::T v[19], typename Traits<P>::T d,
+                                             typename Traits<P>::T rho, typename Traits<P>::T ux,
+                                             typename Traits<P>::T uy, typename Traits<P>::T uz,
+                                             typename Traits<P>::T fx, typename Traits<P>::T fy,
+                                             typename Traits<P>::T fz, const KConsts<typename Traits<P>::T> &c) {
   if constexpr (!Traits<P>::dev) {
 #pragma unroll
     for (int q = 0; q < 19; ++q) v[q] = collide_exact(q, v[q], rho, ux, uy, uz, fx, fy, fz, c);
   } else {
-    // Deviation form: with g_q = f_q - w_q and rho = 1 + Sum g,
-    //   feq_q - w_q = w_q (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2))
-    //   g*_q = (1 - 1/tau) g_q + (1/tau)(feq_q - w_q) + S_q,
-    // S_q = w_q (1 - 1/(2 tau)) (3 (c_q - u).F + 9 (c_q.u)(c_q.F)).
-    const T drho = rho - (T)1;
-    const T usq15 = (T)1.5 * (ux * ux + uy * uy + uz * uz);
-    const T fzg = fz + rho * c.g;
-    const T uF = ux * fx + uy * fy + uz * fzg;
+    // Deviation form on h_q = s (f_q - w_q), d = rho - 1:
+    //   h*_q = (1 - 1/tau) h_q + s/tau (feq_q - w_q) + s S_q
+    //   feq_q - w_q = w_q (d + rho (3 cu + 4.5 cu^2 - 1.5 u^2))
+    //   S_q = w_q (1 - 1/(2 tau)) (3 (c_q - u).F + 9 (c_q.u)(c_q.F))
+    // Opposite directions share the even part E and flip the odd part O:
+    //   h*_q = omc h_q + E + O,  h*_opp = omc h_opp + E - O
+    //   E = wa (d + rho (4.5 cu^2 - 1.5 u^2)) + wb (9 cu cF - 3 u.F)
+    //   O = 3 wa rho cu + 3 wb cF
+    const float fzg = __builtin_fmaf(rho, c.g, fz);
+    const float usq15 = 1.5f * __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz));
+    const float uF3 = 3.0f * __builtin_fmaf(ux, fx, __builtin_fmaf(uy, fy, uz * fzg));
+    const float dr = __builtin_fmaf(-rho, usq15, d);  // d - 1.5 rho u^2
+    const float r45 = 4.5f * rho;
+    v[0] = __builtin_fmaf(c.omc, v[0], __builtin_fmaf(c.wa[0], dr, -c.wb[0] * uF3));
+    const float ar3[3] = {0.0f, 3.0f * c.wa[1] * rho, 3.0f * c.wa[2] * rho};
+    const float wb3[3] = {0.0f, 3.0f * c.wb[1], 3.0f * c.wb[2]};
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const T cu = (T)kCX[q] * ux + (T)kCY[q] * uy + (T)kCZ[q] * uz;
-      const T cF = (T)kCX[q] * fx + (T)kCY[q] * fy + (T)kCZ[q] * fzg;
-      const T w = (T)1 / (T)kWDEN[q];
-      const T geq = w * (drho + rho * ((T)3 * cu + (T)4.5 * cu * cu - usq15));
-      const T S = c.pw[q] * ((T)3 * (cF - uF) + (T)9 * cu * cF);
-      v[q] = c.omc * v[q] + c.om * geq + S;
+    for (int q = 1; q < 19; q += 2) {
+      const int k = kCLS[q];
+      const float cu = cdot(q, ux, uy, uz);
+      const float cF = cdot(q, fx, fy, fzg);
+      const float base = __builtin_fmaf(r45, cu * cu, dr);
+      const float s9 = __builtin_fmaf(9.0f * cu, cF, -uF3);
+      const float E = __builtin_fmaf(c.wa[k], base, c.wb[k] * s9);
+      const float O = __builtin_fmaf(ar3[k], cu, wb3[k] * cF);
+      v[q] = __builtin_fmaf(c.omc, v[q], E + O);
+      v[q + 1] = __builtin_fmaf(c.omc, v[q + 1], E - O);
     }
   }
 }
 
-// The whole update of one active cell: arriving populations (inlet/outlet
-// table on the frozen planes, own populations on the first step, pull
-// otherwise), lagged moments, collision with the current force, store.
+// Arriving populations (kernel units) and lagged moments of one active cell
+// by the general rules: inlet/outlet table on the frozen planes, own
+// populations on the first step, pull otherwise.
+template <int P, bool NTL>
+__device__ __forceinline__ void arrive_general(const Geometry &g, const typename Traits<P>::S *__restrict__ fin,
+                                               int own, const KConsts<typename Traits<P>::T> &c, int nsheets,
+                                               const int *__restrict__ box_prev,
+                                               const typename Traits<P>::FT *__restrict__ Fprev, int x, int y,
+                                               int z, long c0, typename Traits<P>::T v[19],
+                                               typename Traits<P>::T &d, typename Traits<P>::T &rho,
+                                               typename Traits<P>::T &ux, typename Traits<P>::T &uy,
+                                               typename Traits<P>::T &uz) {
+  using T = typename Traits<P>::T;
+  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+  if (frozen || own) {
+    // planes x = 2 and N-3 hold the inlet/outlet tables after
+    // copy_inout_to_df2, and their rho/vel are never updated (the moment box
+    // starts at x = 3, lbm_ib.c:2482-2483); on the first step every cell
+    // keeps init_gv's rho/u (lbm_ib.c:837-849).
+    if (frozen) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) v[q] = ld_h<P>(fin + q * g.pitch + c0);
+    }
+    rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
+    return;
+  }
+  pull<P, NTL, true>(g, fin, x, y, z, c0, v);
+  T fpx = 0, fpy = 0, fpz = 0;
+  if (in_any_box(box_prev, nsheets, x, y, z)) {
+    fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
+  }
+  lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
+}
+
+// The whole update of one active cell by the general rules.
 template <int P, bool NT, bool NTL = false>
 __device__ __forceinline__ void update_cell(const Geometry &g, const typename Traits<P>::S *__restrict__ fin,
                                             typename Traits<P>::S *__restrict__ fout, int own,
-                                            const FluidConsts<typename Traits<P>::T> &c, int nsheets,
+                                            const KConsts<typename Traits<P>::T> &c, int nsheets,
                                             const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                                             const typename Traits<P>::FT *__restrict__ Fcur,
                                             const typename Traits<P>::FT *__restrict__ Fprev, int x, int y, int z) {
@@ -153,63 +218,156 @@ __device__ __forceinline__ void update_cell(const Geometry &g, const typename Tr
   if (x < g.lo[0] || x > g.hi[0] || y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) return;
   const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
   T v[19];
-  T rho, ux, uy, uz;
-  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
-  if (frozen) {
-    // planes x = 2 and N-3 hold the inlet/outlet tables after
-    // copy_inout_to_df2, and their rho/vel are never updated (the moment box
-    // starts at x = 3, lbm_ib.c:2482-2483).
-    const T *tab = (x == g.lo[0]) ? c.tab_in : c.tab_out;
-#pragma unroll
-    for (int q = 0; q < 19; ++q) v[q] = tab[q];
-    rho = c.rho0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
-  } else if (own) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) v[q] = ld<P>(fin + q * g.pitch + c0);
-    rho = c.rho0; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // init_gv values, lbm_ib.c:837-849
-  } else {
-    pull<P, NTL>(g, fin, x, y, z, c0, v);
-    T fpx = 0, fpy = 0, fpz = 0;
-    if (in_any_box(box_prev, nsheets, x, y, z)) {
-      fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
-    }
-    lagged_moments<P>(v, fpx, fpy, fpz, c, rho, ux, uy, uz);
-  }
+  T d, rho, ux, uy, uz;
+  arrive_general<P, NTL>(g, fin, own, c, nsheets, box_prev, Fprev, x, y, z, c0, v, d, rho, ux, uy, uz);
   T fx = 0, fy = 0, fz = 0;
   if (in_any_box(box_cur, nsheets, x, y, z)) {
     fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
   }
-  collide_cell<P>(v, rho, ux, uy, uz, fx, fy, fz, c);
+  collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
 #pragma unroll
-  for (int q = 0; q < 19; ++q) st<P, NT>(fout + q * g.pitch + c0, v[q]);
+  for (int q = 0; q < 19; ++q) st_h<P, NT>(fout + q * g.pitch + c0, v[q]);
 }
 
-// One cell per thread, row-aligned blocks (any grid).
-template <int P, bool NT>
-__global__ void __launch_bounds__(1024)
-    k_collide_stream(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
-                     int own, FluidConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box_cur,
-                     const int *__restrict__ box_prev, const typename Traits<P>::FT *__restrict__ Fcur,
-                     const typename Traits<P>::FT *__restrict__ Fprev) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  update_cell<P, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, blockIdx.y, blockIdx.z);
+// Byte offsets, in a population array, of the 19 sources of cell (0, y, z)
+// of every row (y, z) -- the pull of stream_distrfunc with the y/z wrap and
+// half-way bounce-back of the previous step -- for the buffer-addressed row
+// path; the lane adds x * sizeof(S).  Built once per context on the host.
+constexpr int kRowTabStride = 20;
+
+int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
+  const long nrows = (long)g.ny * g.nz;
+  tab.assign((size_t)nrows * kRowTabStride, 0u);
+  const uint64_t pitch_b = (uint64_t)g.pitch * es, sy_b = (uint64_t)g.nx * es;
+  for (long row = 0; row < nrows; ++row) {
+    const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+    if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
+    for (int q = 0; q < 19; ++q) {
+      int py = y - kCY[q], pz = z - kCZ[q];
+      bool bb = false;
+      if (g.bc[1] == LBMIB_BC_PERIODIC) py = (py + g.ny) % g.ny;
+      else bb |= py < g.lo[1] || py > g.hi[1];
+      if (g.bc[2] == LBMIB_BC_PERIODIC) pz = (pz + g.nz) % g.nz;
+      else bb |= pz < g.lo[2] || pz > g.hi[2];
+      const int64_t off = bb ? (int64_t)(kOPP[q] * pitch_b + row * sy_b)
+                             : (int64_t)(q * pitch_b + (uint64_t)(py + (long)g.ny * pz) * sy_b) - kCX[q] * (int64_t)es;
+      if (off < 0 || off > 0xFFFFFFFFll) return LBMIB_ERR_ARG;
+      tab[(size_t)row * kRowTabStride + q] = (uint32_t)off;
+    }
+  }
+  return LBMIB_OK;
 }
 
-// Persistent form: a fixed grid of a few blocks per CU walks the (y, z) rows
-// with a stride, each block updating whole rows; optional nontemporal loads.
-template <int P, bool NT, bool NTL>
+// Row form: blocks walk whole (y, z) rows with a stride (one block per row
+// when the grid has as many blocks as rows).
+//
+// BUF (buffer addressing, needs 19 * pitch * sizeof(S) < 2^32): for each
+// population the source row -- wrapped on periodic axes, or the own row's
+// opposite population where the source lies beyond a y/z wall (half-way
+// bounce-back) -- is the same for the whole row, so its byte offset is a
+// wave-uniform (SGPR) value and the 19 loads and 19 stores of a lane share one
+// VGPR offset, x * sizeof(S) (one cell per thread, blockDim.x >= nx).  Lanes
+// on the frozen inlet/outlet planes take the table; the two x-edge cells of a
+// row on a wall or periodic x axis are left to k_collide_xedges.  Without BUF
+// every cell takes update_cell.
+template <int P, bool NT, bool NTL, bool BUF>
 __global__ void __launch_bounds__(1024)
     k_collide_stream_rows(Geometry g, const typename Traits<P>::S *__restrict__ fin,
-                          typename Traits<P>::S *__restrict__ fout, int own, FluidConsts<typename Traits<P>::T> c,
+                          typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
                           int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                           const typename Traits<P>::FT *__restrict__ Fcur,
-                          const typename Traits<P>::FT *__restrict__ Fprev) {
+                          const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab) {
+  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
   const int nrows = g.ny * g.nz;
-  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
-    const int y = row % g.ny, z = row / g.ny;
-    for (int x = threadIdx.x; x < g.nx; x += blockDim.x)
-      update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
+  if constexpr (!BUF) {
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+      const int y = row % g.ny, z = row / g.ny;
+      for (int x = threadIdx.x; x < g.nx; x += blockDim.x)
+        update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
+    }
+  } else {
+    // one cell per thread: blockDim.x >= nx (checked by the launcher)
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
+    constexpr uint32_t es = sizeof(S);
+    const uint32_t pitch_b = (uint32_t)g.pitch * es;
+    const uint32_t sy_b = (uint32_t)g.nx * es;
+    const int x = threadIdx.x;
+    const bool edge = x == g.lo[0] || x == g.hi[0];
+    // frozen inlet/outlet planes; x edges of wall/periodic x go to k_collide_xedges
+    const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
+    if (x < g.lo[0] || x > g.hi[0] || (edge && !frozen)) return;
+    const uint32_t vo = (uint32_t)x * es;
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+      const int y = row % g.ny, z = row / g.ny;
+      if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
+      const uint32_t row_b = (uint32_t)row * sy_b;  // byte offset of (0, y, z) in a plane
+      const long c0 = (long)x + (long)g.nx * row;
+      T v[19];
+      T d, rho, ux, uy, uz;
+      if (frozen) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
+        rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
+      } else {
+        // per-row source offsets (row_source_table): wrap and bounce-back
+        // of the y/z axes and the x shift folded in
+        if (own) {
+          uint32_t qb = row_b;  // first step: the cell's own populations
+#pragma unroll
+          for (int q = 0; q < 19; ++q) {
+            v[q] = bld_h<P, NTL>(rin, vo, qb);
+            qb += pitch_b;
+          }
+        } else {
+          const uint32_t *rt = rowtab + (long)row * kRowTabStride;
+          uint32_t so[19];
+#pragma unroll
+          for (int q = 0; q < 19; ++q) so[q] = rt[q];
+#pragma unroll
+          for (int q = 0; q < 19; ++q) v[q] = bld_h<P, NTL>(rin, vo, so[q]);
+        }
+        if (own) {
+          rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
+        } else {
+          T fpx = 0, fpy = 0, fpz = 0;
+          if (in_any_box(box_prev, nsheets, x, y, z)) {
+            fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
+          }
+          lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
+        }
+      }
+      T fx = 0, fy = 0, fz = 0;
+      if (in_any_box(box_cur, nsheets, x, y, z)) {
+        fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
+      }
+      collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
+      uint32_t sb = row_b;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        bst_h<P, NT>(rout, vo, sb, v[q]);
+        sb += pitch_b;
+      }
+    }
   }
+}
+
+// The x-edge cells (x = lo, hi) of every active row when x is a wall or a
+// periodic axis, by the general rules; completes the buffer-addressed rows.
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(256)
+    k_collide_xedges(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                     typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
+                     int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                     const typename Traits<P>::FT *__restrict__ Fcur,
+                     const typename Traits<P>::FT *__restrict__ Fprev) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2L * g.ny * g.nz) return;
+  const long row = e >> 1;
+  const int x = (e & 1) ? g.hi[0] : g.lo[0];
+  if ((e & 1) && g.lo[0] == g.hi[0]) return;  // one-cell-wide axis: done by the even item
+  update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, (int)(row % g.ny),
+                          (int)(row / g.ny));
 }
 
 // Bulk of a split step: every cell outside the slabs, no force (the slabs
@@ -218,7 +376,7 @@ __global__ void __launch_bounds__(1024)
 template <int P, bool NT, bool NTL>
 __global__ void __launch_bounds__(1024)
     k_collide_bulk(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
-                   int own, FluidConsts<typename Traits<P>::T> c, Slabs slabs) {
+                   int own, KConsts<typename Traits<P>::T> c, Slabs slabs) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   bool skip = false;
   for (int k = 0; k < slabs.n; ++k) skip |= x >= slabs.lo[k] && x <= slabs.hi[k];
@@ -231,7 +389,7 @@ __global__ void __launch_bounds__(1024)
 template <int P, bool NT, bool NTL>
 __global__ void __launch_bounds__(256)
     k_collide_band(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
-                   int own, FluidConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box_cur,
+                   int own, KConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box_cur,
                    const int *__restrict__ box_prev, const typename Traits<P>::FT *__restrict__ Fcur,
                    const typename Traits<P>::FT *__restrict__ Fprev, Slabs slabs) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -292,7 +450,7 @@ __device__ __forceinline__ void store_vec(VT *dst, const VT &t) {
 template <int P, int V, bool NT>
 __global__ void __launch_bounds__(256)
     k_collide_stream_vec(Geometry g, const typename Traits<P>::S *__restrict__ fin,
-                         typename Traits<P>::S *__restrict__ fout, int own, FluidConsts<typename Traits<P>::T> c,
+                         typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
                          int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                          const typename Traits<P>::FT *__restrict__ Fcur,
                          const typename Traits<P>::FT *__restrict__ Fprev) {
@@ -319,7 +477,7 @@ __global__ void __launch_bounds__(256)
     if (kCX[q] == 0) t = *reinterpret_cast<const VecT<S, V> *>(p);
     else __builtin_memcpy(&t, p, sizeof(t));
 #pragma unroll
-    for (int k = 0; k < V; ++k) v[k][q] = ld<P>(&t.a[k]);
+    for (int k = 0; k < V; ++k) v[k][q] = from_bits<P>(t.a[k]);
   }
   const bool inprev = nsheets > 0 && any_box_overlaps(box_prev, nsheets, x0, x0 + V - 1, y, z);
   const bool incur = nsheets > 0 && any_box_overlaps(box_cur, nsheets, x0, x0 + V - 1, y, z);
@@ -333,15 +491,15 @@ __global__ void __launch_bounds__(256)
     if (incur && in_any_box(box_cur, nsheets, x, y, z)) {
       fx = (T)Fcur[c0 + k]; fy = (T)Fcur[g.n + c0 + k]; fz = (T)Fcur[2 * g.n + c0 + k];
     }
-    T rho, ux, uy, uz;
-    lagged_moments<P>(v[k], fpx, fpy, fpz, c, rho, ux, uy, uz);
-    collide_cell<P>(v[k], rho, ux, uy, uz, fx, fy, fz, c);
+    T d, rho, ux, uy, uz;
+    lagged_moments<P>(v[k], fpx, fpy, fpz, c, d, rho, ux, uy, uz);
+    collide_cell<P>(v[k], d, rho, ux, uy, uz, fx, fy, fz, c);
   }
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     VecT<S, V> t;
 #pragma unroll
-    for (int k = 0; k < V; ++k) st<P, false>(&t.a[k], v[k][q]);
+    for (int k = 0; k < V; ++k) t.a[k] = to_bits<P>(v[k][q]);
     store_vec<NT>(reinterpret_cast<VecT<S, V> *>(fout + q * g.pitch + c0), t);
   }
 }
@@ -349,21 +507,21 @@ __global__ void __launch_bounds__(256)
 // Fills the active box with the equilibrium (init_eqlbrmdistrfuncDF0 +
 // init_df1, lbm_ib.c:1528-1606, 2725-2767); other cells are zero.
 template <int P>
-__global__ void k_fill_init(Geometry g, typename Traits<P>::S *__restrict__ f, FluidConsts<typename Traits<P>::T> c) {
+__global__ void k_fill_init(Geometry g, typename Traits<P>::S *__restrict__ f, KConsts<typename Traits<P>::T> c) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, z = blockIdx.z;
   if (x >= g.nx) return;
   const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
   const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
 #pragma unroll
-  for (int q = 0; q < 19; ++q) st<P>(f + q * g.pitch + c0, act ? c.tab_in[q] : (typename Traits<P>::T)0);
+  for (int q = 0; q < 19; ++q) st_h<P>(f + q * g.pitch + c0, act ? c.tab[q] : (typename Traits<P>::T)0);
 }
 
 // Materialises df2 (= df1 after the swap), rho and vel of the last completed
 // step for every cell, in FP64 and in the reference's conventions.
 template <int P>
 __global__ void k_materialize(Geometry g, const typename Traits<P>::S *__restrict__ f, int own,
-                              FluidConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box,
+                              KConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box,
                               const typename Traits<P>::FT *__restrict__ F, double *__restrict__ df,
                               double *__restrict__ rho_out, double *__restrict__ vel) {
   using T = typename Traits<P>::T;
@@ -378,8 +536,7 @@ __global__ void k_materialize(Geometry g, const typename Traits<P>::S *__restric
     T v[19];
     const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
     if (frozen) {
-      const T *tab = (x == g.lo[0]) ? c.tab_in : c.tab_out;
-      for (int q = 0; q < 19; ++q) v[q] = tab[q];
+      for (int q = 0; q < 19; ++q) v[q] = c.tab[q] * Traits<P>::inv_scale;
     } else if (own) {
       for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pitch + c0);
     } else {
@@ -423,7 +580,7 @@ constexpr int kTotRowsPerBlock = 16;
 template <int P>
 __global__ void __launch_bounds__(kTotThreads)
     k_totals_partial(Geometry g, const typename Traits<P>::S *__restrict__ f, int own,
-                     FluidConsts<typename Traits<P>::T> c, double *__restrict__ partials) {
+                     KConsts<typename Traits<P>::T> c, double *__restrict__ partials) {
   using T = typename Traits<P>::T;
   __shared__ double red[4][kTotThreads];
   double acc[4] = {0, 0, 0, 0};
@@ -438,8 +595,7 @@ __global__ void __launch_bounds__(kTotThreads)
       T v[19];
       const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
       if (frozen) {
-        const T *tab = (x == g.lo[0]) ? c.tab_in : c.tab_out;
-        for (int q = 0; q < 19; ++q) v[q] = tab[q];
+        for (int q = 0; q < 19; ++q) v[q] = c.tab[q] * Traits<P>::inv_scale;
       } else if (own) {
         for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pitch + c0);
       } else {
@@ -509,8 +665,7 @@ static inline dim3 row_grid(const Geometry &g, dim3 b) { return dim3((g.nx + b.x
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s) {
   dim3 b = row_block(g), gr = row_grid(g, b);
   LBMIB_DISPATCH(prec, ({
-    using TT = typename Traits<P>::T;
-    k_fill_init<P><<<gr, b, 0, s>>>(g, (typename Traits<P>::S *)f, cast_consts<TT>(c));
+    k_fill_init<P><<<gr, b, 0, s>>>(g, (typename Traits<P>::S *)f, kconsts<P>(c));
   }));
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
@@ -521,7 +676,7 @@ int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<dou
 // than 256 threads.
 int collide_vec_width(int prec, int nx, int vec_req) {
   int v = vec_req;
-  if (v <= 0) v = (prec == LBMIB_F16 || prec == LBMIB_BF16) ? 8 : 1;
+  if (v <= 0) v = 1;
   const int max_v = prec == LBMIB_F64 ? 2 : (prec == LBMIB_F32 ? 4 : 8);
   const int min_v = (prec == LBMIB_F16 || prec == LBMIB_BF16) ? 4 : 2;
   if (v == 1 || v > max_v || v < min_v || (v & (v - 1))) return 1;
@@ -533,40 +688,44 @@ template <int P, int V, bool NT>
 static void launch_vec(const Geometry &g, const void *fin, void *fout, int own, const FluidConsts<double> &c,
                        int nsheets, const int *box_cur, const int *box_prev, const void *Fcur, const void *Fprev,
                        hipStream_t s) {
-  using TT = typename Traits<P>::T;
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const int bx = g.nx / V;
   const int by = bx >= 256 ? 1 : 256 / bx;
   dim3 b(bx, by, 1), gr((g.ny + by - 1) / by, g.nz, 1);
-  k_collide_stream_vec<P, V, NT><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, cast_consts<TT>(c), nsheets,
+  k_collide_stream_vec<P, V, NT><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
                                                   box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev);
 }
 
-template <int P, bool NT>
-static void launch_scalar(const Geometry &g, const void *fin, void *fout, int own, const FluidConsts<double> &c,
-                          int nsheets, const int *box_cur, const int *box_prev, const void *Fcur, const void *Fprev,
-                          hipStream_t s) {
-  using TT = typename Traits<P>::T;
-  using FT = typename Traits<P>::FT;
-  using S = typename Traits<P>::S;
-  dim3 b = row_block(g), gr = row_grid(g, b);
-  k_collide_stream<P, NT><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, cast_consts<TT>(c), nsheets, box_cur,
-                                           box_prev, (const FT *)Fcur, (const FT *)Fprev);
+// Whether the buffer-addressed row path applies: every byte offset of a
+// population array fits 32 bits and one block covers a row.
+bool row_path_buffered(const Geometry &g, size_t es) {
+  return 19ull * (unsigned long long)g.pitch * es <= 0xFFFFFFFFull && (int)row_block(g).x >= g.nx;
 }
 
 template <int P, bool NT, bool NTL>
 static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                         const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                        const void *Fcur, const void *Fprev, hipStream_t s) {
-  using TT = typename Traits<P>::T;
+                        const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const dim3 b = row_block(g);
   const int nrows = g.ny * g.nz;
-  k_collide_stream_rows<P, NT, NTL><<<dim3(nblocks < nrows ? nblocks : nrows), b, 0, s>>>(
-      g, (const S *)fin, (S *)fout, own, cast_consts<TT>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
-      (const FT *)Fprev);
+  const dim3 gr(nblocks < nrows ? nblocks : nrows);
+  if (rowtab && row_path_buffered(g, sizeof(S))) {
+    k_collide_stream_rows<P, NT, NTL, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
+                                                             box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev,
+                                                             rowtab);
+    if (g.bc[0] != LBMIB_BC_INOUT) {
+      const long items = 2L * g.ny * g.nz;
+      k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
+          g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
+          (const FT *)Fprev);
+    }
+  } else
+    k_collide_stream_rows<P, NT, NTL, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c),
+                                                              nsheets, box_cur, box_prev, (const FT *)Fcur,
+                                                              (const FT *)Fprev, nullptr);
 }
 
 template <int P, bool NT>
@@ -582,22 +741,25 @@ static void launch_any(int v, const Geometry &g, const void *fin, void *fout, in
     if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   }
-  launch_scalar<P, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+  launch_rows<P, NT, false>(g.ny * g.nz, g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, nullptr, s);
 }
 
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                          const void *Fcur, const void *Fprev, const KernelVariant &kv, hipStream_t s) {
-  const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);  // nontemporal stores by default
-  const int ntl = kv.nt_mask < 0 ? 0 : ((kv.nt_mask >> 1) & 1);
+                          const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
+                          hipStream_t s) {
+  // nontemporal stores and loads by default: the populations are touched
+  // once per step, and keeping them out of the caches was fastest measured
+  const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
+  const int ntl = kv.nt_mask < 0 ? 1 : ((kv.nt_mask >> 1) & 1);
   const int v = collide_vec_width(prec, g.nx, kv.cells_per_thread);
-  if (v == 1 && (kv.blocks > 0 || ntl)) {
+  if (v == 1) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
-      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
-      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
-      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
+      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
@@ -616,7 +778,7 @@ int launch_collide_bulk(int prec, const Geometry &g, const void *fin, void *fout
   LBMIB_DISPATCH(prec, ({
     using TT = typename Traits<P>::T;
     using S = typename Traits<P>::S;
-    const FluidConsts<TT> cc = cast_consts<TT>(c);
+    const KConsts<TT> cc = kconsts<P>(c);
     if (nt && ntl) k_collide_bulk<P, true, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
     else if (nt) k_collide_bulk<P, true, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
     else if (ntl) k_collide_bulk<P, false, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
@@ -638,7 +800,7 @@ int launch_collide_band(int prec, const Geometry &g, const void *fin, void *fout
     using TT = typename Traits<P>::T;
     using FT = typename Traits<P>::FT;
     using S = typename Traits<P>::S;
-    const FluidConsts<TT> cc = cast_consts<TT>(c);
+    const KConsts<TT> cc = kconsts<P>(c);
     if (nt && ntl)
       k_collide_band<P, true, true><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
                                                            box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
@@ -660,9 +822,8 @@ int launch_materialize(int prec, const Geometry &g, const void *f, int own_mode,
                        hipStream_t s) {
   dim3 b = row_block(g), gr = row_grid(g, b);
   LBMIB_DISPATCH(prec, ({
-    using TT = typename Traits<P>::T;
     using FT = typename Traits<P>::FT;
-    k_materialize<P><<<gr, b, 0, s>>>(g, (const typename Traits<P>::S *)f, own_mode, cast_consts<TT>(c), nsheets,
+    k_materialize<P><<<gr, b, 0, s>>>(g, (const typename Traits<P>::S *)f, own_mode, kconsts<P>(c), nsheets,
                                        box, (const FT *)F, df, rho, vel);
   }));
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
@@ -678,9 +839,8 @@ int launch_totals(int prec, const Geometry &g, const void *f, int own_mode, cons
   const int nb = totals_blocks(g);
   if (nb > nblocks_max) return LBMIB_ERR_ARG;
   LBMIB_DISPATCH(prec, ({
-    using TT = typename Traits<P>::T;
     k_totals_partial<P><<<nb, kTotThreads, 0, s>>>(g, (const typename Traits<P>::S *)f, own_mode,
-                                                    cast_consts<TT>(c), partials);
+                                                    kconsts<P>(c), partials);
   }));
   k_totals_final<<<1, kTotThreads, 0, s>>>(partials, nb, out4);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;

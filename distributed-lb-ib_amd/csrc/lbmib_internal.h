@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/lbmib.h"
 
 namespace lbmib {
@@ -18,6 +20,9 @@ constexpr int kOPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 
 // 1/w_q denominators (lbm_ib.c:1742, 1773, 1804)
 constexpr int kWDEN[19] = {3, 18, 18, 18, 18, 18, 18, 36, 36, 36,
                                                 36, 36, 36, 36, 36, 36, 36, 36, 36};
+// weight class of each direction: 0 rest (1/3), 1 axis (1/18), 2 diagonal (1/36)
+constexpr int kCLS[19] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+constexpr int kWDEN_CLS[3] = {3, 18, 36};
 
 // The reference's truncated pi (lbm_ib.c:33).
 constexpr double kRefPi = 3.14159265358979;
@@ -51,7 +56,7 @@ struct SheetDesc {
   double str_col_c;      // Ks / ds2^2 (lbm_ib.c:1235)
 };
 
-// Everything the fused fluid kernels need, passed by value.
+// Host copy of the fluid constants in FP64 (kernels receive KConsts).
 template <class T>
 struct FluidConsts {
   T omc, om;          // 1 - 1/tau, 1/tau
@@ -61,6 +66,20 @@ struct FluidConsts {
   T g;
   T rho0, u0x, u0y, u0z;
   T tab_in[19], tab_out[19];  // f (F64) or f - w (deviation forms)
+  const void *dtab;           // device copy of tab_in in kernel units (see KConsts::tab)
+};
+
+// Constants of the fluid kernels, passed by value, in kernel units: the F64
+// form works on f; the deviation forms on h = s (f - w) with s = 1 (F32) or
+// 2^8 (16-bit storage), the stored value itself.
+template <class T>
+struct KConsts {
+  T omc, om;          // 1 - 1/tau, 1/tau
+  T wa[3];            // deviation forms, per weight class: s om w
+  T wb[3];            // deviation forms: s w (1 - 1/(2 tau)); F64: dt (1 - 1/(2 tau)) / wden
+  T cs2, cs4, g;
+  T rho0, u0x, u0y, u0z;
+  const T *tab;       // [19] populations of the frozen inlet/outlet planes, kernel units
 };
 
 struct Geometry {
@@ -100,9 +119,14 @@ struct KernelVariant {
   int blocks;            // > 0: persistent grid of this many blocks walking rows (one cell per thread)
 };
 
+// rowtab: the row source table (row_source_table) on the device, or null for
+// the general path.
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                          const void *Fcur, const void *Fprev, const KernelVariant &kv, hipStream_t s);
+                          const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
+                          hipStream_t s);
+bool row_path_buffered(const Geometry &g, size_t es);
+int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab);
 int collide_vec_width(int prec, int nx, int vec_req);
 int launch_materialize(int prec, const Geometry &g, const void *f, int own_mode, const FluidConsts<double> &c,
                        int nsheets, const int *box, const void *F, double *df, double *rho, double *vel,
