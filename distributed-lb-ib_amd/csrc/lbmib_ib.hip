@@ -6,20 +6,20 @@
 //                  floor(X-2)+1 and its twelve 1+cos(pi r/2) factors, its
 //                  (y,z) bin and the sheet's support box.  One launch covers
 //                  every sheet.
-//   k_gather       cell-owned force spreading: each workgroup owns an 8x8
+//   k_gather       cell-owned force spreading: each workgroup owns a 4x4
 //                  (y,z) column tile of a sheet's support box, loads the
-//                  candidate node ids of the 3x3 bins that can reach it into
+//                  candidate node ids of the 2x2 bins that can reach it into
 //                  LDS, rank-sorts them, and every cell sums F_node * delta
 //                  over the candidates in ascending (sheet, row, col) order --
 //                  the order of the sequential spread (lbm_ib.c:1420-1495).
 //                  No float atomics; the result is run-to-run identical.
-//   k_move         one wave per node: rho/u of the new populations on the
-//                  node's 64 support cells (compute_rho_and_u,
-//                  lbm_ib.c:2440-2538, evaluated only where the
-//                  interpolation reads), u(X) = Sum u delta summed in the
-//                  reference's order (lbm_ib.c:2670-2673), X += dt u(X) in
-//                  FP64; also clears the bins and the box slot that the next
-//                  step will fill.
+//   k_band_u       rho/u of the new populations on each support-box cell
+//                  (compute_rho_and_u, lbm_ib.c:2440-2538, evaluated only
+//                  where the interpolation reads), FP64.
+//   k_move         one wave per node: u(X) = Sum u delta over the node's 64
+//                  support cells summed in the reference's order
+//                  (lbm_ib.c:2670-2673), X += dt u(X) in FP64; also clears
+//                  the bins and the box slot that the next step will fill.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
@@ -36,43 +36,42 @@ __device__ __forceinline__ int find_sheet(const IBState &ib, int n) {
 
 // --------------------------------------------------------------- k_fiber
 
+// The stencils below read a node's line neighbourhood from registers:
+// W[k + 2] = coordinate of line element j + k (k = -2..2; entries outside
+// the line are zero and never used).
+
 // Row stencil of compute_bendingforce (lbm_ib.c:1004-1054), one component.
-__device__ __forceinline__ double bend_1d(const double *X, long stride, int j, int C, double bc, bool slip) {
-  // X points at element 0 of the line, elements stride apart.
-#define P(k) X[(long)(k) * stride]
-  if (j == 0) return bc * (-P(2) + 2 * P(1) - P(0));
-  if (j == 1) return bc * (-P(3) + 4 * P(2) - 5 * P(1) + 2 * P(0));
-  if (j == C - 2) return bc * (2 * P(j + 1) - 5 * P(j) + 4 * P(j - 1) - (slip ? P(j) : P(j - 2)));
-  if (j == C - 1) return bc * (-P(j) + 2 * P(j - 1) - P(j - 2));
-  return bc * (-P(j + 2) + 4.0 * P(j + 1) - 6.0 * P(j) + 4.0 * P(j - 1) - P(j - 2));
-#undef P
+__device__ __forceinline__ double bend_w(const double W[5], int j, int C, double bc, bool slip) {
+  if (j == 0) return bc * (-W[4] + 2 * W[3] - W[2]);
+  if (j == 1) return bc * (-W[4] + 4 * W[3] - 5 * W[2] + 2 * W[1]);
+  if (j == C - 2) return bc * (2 * W[3] - 5 * W[2] + 4 * W[1] - (slip ? W[2] : W[0]));
+  if (j == C - 1) return bc * (-W[2] + 2 * W[1] - W[0]);
+  return bc * (-W[4] + 4.0 * W[3] - 6.0 * W[2] + 4.0 * W[1] - W[0]);
 }
 
 // Spring term of compute_stretchingforce for one line (lbm_ib.c:1188-1221,
-// :1236-1276).  Writes the three components of the force on element j.
-__device__ __forceinline__ void stretch_1d(const double *X0, const double *X1, const double *X2, long stride, int j,
-                                           int C, double ds, double sc, bool quirk, double out[3]) {
-  const double *X[3] = {X0, X1, X2};
-  auto at = [&](int d, int k) { return X[d][(long)k * stride]; };
+// :1236-1276): the three components of the force on element j.
+__device__ __forceinline__ void stretch_w(const double W[3][5], int j, int C, double ds, double sc, bool quirk,
+                                          double out[3]) {
   if (j > 0 && j < C - 1) {
     double tr[3], tl[3];
     for (int d = 0; d < 3; ++d) {
-      tr[d] = at(d, j + 1) - at(d, j);
-      tl[d] = at(d, j - 1) - at(d, j);
+      tr[d] = W[d][3] - W[d][2];
+      tl[d] = W[d][1] - W[d][2];
     }
     const double dr = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2]);
     const double dl = sqrt(tl[0] * tl[0] + tl[1] * tl[1] + tl[2] * tl[2]);
     for (int d = 0; d < 3; ++d) out[d] = sc * ((dr - ds) * (tr[d] / dr) + (dl - ds) * (tl[d] / dl));
   } else if (j == 0) {
     double tr[3];
-    for (int d = 0; d < 3; ++d) tr[d] = at(d, 1) - at(d, 0);
+    for (int d = 0; d < 3; ++d) tr[d] = W[d][3] - W[d][2];
     const double dr = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2]);
     for (int d = 0; d < 3; ++d) out[d] = sc * ((dr - ds) * (tr[d] / dr));
   } else {
     // last node: the reference's tangent points away from the neighbour
     // (lbm_ib.c:1215-1217, :1269-1271); quirk=0 points it at the neighbour.
     double tl[3];
-    for (int d = 0; d < 3; ++d) tl[d] = quirk ? at(d, C - 1) - at(d, C - 2) : at(d, C - 2) - at(d, C - 1);
+    for (int d = 0; d < 3; ++d) tl[d] = quirk ? W[d][2] - W[d][1] : W[d][1] - W[d][2];
     const double dl = sqrt(tl[0] * tl[0] + tl[1] * tl[1] + tl[2] * tl[2]);
     for (int d = 0; d < 3; ++d) out[d] = sc * ((dl - ds) * (tl[d] / dl));
   }
@@ -89,10 +88,12 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-__global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask, int slot, Slabs guard) {
+__global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g, int mask, int slot, Slabs guard) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = n < ib.nnodes;
   const int M = ib.nnodes;
+  const double *__restrict__ pos = ib.pos;
+  double *__restrict__ frc = ib.frc;
   int s = 0;
   bool ok = false;
   int st[3] = {INT_MAX, INT_MAX, INT_MAX};
@@ -100,32 +101,46 @@ __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask,
     s = find_sheet(ib, n);
     const SheetDesc sd = ib.sheets[s];
     const int k = n - sd.off, i = k / sd.cols, j = k % sd.cols;
-    const double *px = ib.pos, *py = ib.pos + M, *pz = ib.pos + 2 * M;
-    const double *lines[3] = {px, py, pz};
     const bool quirk = ib.quirks != 0;
-
-    if (mask & FK_BEND) {
-      for (int d = 0; d < 3; ++d) {
-        const double *row = lines[d] + sd.off + (long)i * sd.cols;  // along cols, stride 1
-        const double *col = lines[d] + sd.off + j;                  // along rows, stride cols
-        const double r = bend_1d(row, 1, j, sd.cols, sd.bend_c, quirk && d == 0);
-        const double c = bend_1d(col, sd.cols, i, sd.rows, sd.bend_c, false);
-        ib.frc[(NF_BX + d) * (long)M + n] = r + c;  // row pass '=', column pass '+=' (lbm_ib.c:1070)
+    // the node's row (along cols) and column (along rows) neighbourhoods,
+    // loaded once
+    double Wr[3][5], Wc[3][5];
+    for (int d = 0; d < 3; ++d) {
+      const double *line = pos + (long)d * M + sd.off;
+      for (int o = -2; o <= 2; ++o) {
+        const int jj = j + o, ii = i + o;
+        Wr[d][o + 2] = (jj >= 0 && jj < sd.cols) ? line[(long)i * sd.cols + jj] : 0.0;
+        Wc[d][o + 2] = (ii >= 0 && ii < sd.rows) ? line[(long)ii * sd.cols + j] : 0.0;
       }
     }
-    if (mask & FK_STRETCH) {
-      double r[3], c[3];
-      const long ro = sd.off + (long)i * sd.cols, co = sd.off + j;
-      stretch_1d(px + ro, py + ro, pz + ro, 1, j, sd.cols, sd.ds1, sd.str_row_c, quirk, r);
-      stretch_1d(px + co, py + co, pz + co, sd.cols, i, sd.rows, sd.ds2, sd.str_col_c, quirk, c);
-      for (int d = 0; d < 3; ++d) ib.frc[(NF_SX + d) * (long)M + n] = r[d] + c[d];
+    double bend[3] = {0, 0, 0}, str[3] = {0, 0, 0};
+    if (mask & (FK_BEND | FK_ELASTIC)) {
+      for (int d = 0; d < 3; ++d) {
+        const double r = bend_w(Wr[d], j, sd.cols, sd.bend_c, quirk && d == 0);
+        const double c = bend_w(Wc[d], i, sd.rows, sd.bend_c, false);
+        bend[d] = r + c;  // row pass '=', column pass '+=' (lbm_ib.c:1070)
+      }
     }
+    if (mask & (FK_STRETCH | FK_ELASTIC)) {
+      double r[3], c[3];
+      stretch_w(Wr, j, sd.cols, sd.ds1, sd.str_row_c, quirk, r);
+      stretch_w(Wc, i, sd.rows, sd.ds2, sd.str_col_c, quirk, c);
+      for (int d = 0; d < 3; ++d) str[d] = r[d] + c[d];
+    }
+    if (mask & FK_BEND)
+      for (int d = 0; d < 3; ++d) frc[(NF_BX + d) * (long)M + n] = bend[d];
+    if (mask & FK_STRETCH)
+      for (int d = 0; d < 3; ++d) frc[(NF_SX + d) * (long)M + n] = str[d];
     if (mask & FK_ELASTIC) {
-      for (int d = 0; d < 3; ++d)
-        ib.frc[(NF_EX + d) * (long)M + n] = ib.frc[(NF_SX + d) * (long)M + n] + ib.frc[(NF_BX + d) * (long)M + n];
+      // elastic = stretch + bend of the stored node forces (lbm_ib.c:1300-1315)
+      for (int d = 0; d < 3; ++d) {
+        const double sv = (mask & FK_STRETCH) ? str[d] : frc[(NF_SX + d) * (long)M + n];
+        const double bv = (mask & FK_BEND) ? bend[d] : frc[(NF_BX + d) * (long)M + n];
+        frc[(NF_EX + d) * (long)M + n] = sv + bv;
+      }
     }
     if (mask & FK_PREP) {
-      const double X[3] = {px[n], py[n], pz[n]};
+      const double X[3] = {Wr[0][2], Wr[1][2], Wr[2][2]};
       const int dims[3] = {g.nx, g.ny, g.nz};
       ok = true;
       for (int d = 0; d < 3; ++d) {
@@ -160,7 +175,7 @@ __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask,
   // the box is deterministic).
   __shared__ int sbox[kMaxSheets * 6];
   __shared__ int s_last;
-  if (threadIdx.x < kMaxSheets * 6) sbox[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
+  for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) sbox[t] = (t & 1) ? INT_MIN : INT_MAX;
   __syncthreads();
   const int s_lo = __shfl(s, 0, 64);
   const bool uniform = __all(!live || s == s_lo);
@@ -184,23 +199,41 @@ __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask,
     }
   }
   __syncthreads();
-  if (threadIdx.x < kMaxSheets * 6) ib.box_part[blockIdx.x * kMaxSheets * 6 + threadIdx.x] = sbox[threadIdx.x];
+  for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) ib.box_part[blockIdx.x * kMaxSheets * 6 + t] = sbox[t];
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == gridDim.x - 1;
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  if (threadIdx.x < ib.nsheets * 6) {
-    const int k = threadIdx.x;
-    int acc = (k & 1) ? INT_MIN : INT_MAX;
-    for (unsigned b = 0; b < gridDim.x; ++b) {
-      const int v = __hip_atomic_load(ib.box_part + b * kMaxSheets * 6 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      acc = (k & 1) ? max(acc, v) : min(acc, v);
+  // min/max over every block's partial, all threads at once: eight
+  // independent loads per thread in flight, combined with LDS atomics
+  for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) sbox[t] = (t & 1) ? INT_MIN : INT_MAX;
+  __syncthreads();
+  const int nk = ib.nsheets * 6;
+  const int items = (int)gridDim.x * nk;
+  for (int base = 0; base < items; base += 8 * blockDim.x) {
+    int v[8], kk[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int it = base + u * blockDim.x + threadIdx.x;
+      kk[u] = -1;
+      v[u] = 0;
+      if (it < items) {
+        const int b = it / nk, k = it % nk;
+        kk[u] = k;
+        v[u] = __hip_atomic_load(ib.box_part + b * kMaxSheets * 6 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
-    ib.box[slot * kMaxSheets * 6 + k] = acc;
-    sbox[k] = acc;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (kk[u] < 0) continue;
+      if (kk[u] & 1) atomicMax(&sbox[kk[u]], v[u]);
+      else atomicMin(&sbox[kk[u]], v[u]);
+    }
   }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) ib.box[slot * kMaxSheets * 6 + k] = sbox[k];
   if (guard.n > 0) {
     // a split step needs every support cell and its pull neighbours inside
     // one slab; latch an error if the sheet got closer than that
@@ -217,8 +250,8 @@ __global__ void __launch_bounds__(256) k_fiber(IBState ib, Geometry g, int mask,
 
 // -------------------------------------------------------------- k_gather
 
-constexpr int kGatherThreads = 256;
-constexpr int kTileBins = 2;              // tile side in 4-cell bins
+constexpr int kGatherThreads = 128;
+constexpr int kTileBins = 1;              // tile side in 4-cell bins
 constexpr int kTile = 4 * kTileBins;      // tile side in cells (y and z)
 constexpr int kCandBins = kTileBins + 1;  // candidate bins per axis
 
@@ -328,20 +361,60 @@ __global__ void __launch_bounds__(kGatherThreads)
 
 // ----------------------------------------------------------------- k_move
 //
-// One wave per fiber node, one lane per cell of its 4x4x4 support (lanes
-// ordered along x for coalescing; products kept in the order i2, j2, k2 of
-// lbm_ib.c:2614-2682).  Each
-// lane computes the cell's rho/u from the new populations exactly as
-// compute_rho_and_u does (lbm_ib.c:2498-2525; cells outside the moment box
-// keep init_gv's velocity), weights it, and lane 0 sums the 64 products in
-// lane order -- the reference's summation order -- before X += dt u(X).
+// Two kernels.  k_band_u evaluates compute_rho_and_u (lbm_ib.c:2498-2525) on
+// every cell of each sheet's support box from the new populations and the
+// current force -- once per cell, where a node-centred evaluation would
+// redo each cell for the ~16 nodes whose support covers it -- into the u
+// field ub (cells outside the moment box keep init_gv's velocity).  k_move
+// then runs one wave per fiber node, one lane per cell of its 4x4x4 support
+// (lanes ordered along x for coalescing), weights u, and lane 0..2 sum the 64
+// products in the reference's (i2, j2, k2) order (lbm_ib.c:2614-2682) before
+// X += dt u(X).
+
+template <int P>
+__global__ void __launch_bounds__(256)
+    k_band_u(IBState ib, Geometry g, const typename Traits<P>::S *__restrict__ f,
+             const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y, double u0z, double gz, int slot) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+  for (int s = 0; s < ib.nsheets; ++s) {
+    const int *b = ib.box + (slot * kMaxSheets + s) * 6;
+    const int x0 = b[0], y0 = b[2], z0 = b[4];
+    if (x0 > b[1]) continue;  // no valid node
+    const int nxs = b[1] - x0 + 1, nys = b[3] - y0 + 1;
+    const long cnt = (long)nxs * nys * (b[5] - z0 + 1);
+    for (long e = tid; e < cnt; e += stride) {
+      const int x = x0 + (int)(e % nxs), y = y0 + (int)((e / nxs) % nys), z = z0 + (int)(e / ((long)nxs * nys));
+      const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
+      const bool mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] &&
+                       z <= g.mhi[2];
+      double ux = u0x, uy = u0y, uz = u0z;
+      if (mom) {
+        typename Traits<P>::T v[19];
+        pull<P>(g, f, x, y, z, c0, v);
+        double fv[19];
+        for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + 1.0 / kWDEN[q] : (double)v[q];
+        double s1 = fv[0], s2 = 0, s3 = 0, s4 = 0;
+        for (int q = 1; q < 19; ++q) {
+          s1 += fv[q];
+          if (kCX[q]) s2 = kCX[q] > 0 ? s2 + fv[q] : s2 - fv[q];
+          if (kCY[q]) s3 = kCY[q] > 0 ? s3 + fv[q] : s3 - fv[q];
+          if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + fv[q] : s4 - fv[q];
+        }
+        const double fx = F[c0], fy = F[g.n + c0], fz = F[2 * g.n + c0];
+        ux = (s2 + 0.5 * fx) / s1;
+        uy = (s3 + 0.5 * fy) / s1;
+        uz = (s4 + 0.5 * (fz + s1 * gz)) / s1;
+      }
+      ib.ub[c0] = ux;
+      ib.ub[g.n + c0] = uy;
+      ib.ub[2 * g.n + c0] = uz;
+    }
+  }
+}
 
 constexpr int kMoveThreads = 256;
 
-template <int P>
-__global__ void __launch_bounds__(kMoveThreads)
-    k_move(IBState ib, Geometry g, const typename Traits<P>::S *__restrict__ f, const typename Traits<P>::FT *__restrict__ F,
-           double u0x, double u0y, double u0z, double gz, int reset_slot) {
+__global__ void __launch_bounds__(kMoveThreads) k_move(IBState ib, Geometry g, int reset_slot) {
   __shared__ double prod[kMoveThreads / 64][3][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = blockIdx.x * (kMoveThreads / 64) + wv;
@@ -354,25 +427,7 @@ __global__ void __launch_bounds__(kMoveThreads)
     const int slot_ref = i2 * 16 + j2 * 4 + k2;
     const int x = ib.start[n] + i2, y = ib.start[M + n] + j2, z = ib.start[2 * M + n] + k2;
     const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
-    const bool mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] && z <= g.mhi[2];
-    double ux = u0x, uy = u0y, uz = u0z;
-    if (mom) {
-      typename Traits<P>::T v[19];
-      pull<P>(g, f, x, y, z, c0, v);
-      double fv[19];
-      for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + 1.0 / kWDEN[q] : (double)v[q];
-      double s1 = fv[0], s2 = 0, s3 = 0, s4 = 0;
-      for (int q = 1; q < 19; ++q) {
-        s1 += fv[q];
-        if (kCX[q]) s2 = kCX[q] > 0 ? s2 + fv[q] : s2 - fv[q];
-        if (kCY[q]) s3 = kCY[q] > 0 ? s3 + fv[q] : s3 - fv[q];
-        if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + fv[q] : s4 - fv[q];
-      }
-      const double fx = F[c0], fy = F[g.n + c0], fz = F[2 * g.n + c0];
-      ux = (s2 + 0.5 * fx) / s1;
-      uy = (s3 + 0.5 * fy) / s1;
-      uz = (s4 + 0.5 * (fz + s1 * gz)) / s1;
-    }
+    const double ux = ib.ub[c0], uy = ib.ub[g.n + c0], uz = ib.ub[2 * g.n + c0];
     const double c_64 = 1.0 / (6.4e1);
     const double wx = ib.wgt[i2 * (long)M + n], wy = ib.wgt[(4 + j2) * (long)M + n], wz = ib.wgt[(8 + k2) * (long)M + n];
     prod[wv][0][slot_ref] = ux * wx * wy * wz * c_64;
@@ -418,7 +473,7 @@ int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, const
   if (ib.nnodes <= 0) return LBMIB_OK;
   Slabs none;
   none.n = 0;
-  k_fiber<<<grid_for(ib.nnodes, 256), 256, 0, s>>>(ib, g, mask, slot, guard ? *guard : none);
+  k_fiber<<<fiber_blocks(ib.nnodes), kFiberThreads, 0, s>>>(ib, g, mask, slot, guard ? *guard : none);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
@@ -426,7 +481,9 @@ int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void
   if (ib.nnodes <= 0) return LBMIB_OK;
   const int mcap = min(kCandBins * kCandBins * ib.cap, kGatherMaxCand);
   const size_t shm = sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * 5 * (size_t)mcap;
-  const int blocks = 512;  // the kernel strides over any further tiles
+  // about one tile (4x4 bin) per 16 nodes at unit spacing; the kernel strides
+  // over any further tiles
+  const int blocks = max(64, min(4096, ib.nnodes / 8));
   if (prec == LBMIB_F64) k_gather<double><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (double *)F, mcap);
   else k_gather<float><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (float *)F, mcap);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
@@ -435,28 +492,31 @@ int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void
 int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
                 const FluidConsts<double> &c, int reset_slot, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
-  const int nodes_per_block = kMoveThreads / 64;
-  const int blocks = grid_for(ib.nnodes, nodes_per_block);
+  const int slot = 1 - reset_slot;  // the box of this step's spread
+  // about 4 support-box cells per node for a flat sheet; the kernel strides
+  const int ublocks = max(64, min(8192, grid_for(4L * ib.nnodes, 256)));
   switch (prec) {
     case LBMIB_F64:
-      k_move<LBMIB_F64><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const double *)f, (const double *)F, c.u0x, c.u0y,
-                                                         c.u0z, c.g, reset_slot);
+      k_band_u<LBMIB_F64><<<ublocks, 256, 0, s>>>(ib, g, (const double *)f, (const double *)F, c.u0x, c.u0y, c.u0z,
+                                                  c.g, slot);
       break;
     case LBMIB_F32:
-      k_move<LBMIB_F32><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const float *)f, (const float *)F, c.u0x, c.u0y,
-                                                         c.u0z, c.g, reset_slot);
+      k_band_u<LBMIB_F32><<<ublocks, 256, 0, s>>>(ib, g, (const float *)f, (const float *)F, c.u0x, c.u0y, c.u0z,
+                                                  c.g, slot);
       break;
     case LBMIB_F16:
-      k_move<LBMIB_F16><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const half_t *)f, (const float *)F, c.u0x, c.u0y,
-                                                         c.u0z, c.g, reset_slot);
+      k_band_u<LBMIB_F16><<<ublocks, 256, 0, s>>>(ib, g, (const half_t *)f, (const float *)F, c.u0x, c.u0y, c.u0z,
+                                                  c.g, slot);
       break;
     case LBMIB_BF16:
-      k_move<LBMIB_BF16><<<blocks, kMoveThreads, 0, s>>>(ib, g, (const uint16_t *)f, (const float *)F, c.u0x,
-                                                          c.u0y, c.u0z, c.g, reset_slot);
+      k_band_u<LBMIB_BF16><<<ublocks, 256, 0, s>>>(ib, g, (const uint16_t *)f, (const float *)F, c.u0x, c.u0y,
+                                                   c.u0z, c.g, slot);
       break;
     default:
       return LBMIB_ERR_ARG;
   }
+  const int nodes_per_block = kMoveThreads / 64;
+  k_move<<<grid_for(ib.nnodes, nodes_per_block), kMoveThreads, 0, s>>>(ib, g, reset_slot);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 

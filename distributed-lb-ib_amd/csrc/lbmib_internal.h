@@ -107,9 +107,15 @@ struct IBState {
   unsigned *done;    // blocks of k_fiber finished (reset by the last one)
   int *err;
   int quirks;
+  double *ub;        // [3][n] fluid velocity on the support boxes (k_band_u -> k_move)
 };
 
 // host-side launchers (defined in lbmib_fluid.hip / lbmib_ib.hip)
+
+// k_fiber: one thread per node, kFiberThreads per block; each block leaves one
+// partial support box per sheet in IBState::box_part.
+constexpr int kFiberThreads = 64;
+inline int fiber_blocks(int nnodes) { return (nnodes + kFiberThreads - 1) / kFiberThreads; }
 
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 // Launch shape of the fused kernel (all forms compute identical bits).
