@@ -292,3 +292,26 @@ def test_split_steps_two_sheets(lbmib):
             outs.append((e.fibers(0), e.fibers(1), e.df()))
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+# ------------------------------------------------- drop-in host layer (CLI)
+
+def test_cli_replays_reference_fixture_refA(tmp_path, fixtures_meta):
+    """The reference's command line (refA flags) through the host layer with
+    the reference's per-step names (do_thread -> lbmib_* C-ABI) writes fiber
+    dumps in the reference's binary format that match the compiled
+    reference's own dumps."""
+    import subprocess
+    meta = fixtures_meta["refA"]
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "distributed-lb-ib_amd", "lbmib")
+    out = subprocess.run([os.path.abspath(exe)] + meta["args"] + ["-dump_fiber"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    print(out.stdout[-400:], out.stderr[-400:])
+    assert out.returncode == 0
+    assert "TOTAL TIME TAKEN IN Seconds:" in out.stdout
+    for step in meta["fiber_steps"]:
+        ij_got, got = read_fiber_dump(os.path.join(tmp_path, f"Fiber1_dump_step{step}.bin"))
+        ij_ref, ref = read_fiber_dump(os.path.join(GOLDEN, f"refA_fiber_step{step}.bin"))
+        assert np.array_equal(ij_got, ij_ref)
+        assert np.abs(got[:, :3] - ref[:, :3]).max() <= 1e-12
+        assert np.abs(got[:, 3:] - ref[:, 3:]).max() <= 1e-9 * max(np.abs(ref[:, 3:]).max(), 1e-12) + 1e-20
