@@ -120,10 +120,10 @@ def cpu_reference_baseline(n, rows, steps_pair):
 
 
 # (cells per thread, nontemporal mask: 1 stores / 2 loads, blocks per CU; 0 = one block per row)
-VARIANTS = {"f64": [(1, 1, 0), (1, 3, 0), (1, 3, 4), (1, 3, 8), (1, 1, 4), (2, 1, 0)],
-            "f32": [(1, 1, 0), (1, 3, 0), (1, 3, 2), (1, 3, 4), (1, 3, 8), (1, 1, 4), (4, 1, 0)],
-            "f16": [(8, 1, 0), (4, 1, 0), (1, 1, 0), (1, 3, 0), (1, 3, 4), (1, 3, 8)],
-            "bf16": [(8, 1, 0), (4, 1, 0), (1, 1, 0), (1, 3, 0), (1, 3, 4), (1, 3, 8)]}
+VARIANTS = {"f64": [(1, 3, 0), (1, 1, 0), (2, 3, 0), (1, 3, 4)],
+            "f32": [(1, 3, 0), (1, 1, 0), (2, 3, 0), (2, 1, 0), (1, 3, 4), (2, 3, 4)],
+            "f16": [(1, 3, 0), (1, 1, 0), (2, 3, 0), (2, 1, 0), (1, 3, 4), (2, 3, 4)],
+            "bf16": [(1, 3, 0), (1, 1, 0), (2, 3, 0), (2, 1, 0), (1, 3, 4), (2, 3, 4)]}
 
 
 def autotune(e, a, lbmib):

@@ -352,6 +352,130 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// Two consecutive cells of a row per lane (the buffer-addressed path with
+// loads and stores twice as wide): lane t updates x = 2t and 2t + 1.  A pair
+// load of plane q reads the two sources (x - c_x, x + 1 - c_x), which are
+// adjacent; for c_x != 0 it is one element off the pair alignment, which the
+// memory system handles as an unaligned access.  Requires nx even, the
+// active x range [lo, hi] with lo even and hi odd (so no pair straddles it),
+// and blockDim.x >= nx / 2 (nx <= 1024).  Edge pairs of a wall or periodic x axis compute
+// their edge cell with the in-row neighbour, and k_collide_xedges, launched
+// after this kernel on the same stream, rewrites that cell.
+template <int P>
+struct PairBits;
+template <>
+struct PairBits<LBMIB_F64> { typedef unsigned int type __attribute__((ext_vector_type(4))); };
+template <>
+struct PairBits<LBMIB_F32> { typedef unsigned int type __attribute__((ext_vector_type(2))); };
+template <>
+struct PairBits<LBMIB_F16> { typedef unsigned int type; };
+template <>
+struct PairBits<LBMIB_BF16> { typedef unsigned int type; };
+
+template <int P, bool NTL>
+__device__ __forceinline__ void bld_pair(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                         typename Traits<P>::T &a, typename Traits<P>::T &b) {
+  using S = typename Traits<P>::S;
+  constexpr int aux = NTL ? 2 : 0;
+  struct Two { S s[2]; };
+  Two t;
+  if constexpr (P == LBMIB_F64) t = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, aux));
+  else if constexpr (P == LBMIB_F32) t = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, aux));
+  else t = __builtin_bit_cast(Two, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, aux));
+  a = from_bits<P>(t.s[0]);
+  b = from_bits<P>(t.s[1]);
+}
+template <int P, bool NT>
+__device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                         typename Traits<P>::T a, typename Traits<P>::T b) {
+  using S = typename Traits<P>::S;
+  constexpr int aux = NT ? 2 : 0;
+  struct Two { S s[2]; };
+  Two t;
+  t.s[0] = to_bits<P>(a);
+  t.s[1] = to_bits<P>(b);
+  const typename PairBits<P>::type u = __builtin_bit_cast(typename PairBits<P>::type, t);
+  if constexpr (P == LBMIB_F64) __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, aux);
+  else if constexpr (P == LBMIB_F32) __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, aux);
+  else __builtin_amdgcn_raw_buffer_store_b32(u, r, voff, soff, aux);
+}
+
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(512)
+    k_collide_stream_pairs(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                           typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
+                           int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                           const typename Traits<P>::FT *__restrict__ Fcur,
+                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab) {
+  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
+  const int nrows = g.ny * g.nz;
+  const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
+  constexpr uint32_t es = sizeof(S);
+  const uint32_t pitch_b = (uint32_t)g.pitch * es;
+  const uint32_t sy_b = (uint32_t)g.nx * es;
+  const int x0 = 2 * (int)threadIdx.x;
+  if (x0 >= g.nx || x0 + 1 < g.lo[0] || x0 > g.hi[0]) return;
+  const bool inout_x = g.bc[0] == LBMIB_BC_INOUT;
+  const bool frozen[2] = {inout_x && x0 == g.lo[0], inout_x && x0 + 1 == g.hi[0]};
+  const uint32_t vo = (uint32_t)x0 * es;
+  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int y = row % g.ny, z = row / g.ny;
+    if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
+    const uint32_t row_b = (uint32_t)row * sy_b;
+    const long c0 = (long)x0 + (long)g.nx * row;
+    T v[2][19];
+    T d[2], rho[2], ux[2], uy[2], uz[2];
+    if (own) {
+      uint32_t qb = row_b;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        bld_pair<P, NTL>(rin, vo, qb, v[0][q], v[1][q]);
+        qb += pitch_b;
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+      }
+    } else {
+      const uint32_t *rt = rowtab + (long)row * kRowTabStride;
+      uint32_t so[19];
+#pragma unroll
+      for (int q = 0; q < 19; ++q) so[q] = rt[q];
+#pragma unroll
+      for (int q = 0; q < 19; ++q) bld_pair<P, NTL>(rin, vo, so[q], v[0][q], v[1][q]);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        T fpx = 0, fpy = 0, fpz = 0;
+        if (in_any_box(box_prev, nsheets, x0 + k, y, z)) {
+          fpx = (T)Fprev[c0 + k]; fpy = (T)Fprev[g.n + c0 + k]; fpz = (T)Fprev[2 * g.n + c0 + k];
+        }
+        lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (frozen[k]) {
+        // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
+        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+      }
+      T fx = 0, fy = 0, fz = 0;
+      if (in_any_box(box_cur, nsheets, x0 + k, y, z)) {
+        fx = (T)Fcur[c0 + k]; fy = (T)Fcur[g.n + c0 + k]; fz = (T)Fcur[2 * g.n + c0 + k];
+      }
+      collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
+    }
+    uint32_t sb = row_b;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      bst_pair<P, NT>(rout, vo, sb, v[0][q], v[1][q]);
+      sb += pitch_b;
+    }
+  }
+}
+
 // The x-edge cells (x = lo, hi) of every active row when x is a wall or a
 // periodic axis, by the general rules; completes the buffer-addressed rows.
 template <int P, bool NT, bool NTL>
@@ -703,6 +827,30 @@ bool row_path_buffered(const Geometry &g, size_t es) {
   return 19ull * (unsigned long long)g.pitch * es <= 0xFFFFFFFFull && (int)row_block(g).x >= g.nx;
 }
 
+// Whether the two-cells-per-lane row path applies (see k_collide_stream_pairs).
+static bool pair_path_ok(const Geometry &g) {
+  return g.nx % 2 == 0 && g.lo[0] % 2 == 0 && g.hi[0] % 2 == 1 && g.nx / 2 <= 512;
+}
+
+template <int P, bool NT, bool NTL>
+static void launch_pairs(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
+                         const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
+  using FT = typename Traits<P>::FT;
+  using S = typename Traits<P>::S;
+  const int nrows = g.ny * g.nz;
+  const dim3 b((unsigned)(((g.nx / 2) + 63) / 64 * 64));
+  const dim3 gr(nblocks < nrows ? nblocks : nrows);
+  k_collide_stream_pairs<P, NT, NTL><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
+                                                      box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab);
+  if (g.bc[0] != LBMIB_BC_INOUT) {
+    const long items = 2L * g.ny * g.nz;
+    k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
+        g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
+        (const FT *)Fprev);
+  }
+}
+
 template <int P, bool NT, bool NTL>
 static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                         const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
@@ -752,6 +900,16 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
   // once per step, and keeping them out of the caches was fastest measured
   const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
   const int ntl = kv.nt_mask < 0 ? 1 : ((kv.nt_mask >> 1) & 1);
+  if (kv.cells_per_thread == 2 && rowtab && pair_path_ok(g)) {
+    const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
+    LBMIB_DISPATCH(prec, ({
+      if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else if (nt) launch_pairs<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else if (ntl) launch_pairs<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else launch_pairs<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+    }));
+    return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+  }
   const int v = collide_vec_width(prec, g.nx, kv.cells_per_thread);
   if (v == 1) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
