@@ -887,9 +887,15 @@ extern "C" int lbmib_set_kernel_variant(lbmib_ctx *ctx, int cells_per_thread, in
   if (!ctx || cells_per_thread < 0 || cells_per_thread > 8 || nontemporal < -1 || nontemporal > 3 ||
       blocks_per_cu < 0 || blocks_per_cu > 64)
     return LBMIB_ERR_ARG;
-  if (cells_per_thread > 1 && collide_vec_width(ctx->fp.precision, ctx->g.nx, cells_per_thread) != cells_per_thread)
-    return LBMIB_ERR_ARG;  // not available for this precision / grid
-  if (cells_per_thread > 1 && (blocks_per_cu > 0 || (nontemporal > 0 && (nontemporal & 2)))) return LBMIB_ERR_ARG;
+  const bool pairs = cells_per_thread == 2 && ctx->d_rowtab && pair_path_ok(ctx->g);
+  const bool rows2 = cells_per_thread == 3 && ctx->d_rowtab && rows2_path_ok(ctx->g);
+  if (cells_per_thread == 3 && !rows2) return LBMIB_ERR_ARG;
+  if (cells_per_thread > 1 && !pairs && !rows2) {
+    // the vector-per-lane kernel: only where its width fits the precision
+    // and the row, one block per row, cached loads
+    if (collide_vec_width(ctx->fp.precision, ctx->g.nx, cells_per_thread) != cells_per_thread) return LBMIB_ERR_ARG;
+    if (blocks_per_cu > 0 || (nontemporal > 0 && (nontemporal & 2))) return LBMIB_ERR_ARG;
+  }
   ctx->kv.cells_per_thread = cells_per_thread;
   ctx->kv.nt_mask = nontemporal;
   ctx->kv.blocks = blocks_per_cu * ctx->ncu;

@@ -476,6 +476,102 @@ __global__ void __launch_bounds__(512)
   }
 }
 
+// Two rows per block (y and y + 1 of one z plane): lane x updates (x, y, z)
+// and (x, y + 1, z) with two independent sets of aligned buffer loads, which
+// doubles the bytes each wave keeps in flight without the unaligned pair
+// loads of k_collide_stream_pairs.  Requires ny even and blockDim.x >= nx;
+// x-edge cells of a wall or periodic x axis are left to k_collide_xedges.
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(512)
+    k_collide_stream_rows2(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                           typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
+                           int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                           const typename Traits<P>::FT *__restrict__ Fcur,
+                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab) {
+  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
+  const int npairs = g.ny * g.nz / 2;
+  const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
+  constexpr uint32_t es = sizeof(S);
+  const uint32_t pitch_b = (uint32_t)g.pitch * es;
+  const uint32_t sy_b = (uint32_t)g.nx * es;
+  const int x = threadIdx.x;
+  const bool edge = x == g.lo[0] || x == g.hi[0];
+  const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
+  if (x < g.lo[0] || x > g.hi[0] || (edge && !frozen)) return;
+  const uint32_t vo = (uint32_t)x * es;
+  for (int rp = blockIdx.x; rp < npairs; rp += gridDim.x) {
+    const int row0 = 2 * rp;
+    const int y0 = row0 % g.ny, z = row0 / g.ny;  // rows row0, row0 + 1 = (y0, z), (y0 + 1, z)
+    if (z < g.lo[2] || z > g.hi[2]) continue;  // dead rows
+    bool live[2];
+    T v[2][19];
+    T d[2], rho[2], ux[2], uy[2], uz[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int y = y0 + k;
+      live[k] = y >= g.lo[1] && y <= g.hi[1];
+    }
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        uint32_t qb = (uint32_t)(row0 + k) * sy_b;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+          v[k][q] = bld_h<P, NTL>(rin, vo, qb);
+          qb += pitch_b;
+        }
+        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+      }
+    } else {
+      const uint32_t *rt = rowtab + (long)row0 * kRowTabStride;
+      uint32_t so[2][19];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) so[k][q] = rt[k * kRowTabStride + q];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[k][q] = bld_h<P, NTL>(rin, vo, so[k][q]);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const long ck = (long)x + (long)g.nx * (row0 + k);
+        T fpx = 0, fpy = 0, fpz = 0;
+        if (in_any_box(box_prev, nsheets, x, y0 + k, z)) {
+          fpx = (T)Fprev[ck]; fpy = (T)Fprev[g.n + ck]; fpz = (T)Fprev[2 * g.n + ck];
+        }
+        lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const long ck = (long)x + (long)g.nx * (row0 + k);
+      if (frozen) {
+        // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
+        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+      }
+      T fx = 0, fy = 0, fz = 0;
+      if (in_any_box(box_cur, nsheets, x, y0 + k, z)) {
+        fx = (T)Fcur[ck]; fy = (T)Fcur[g.n + ck]; fz = (T)Fcur[2 * g.n + ck];
+      }
+      collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!live[k]) continue;  // a dead row of a wall y axis
+      uint32_t sb = (uint32_t)(row0 + k) * sy_b;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        bst_h<P, NT>(rout, vo, sb, v[k][q]);
+        sb += pitch_b;
+      }
+    }
+  }
+}
+
 // The x-edge cells (x = lo, hi) of every active row when x is a wall or a
 // periodic axis, by the general rules; completes the buffer-addressed rows.
 template <int P, bool NT, bool NTL>
@@ -827,8 +923,30 @@ bool row_path_buffered(const Geometry &g, size_t es) {
   return 19ull * (unsigned long long)g.pitch * es <= 0xFFFFFFFFull && (int)row_block(g).x >= g.nx;
 }
 
+template <int P, bool NT, bool NTL>
+static void launch_rows2(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
+                         const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
+  using FT = typename Traits<P>::FT;
+  using S = typename Traits<P>::S;
+  const int npairs = g.ny * g.nz / 2;
+  const dim3 b((unsigned)((g.nx + 63) / 64 * 64));
+  const dim3 gr(nblocks < npairs ? nblocks : npairs);
+  k_collide_stream_rows2<P, NT, NTL><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
+                                                      box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab);
+  if (g.bc[0] != LBMIB_BC_INOUT) {
+    const long items = 2L * g.ny * g.nz;
+    k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
+        g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
+        (const FT *)Fprev);
+  }
+}
+
+// Whether the two-rows-per-block path applies (see k_collide_stream_rows2).
+bool rows2_path_ok(const Geometry &g) { return g.ny % 2 == 0 && g.nx <= 512; }
+
 // Whether the two-cells-per-lane row path applies (see k_collide_stream_pairs).
-static bool pair_path_ok(const Geometry &g) {
+bool pair_path_ok(const Geometry &g) {
   return g.nx % 2 == 0 && g.lo[0] % 2 == 0 && g.hi[0] % 2 == 1 && g.nx / 2 <= 512;
 }
 
@@ -900,6 +1018,16 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
   // once per step, and keeping them out of the caches was fastest measured
   const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
   const int ntl = kv.nt_mask < 0 ? 1 : ((kv.nt_mask >> 1) & 1);
+  if (kv.cells_per_thread == 3 && rowtab && rows2_path_ok(g)) {
+    const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz / 2;
+    LBMIB_DISPATCH(prec, ({
+      if (nt && ntl) launch_rows2<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else if (nt) launch_rows2<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else if (ntl) launch_rows2<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      else launch_rows2<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+    }));
+    return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+  }
   if (kv.cells_per_thread == 2 && rowtab && pair_path_ok(g)) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({

@@ -120,7 +120,7 @@ inline int fiber_blocks(int nnodes) { return (nnodes + kFiberThreads - 1) / kFib
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 // Launch shape of the fused kernel (all forms compute identical bits).
 struct KernelVariant {
-  int cells_per_thread;  // 0 automatic, 1, 2, 4, 8
+  int cells_per_thread;  // 0 automatic, 1, 2 (x pairs), 3 (two rows per block), 4, 8 (x vectors)
   int nt_mask;           // -1 automatic; bit 0 nontemporal stores, bit 1 nontemporal loads
   int blocks;            // > 0: persistent grid of this many blocks walking rows (one cell per thread)
 };
@@ -132,6 +132,8 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
                           hipStream_t s);
 bool row_path_buffered(const Geometry &g, size_t es);
+bool pair_path_ok(const Geometry &g);
+bool rows2_path_ok(const Geometry &g);
 int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab);
 int collide_vec_width(int prec, int nx, int vec_req);
 int launch_materialize(int prec, const Geometry &g, const void *f, int own_mode, const FluidConsts<double> &c,

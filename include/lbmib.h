@@ -152,13 +152,19 @@ int lbmib_get_band_box(lbmib_ctx *ctx, int sheet, int *box6);
  * of the fused collide-stream kernel (timed in a separate pass of the same
  * length with one event pair per launch). */
 int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *ms_total, double *ms_main_mean);
-/* Launch shape of the fused collide-stream kernel: cells per thread along x
- * (1; 2 for F64/F32; 4 for F32/16-bit; 8 for 16-bit -- at most 16 bytes per
- * lane; 0 = automatic), nontemporal mask (bit 0 stores, bit 1 loads; -1 =
- * automatic) and, for one cell per thread, blocks_per_cu > 0 for a
- * persistent grid of that many blocks per compute unit walking the rows
- * (0 = one block per row).  Not in the reference: the knobs bench.py's
- * autotuning sweep sets.  Every variant computes bit-identical results. */
+/* Launch shape of the fused collide-stream kernel.  cells_per_thread:
+ *   0 automatic (= 1);
+ *   1 one cell per lane, one block per (y, z) row (or a persistent grid);
+ *   2 two x-adjacent cells per lane (needs nx even; wall/periodic x edges
+ *     then run in a small extra kernel);
+ *   3 two rows (y, y+1) per block, one cell of each per lane (needs ny even);
+ *   4, 8 x-vectors per lane (16 bytes per lane at most; one block per row,
+ *     cached loads).
+ * nontemporal: bit 0 stores, bit 1 loads; -1 automatic (both).
+ * blocks_per_cu > 0: a persistent grid of that many blocks per compute unit
+ * walking the rows (0 = one block per row or row pair).  Not in the
+ * reference: the knobs bench.py's autotuning sweep sets.  Every variant
+ * computes bit-identical results. */
 int lbmib_set_kernel_variant(lbmib_ctx *ctx, int cells_per_thread, int nontemporal, int blocks_per_cu);
 /* Bytes per cell update of the population storage (2 * 19 * sizeof). */
 int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes);

@@ -209,11 +209,11 @@ def test_fiber_leaving_grid_reports_error(lbmib):
 
 
 VARIANTS = {  # (cells per thread, nontemporal mask, blocks per CU)
-    "f64": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 3, 4), (1, 3, 0), (1, 3, 4), (1, 1, 8)],
-    "f32": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 3, 0), (2, 3, 4), (4, 1, 0), (1, 3, 0), (1, 3, 4), (1, 1, 8),
-            (1, 2, 2)],
-    "f16": [(1, 1, 0), (2, 3, 0), (2, 0, 2), (4, 1, 0), (8, 1, 0), (8, 0, 0), (1, 3, 4)],
-    "bf16": [(1, 1, 0), (2, 3, 0), (2, 1, 4), (4, 1, 0), (8, 1, 0), (1, 3, 4)],
+    "f64": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 3, 4), (3, 3, 0), (1, 3, 0), (1, 3, 4), (1, 1, 8)],
+    "f32": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 3, 0), (2, 3, 4), (3, 3, 0), (3, 1, 2), (4, 1, 0), (1, 3, 0),
+            (1, 3, 4), (1, 1, 8), (1, 2, 2)],
+    "f16": [(1, 1, 0), (2, 3, 0), (2, 0, 2), (3, 3, 0), (4, 1, 0), (8, 1, 0), (8, 0, 0), (1, 3, 4)],
+    "bf16": [(1, 1, 0), (2, 3, 0), (2, 1, 4), (3, 3, 0), (3, 3, 4), (4, 1, 0), (8, 1, 0), (1, 3, 4)],
 }
 
 
@@ -242,12 +242,12 @@ def test_kernel_variants_are_bitwise_identical(lbmib, precision):
                                 (BC_INOUT, BC_WALL, BC_WALL)])
 @pytest.mark.parametrize("precision", ["f32", "bf16"])
 def test_row_and_pair_paths_agree_on_every_boundary(lbmib, bc, precision):
-    """The one- and two-cells-per-lane row kernels, one block per row or
-    persistent, with the x-edge kernel completing wall / periodic x rows,
-    give the same bits on every boundary combination."""
+    """The row kernels -- one cell per lane, x pairs, two rows per block; one
+    block per row or persistent -- with the x-edge kernel completing wall /
+    periodic x rows, give the same bits on every boundary combination."""
     kw = small_ib_config(lbmib, precision, bc=bc, n=(62, 30, 34))
     outs = []
-    for v in ((1, 3, 0), (2, 3, 0), (2, 1, 3)):
+    for v in ((1, 3, 0), (2, 3, 0), (2, 1, 3), (3, 3, 0), (3, 1, 2)):
         with lbmib.Engine(**kw) as e:
             e.set_kernel_variant(*v)
             e.init()
