@@ -60,11 +60,20 @@ __device__ __forceinline__ typename Traits<P>::T from_bits(typename Traits<P>::S
   else if constexpr (P == LBMIB_BF16) return bf16_to_f32(s);
   else return s;
 }
+// The 16-bit forms round the FP32 value h once more to 16 bits.  The empty
+// asm pins h as an FP32 register value, so the compiler cannot fold the
+// conversion into the FMA that produced h (v_fma_mix rounds the exact
+// product-sum to 16 bits directly): every kernel shape then stores the same
+// bits.
 template <int P>
 __device__ __forceinline__ typename Traits<P>::S to_bits(typename Traits<P>::T h) {
-  if constexpr (P == LBMIB_F16) return (half_t)h;
-  else if constexpr (P == LBMIB_BF16) return f32_to_bf16(h);
-  else return h;
+  if constexpr (P == LBMIB_F16 || P == LBMIB_BF16) {
+    __asm__ volatile("" : "+v"(h));
+    if constexpr (P == LBMIB_F16) return (half_t)h;
+    else return f32_to_bf16(h);
+  } else {
+    return h;
+  }
 }
 template <int P, bool NTL = false>
 __device__ __forceinline__ typename Traits<P>::T ld_h(const typename Traits<P>::S *p) {

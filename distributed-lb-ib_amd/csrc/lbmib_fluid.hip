@@ -293,10 +293,15 @@ __global__ void __launch_bounds__(1024)
     const uint32_t pitch_b = (uint32_t)g.pitch * es;
     const uint32_t sy_b = (uint32_t)g.nx * es;
     const int x = threadIdx.x;
+    if (x >= g.nx) return;
     const bool edge = x == g.lo[0] || x == g.hi[0];
     // frozen inlet/outlet planes; x edges of wall/periodic x go to k_collide_xedges
     const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
-    if (x < g.lo[0] || x > g.hi[0] || (edge && !frozen)) return;
+    // Filler lanes (dead x, and the x edges left to k_collide_xedges) store
+    // zeros, so that every store instruction writes whole cache lines of the
+    // row: partially written lines cost HBM write bandwidth.  Nothing reads
+    // a dead cell; k_collide_xedges rewrites its edge cells afterwards.
+    const bool filler = x < g.lo[0] || x > g.hi[0] || (edge && !frozen);
     const uint32_t vo = (uint32_t)x * es;
     for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
       const int y = row % g.ny, z = row / g.ny;
@@ -304,6 +309,10 @@ __global__ void __launch_bounds__(1024)
       const uint32_t row_b = (uint32_t)row * sy_b;  // byte offset of (0, y, z) in a plane
       const long c0 = (long)x + (long)g.nx * row;
       T v[19];
+      if (filler) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[q] = (T)0;
+      } else {
       T d, rho, ux, uy, uz;
       if (frozen) {
 #pragma unroll
@@ -342,6 +351,7 @@ __global__ void __launch_bounds__(1024)
         fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
       }
       collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
+      }
       uint32_t sb = row_b;
 #pragma unroll
       for (int q = 0; q < 19; ++q) {
@@ -415,7 +425,9 @@ __global__ void __launch_bounds__(512)
   const uint32_t pitch_b = (uint32_t)g.pitch * es;
   const uint32_t sy_b = (uint32_t)g.nx * es;
   const int x0 = 2 * (int)threadIdx.x;
-  if (x0 >= g.nx || x0 + 1 < g.lo[0] || x0 > g.hi[0]) return;
+  if (x0 >= g.nx) return;
+  // dead pairs store zeros so that the row's cache lines are written whole
+  const bool filler = x0 + 1 < g.lo[0] || x0 > g.hi[0];
   const bool inout_x = g.bc[0] == LBMIB_BC_INOUT;
   const bool frozen[2] = {inout_x && x0 == g.lo[0], inout_x && x0 + 1 == g.hi[0]};
   const uint32_t vo = (uint32_t)x0 * es;
@@ -425,6 +437,10 @@ __global__ void __launch_bounds__(512)
     const uint32_t row_b = (uint32_t)row * sy_b;
     const long c0 = (long)x0 + (long)g.nx * row;
     T v[2][19];
+    if (filler) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) v[0][q] = v[1][q] = (T)0;
+    } else {
     T d[2], rho[2], ux[2], uy[2], uz[2];
     if (own) {
       uint32_t qb = row_b;
@@ -467,6 +483,7 @@ __global__ void __launch_bounds__(512)
       }
       collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
     }
+    }
     uint32_t sb = row_b;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
@@ -496,9 +513,11 @@ __global__ void __launch_bounds__(512)
   const uint32_t pitch_b = (uint32_t)g.pitch * es;
   const uint32_t sy_b = (uint32_t)g.nx * es;
   const int x = threadIdx.x;
+  if (x >= g.nx) return;
   const bool edge = x == g.lo[0] || x == g.hi[0];
   const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
-  if (x < g.lo[0] || x > g.hi[0] || (edge && !frozen)) return;
+  // filler lanes store zeros so that the rows' cache lines are written whole
+  const bool filler = x < g.lo[0] || x > g.hi[0] || (edge && !frozen);
   const uint32_t vo = (uint32_t)x * es;
   for (int rp = blockIdx.x; rp < npairs; rp += gridDim.x) {
     const int row0 = 2 * rp;
@@ -506,12 +525,16 @@ __global__ void __launch_bounds__(512)
     if (z < g.lo[2] || z > g.hi[2]) continue;  // dead rows
     bool live[2];
     T v[2][19];
-    T d[2], rho[2], ux[2], uy[2], uz[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int y = y0 + k;
       live[k] = y >= g.lo[1] && y <= g.hi[1];
     }
+    if (filler) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) v[0][q] = v[1][q] = (T)0;
+    } else {
+    T d[2], rho[2], ux[2], uy[2], uz[2];
     if (own) {
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
@@ -558,6 +581,7 @@ __global__ void __launch_bounds__(512)
         fx = (T)Fcur[ck]; fy = (T)Fcur[g.n + ck]; fz = (T)Fcur[2 * g.n + ck];
       }
       collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
+    }
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
