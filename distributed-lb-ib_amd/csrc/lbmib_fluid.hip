@@ -313,44 +313,44 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
         for (int q = 0; q < 19; ++q) v[q] = (T)0;
       } else {
-      T d, rho, ux, uy, uz;
-      if (frozen) {
+        T d, rho, ux, uy, uz;
+        if (frozen) {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
-        rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
-      } else {
-        // per-row source offsets (row_source_table): wrap and bounce-back
-        // of the y/z axes and the x shift folded in
-        if (own) {
-          uint32_t qb = row_b;  // first step: the cell's own populations
-#pragma unroll
-          for (int q = 0; q < 19; ++q) {
-            v[q] = bld_h<P, NTL>(rin, vo, qb);
-            qb += pitch_b;
-          }
+          for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
+          rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
         } else {
-          const uint32_t *rt = rowtab + (long)row * kRowTabStride;
-          uint32_t so[19];
+          // per-row source offsets (row_source_table): wrap and bounce-back
+          // of the y/z axes and the x shift folded in
+          if (own) {
+            uint32_t qb = row_b;  // first step: the cell's own populations
 #pragma unroll
-          for (int q = 0; q < 19; ++q) so[q] = rt[q];
+            for (int q = 0; q < 19; ++q) {
+              v[q] = bld_h<P, NTL>(rin, vo, qb);
+              qb += pitch_b;
+            }
+          } else {
+            const uint32_t *rt = rowtab + (long)row * kRowTabStride;
+            uint32_t so[19];
 #pragma unroll
-          for (int q = 0; q < 19; ++q) v[q] = bld_h<P, NTL>(rin, vo, so[q]);
-        }
-        if (own) {
-          rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
-        } else {
-          T fpx = 0, fpy = 0, fpz = 0;
-          if (in_any_box(box_prev, nsheets, x, y, z)) {
-            fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
+            for (int q = 0; q < 19; ++q) so[q] = rt[q];
+#pragma unroll
+            for (int q = 0; q < 19; ++q) v[q] = bld_h<P, NTL>(rin, vo, so[q]);
           }
-          lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
+          if (own) {
+            rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
+          } else {
+            T fpx = 0, fpy = 0, fpz = 0;
+            if (in_any_box(box_prev, nsheets, x, y, z)) {
+              fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
+            }
+            lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
+          }
         }
-      }
-      T fx = 0, fy = 0, fz = 0;
-      if (in_any_box(box_cur, nsheets, x, y, z)) {
-        fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
-      }
-      collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
+        T fx = 0, fy = 0, fz = 0;
+        if (in_any_box(box_cur, nsheets, x, y, z)) {
+          fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
+        }
+        collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
       }
       uint32_t sb = row_b;
 #pragma unroll
@@ -441,48 +441,48 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
       for (int q = 0; q < 19; ++q) v[0][q] = v[1][q] = (T)0;
     } else {
-    T d[2], rho[2], ux[2], uy[2], uz[2];
-    if (own) {
-      uint32_t qb = row_b;
+      T d[2], rho[2], ux[2], uy[2], uz[2];
+      if (own) {
+        uint32_t qb = row_b;
 #pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        bld_pair<P, NTL>(rin, vo, qb, v[0][q], v[1][q]);
-        qb += pitch_b;
-      }
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
-      }
-    } else {
-      const uint32_t *rt = rowtab + (long)row * kRowTabStride;
-      uint32_t so[19];
-#pragma unroll
-      for (int q = 0; q < 19; ++q) so[q] = rt[q];
-#pragma unroll
-      for (int q = 0; q < 19; ++q) bld_pair<P, NTL>(rin, vo, so[q], v[0][q], v[1][q]);
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        T fpx = 0, fpy = 0, fpz = 0;
-        if (in_any_box(box_prev, nsheets, x0 + k, y, z)) {
-          fpx = (T)Fprev[c0 + k]; fpy = (T)Fprev[g.n + c0 + k]; fpz = (T)Fprev[2 * g.n + c0 + k];
+        for (int q = 0; q < 19; ++q) {
+          bld_pair<P, NTL>(rin, vo, qb, v[0][q], v[1][q]);
+          qb += pitch_b;
         }
-        lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
-      }
-    }
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      if (frozen[k]) {
-        // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
+        for (int k = 0; k < 2; ++k) {
+          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+        }
+      } else {
+        const uint32_t *rt = rowtab + (long)row * kRowTabStride;
+        uint32_t so[19];
 #pragma unroll
-        for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
-        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+        for (int q = 0; q < 19; ++q) so[q] = rt[q];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) bld_pair<P, NTL>(rin, vo, so[q], v[0][q], v[1][q]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          T fpx = 0, fpy = 0, fpz = 0;
+          if (in_any_box(box_prev, nsheets, x0 + k, y, z)) {
+            fpx = (T)Fprev[c0 + k]; fpy = (T)Fprev[g.n + c0 + k]; fpz = (T)Fprev[2 * g.n + c0 + k];
+          }
+          lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
+        }
       }
-      T fx = 0, fy = 0, fz = 0;
-      if (in_any_box(box_cur, nsheets, x0 + k, y, z)) {
-        fx = (T)Fcur[c0 + k]; fy = (T)Fcur[g.n + c0 + k]; fz = (T)Fcur[2 * g.n + c0 + k];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (frozen[k]) {
+          // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
+#pragma unroll
+          for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
+          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+        }
+        T fx = 0, fy = 0, fz = 0;
+        if (in_any_box(box_cur, nsheets, x0 + k, y, z)) {
+          fx = (T)Fcur[c0 + k]; fy = (T)Fcur[g.n + c0 + k]; fz = (T)Fcur[2 * g.n + c0 + k];
+        }
+        collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
       }
-      collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
-    }
     }
     uint32_t sb = row_b;
 #pragma unroll
@@ -534,54 +534,54 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
       for (int q = 0; q < 19; ++q) v[0][q] = v[1][q] = (T)0;
     } else {
-    T d[2], rho[2], ux[2], uy[2], uz[2];
-    if (own) {
+      T d[2], rho[2], ux[2], uy[2], uz[2];
+      if (own) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        uint32_t qb = (uint32_t)(row0 + k) * sy_b;
+        for (int k = 0; k < 2; ++k) {
+          uint32_t qb = (uint32_t)(row0 + k) * sy_b;
 #pragma unroll
-        for (int q = 0; q < 19; ++q) {
-          v[k][q] = bld_h<P, NTL>(rin, vo, qb);
-          qb += pitch_b;
+          for (int q = 0; q < 19; ++q) {
+            v[k][q] = bld_h<P, NTL>(rin, vo, qb);
+            qb += pitch_b;
+          }
+          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
-        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+      } else {
+        const uint32_t *rt = rowtab + (long)row0 * kRowTabStride;
+        uint32_t so[2][19];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int q = 0; q < 19; ++q) so[k][q] = rt[k * kRowTabStride + q];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int q = 0; q < 19; ++q) v[k][q] = bld_h<P, NTL>(rin, vo, so[k][q]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const long ck = (long)x + (long)g.nx * (row0 + k);
+          T fpx = 0, fpy = 0, fpz = 0;
+          if (in_any_box(box_prev, nsheets, x, y0 + k, z)) {
+            fpx = (T)Fprev[ck]; fpy = (T)Fprev[g.n + ck]; fpz = (T)Fprev[2 * g.n + ck];
+          }
+          lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
+        }
       }
-    } else {
-      const uint32_t *rt = rowtab + (long)row0 * kRowTabStride;
-      uint32_t so[2][19];
-#pragma unroll
-      for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int q = 0; q < 19; ++q) so[k][q] = rt[k * kRowTabStride + q];
-#pragma unroll
-      for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int q = 0; q < 19; ++q) v[k][q] = bld_h<P, NTL>(rin, vo, so[k][q]);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const long ck = (long)x + (long)g.nx * (row0 + k);
-        T fpx = 0, fpy = 0, fpz = 0;
-        if (in_any_box(box_prev, nsheets, x, y0 + k, z)) {
-          fpx = (T)Fprev[ck]; fpy = (T)Fprev[g.n + ck]; fpz = (T)Fprev[2 * g.n + ck];
+        if (frozen) {
+          // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
+#pragma unroll
+          for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
+          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
-        lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
+        T fx = 0, fy = 0, fz = 0;
+        if (in_any_box(box_cur, nsheets, x, y0 + k, z)) {
+          fx = (T)Fcur[ck]; fy = (T)Fcur[g.n + ck]; fz = (T)Fcur[2 * g.n + ck];
+        }
+        collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
       }
-    }
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const long ck = (long)x + (long)g.nx * (row0 + k);
-      if (frozen) {
-        // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
-#pragma unroll
-        for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
-        rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
-      }
-      T fx = 0, fy = 0, fz = 0;
-      if (in_any_box(box_cur, nsheets, x, y0 + k, z)) {
-        fx = (T)Fcur[ck]; fy = (T)Fcur[g.n + ck]; fz = (T)Fcur[2 * g.n + ck];
-      }
-      collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
-    }
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -1042,7 +1042,11 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
   // once per step, and keeping them out of the caches was fastest measured
   const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
   const int ntl = kv.nt_mask < 0 ? 1 : ((kv.nt_mask >> 1) & 1);
-  if (kv.cells_per_thread == 3 && rowtab && rows2_path_ok(g)) {
+  // automatic shape: x pairs for the 16-bit forms (4-byte lanes), one cell
+  // per lane otherwise -- the fastest measured at 256^3
+  int cpt = kv.cells_per_thread;
+  if (cpt <= 0) cpt = ((prec == LBMIB_F16 || prec == LBMIB_BF16) && rowtab && pair_path_ok(g)) ? 2 : 1;
+  if (cpt == 3 && rowtab && rows2_path_ok(g)) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz / 2;
     LBMIB_DISPATCH(prec, ({
       if (nt && ntl) launch_rows2<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
@@ -1052,7 +1056,7 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
-  if (kv.cells_per_thread == 2 && rowtab && pair_path_ok(g)) {
+  if (cpt == 2 && rowtab && pair_path_ok(g)) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({
       if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
@@ -1062,7 +1066,7 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
-  const int v = collide_vec_width(prec, g.nx, kv.cells_per_thread);
+  const int v = collide_vec_width(prec, g.nx, cpt);
   if (v == 1) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({

@@ -153,7 +153,7 @@ int lbmib_get_band_box(lbmib_ctx *ctx, int sheet, int *box6);
  * length with one event pair per launch). */
 int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *ms_total, double *ms_main_mean);
 /* Launch shape of the fused collide-stream kernel.  cells_per_thread:
- *   0 automatic (= 1);
+ *   0 automatic (2 for the 16-bit forms where it applies, else 1);
  *   1 one cell per lane, one block per (y, z) row (or a persistent grid);
  *   2 two x-adjacent cells per lane (needs nx even; wall/periodic x edges
  *     then run in a small extra kernel);
