@@ -41,6 +41,8 @@ def parse_args():
     p.add_argument("--cpu-steps", default="2,6", help="N1,N2 steps of the differenced CPU timing")
     p.add_argument("--variant", default="auto",
                    help="fused-kernel launch shape 'cells,ntmask,blocks_per_cu' or 'auto' (autotune)")
+    p.add_argument("--overlap", type=int, default=-1, choices=[-1, 0, 1],
+                   help="split steps so the IB chain overlaps the bulk fluid update: -1 library default, 0 off, 1 on")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -183,6 +185,7 @@ def main():
     print(f"[bench] {a.workload} {kw['nx']}x{kw['ny']}x{kw['nz']} {a.precision} nodes={nodes} "
           f"warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
     with lbmib.Engine(**kw) as e:
+        e.set_overlap(a.overlap)
         e.init()
         variant = autotune(e, a, lbmib)
         ms_total, ms_main = e.time_steps(a.warmup, a.steps)

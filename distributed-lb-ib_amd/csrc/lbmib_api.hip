@@ -54,19 +54,16 @@ struct lbmib_ctx {
   int latched = LBMIB_OK;
   KernelVariant kv = {0, -1, 0};  // fused-kernel launch shape
   int ncu = 0;                    // compute units of the device
-  // split steps: bulk on `stream`, sheet band + IB chain on `stream_b`
-  int overlap_req = -1;           // -1 automatic (currently off), 0 off, 1 on
+  // split steps (lbmib_set_overlap): band rows, then bulk rows on `stream`;
+  // band velocity, move and the next step's fiber forces and spreading on
+  // `stream_b`, overlapping the bulk
+  int overlap_req = -1;           // -1 automatic, 0 off, 1 on
   hipStream_t stream_b = nullptr;
-  hipEvent_t evA[2] = {nullptr, nullptr}, evB[2] = {nullptr, nullptr}, evJoin = nullptr;
+  hipEvent_t evIB = nullptr, evBand = nullptr, evJoin = nullptr;
   int split_run = 0;              // steps taken in the current run of split steps
-  Slabs slabs = {0, {0}, {0}};
-  int slab_age = 0;               // steps since the slabs were planned
-  // ring of asynchronous support-box readbacks (pinned host memory)
-  static constexpr int kRing = 16;
-  int *h_box_ring = nullptr;      // [kRing][kMaxSheets * 6]
-  hipEvent_t ev_box[kRing] = {};
-  int ring_step[kRing];           // step whose box each entry holds (-1: none)
-  double hx[kMaxSheets][2];       // host x-extent of each sheet's nodes (from the uploaded positions)
+  int ib_ready = 0;               // fiber forces and spreading of the current step done (stream_b)
+  int *d_rect = nullptr;          // band rectangles of the last band update
+  double hyz[kMaxSheets][4];      // host y/z extent of each sheet's nodes (from the uploaded positions)
   std::vector<hipEvent_t> evs;
 };
 
@@ -98,7 +95,6 @@ static void axis_ranges(int bc, int n, int *lo, int *hi, int *mlo, int *mhi) {
 static int map_device_err(int bits) {
   if (bits & DERR_FIBER_OOB) return LBMIB_ERR_FIBER_OOB;
   if (bits & DERR_BIN_OVERFLOW) return LBMIB_ERR_BIN_OVERFLOW;
-  if (bits & DERR_BAND) return LBMIB_ERR_BAND;
   return LBMIB_OK;
 }
 
@@ -117,15 +113,12 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->evs) (void)hipEventDestroy(e);
   if (ctx->stream_b) (void)hipStreamSynchronize(ctx->stream_b);
-  for (hipEvent_t e : {ctx->evA[0], ctx->evA[1], ctx->evB[0], ctx->evB[1], ctx->evJoin})
+  for (hipEvent_t e : {ctx->evIB, ctx->evBand, ctx->evJoin})
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : ctx->ev_box)
-    if (e) (void)hipEventDestroy(e);
-  if (ctx->h_box_ring) (void)hipHostFree(ctx->h_box_ring);
   if (ctx->stream_b) (void)hipStreamDestroy(ctx->stream_b);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
                   ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.box_part, ctx->ib.done, ctx->ib.ub, ctx->d_sheets,
-                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab};
+                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab, ctx->d_rect};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -206,15 +199,8 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   };
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
   if (!rc && hipStreamCreateWithFlags(&ctx->stream_b, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
-  for (hipEvent_t *e : {&ctx->evA[0], &ctx->evA[1], &ctx->evB[0], &ctx->evB[1], &ctx->evJoin})
+  for (hipEvent_t *e : {&ctx->evIB, &ctx->evBand, &ctx->evJoin})
     if (!rc && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = LBMIB_ERR_HIP;
-  for (int k = 0; k < lbmib_ctx::kRing; ++k) {
-    ctx->ring_step[k] = -1;
-    if (!rc && hipEventCreateWithFlags(&ctx->ev_box[k], hipEventDisableTiming) != hipSuccess) rc = LBMIB_ERR_HIP;
-  }
-  if (!rc && hipHostMalloc((void **)&ctx->h_box_ring, sizeof(int) * lbmib_ctx::kRing * kMaxSheets * 6,
-                           hipHostMallocDefault) != hipSuccess)
-    rc = LBMIB_ERR_ALLOC;
   {
     int dev = 0;
     if (!rc && (hipGetDevice(&dev) != hipSuccess ||
@@ -230,6 +216,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   alloc((void **)&ctx->d_tot, sizeof(double) * 4);
   alloc((void **)&ctx->ib.box, sizeof(int) * 2 * kMaxSheets * 6);
   alloc(&ctx->d_tab, sizeof(double) * 19);
+  alloc((void **)&ctx->d_rect, sizeof(int) * kRectBufInts);
   if (!rc) {
     // kernel units: f (F64), f - w (F32), 2^8 (f - w) (16-bit storage)
     double td[19];
@@ -309,14 +296,14 @@ extern "C" int lbmib_set_fiber_positions(lbmib_ctx *ctx, int sheet, const double
   std::vector<double> soa((size_t)3 * m);
   for (int k = 0; k < m; ++k)
     for (int a = 0; a < 3; ++a) soa[(size_t)a * m + k] = xyz[3 * (size_t)k + a];
-  double lo = 1e300, hi = -1e300;
+  double ext[4] = {1e300, -1e300, 1e300, -1e300};
   for (int k = 0; k < m; ++k) {
-    lo = fmin(lo, xyz[3 * (size_t)k]);
-    hi = fmax(hi, xyz[3 * (size_t)k]);
+    ext[0] = fmin(ext[0], xyz[3 * (size_t)k + 1]);
+    ext[1] = fmax(ext[1], xyz[3 * (size_t)k + 1]);
+    ext[2] = fmin(ext[2], xyz[3 * (size_t)k + 2]);
+    ext[3] = fmax(ext[3], xyz[3 * (size_t)k + 2]);
   }
-  ctx->hx[sheet][0] = lo;
-  ctx->hx[sheet][1] = hi;
-  ctx->slabs.n = 0;  // re-plan the split at the next step
+  for (int a = 0; a < 4; ++a) ctx->hyz[sheet][a] = ext[a];
   for (int a = 0; a < 3; ++a)
     HIPCHK(hipMemcpyAsync(ctx->ib.pos + (size_t)a * M + d.off, soa.data() + (size_t)a * m, sizeof(double) * m,
                           hipMemcpyHostToDevice, ctx->stream));
@@ -379,26 +366,26 @@ static int ensure_ib_clean(lbmib_ctx *ctx) {
 
 extern "C" int lbmib_compute_bendingforce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_BEND, slot_now(ctx), nullptr, ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_BEND, slot_now(ctx), ctx->stream);
 }
 extern "C" int lbmib_compute_stretchingforce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_STRETCH, slot_now(ctx), nullptr, ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_STRETCH, slot_now(ctx), ctx->stream);
 }
 extern "C" int lbmib_compute_elasticforce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_ELASTIC, slot_now(ctx), nullptr, ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_ELASTIC, slot_now(ctx), ctx->stream);
 }
 extern "C" int lbmib_compute_fiber_forces(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
-  return launch_fiber(ctx->ib, ctx->g, FK_BEND | FK_STRETCH | FK_ELASTIC, slot_now(ctx), nullptr, ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_BEND | FK_STRETCH | FK_ELASTIC, slot_now(ctx), ctx->stream);
 }
 
 extern "C" int lbmib_fiber_SpreadForce(lbmib_ctx *ctx) {
   RCHK(ready(ctx));
   RCHK(ensure_ib_clean(ctx));
   ctx->ib_clean = 0;
-  return launch_fiber(ctx->ib, ctx->g, FK_PREP, slot_now(ctx), nullptr, ctx->stream);
+  return launch_fiber(ctx->ib, ctx->g, FK_PREP, slot_now(ctx), ctx->stream);
 }
 
 extern "C" int lbmib_fluid_get_SpreadForce(lbmib_ctx *ctx) {
@@ -479,7 +466,7 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
   const int prec = ctx->fp.precision;
   if (ctx->nnodes > 0) {
     RCHK(ensure_ib_clean(ctx));
-    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, nullptr, ctx->stream));
+    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, ctx->stream));
     RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream));
     ctx->spread_slot = s;
   }
@@ -503,147 +490,74 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
 
 // ------------------------------------------------------------ split steps
 //
-// With sheets present, a step is split so that the IB chain hides behind
-// the bulk of the fluid update:
-//   stream   : bulk(t)  -- cells outside the sheets' x-slabs, no force
-//   stream_b : fiber(t), gather(t) | band(t) -- slab cells, with force |
-//              move(t)
-// bulk(t) reads the previous step's populations of the whole grid, so it
-// waits for band(t-1); band(t) waits for bulk(t-1).  k_move(t) reads the
-// new populations only on support cells and their pull neighbours, which
-// the slabs contain with a margin of kSlabMargin cells.  Every split step
-// copies the support boxes to pinned host memory; the host checks the newest
-// completed copy (at most kRing steps old) and moves the slabs once a box
-// comes within kSlabMargin/2 of an edge.  A sheet moves less than one cell
-// per step, and k_fiber latches LBMIB_ERR_BAND if a support box ever gets
-// within one cell of a slab edge.
-static constexpr int kSlabMargin = 10;
+// With sheets present, a split step updates the band rows -- every (y, z)
+// row near the sheets' support boxes of this and the previous step, all x --
+// first, then the remaining rows (lbmib_fluid.hip, "split steps"):
+//   stream   : wait IB(t) | band(t) | bulk(t)
+//   stream_b : wait band(t) | band velocity + move(t), fiber forces and
+//              spreading of t+1 -> IB(t+1)
+// The band velocity reads the new populations only on support cells and
+// their neighbours, which the band holds, so the IB chain of the next step
+// runs while the bulk rows are updated.  Results are bit-identical to
+// unsplit steps: every cell does the same arithmetic.
 
-// Automatic mode keeps steps unsplit: measured at 256^3 with one 52x52
-// sheet, the concurrent bulk kernel saturates HBM and stretches the
-// latency-bound IB chain on the second stream past the bulk's own duration.
 static bool split_enabled(const lbmib_ctx *ctx) {
-  if (ctx->nnodes == 0) return false;
+  if (ctx->nnodes == 0 || !ctx->d_rowtab) return false;
   return ctx->overlap_req == 1;
 }
 
-// Slabs around the given per-sheet support x-ranges.
-static void plan_slabs(lbmib_ctx *ctx, const int (*xr)[2]) {
-  int lo[kMaxSheets], hi[kMaxSheets], n = 0;
-  for (int s = 0; s < ctx->nsheets; ++s) {
-    if (xr[s][0] > xr[s][1]) continue;
-    lo[n] = std::max(0, xr[s][0] - kSlabMargin);
-    hi[n] = std::min(ctx->g.nx - 1, xr[s][1] + kSlabMargin);
-    ++n;
-  }
-  // sort and merge overlapping / touching ranges
-  for (int i = 0; i < n; ++i)
-    for (int j = i + 1; j < n; ++j)
-      if (lo[j] < lo[i]) { std::swap(lo[i], lo[j]); std::swap(hi[i], hi[j]); }
-  int m = 0;
-  for (int i = 0; i < n; ++i) {
-    if (m > 0 && lo[i] <= hi[m - 1] + 1) hi[m - 1] = std::max(hi[m - 1], hi[i]);
-    else { lo[m] = lo[i]; hi[m] = hi[i]; ++m; }
-  }
-  while (m > kMaxSlabs) {  // join the closest neighbours
-    int best = 0;
-    for (int i = 1; i + 1 < m; ++i)
-      if (lo[i + 1] - hi[i] < lo[best + 1] - hi[best]) best = i;
-    hi[best] = hi[best + 1];
-    for (int i = best + 1; i + 1 < m; ++i) { lo[i] = lo[i + 1]; hi[i] = hi[i + 1]; }
-    --m;
-  }
-  ctx->slabs.n = m;
-  for (int i = 0; i < m; ++i) { ctx->slabs.lo[i] = lo[i]; ctx->slabs.hi[i] = hi[i]; }
-  ctx->slab_age = 0;
+// Grid of the band update: about the rows the sheets' support spans, from
+// the uploaded positions (the kernel strides over any further rows).
+static int band_blocks(const lbmib_ctx *ctx) {
+  double rows = 64;
+  for (int s = 0; s < ctx->nsheets; ++s)
+    rows += (ctx->hyz[s][1] - ctx->hyz[s][0] + 10.0) * (ctx->hyz[s][3] - ctx->hyz[s][2] + 10.0);
+  const double nrows = (double)ctx->g.ny * ctx->g.nz;
+  return (int)std::min(rows * 1.25, nrows);
 }
 
-// Slab check on the support boxes read back asynchronously (one ring entry
-// per split step): the newest completed entry is compared with the slabs;
-// the host blocks only if all entries of the last kRing steps are pending.
-static int check_slabs(lbmib_ctx *ctx) {
-  int xr[kMaxSheets][2];
-  if (ctx->slabs.n == 0) {
-    // first plan: from the uploaded positions (support start floor(X-1))
-    for (int s = 0; s < ctx->nsheets; ++s) {
-      xr[s][0] = (int)floor(ctx->hx[s][0] - 1.0);
-      xr[s][1] = (int)floor(ctx->hx[s][1] - 1.0) + 3;
-    }
-    plan_slabs(ctx, xr);
-    for (int k = 0; k < lbmib_ctx::kRing; ++k) ctx->ring_step[k] = -1;
-    return LBMIB_OK;
-  }
-  ++ctx->slab_age;
-  int newest = -1, oldest = -1;
-  for (int k = 0; k < lbmib_ctx::kRing; ++k) {
-    if (ctx->ring_step[k] < 0) continue;
-    if (oldest < 0 || ctx->ring_step[k] < ctx->ring_step[oldest]) oldest = k;
-    if (hipEventQuery(ctx->ev_box[k]) == hipSuccess && (newest < 0 || ctx->ring_step[k] > ctx->ring_step[newest]))
-      newest = k;
-  }
-  if (newest < 0) {
-    if (oldest < 0 || ctx->time - ctx->ring_step[oldest] < lbmib_ctx::kRing) return LBMIB_OK;
-    HIPCHK(hipEventSynchronize(ctx->ev_box[oldest]));
-    newest = oldest;
-  }
-  const int *box = ctx->h_box_ring + newest * kMaxSheets * 6;
-  bool replan = false;
-  for (int s = 0; s < ctx->nsheets; ++s) {
-    xr[s][0] = box[6 * s];
-    xr[s][1] = box[6 * s + 1];
-    if (xr[s][0] > xr[s][1]) continue;
-    bool ok = false;
-    for (int k = 0; k < ctx->slabs.n; ++k)
-      ok |= xr[s][0] >= ctx->slabs.lo[k] + kSlabMargin / 2 && xr[s][1] <= ctx->slabs.hi[k] - kSlabMargin / 2;
-    replan |= !ok;
-  }
-  // entries older than the one checked carry no newer information
-  const int checked = ctx->ring_step[newest];
-  for (int k = 0; k < lbmib_ctx::kRing; ++k)
-    if (ctx->ring_step[k] >= 0 && ctx->ring_step[k] <= checked) ctx->ring_step[k] = -1;
-  if (replan) plan_slabs(ctx, xr);
-  return LBMIB_OK;
-}
-
-static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
-  RCHK(check_slabs(ctx));
-  const int t = ctx->time, s = t & 1, prec = ctx->fp.precision;
+static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, bool more) {
+  const int s = slot_now(ctx), prec = ctx->fp.precision;
   const int *box = ctx->ib.box;
   if (ctx->split_run == 0) {
-    // sB starts after everything already queued on the main stream
+    // stream_b starts after everything already queued on the main stream
     HIPCHK(hipEventRecord(ctx->evJoin, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evJoin, 0));
   }
-  const bool prev = ctx->split_run > 0;
-  // main stream: bulk(t)
-  if (prev) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evB[1 - s], 0));
-  if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
-  RCHK(launch_collide_bulk(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->slabs,
-                           ctx->kv.nt_mask, ctx->stream));
-  if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
-  HIPCHK(hipEventRecord(ctx->evA[s], ctx->stream));
-  // second stream: IB chain and the band
-  if (!ctx->ib_clean) {
-    RCHK(launch_ib_reset(ctx->ib, 1, 1 << s, ctx->stream_b));
-    ctx->ib_clean = 1;
+  if (!ctx->ib_ready) {
+    if (!ctx->ib_clean) {
+      RCHK(launch_ib_reset(ctx->ib, 1, 1 << s, ctx->stream_b));
+      ctx->ib_clean = 1;
+    }
+    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, ctx->stream_b));
+    RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream_b));
+    HIPCHK(hipEventRecord(ctx->evIB, ctx->stream_b));
   }
-  RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, &ctx->slabs, ctx->stream_b));
-  {
-    const int r = t % lbmib_ctx::kRing;
-    HIPCHK(hipMemcpyAsync(ctx->h_box_ring + r * kMaxSheets * 6, box + s * kMaxSheets * 6,
-                          sizeof(int) * 6 * ctx->nsheets, hipMemcpyDeviceToHost, ctx->stream_b));
-    HIPCHK(hipEventRecord(ctx->ev_box[r], ctx->stream_b));
-    ctx->ring_step[r] = t;
-  }
-  RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream_b));
   ctx->spread_slot = s;
-  if (prev) HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evA[1 - s], 0));
-  RCHK(launch_collide_band(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
-                           box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
-                           ctx->slabs, ctx->kv.nt_mask, ctx->stream_b));
-  HIPCHK(hipEventRecord(ctx->evB[s], ctx->stream_b));
+  // main stream: the band rows after this step's spreading, then the bulk
+  // rows (no force there)
+  HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evIB, 0));
+  if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
+  RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
+                             box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
+                             ctx->d_rowtab, ctx->kv, ctx->stream, 1, ctx->d_rect, band_blocks(ctx)));
+  HIPCHK(hipEventRecord(ctx->evBand, ctx->stream));
+  RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, 0,
+                             box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
+                             ctx->d_rowtab, ctx->kv, ctx->stream, 2, ctx->d_rect, 0));
+  if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
+  // second stream: band velocity and move, then the next step's fiber
+  // forces and spreading (slot 1 - s, reset by the move)
+  HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evBand, 0));
   RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream_b));
   ctx->ib_clean = 1;
+  ctx->ib_ready = 0;
+  if (more) {
+    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, 1 - s, ctx->stream_b));
+    RCHK(launch_gather(prec, ctx->ib, ctx->g, 1 - s, ctx->F[1 - s], ctx->stream_b));
+    ctx->ib_ready = 1;
+  }
+  HIPCHK(hipEventRecord(ctx->evIB, ctx->stream_b));
   ctx->cur = 1 - ctx->cur;
   ctx->own = 0;
   ctx->pending = 0;
@@ -671,13 +585,7 @@ extern "C" int lbmib_set_overlap(lbmib_ctx *ctx, int mode) {
 
 extern "C" int lbmib_timed_kernel_cells(lbmib_ctx *ctx, double *cells) {
   if (!ctx || !cells) return LBMIB_ERR_ARG;
-  double n = (double)ctx->g.n;
-  if (split_enabled(ctx)) {
-    if (ctx->slabs.n == 0) RCHK(check_slabs(ctx));
-    for (int k = 0; k < ctx->slabs.n; ++k)
-      n -= (double)(ctx->slabs.hi[k] - ctx->slabs.lo[k] + 1) * ctx->g.ny * ctx->g.nz;
-  }
-  *cells = n;
+  *cells = (double)ctx->g.n;  // split steps time the band and bulk updates together
   return LBMIB_OK;
 }
 
@@ -685,7 +593,7 @@ extern "C" int lbmib_step(lbmib_ctx *ctx, int nsteps) {
   RCHK(ready(ctx));
   if (ctx->pending || nsteps < 0) return LBMIB_ERR_STATE;
   if (split_enabled(ctx)) {
-    for (int k = 0; k < nsteps; ++k) RCHK(split_step(ctx, nullptr, nullptr));
+    for (int k = 0; k < nsteps; ++k) RCHK(split_step(ctx, nullptr, nullptr, k + 1 < nsteps));
     return split_join(ctx);
   }
   for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, nullptr, nullptr));
@@ -856,12 +764,14 @@ extern "C" int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *m
     ctx->evs.push_back(e);
   }
   const bool split = split_enabled(ctx);
-  auto step_once = [&](hipEvent_t a, hipEvent_t b) { return split ? split_step(ctx, a, b) : one_step(ctx, a, b); };
-  for (int k = 0; k < nwarm; ++k) RCHK(step_once(nullptr, nullptr));
+  auto step_once = [&](hipEvent_t a, hipEvent_t b, bool more) {
+    return split ? split_step(ctx, a, b, more) : one_step(ctx, a, b);
+  };
+  for (int k = 0; k < nwarm; ++k) RCHK(step_once(nullptr, nullptr, k + 1 < nwarm));
   RCHK(split_join(ctx));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipEventRecord(ctx->evs[0], ctx->stream));
-  for (int k = 0; k < nsteps; ++k) RCHK(step_once(nullptr, nullptr));
+  for (int k = 0; k < nsteps; ++k) RCHK(step_once(nullptr, nullptr, k + 1 < nsteps));
   RCHK(split_join(ctx));
   HIPCHK(hipEventRecord(ctx->evs[1], ctx->stream));
   HIPCHK(hipEventSynchronize(ctx->evs[1]));
@@ -869,7 +779,7 @@ extern "C" int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *m
   HIPCHK(hipEventElapsedTime(&ms, ctx->evs[0], ctx->evs[1]));
   if (ms_total) *ms_total = ms;
   if (ms_main_mean) {
-    for (int k = 0; k < nsteps; ++k) RCHK(step_once(ctx->evs[2 + 2 * k], ctx->evs[3 + 2 * k]));
+    for (int k = 0; k < nsteps; ++k) RCHK(step_once(ctx->evs[2 + 2 * k], ctx->evs[3 + 2 * k], k + 1 < nsteps));
     RCHK(split_join(ctx));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     double acc = 0;

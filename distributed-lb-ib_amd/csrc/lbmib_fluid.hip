@@ -258,6 +258,97 @@ int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
   return LBMIB_OK;
 }
 
+// ---------------------------------------------------------- split steps
+//
+// A split step (lbmib_set_overlap) updates the rows (y, z) near the sheets
+// first -- the band -- and the remaining rows -- the bulk -- afterwards, so
+// that the next step's IB chain, which needs only the band's new
+// populations, overlaps the bulk.  The band holds, per sheet, the (y, z)
+// rectangle spanned by the support boxes of this and the previous step (all
+// cells with a force in either), widened by one row on each side for the
+// pulls of the band velocity; on a periodic axis a rectangle touching an end
+// takes the whole axis.  The band kernel's first block stores the
+// rectangles in rect_buf ({n, (y0, y1, z0, z1) x n}) for the bulk kernel,
+// which runs while the next step's IB chain rewrites the box slots.
+enum { kRowsAll = 0, kRowsBand = 1, kRowsBulk = 2 };
+
+__device__ __forceinline__ bool band_rect(const Geometry &g, const int *box_cur, const int *box_prev, int s,
+                                          int r[4]) {
+  const int *b1 = box_cur + 6 * s, *b0 = box_prev + 6 * s;
+  const bool v1 = b1[0] <= b1[1], v0 = b0[0] <= b0[1];
+  if (!v1 && !v0) return false;
+  int y0 = INT_MAX, y1 = INT_MIN, z0 = INT_MAX, z1 = INT_MIN;
+  if (v1) { y0 = min(y0, b1[2]); y1 = max(y1, b1[3]); z0 = min(z0, b1[4]); z1 = max(z1, b1[5]); }
+  if (v0) { y0 = min(y0, b0[2]); y1 = max(y1, b0[3]); z0 = min(z0, b0[4]); z1 = max(z1, b0[5]); }
+  y0 -= 1; y1 += 1; z0 -= 1; z1 += 1;
+  if (g.bc[1] == LBMIB_BC_PERIODIC && (y0 < 0 || y1 >= g.ny)) { y0 = 0; y1 = g.ny - 1; }
+  if (g.bc[2] == LBMIB_BC_PERIODIC && (z0 < 0 || z1 >= g.nz)) { z0 = 0; z1 = g.nz - 1; }
+  r[0] = max(y0, 0); r[1] = min(y1, g.ny - 1); r[2] = max(z0, 0); r[3] = min(z1, g.nz - 1);
+  return true;
+}
+
+// Row index of band item `it` (rectangles in sheet order, y fastest); -1 for
+// a row an earlier rectangle already holds, -2 past the last item.
+__device__ __forceinline__ int band_row_at(const Geometry &g, const int *box_cur, const int *box_prev, int nsheets,
+                                           long it) {
+  for (int s = 0; s < nsheets; ++s) {
+    int r[4];
+    if (!band_rect(g, box_cur, box_prev, s, r)) continue;
+    const long h = r[1] - r[0] + 1, area = h * (r[3] - r[2] + 1);
+    if (it < area) {
+      const int y = r[0] + (int)(it % h), z = r[2] + (int)(it / h);
+      for (int t = 0; t < s; ++t) {
+        int q[4];
+        if (band_rect(g, box_cur, box_prev, t, q) && y >= q[0] && y <= q[1] && z >= q[2] && z <= q[3]) return -1;
+      }
+      return y + g.ny * z;
+    }
+    it -= area;
+  }
+  return -2;
+}
+
+__device__ __forceinline__ void store_band_rects(const Geometry &g, const int *box_cur, const int *box_prev,
+                                                 int nsheets, int *rect_buf) {
+  int n = 0;
+  for (int s = 0; s < nsheets; ++s) {
+    int r[4];
+    if (!band_rect(g, box_cur, box_prev, s, r)) continue;
+    for (int k = 0; k < 4; ++k) rect_buf[1 + 4 * n + k] = r[k];
+    ++n;
+  }
+  rect_buf[0] = n;
+}
+
+__device__ __forceinline__ bool in_band_rects(const int *rect_buf, int y, int z) {
+  bool in = false;
+  const int n = rect_buf[0];
+  for (int k = 0; k < n; ++k) {
+    const int *r = rect_buf + 1 + 4 * k;
+    in |= y >= r[0] && y <= r[1] && z >= r[2] && z <= r[3];
+  }
+  return in;
+}
+
+// Next row of this block's walk over the rows of `sel` (all, band, bulk);
+// false when the walk is over.  `it` starts at blockIdx.x.
+__device__ __forceinline__ bool next_row(const Geometry &g, int sel, const int *box_cur, const int *box_prev,
+                                         int nsheets, const int *rect_buf, long &it, int &row) {
+  const long nrows = (long)g.ny * g.nz;
+  for (;; it += gridDim.x) {
+    if (sel == kRowsBand) {
+      row = band_row_at(g, box_cur, box_prev, nsheets, it);
+      if (row == -2) return false;
+      if (row < 0) continue;
+      return true;
+    }
+    if (it >= nrows) return false;
+    row = (int)it;
+    if (sel == kRowsBulk && in_band_rects(rect_buf, row % g.ny, row / g.ny)) continue;
+    return true;
+  }
+}
+
 // Row form: blocks walk whole (y, z) rows with a stride (one block per row
 // when the grid has as many blocks as rows).
 //
@@ -276,12 +367,15 @@ __global__ void __launch_bounds__(1024)
                           typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
                           int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                           const typename Traits<P>::FT *__restrict__ Fcur,
-                          const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab) {
+                          const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
+                          int sel, int *__restrict__ rect_buf) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
-  const int nrows = g.ny * g.nz;
+  if (sel == kRowsBand && blockIdx.x == 0 && threadIdx.x == 0)
+    store_band_rects(g, box_cur, box_prev, nsheets, rect_buf);
   if constexpr (!BUF) {
-    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+    int row;
+    for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x) {
       const int y = row % g.ny, z = row / g.ny;
       for (int x = threadIdx.x; x < g.nx; x += blockDim.x)
         update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
@@ -303,7 +397,8 @@ __global__ void __launch_bounds__(1024)
     // a dead cell; k_collide_xedges rewrites its edge cells afterwards.
     const bool filler = x < g.lo[0] || x > g.hi[0] || (edge && !frozen);
     const uint32_t vo = (uint32_t)x * es;
-    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+    int row;
+    for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x) {
       const int y = row % g.ny, z = row / g.ny;
       if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
       const uint32_t row_b = (uint32_t)row * sy_b;  // byte offset of (0, y, z) in a plane
@@ -416,14 +511,16 @@ __global__ void __launch_bounds__(512)
                            typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
                            int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                            const typename Traits<P>::FT *__restrict__ Fcur,
-                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab) {
+                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
+                           int sel, int *__restrict__ rect_buf) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
-  const int nrows = g.ny * g.nz;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
   constexpr uint32_t es = sizeof(S);
   const uint32_t pitch_b = (uint32_t)g.pitch * es;
   const uint32_t sy_b = (uint32_t)g.nx * es;
+  if (sel == kRowsBand && blockIdx.x == 0 && threadIdx.x == 0)
+    store_band_rects(g, box_cur, box_prev, nsheets, rect_buf);
   const int x0 = 2 * (int)threadIdx.x;
   if (x0 >= g.nx) return;
   // dead pairs store zeros so that the row's cache lines are written whole
@@ -431,7 +528,8 @@ __global__ void __launch_bounds__(512)
   const bool inout_x = g.bc[0] == LBMIB_BC_INOUT;
   const bool frozen[2] = {inout_x && x0 == g.lo[0], inout_x && x0 + 1 == g.hi[0]};
   const uint32_t vo = (uint32_t)x0 * es;
-  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+  int row;
+  for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x) {
     const int y = row % g.ny, z = row / g.ny;
     if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
     const uint32_t row_b = (uint32_t)row * sy_b;
@@ -604,52 +702,15 @@ __global__ void __launch_bounds__(256)
                      typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
                      int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                      const typename Traits<P>::FT *__restrict__ Fcur,
-                     const typename Traits<P>::FT *__restrict__ Fprev) {
+                     const typename Traits<P>::FT *__restrict__ Fprev, int sel, const int *__restrict__ rect_buf) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 2L * g.ny * g.nz) return;
   const long row = e >> 1;
   const int x = (e & 1) ? g.hi[0] : g.lo[0];
   if ((e & 1) && g.lo[0] == g.hi[0]) return;  // one-cell-wide axis: done by the even item
+  if (sel != kRowsAll && in_band_rects(rect_buf, (int)(row % g.ny), (int)(row / g.ny)) != (sel == kRowsBand)) return;
   update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, (int)(row % g.ny),
                           (int)(row / g.ny));
-}
-
-// Bulk of a split step: every cell outside the slabs, no force (the slabs
-// contain all cells where the force of this or the previous step is
-// non-zero).  Row-aligned blocks as k_collide_stream.
-template <int P, bool NT, bool NTL>
-__global__ void __launch_bounds__(1024)
-    k_collide_bulk(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
-                   int own, KConsts<typename Traits<P>::T> c, Slabs slabs) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  bool skip = false;
-  for (int k = 0; k < slabs.n; ++k) skip |= x >= slabs.lo[k] && x <= slabs.hi[k];
-  if (skip) return;
-  update_cell<P, NT, NTL>(g, fin, fout, own, c, 0, nullptr, nullptr, nullptr, nullptr, x, blockIdx.y, blockIdx.z);
-}
-
-// Band of a split step: the cells of the slabs, enumerated flat (slab by
-// slab, x fastest), with the force.
-template <int P, bool NT, bool NTL>
-__global__ void __launch_bounds__(256)
-    k_collide_band(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
-                   int own, KConsts<typename Traits<P>::T> c, int nsheets, const int *__restrict__ box_cur,
-                   const int *__restrict__ box_prev, const typename Traits<P>::FT *__restrict__ Fcur,
-                   const typename Traits<P>::FT *__restrict__ Fprev, Slabs slabs) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long plane = (long)g.ny * g.nz;
-  for (int k = 0; k < slabs.n; ++k) {
-    const int w = slabs.hi[k] - slabs.lo[k] + 1;
-    const long cnt = (long)w * plane;
-    if (e < cnt) {
-      const int x = slabs.lo[k] + (int)(e % w);
-      const long r = e / w;
-      update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, (int)(r % g.ny),
-                              (int)(r / g.ny));
-      return;
-    }
-    e -= cnt;
-  }
 }
 
 // V consecutive cells of one x-row per thread: the 9 populations with
@@ -951,6 +1012,8 @@ template <int P, bool NT, bool NTL>
 static void launch_rows2(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                          const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
+  const int sel = kRowsAll;  // whole-grid steps only
+  int *rect_buf = nullptr;
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const int npairs = g.ny * g.nz / 2;
@@ -962,7 +1025,7 @@ static void launch_rows2(int nblocks, const Geometry &g, const void *fin, void *
     const long items = 2L * g.ny * g.nz;
     k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
         g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
-        (const FT *)Fprev);
+        (const FT *)Fprev, sel, rect_buf);
   }
 }
 
@@ -977,26 +1040,29 @@ bool pair_path_ok(const Geometry &g) {
 template <int P, bool NT, bool NTL>
 static void launch_pairs(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
+                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, int sel, int *rect_buf,
+                         hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const int nrows = g.ny * g.nz;
   const dim3 b((unsigned)(((g.nx / 2) + 63) / 64 * 64));
   const dim3 gr(nblocks < nrows ? nblocks : nrows);
   k_collide_stream_pairs<P, NT, NTL><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
-                                                      box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab);
+                                                      box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab,
+                                                      sel, rect_buf);
   if (g.bc[0] != LBMIB_BC_INOUT) {
     const long items = 2L * g.ny * g.nz;
     k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
         g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
-        (const FT *)Fprev);
+        (const FT *)Fprev, sel, rect_buf);
   }
 }
 
 template <int P, bool NT, bool NTL>
 static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                         const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                        const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
+                        const void *Fcur, const void *Fprev, const uint32_t *rowtab, int sel, int *rect_buf,
+                        hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const dim3 b = row_block(g);
@@ -1005,17 +1071,17 @@ static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *f
   if (rowtab && row_path_buffered(g, sizeof(S))) {
     k_collide_stream_rows<P, NT, NTL, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
                                                              box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev,
-                                                             rowtab);
+                                                             rowtab, sel, rect_buf);
     if (g.bc[0] != LBMIB_BC_INOUT) {
       const long items = 2L * g.ny * g.nz;
       k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
           g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
-          (const FT *)Fprev);
+          (const FT *)Fprev, sel, rect_buf);
     }
   } else
     k_collide_stream_rows<P, NT, NTL, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c),
                                                               nsheets, box_cur, box_prev, (const FT *)Fcur,
-                                                              (const FT *)Fprev, nullptr);
+                                                              (const FT *)Fprev, nullptr, sel, rect_buf);
 }
 
 template <int P, bool NT>
@@ -1031,13 +1097,15 @@ static void launch_any(int v, const Geometry &g, const void *fin, void *fout, in
     if (v == 4) return launch_vec<P, 4, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   }
-  launch_rows<P, NT, false>(g.ny * g.nz, g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, nullptr, s);
+  launch_rows<P, NT, false>(g.ny * g.nz, g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, nullptr,
+                            kRowsAll, nullptr, s);
 }
 
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
-                          hipStream_t s) {
+                          hipStream_t s, int sel, int *rect_buf, int band_blocks) {
+  if (sel != kRowsAll && !rowtab) return LBMIB_ERR_ARG;
   // nontemporal stores and loads by default: the populations are touched
   // once per step, and keeping them out of the caches was fastest measured
   const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
@@ -1046,6 +1114,7 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
   // per lane otherwise -- the fastest measured at 256^3
   int cpt = kv.cells_per_thread;
   if (cpt <= 0) cpt = ((prec == LBMIB_F16 || prec == LBMIB_BF16) && rowtab && pair_path_ok(g)) ? 2 : 1;
+  if (cpt == 3 && sel != kRowsAll) cpt = 1;  // split steps use the row and pair kernels
   if (cpt == 3 && rowtab && rows2_path_ok(g)) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz / 2;
     LBMIB_DISPATCH(prec, ({
@@ -1057,76 +1126,29 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
   if (cpt == 2 && rowtab && pair_path_ok(g)) {
-    const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
+    const int nb = sel == kRowsBand ? band_blocks : (kv.blocks > 0 ? kv.blocks : g.ny * g.nz);
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else if (nt) launch_pairs<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else if (ntl) launch_pairs<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else launch_pairs<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      else if (nt) launch_pairs<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      else if (ntl) launch_pairs<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      else launch_pairs<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
-  const int v = collide_vec_width(prec, g.nx, cpt);
+  const int v = sel != kRowsAll ? 1 : collide_vec_width(prec, g.nx, cpt);
   if (v == 1) {
-    const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
+    const int nb = sel == kRowsBand ? band_blocks : (kv.blocks > 0 ? kv.blocks : g.ny * g.nz);
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
   LBMIB_DISPATCH(prec, ({
     if (nt) launch_any<P, true>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     else launch_any<P, false>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
-  }));
-  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
-}
-
-int launch_collide_bulk(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
-                        const FluidConsts<double> &c, const Slabs &slabs, int nt_mask, hipStream_t s) {
-  const int nt = nt_mask < 0 ? 1 : (nt_mask & 1);
-  const int ntl = nt_mask < 0 ? 0 : ((nt_mask >> 1) & 1);
-  const dim3 b = row_block(g), gr = row_grid(g, b);
-  LBMIB_DISPATCH(prec, ({
-    using TT = typename Traits<P>::T;
-    using S = typename Traits<P>::S;
-    const KConsts<TT> cc = kconsts<P>(c);
-    if (nt && ntl) k_collide_bulk<P, true, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
-    else if (nt) k_collide_bulk<P, true, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
-    else if (ntl) k_collide_bulk<P, false, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
-    else k_collide_bulk<P, false, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, slabs);
-  }));
-  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
-}
-
-int launch_collide_band(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
-                        const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                        const void *Fcur, const void *Fprev, const Slabs &slabs, int nt_mask, hipStream_t s) {
-  const int nt = nt_mask < 0 ? 1 : (nt_mask & 1);
-  const int ntl = nt_mask < 0 ? 0 : ((nt_mask >> 1) & 1);
-  long cells = 0;
-  for (int k = 0; k < slabs.n; ++k) cells += (long)(slabs.hi[k] - slabs.lo[k] + 1) * g.ny * g.nz;
-  if (cells == 0) return LBMIB_OK;
-  const int blocks = (int)((cells + 255) / 256);
-  LBMIB_DISPATCH(prec, ({
-    using TT = typename Traits<P>::T;
-    using FT = typename Traits<P>::FT;
-    using S = typename Traits<P>::S;
-    const KConsts<TT> cc = kconsts<P>(c);
-    if (nt && ntl)
-      k_collide_band<P, true, true><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
-                                                           box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
-    else if (nt)
-      k_collide_band<P, true, false><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
-                                                            box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
-    else if (ntl)
-      k_collide_band<P, false, true><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
-                                                            box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
-    else
-      k_collide_band<P, false, false><<<blocks, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, cc, nsheets,
-                                                             box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, slabs);
   }));
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }

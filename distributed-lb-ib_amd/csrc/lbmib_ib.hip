@@ -88,7 +88,7 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-__global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g, int mask, int slot, Slabs guard) {
+__global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g, int mask, int slot) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = n < ib.nnodes;
   const int M = ib.nnodes;
@@ -234,17 +234,6 @@ __global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g,
   }
   __syncthreads();
   for (int k = threadIdx.x; k < nk; k += blockDim.x) ib.box[slot * kMaxSheets * 6 + k] = sbox[k];
-  if (guard.n > 0) {
-    // a split step needs every support cell and its pull neighbours inside
-    // one slab; latch an error if the sheet got closer than that
-    __syncthreads();
-    if (threadIdx.x < ib.nsheets) {
-      const int x0 = sbox[threadIdx.x * 6], x1 = sbox[threadIdx.x * 6 + 1];
-      bool inside = x0 > x1;  // empty box
-      for (int k = 0; k < guard.n; ++k) inside |= x0 >= guard.lo[k] + 1 && x1 <= guard.hi[k] - 1;
-      if (!inside) atomicOr(ib.err, DERR_BAND);
-    }
-  }
   if (threadIdx.x == 0) atomicExch(ib.done, 0u);
 }
 
@@ -469,11 +458,9 @@ __global__ void k_ib_reset(IBState ib, int reset_bins, int slot_mask) {
 
 static inline int grid_for(long n, int b) { return (int)((n + b - 1) / b); }
 
-int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, const Slabs *guard, hipStream_t s) {
+int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
-  Slabs none;
-  none.n = 0;
-  k_fiber<<<fiber_blocks(ib.nnodes), kFiberThreads, 0, s>>>(ib, g, mask, slot, guard ? *guard : none);
+  k_fiber<<<fiber_blocks(ib.nnodes), kFiberThreads, 0, s>>>(ib, g, mask, slot);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 

@@ -37,15 +37,7 @@ enum { NF_BX = 0, NF_BY, NF_BZ, NF_SX, NF_SY, NF_SZ, NF_EX, NF_EY, NF_EZ, NF_COU
 enum { FK_BEND = 1, FK_STRETCH = 2, FK_ELASTIC = 4, FK_PREP = 8, FK_ALL = 15 };
 
 // Device error bits (combined with atomicOr, mapped to LBMIB_ERR_* on the host).
-enum { DERR_FIBER_OOB = 1, DERR_BIN_OVERFLOW = 2, DERR_BAND = 4 };
-
-// x-slabs holding the sheets' support (with margin) when a step is split
-// into a force-free bulk update and a band update on a second stream.
-constexpr int kMaxSlabs = 4;
-struct Slabs {
-  int n;
-  int lo[kMaxSlabs], hi[kMaxSlabs];  // inclusive x ranges, disjoint, ascending
-};
+enum { DERR_FIBER_OOB = 1, DERR_BIN_OVERFLOW = 2 };
 
 struct SheetDesc {
   int rows, cols, off;   // node range [off, off + rows*cols)
@@ -126,11 +118,14 @@ struct KernelVariant {
 };
 
 // rowtab: the row source table (row_source_table) on the device, or null for
-// the general path.
+// the general path.  sel: which rows to update -- 0 all, 1 the band of a
+// split step (grid of band_blocks blocks; stores the band rectangles in
+// rect_buf), 2 the rest (reads rect_buf); 1 and 2 need rowtab.
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
-                          hipStream_t s);
+                          hipStream_t s, int sel = 0, int *rect_buf = nullptr, int band_blocks = 0);
+constexpr int kRectBufInts = 1 + 4 * kMaxSheets;
 bool row_path_buffered(const Geometry &g, size_t es);
 bool pair_path_ok(const Geometry &g);
 bool rows2_path_ok(const Geometry &g);
@@ -143,14 +138,7 @@ int launch_totals(int prec, const Geometry &g, const void *f, int own_mode, cons
                   double *partials, int nblocks_max, double *out4, hipStream_t s);
 int totals_blocks(const Geometry &g);
 
-int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, const Slabs *guard, hipStream_t s);
-// Split step: the bulk update (cells outside the slabs, no force) and the
-// band update (cells inside the slabs, with force).
-int launch_collide_bulk(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
-                        const FluidConsts<double> &c, const Slabs &slabs, int nt_mask, hipStream_t s);
-int launch_collide_band(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
-                        const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                        const void *Fcur, const void *Fprev, const Slabs &slabs, int nt_mask, hipStream_t s);
+int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s);
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s);
 int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
                 const FluidConsts<double> &c, int reset_slot, hipStream_t s);

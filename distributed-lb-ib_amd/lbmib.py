@@ -15,9 +15,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbmib.so")
 
-OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_FIBER_OOB, ERR_BIN_OVERFLOW, ERR_NONFINITE, ERR_STATE, ERR_BAND = range(9)
+OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_FIBER_OOB, ERR_BIN_OVERFLOW, ERR_NONFINITE, ERR_STATE = range(8)
 ERR_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_ALLOC", 4: "ERR_FIBER_OOB",
-             5: "ERR_BIN_OVERFLOW", 6: "ERR_NONFINITE", 7: "ERR_STATE", 8: "ERR_BAND"}
+             5: "ERR_BIN_OVERFLOW", 6: "ERR_NONFINITE", 7: "ERR_STATE"}
 BC_INOUT, BC_WALL, BC_PERIODIC = 0, 1, 2
 PRECISIONS = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3}
 STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}

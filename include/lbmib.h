@@ -34,7 +34,6 @@ extern "C" {
 #define LBMIB_ERR_BIN_OVERFLOW 5 /* more fiber nodes in one spread bin than its capacity */
 #define LBMIB_ERR_NONFINITE 6    /* NaN/Inf in the mass/momentum watchdog */
 #define LBMIB_ERR_STATE 7        /* call out of order (e.g. step before init) */
-#define LBMIB_ERR_BAND 8         /* a sheet outran the x-slab of a split step (see lbmib_set_overlap) */
 
 /* ---- boundary kinds per axis ---- */
 #define LBMIB_BC_INOUT 0    /* x only: fixed-equilibrium planes x=1,2 and N-3,N-2 (lbm_ib.c:2769-2907) */
@@ -115,14 +114,14 @@ int lbmib_replace_old_DF(lbmib_ctx *ctx);             /* lbm_ib.c:2909-2968: buf
 /* ---- whole steps: the do_thread loop body, lbm_ib.c:3115-3563 ---- */
 int lbmib_step(lbmib_ctx *ctx, int nsteps);
 /* Split steps for lbmib_step / lbmib_time_steps (not in the reference): with
- * sheets present, each step updates the cells outside x-slabs around the
- * sheets on the main stream while a second stream runs the fiber forces,
- * spreading, the slab cells and the move, so the IB work overlaps the bulk
- * of the fluid update.  Results are bit-identical to unsplit steps.
- * mode: -1 automatic (the default; currently never splits), 0 never, 1 always. */
+ * sheets present, each step first updates the rows (y, z) near the sheets'
+ * support, then the remaining rows, while a second stream moves the sheets
+ * and computes the next step's fiber forces and spreading -- the IB chain
+ * overlaps the bulk of the fluid update.  Results are bit-identical to
+ * unsplit steps.  mode: -1 automatic (the default), 0 never, 1 always. */
 int lbmib_set_overlap(lbmib_ctx *ctx, int mode);
-/* Cells updated by the kernel lbmib_time_steps times (all cells, or the
- * cells outside the slabs when steps are split). */
+/* Cells updated by the kernels lbmib_time_steps times per step (all cells:
+ * split steps time their band and bulk updates together). */
 int lbmib_timed_kernel_cells(lbmib_ctx *ctx, double *cells);
 
 /* ---- queries (synchronising) ---- */
