@@ -57,7 +57,7 @@ struct lbmib_ctx {
   // split steps (lbmib_set_overlap): band rows, then bulk rows on `stream`;
   // band velocity, move and the next step's fiber forces and spreading on
   // `stream_b`, overlapping the bulk
-  int overlap_req = -1;           // -1 automatic, 0 off, 1 on
+  int overlap_req = -1;           // -1 automatic (on with sheets), 0 off, 1 on
   hipStream_t stream_b = nullptr;
   hipEvent_t evIB = nullptr, evBand = nullptr, evJoin = nullptr;
   int split_run = 0;              // steps taken in the current run of split steps
@@ -501,9 +501,12 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
 // runs while the bulk rows are updated.  Results are bit-identical to
 // unsplit steps: every cell does the same arithmetic.
 
+// Automatic mode splits whenever there are sheets and the buffer-addressed
+// row kernels apply (measured at 256^3 with a 104x104 sheet: 4% faster in
+// FP32, 2% with 16-bit storage, than unsplit steps).
 static bool split_enabled(const lbmib_ctx *ctx) {
   if (ctx->nnodes == 0 || !ctx->d_rowtab) return false;
-  return ctx->overlap_req == 1;
+  return ctx->overlap_req != 0;
 }
 
 // Grid of the band update: about the rows the sheets' support spans, from

@@ -118,7 +118,8 @@ int lbmib_step(lbmib_ctx *ctx, int nsteps);
  * support, then the remaining rows, while a second stream moves the sheets
  * and computes the next step's fiber forces and spreading -- the IB chain
  * overlaps the bulk of the fluid update.  Results are bit-identical to
- * unsplit steps.  mode: -1 automatic (the default), 0 never, 1 always. */
+ * unsplit steps.  mode: -1 automatic (the default: split when there are
+ * sheets and the buffer-addressed row kernels apply), 0 never, 1 always. */
 int lbmib_set_overlap(lbmib_ctx *ctx, int mode);
 /* Cells updated by the kernels lbmib_time_steps times per step (all cells:
  * split steps time their band and bulk updates together). */
