@@ -125,11 +125,13 @@ def cpu_reference_baseline(n, rows, steps_pair):
 
 
 # (cells per thread, nontemporal mask: 1 stores / 2 loads, blocks per CU; 0 = one block per row)
-# cells per thread: 1 one cell per lane, 2 x pairs, 3 two rows per block
-VARIANTS = {"f64": [(1, 3, 0), (1, 1, 0), (3, 3, 0), (3, 1, 0)],
-            "f32": [(1, 3, 0), (1, 1, 0), (3, 3, 0), (3, 1, 0), (2, 3, 0), (3, 3, 8)],
-            "f16": [(1, 3, 0), (1, 1, 0), (3, 3, 0), (3, 1, 0), (2, 3, 0), (3, 3, 8)],
-            "bf16": [(1, 3, 0), (1, 1, 0), (3, 3, 0), (3, 1, 0), (2, 3, 0), (3, 3, 8)]}
+# (cells per thread: 1 one cell per lane, 2 x pairs; nontemporal mask: 1
+# stores, 2 loads; blocks per CU, 0 = one block per row).  The two-rows
+# shape (3) is never faster and is left out.
+VARIANTS = {"f64": [(1, 3, 0), (1, 1, 0), (2, 3, 0)],
+            "f32": [(1, 3, 0), (1, 1, 0), (2, 3, 0), (2, 1, 0), (1, 3, 8)],
+            "f16": [(2, 3, 0), (2, 1, 0), (1, 3, 0), (1, 1, 0), (2, 3, 8)],
+            "bf16": [(2, 3, 0), (2, 1, 0), (1, 3, 0), (1, 1, 0), (2, 3, 8)]}
 
 
 def autotune(e, a, lbmib):
