@@ -17,55 +17,8 @@
 #include <algorithm>
 #include <vector>
 
+#include "lbmib_ctx.h"
 #include "lbmib_internal.h"
-
-using namespace lbmib;
-
-struct lbmib_ctx {
-  lbmib_fluid_params fp;
-  int nsheets = 0;
-  lbmib_sheet_params sp[kMaxSheets];
-  SheetDesc sd[kMaxSheets];
-  int nnodes = 0;
-  Geometry g;
-  FluidConsts<double> c;
-  size_t esz = 0, fsz = 0;  // population / force element sizes
-  void *f[2] = {nullptr, nullptr};
-  int cur = 0;
-  int own = 1;       // f[cur] holds the initial f (not a post-collision state)
-  int pending = 0;   // collide done, new state in f[1-cur], swap pending
-  int band_done = 0; // band moments of the pending state computed
-  int ib_clean = 1;  // bins empty and this step's box slot reset
-  int spread_slot = -1;  // slot of the last completed gather
-  void *F[2] = {nullptr, nullptr};
-  IBState ib;
-  SheetDesc *d_sheets = nullptr;
-  int *d_err = nullptr;
-  double *d_partials = nullptr;
-  int nparts = 0;
-  double *d_tot = nullptr;
-  double *d_mat = nullptr;   // materialisation scratch (lazy)
-  void *d_tab = nullptr;     // frozen-plane populations in kernel units (19 values)
-  uint32_t *d_rowtab = nullptr;  // row source table of the buffer-addressed fused kernel
-  size_t mat_bytes = 0;
-  hipStream_t stream = nullptr;
-  int time = 1;
-  int init_bits = 0;  // 1: DF0, 2: df1, 4: inout
-  int latched = LBMIB_OK;
-  KernelVariant kv = {0, -1, 0};  // fused-kernel launch shape
-  int ncu = 0;                    // compute units of the device
-  // split steps (lbmib_set_overlap): band rows, then bulk rows on `stream`;
-  // band velocity, move and the next step's fiber forces and spreading on
-  // `stream_b`, overlapping the bulk
-  int overlap_req = -1;           // -1 automatic (on with sheets), 0 off, 1 on
-  hipStream_t stream_b = nullptr;
-  hipEvent_t evIB = nullptr, evBand = nullptr, evJoin = nullptr;
-  int split_run = 0;              // steps taken in the current run of split steps
-  int ib_ready = 0;               // fiber forces and spreading of the current step done (stream_b)
-  int *d_rect = nullptr;          // band rectangles of the last band update
-  double hyz[kMaxSheets][4];      // host y/z extent of each sheet's nodes (from the uploaded positions)
-  std::vector<hipEvent_t> evs;
-};
 
 #define HIPCHK(x)                               \
   do {                                          \

@@ -1,0 +1,295 @@
+// lbmib_batch.hip -- batched ensembles (lbmib_batch_* of include/lbmib.h).
+//
+// An ensemble is a set of independent simulations, each a full lbmib_ctx
+// with its own grid, fluid constants, sheets and buffers.  A batched step
+// runs every kernel of the step once for the whole ensemble: the launch's
+// blocks are mapped to (instance, block within the instance) by a table per
+// kernel, so 64 instances of uneven size cost five launches per step instead
+// of 320.  The instances step in lockstep and their results are those of
+// lbmib_step on each alone, bit for bit: the batched kernels run the same
+// per-cell and per-node code.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "lbmib_ctx.h"
+#include "lbmib_internal.h"
+
+namespace {
+
+enum { kMapFused = 0, kMapFiber, kMapGather, kMapBandU, kMapMove, kMapCount };
+
+}  // namespace
+
+struct lbmib_batch {
+  int n = 0;
+  int prec = 0;
+  std::vector<lbmib_ctx *> inst;
+  BatchDesc *d_desc = nullptr;
+  int *d_map[kMapCount] = {};
+  int nblk[kMapCount] = {};
+  int wpb = 0;        // waves per block of the fused launch (tile shape)
+  int nt_mask = -1;   // nontemporal loads/stores of the fused launch
+  int mcap_max = 0;   // largest gather candidate capacity
+  int nnodes = 0;     // fiber nodes over all instances
+  double cells = 0;   // grid cells over all instances
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr, ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> ev_inst;
+};
+
+#define HIPCHK(x)                                \
+  do {                                           \
+    if ((x) != hipSuccess) return LBMIB_ERR_HIP; \
+  } while (0)
+#define RCHK(x)        \
+  do {                 \
+    int _r = (x);      \
+    if (_r) return _r; \
+  } while (0)
+
+static int waves_per_row(const lbmib_ctx *c) { return (c->g.nx + 63) / 64; }
+
+// Block tables of every batched launch for the current tile shape.
+static int build_maps(lbmib_batch *b) {
+  std::vector<int> maps[kMapCount];
+  for (int i = 0; i < b->n; ++i) {
+    const lbmib_ctx *c = b->inst[i];
+    const int rpb = b->wpb / waves_per_row(c);
+    const int nrows = c->g.ny * c->g.nz;
+    const int M = c->nnodes;
+    const int counts[kMapCount] = {(nrows + rpb - 1) / rpb, M ? fiber_blocks(M) : 0, M ? gather_blocks(M) : 0,
+                                   M ? band_u_blocks(M) : 0, M ? move_blocks(M) : 0};
+    for (int k = 0; k < kMapCount; ++k) {
+      if (counts[k] >= (1 << kBatchShift)) return LBMIB_ERR_ARG;
+      for (int lb = 0; lb < counts[k]; ++lb) maps[k].push_back(i << kBatchShift | lb);
+    }
+  }
+  for (int k = 0; k < kMapCount; ++k) {
+    if (b->d_map[k]) (void)hipFree(b->d_map[k]);
+    b->d_map[k] = nullptr;
+    b->nblk[k] = (int)maps[k].size();
+    if (maps[k].empty()) continue;
+    if (hipMalloc((void **)&b->d_map[k], sizeof(int) * maps[k].size()) != hipSuccess) return LBMIB_ERR_ALLOC;
+    HIPCHK(hipMemcpy(b->d_map[k], maps[k].data(), sizeof(int) * maps[k].size(), hipMemcpyHostToDevice));
+  }
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_destroy(lbmib_batch *b) {
+  if (!b) return LBMIB_ERR_ARG;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (lbmib_ctx *c : b->inst)
+    if (c) (void)lbmib_destroy(c);
+  for (int k = 0; k < kMapCount; ++k)
+    if (b->d_map[k]) (void)hipFree(b->d_map[k]);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  for (hipEvent_t e : {b->ev, b->ev0, b->ev1})
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->ev_inst)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_create(int n, const lbmib_fluid_params *fluids, const int *nsheets,
+                                  const lbmib_sheet_params *sheets, lbmib_batch **out) {
+  if (!out || n <= 0 || n >= (1 << (31 - kBatchShift)) || !fluids || !nsheets) return LBMIB_ERR_ARG;
+  *out = nullptr;
+  lbmib_batch *b = new lbmib_batch();
+  b->n = n;
+  b->prec = fluids[0].precision;
+  int rc = LBMIB_OK, off = 0;
+  for (int i = 0; i < n && !rc; ++i) {
+    // the batched fused kernel has no x-edge pass: inlet/outlet x only
+    if (fluids[i].precision != b->prec || fluids[i].bc_x != LBMIB_BC_INOUT || nsheets[i] < 0) rc = LBMIB_ERR_ARG;
+    lbmib_ctx *c = nullptr;
+    if (!rc) rc = lbmib_create(fluids + i, nsheets[i], nsheets[i] ? sheets + off : nullptr, &c);
+    if (c) b->inst.push_back(c);
+    off += nsheets[i];
+    if (!rc && !c->d_rowtab) rc = LBMIB_ERR_ARG;  // needs the buffer-addressed row path
+  }
+  if (rc) {
+    lbmib_batch_destroy(b);
+    return rc;
+  }
+  std::vector<BatchDesc> desc(n);
+  for (int i = 0; i < n; ++i) {
+    lbmib_ctx *c = b->inst[i];
+    BatchDesc &d = desc[i];
+    memset(&d, 0, sizeof(d));
+    d.g = c->g;
+    d.ib = c->ib;
+    d.f[0] = c->f[0];
+    d.f[1] = c->f[1];
+    d.F[0] = c->F[0];
+    d.F[1] = c->F[1];
+    d.rowtab = c->d_rowtab;
+    d.nsheets = c->nsheets;
+    d.mcap = gather_mcap(c->ib.cap);
+    d.wpr = waves_per_row(c);
+    d.nrows = c->g.ny * c->g.nz;
+    d.u0[0] = c->c.u0x;
+    d.u0[1] = c->c.u0y;
+    d.u0[2] = c->c.u0z;
+    d.gz = c->c.g;
+    kconsts_bytes(b->prec, c->c, d.kc);
+    b->wpb = std::max(b->wpb, d.wpr);
+    b->mcap_max = std::max(b->mcap_max, d.mcap);
+    b->nnodes += c->nnodes;
+    b->cells += (double)c->g.n;
+  }
+  if (hipMalloc((void **)&b->d_desc, sizeof(BatchDesc) * n) != hipSuccess) rc = LBMIB_ERR_ALLOC;
+  if (!rc && hipMemcpy(b->d_desc, desc.data(), sizeof(BatchDesc) * n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = LBMIB_ERR_HIP;
+  if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
+  for (hipEvent_t *e : {&b->ev, &b->ev0, &b->ev1})
+    if (!rc && hipEventCreate(e) != hipSuccess) rc = LBMIB_ERR_HIP;
+  b->ev_inst.assign(n, nullptr);
+  for (int i = 0; i < n && !rc; ++i)
+    if (hipEventCreateWithFlags(&b->ev_inst[i], hipEventDisableTiming) != hipSuccess) rc = LBMIB_ERR_HIP;
+  if (!rc) rc = build_maps(b);
+  if (rc) {
+    lbmib_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_instance(lbmib_batch *b, int i, lbmib_ctx **ctx) {
+  if (!b || !ctx || i < 0 || i >= b->n) return LBMIB_ERR_ARG;
+  *ctx = b->inst[i];
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_init(lbmib_batch *b) {
+  if (!b) return LBMIB_ERR_ARG;
+  for (lbmib_ctx *c : b->inst) {
+    RCHK(lbmib_init_eqlbrmdistrfuncDF0(c));
+    RCHK(lbmib_init_df1(c));
+    RCHK(lbmib_init_df_inout(c));
+  }
+  for (lbmib_ctx *c : b->inst) RCHK(lbmib_sync(c));
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_set_tile(lbmib_batch *b, int waves_per_block, int nontemporal) {
+  if (!b || waves_per_block < 1 || waves_per_block > 8 || nontemporal < -1 || nontemporal > 3) return LBMIB_ERR_ARG;
+  for (lbmib_ctx *c : b->inst)
+    if (waves_per_row(c) > waves_per_block) return LBMIB_ERR_ARG;  // a block must hold a whole row
+  b->wpb = waves_per_block;
+  b->nt_mask = nontemporal;
+  return build_maps(b);
+}
+
+extern "C" int lbmib_batch_cells(lbmib_batch *b, double *cells, int *nodes) {
+  if (!b) return LBMIB_ERR_ARG;
+  if (cells) *cells = b->cells;
+  if (nodes) *nodes = b->nnodes;
+  return LBMIB_OK;
+}
+
+// Every instance must be at the same step, ready for a whole step (no
+// per-step entry point half done).
+static int batch_state_ok(const lbmib_batch *b) {
+  const lbmib_ctx *c0 = b->inst[0];
+  for (const lbmib_ctx *c : b->inst) {
+    if (c->init_bits != 7 || c->pending || !c->ib_clean || c->split_run || c->ib_ready) return LBMIB_ERR_STATE;
+    if (c->time != c0->time || c->cur != c0->cur || c->own != c0->own) return LBMIB_ERR_STATE;
+  }
+  return LBMIB_OK;
+}
+
+// e0/e1 bracket the whole run; with kev, events kev[2k], kev[2k+1] bracket
+// the fused launch of step k.
+static int batch_steps(lbmib_batch *b, int nsteps, hipEvent_t e0, hipEvent_t e1, const hipEvent_t *kev = nullptr) {
+  RCHK(batch_state_ok(b));
+  // the batch stream starts after the instances' own queued work
+  for (int i = 0; i < b->n; ++i) {
+    HIPCHK(hipEventRecord(b->ev_inst[i], b->inst[i]->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ev_inst[i], 0));
+  }
+  lbmib_ctx *c0 = b->inst[0];
+  int cur = c0->cur, own = c0->own, time = c0->time, slot = 0;
+  if (e0) HIPCHK(hipEventRecord(e0, b->stream));
+  for (int k = 0; k < nsteps; ++k) {
+    slot = time & 1;
+    RCHK(launch_fiber_batch(b->d_desc, b->d_map[kMapFiber], b->nblk[kMapFiber], slot, b->stream));
+    RCHK(launch_gather_batch(b->prec, b->d_desc, b->d_map[kMapGather], b->nblk[kMapGather], slot, b->mcap_max,
+                             b->stream));
+    if (kev) HIPCHK(hipEventRecord(kev[2 * k], b->stream));
+    RCHK(launch_collide_batch(b->prec, b->d_desc, b->d_map[kMapFused], b->nblk[kMapFused], b->wpb, cur, own, slot,
+                              b->nt_mask, b->stream));
+    if (kev) HIPCHK(hipEventRecord(kev[2 * k + 1], b->stream));
+    RCHK(launch_band_u_batch(b->prec, b->d_desc, b->d_map[kMapBandU], b->nblk[kMapBandU], 1 - cur, slot,
+                             b->stream));
+    RCHK(launch_move_batch(b->d_desc, b->d_map[kMapMove], b->nblk[kMapMove], 1 - slot, b->stream));
+    cur = 1 - cur;
+    own = 0;
+    ++time;
+  }
+  if (e1) HIPCHK(hipEventRecord(e1, b->stream));
+  // the instances' own streams (readbacks, status) follow the batch
+  HIPCHK(hipEventRecord(b->ev, b->stream));
+  for (lbmib_ctx *c : b->inst) {
+    HIPCHK(hipStreamWaitEvent(c->stream, b->ev, 0));
+    c->cur = cur;
+    c->own = own;
+    c->time = time;
+    c->pending = 0;
+    c->band_done = 0;
+    c->ib_clean = 1;
+    if (nsteps > 0 && c->nnodes > 0) c->spread_slot = slot;
+  }
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_step(lbmib_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return LBMIB_ERR_ARG;
+  return batch_steps(b, nsteps, nullptr, nullptr);
+}
+
+extern "C" int lbmib_batch_status(lbmib_batch *b) {
+  if (!b) return LBMIB_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (lbmib_ctx *c : b->inst) {
+    const int rc = lbmib_status(c);
+    if (rc) return rc;
+  }
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_time_steps(lbmib_batch *b, int nwarm, int nsteps, double *ms_total,
+                                      double *ms_fused_mean) {
+  if (!b || nwarm < 0 || nsteps <= 0) return LBMIB_ERR_ARG;
+  RCHK(batch_steps(b, nwarm, nullptr, nullptr));
+  RCHK(batch_steps(b, nsteps, b->ev0, b->ev1));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  if (ms_total) *ms_total = ms;
+  if (ms_fused_mean) {
+    // a second pass of the same length with an event pair per fused launch
+    std::vector<hipEvent_t> kev(2 * (size_t)nsteps, nullptr);
+    int rc = LBMIB_OK;
+    for (auto &e : kev)
+      if (!rc && hipEventCreate(&e) != hipSuccess) rc = LBMIB_ERR_HIP;
+    if (!rc) rc = batch_steps(b, nsteps, nullptr, nullptr, kev.data());
+    double acc = 0;
+    if (!rc && hipStreamSynchronize(b->stream) != hipSuccess) rc = LBMIB_ERR_HIP;
+    for (int k = 0; k < nsteps && !rc; ++k) {
+      float t = 0;
+      if (hipEventElapsedTime(&t, kev[2 * k], kev[2 * k + 1]) != hipSuccess) rc = LBMIB_ERR_HIP;
+      acc += t;
+    }
+    for (auto e : kev)
+      if (e) (void)hipEventDestroy(e);
+    RCHK(rc);
+    *ms_fused_mean = acc / nsteps;
+  }
+  return lbmib_batch_status(b);
+}

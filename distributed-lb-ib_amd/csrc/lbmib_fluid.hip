@@ -23,6 +23,7 @@
 //   F16/BF16  g * 2^8 in 16 bits; collision in FP32.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <vector>
 
@@ -349,6 +350,87 @@ __device__ __forceinline__ bool next_row(const Geometry &g, int sel, const int *
   }
 }
 
+// One lane of the buffer-addressed row path: the update of cell (x, row) by
+// a wave whose lanes hold consecutive x of the same row (SGPR row offsets).
+// Dead lanes of an active row store zeros (whole cache lines); x edges of a
+// wall or periodic x axis are left to k_collide_xedges.
+template <int P, bool NT, bool NTL>
+__device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc_t rin, __amdgpu_buffer_rsrc_t rout,
+                                         int own, const KConsts<typename Traits<P>::T> &c, int nsheets,
+                                         const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                                         const typename Traits<P>::FT *__restrict__ Fcur,
+                                         const typename Traits<P>::FT *__restrict__ Fprev,
+                                         const uint32_t *__restrict__ rowtab, int row, int x) {
+  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
+  constexpr uint32_t es = sizeof(S);
+  const uint32_t pitch_b = (uint32_t)g.pitch * es;
+  const uint32_t sy_b = (uint32_t)g.nx * es;
+  const bool edge = x == g.lo[0] || x == g.hi[0];
+  // frozen inlet/outlet planes; x edges of wall/periodic x go to k_collide_xedges
+  const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
+  // Filler lanes (dead x, and the x edges left to k_collide_xedges) store
+  // zeros, so that every store instruction writes whole cache lines of the
+  // row: partially written lines cost HBM write bandwidth.  Nothing reads
+  // a dead cell; k_collide_xedges rewrites its edge cells afterwards.
+  const bool filler = x < g.lo[0] || x > g.hi[0] || (edge && !frozen);
+  const uint32_t vo = (uint32_t)x * es;
+  const int y = row % g.ny, z = row / g.ny;
+  if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) return;  // dead row
+  const uint32_t row_b = (uint32_t)row * sy_b;  // byte offset of (0, y, z) in a plane
+  const long c0 = (long)x + (long)g.nx * row;
+  T v[19];
+  if (filler) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) v[q] = (T)0;
+  } else {
+    T d, rho, ux, uy, uz;
+    if (frozen) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
+      rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
+    } else {
+      // per-row source offsets (row_source_table): wrap and bounce-back
+      // of the y/z axes and the x shift folded in
+      if (own) {
+        uint32_t qb = row_b;  // first step: the cell's own populations
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+          v[q] = bld_h<P, NTL>(rin, vo, qb);
+          qb += pitch_b;
+        }
+      } else {
+        const uint32_t *rt = rowtab + (long)row * kRowTabStride;
+        uint32_t so[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) so[q] = rt[q];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[q] = bld_h<P, NTL>(rin, vo, so[q]);
+      }
+      if (own) {
+        rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
+      } else {
+        T fpx = 0, fpy = 0, fpz = 0;
+        if (in_any_box(box_prev, nsheets, x, y, z)) {
+          fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
+        }
+        lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
+      }
+    }
+    T fx = 0, fy = 0, fz = 0;
+    if (in_any_box(box_cur, nsheets, x, y, z)) {
+      fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
+    }
+    collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
+  }
+  uint32_t sb = row_b;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    bst_h<P, NT>(rout, vo, sb, v[q]);
+    sb += pitch_b;
+  }
+}
+
 // Row form: blocks walk whole (y, z) rows with a stride (one block per row
 // when the grid has as many blocks as rows).
 //
@@ -383,77 +465,11 @@ __global__ void __launch_bounds__(1024)
   } else {
     // one cell per thread: blockDim.x >= nx (checked by the launcher)
     const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
-    constexpr uint32_t es = sizeof(S);
-    const uint32_t pitch_b = (uint32_t)g.pitch * es;
-    const uint32_t sy_b = (uint32_t)g.nx * es;
     const int x = threadIdx.x;
     if (x >= g.nx) return;
-    const bool edge = x == g.lo[0] || x == g.hi[0];
-    // frozen inlet/outlet planes; x edges of wall/periodic x go to k_collide_xedges
-    const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
-    // Filler lanes (dead x, and the x edges left to k_collide_xedges) store
-    // zeros, so that every store instruction writes whole cache lines of the
-    // row: partially written lines cost HBM write bandwidth.  Nothing reads
-    // a dead cell; k_collide_xedges rewrites its edge cells afterwards.
-    const bool filler = x < g.lo[0] || x > g.hi[0] || (edge && !frozen);
-    const uint32_t vo = (uint32_t)x * es;
     int row;
-    for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x) {
-      const int y = row % g.ny, z = row / g.ny;
-      if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
-      const uint32_t row_b = (uint32_t)row * sy_b;  // byte offset of (0, y, z) in a plane
-      const long c0 = (long)x + (long)g.nx * row;
-      T v[19];
-      if (filler) {
-#pragma unroll
-        for (int q = 0; q < 19; ++q) v[q] = (T)0;
-      } else {
-        T d, rho, ux, uy, uz;
-        if (frozen) {
-#pragma unroll
-          for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
-          rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
-        } else {
-          // per-row source offsets (row_source_table): wrap and bounce-back
-          // of the y/z axes and the x shift folded in
-          if (own) {
-            uint32_t qb = row_b;  // first step: the cell's own populations
-#pragma unroll
-            for (int q = 0; q < 19; ++q) {
-              v[q] = bld_h<P, NTL>(rin, vo, qb);
-              qb += pitch_b;
-            }
-          } else {
-            const uint32_t *rt = rowtab + (long)row * kRowTabStride;
-            uint32_t so[19];
-#pragma unroll
-            for (int q = 0; q < 19; ++q) so[q] = rt[q];
-#pragma unroll
-            for (int q = 0; q < 19; ++q) v[q] = bld_h<P, NTL>(rin, vo, so[q]);
-          }
-          if (own) {
-            rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
-          } else {
-            T fpx = 0, fpy = 0, fpz = 0;
-            if (in_any_box(box_prev, nsheets, x, y, z)) {
-              fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
-            }
-            lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
-          }
-        }
-        T fx = 0, fy = 0, fz = 0;
-        if (in_any_box(box_cur, nsheets, x, y, z)) {
-          fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
-        }
-        collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
-      }
-      uint32_t sb = row_b;
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        bst_h<P, NT>(rout, vo, sb, v[q]);
-        sb += pitch_b;
-      }
-    }
+    for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x)
+      row_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, row, x);
   }
 }
 
@@ -692,6 +708,34 @@ __global__ void __launch_bounds__(512)
       }
     }
   }
+}
+
+// Batched ensemble: every instance's rows in one launch.  Block b serves
+// instance map[b] >> kBatchShift; each wave of it one row (or one 64-lane x
+// chunk of a row wider than 64: wpr waves per row), so the row offsets stay
+// wave-uniform for any mix of grid sizes; wpb waves per block is the tile
+// shape of the launch.  Instances need an inlet/outlet x axis (no x-edge
+// kernel).
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(512)
+    k_collide_batch(const BatchDesc *__restrict__ descs, const int *__restrict__ map, int wpb, int cur, int own,
+                    int slot) {
+  using T = typename Traits<P>::T;
+  using FT = typename Traits<P>::FT;
+  const int m = map[blockIdx.x];
+  const int inst = m >> kBatchShift, lb = m & ((1 << kBatchShift) - 1);
+  const BatchDesc &d = descs[inst];
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int rpb = wpb / d.wpr;
+  if (wave >= rpb * d.wpr) return;
+  const int row = lb * rpb + wave / d.wpr;
+  const int x = (wave % d.wpr) * 64 + lane;
+  if (row >= d.nrows || x >= d.g.nx) return;
+  const KConsts<T> &c = *reinterpret_cast<const KConsts<T> *>(d.kc);
+  const int *box = d.ib.box;
+  row_cell<P, NT, NTL>(d.g, make_rsrc(d.f[cur]), make_rsrc(d.f[1 - cur]), own, c, d.nsheets,
+                       box + slot * kMaxSheets * 6, box + (1 - slot) * kMaxSheets * 6, (const FT *)d.F[slot],
+                       (const FT *)d.F[1 - slot], d.rowtab, row, x);
 }
 
 // The x-edge cells (x = lo, hi) of every active row when x is a wall or a
@@ -1179,6 +1223,32 @@ int launch_totals(int prec, const Geometry &g, const void *f, int own_mode, cons
                                                     kconsts<P>(c), partials);
   }));
   k_totals_final<<<1, kTotThreads, 0, s>>>(partials, nb, out4);
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+
+int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[128]) {
+  memset(out, 0, 128);
+  LBMIB_DISPATCH(prec, ({
+    const KConsts<typename Traits<P>::T> k = kconsts<P>(c);
+    static_assert(sizeof(k) <= 128, "KConsts fits the batch descriptor");
+    memcpy(out, &k, sizeof(k));
+  }));
+  return LBMIB_OK;
+}
+
+int launch_collide_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int wpb, int cur, int own,
+                         int slot, int nt_mask, hipStream_t s) {
+  if (nblocks <= 0) return LBMIB_OK;
+  if (wpb < 1 || wpb > 8) return LBMIB_ERR_ARG;
+  const int nt = nt_mask < 0 ? 1 : (nt_mask & 1);
+  const int ntl = nt_mask < 0 ? 1 : ((nt_mask >> 1) & 1);
+  const dim3 b(64 * wpb);
+  LBMIB_DISPATCH(prec, ({
+    if (nt && ntl) k_collide_batch<P, true, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+    else if (nt) k_collide_batch<P, true, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+    else if (ntl) k_collide_batch<P, false, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+    else k_collide_batch<P, false, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+  }));
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 

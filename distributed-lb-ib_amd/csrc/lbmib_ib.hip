@@ -88,8 +88,9 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-__global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g, int mask, int slot) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g, int mask, int slot, int bid,
+                                           int nblk) {
+  const int n = bid * blockDim.x + threadIdx.x;
   const bool live = n < ib.nnodes;
   const int M = ib.nnodes;
   const double *__restrict__ pos = ib.pos;
@@ -199,10 +200,10 @@ __global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g,
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) ib.box_part[blockIdx.x * kMaxSheets * 6 + t] = sbox[t];
+  for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) ib.box_part[bid * kMaxSheets * 6 + t] = sbox[t];
   __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == gridDim.x - 1;
+  if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == nblk - 1;
   __syncthreads();
   if (!s_last) return;
   __threadfence();
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g,
   for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) sbox[t] = (t & 1) ? INT_MIN : INT_MAX;
   __syncthreads();
   const int nk = ib.nsheets * 6;
-  const int items = (int)gridDim.x * nk;
+  const int items = (int)nblk * nk;
   for (int base = 0; base < items; base += 8 * blockDim.x) {
     int v[8], kk[8];
 #pragma unroll
@@ -235,6 +236,10 @@ __global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g,
   __syncthreads();
   for (int k = threadIdx.x; k < nk; k += blockDim.x) ib.box[slot * kMaxSheets * 6 + k] = sbox[k];
   if (threadIdx.x == 0) atomicExch(ib.done, 0u);
+}
+
+__global__ void __launch_bounds__(kFiberThreads) k_fiber(IBState ib, Geometry g, int mask, int slot) {
+  fiber_body(ib, g, mask, slot, blockIdx.x, gridDim.x);
 }
 
 // -------------------------------------------------------------- k_gather
@@ -266,8 +271,8 @@ constexpr int kGatherMaxCand = 1536;   // ids, sorted ids, 3 starts: 30 KB of LD
 constexpr int kGatherStage = 256;      // candidates whose delta factors / forces are staged: 30 KB
 
 template <class FT>
-__global__ void __launch_bounds__(kGatherThreads)
-    k_gather(IBState ib, Geometry g, int slot, FT *__restrict__ F, int mcap) {
+__device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g, int slot, FT *__restrict__ F, int mcap,
+                                            int bid, int nblk) {
   // LDS: raw candidate ids, then the sorted ids and their support starts
   extern __shared__ double smem_d[];
   double *sw = smem_d;                      // [kGatherStage][12] delta factors
@@ -277,7 +282,7 @@ __global__ void __launch_bounds__(kGatherThreads)
   int *sx = ids + mcap, *sy = sx + mcap, *sz = sy + mcap;
   const int M = ib.nnodes;
   const double cf = 1.0 / (64.0 * 1.0 * 1.0 * 1.0);  // lbm_ib.c:1357
-  for (int w = blockIdx.x;; w += gridDim.x) {
+  for (int w = bid;; w += nblk) {
     int s, ty, tz;
     if (!tile_of(ib, slot, w, s, ty, tz)) break;
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
@@ -348,6 +353,12 @@ __global__ void __launch_bounds__(kGatherThreads)
   }
 }
 
+template <class FT>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_gather(IBState ib, Geometry g, int slot, FT *__restrict__ F, int mcap) {
+  gather_body<FT>(ib, g, slot, F, mcap, blockIdx.x, gridDim.x);
+}
+
 // ----------------------------------------------------------------- k_move
 //
 // Two kernels.  k_band_u evaluates compute_rho_and_u (lbm_ib.c:2498-2525) on
@@ -361,10 +372,10 @@ __global__ void __launch_bounds__(kGatherThreads)
 // X += dt u(X).
 
 template <int P>
-__global__ void __launch_bounds__(256)
-    k_band_u(IBState ib, Geometry g, const typename Traits<P>::S *__restrict__ f,
-             const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y, double u0z, double gz, int slot) {
-  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+__device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g, const typename Traits<P>::S *__restrict__ f,
+                                            const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y,
+                                            double u0z, double gz, int slot, int bid, int nblk) {
+  const long tid = (long)bid * blockDim.x + threadIdx.x, stride = (long)nblk * blockDim.x;
   for (int s = 0; s < ib.nsheets; ++s) {
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
     const int x0 = b[0], y0 = b[2], z0 = b[4];
@@ -401,12 +412,19 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+template <int P>
+__global__ void __launch_bounds__(256)
+    k_band_u(IBState ib, Geometry g, const typename Traits<P>::S *__restrict__ f,
+             const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y, double u0z, double gz, int slot) {
+  band_u_body<P>(ib, g, f, F, u0x, u0y, u0z, gz, slot, blockIdx.x, gridDim.x);
+}
+
 constexpr int kMoveThreads = 256;
 
-__global__ void __launch_bounds__(kMoveThreads) k_move(IBState ib, Geometry g, int reset_slot) {
+__device__ __forceinline__ void move_body(const IBState &ib, const Geometry &g, int reset_slot, int bid, int nblk) {
   __shared__ double prod[kMoveThreads / 64][3][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int n = blockIdx.x * (kMoveThreads / 64) + wv;
+  const int n = bid * (kMoveThreads / 64) + wv;
   const int M = ib.nnodes;
   const bool live = n < M && ib.start[n] >= 0;
   if (live) {
@@ -430,13 +448,17 @@ __global__ void __launch_bounds__(kMoveThreads) k_move(IBState ib, Geometry g, i
     ib.pos[(long)lane * M + n] += 1 * acc;  // X += dt * u(X), lbm_ib.c:2697-2699
   }
   // clear the bins for the next spread and the box slot it will fill
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = bid * blockDim.x + threadIdx.x;
   const int nbins = ib.nby * ib.nbz;
-  for (int b = tid; b < nbins; b += gridDim.x * blockDim.x) ib.bin_count[b] = 0;
+  for (int b = tid; b < nbins; b += nblk * blockDim.x) ib.bin_count[b] = 0;
   if (tid < ib.nsheets * 6) {
     const int s = tid / 6, k = tid % 6;
     ib.box[(reset_slot * kMaxSheets + s) * 6 + k] = (k & 1) ? INT_MIN : INT_MAX;
   }
+}
+
+__global__ void __launch_bounds__(kMoveThreads) k_move(IBState ib, Geometry g, int reset_slot) {
+  move_body(ib, g, reset_slot, blockIdx.x, gridDim.x);
 }
 
 __global__ void k_ib_reset(IBState ib, int reset_bins, int slot_mask) {
@@ -454,6 +476,54 @@ __global__ void k_ib_reset(IBState ib, int reset_bins, int slot_mask) {
   }
 }
 
+// ------------------------------------------------------ batched ensemble
+//
+// The same kernels over every instance of an ensemble in one launch: block b
+// serves instance map[b] >> kBatchShift as its block map[b] & mask of that
+// instance's own grid (fiber_blocks, gather_blocks, band_u_blocks,
+// move_blocks of its node count).
+
+__device__ __forceinline__ void batch_block(const int *map, int &inst, int &lb) {
+  const int m = map[blockIdx.x];
+  inst = m >> kBatchShift;
+  lb = m & ((1 << kBatchShift) - 1);
+}
+
+__global__ void __launch_bounds__(kFiberThreads) k_fiber_batch(const BatchDesc *__restrict__ descs,
+                                                               const int *__restrict__ map, int slot) {
+  int inst, lb;
+  batch_block(map, inst, lb);
+  const BatchDesc &d = descs[inst];
+  fiber_body(d.ib, d.g, FK_ALL, slot, lb, fiber_blocks(d.ib.nnodes));
+}
+
+template <class FT>
+__global__ void __launch_bounds__(kGatherThreads) k_gather_batch(const BatchDesc *__restrict__ descs,
+                                                                 const int *__restrict__ map, int slot) {
+  int inst, lb;
+  batch_block(map, inst, lb);
+  const BatchDesc &d = descs[inst];
+  gather_body<FT>(d.ib, d.g, slot, (FT *)d.F[slot], d.mcap, lb, gather_blocks(d.ib.nnodes));
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_band_u_batch(const BatchDesc *__restrict__ descs,
+                                                      const int *__restrict__ map, int cur_new, int slot) {
+  int inst, lb;
+  batch_block(map, inst, lb);
+  const BatchDesc &d = descs[inst];
+  band_u_body<P>(d.ib, d.g, (const typename Traits<P>::S *)d.f[cur_new], (const typename Traits<P>::FT *)d.F[slot],
+                 d.u0[0], d.u0[1], d.u0[2], d.gz, slot, lb, band_u_blocks(d.ib.nnodes));
+}
+
+__global__ void __launch_bounds__(kMoveThreads) k_move_batch(const BatchDesc *__restrict__ descs,
+                                                             const int *__restrict__ map, int reset_slot) {
+  int inst, lb;
+  batch_block(map, inst, lb);
+  const BatchDesc &d = descs[inst];
+  move_body(d.ib, d.g, reset_slot, lb, move_blocks(d.ib.nnodes));
+}
+
 // ------------------------------------------------------------- launchers
 
 static inline int grid_for(long n, int b) { return (int)((n + b - 1) / b); }
@@ -466,11 +536,9 @@ int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipSt
 
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
-  const int mcap = min(kCandBins * kCandBins * ib.cap, kGatherMaxCand);
+  const int mcap = gather_mcap(ib.cap);
   const size_t shm = sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * 5 * (size_t)mcap;
-  // about one tile (4x4 bin) per 16 nodes at unit spacing; the kernel strides
-  // over any further tiles
-  const int blocks = max(64, min(4096, ib.nnodes / 8));
+  const int blocks = gather_blocks(ib.nnodes);
   if (prec == LBMIB_F64) k_gather<double><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (double *)F, mcap);
   else k_gather<float><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (float *)F, mcap);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
@@ -480,8 +548,7 @@ int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, c
                 const FluidConsts<double> &c, int reset_slot, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
   const int slot = 1 - reset_slot;  // the box of this step's spread
-  // about 4 support-box cells per node for a flat sheet; the kernel strides
-  const int ublocks = max(64, min(8192, grid_for(4L * ib.nnodes, 256)));
+  const int ublocks = band_u_blocks(ib.nnodes);
   switch (prec) {
     case LBMIB_F64:
       k_band_u<LBMIB_F64><<<ublocks, 256, 0, s>>>(ib, g, (const double *)f, (const double *)F, c.u0x, c.u0y, c.u0z,
@@ -502,8 +569,7 @@ int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, c
     default:
       return LBMIB_ERR_ARG;
   }
-  const int nodes_per_block = kMoveThreads / 64;
-  k_move<<<grid_for(ib.nnodes, nodes_per_block), kMoveThreads, 0, s>>>(ib, g, reset_slot);
+  k_move<<<move_blocks(ib.nnodes), kMoveThreads, 0, s>>>(ib, g, reset_slot);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
@@ -511,6 +577,39 @@ int launch_ib_reset(const IBState &ib, int reset_bins, int reset_slot_mask, hipS
   const int nbins = ib.nby * ib.nbz;
   const long work = nbins > kMaxSheets * 6 ? nbins : kMaxSheets * 6;
   k_ib_reset<<<grid_for(work, 256), 256, 0, s>>>(ib, reset_bins, reset_slot_mask);
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+
+int gather_mcap(int cap) { return min(kCandBins * kCandBins * cap, kGatherMaxCand); }
+
+int launch_fiber_batch(const BatchDesc *d, const int *map, int nblocks, int slot, hipStream_t s) {
+  if (nblocks <= 0) return LBMIB_OK;
+  k_fiber_batch<<<nblocks, kFiberThreads, 0, s>>>(d, map, slot);
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+int launch_gather_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int slot, int mcap_max,
+                        hipStream_t s) {
+  if (nblocks <= 0) return LBMIB_OK;
+  const size_t shm = sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * 5 * (size_t)mcap_max;
+  if (prec == LBMIB_F64) k_gather_batch<double><<<nblocks, kGatherThreads, shm, s>>>(d, map, slot);
+  else k_gather_batch<float><<<nblocks, kGatherThreads, shm, s>>>(d, map, slot);
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+int launch_band_u_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int cur_new, int slot,
+                        hipStream_t s) {
+  if (nblocks <= 0) return LBMIB_OK;
+  switch (prec) {
+    case LBMIB_F64: k_band_u_batch<LBMIB_F64><<<nblocks, 256, 0, s>>>(d, map, cur_new, slot); break;
+    case LBMIB_F32: k_band_u_batch<LBMIB_F32><<<nblocks, 256, 0, s>>>(d, map, cur_new, slot); break;
+    case LBMIB_F16: k_band_u_batch<LBMIB_F16><<<nblocks, 256, 0, s>>>(d, map, cur_new, slot); break;
+    case LBMIB_BF16: k_band_u_batch<LBMIB_BF16><<<nblocks, 256, 0, s>>>(d, map, cur_new, slot); break;
+    default: return LBMIB_ERR_ARG;
+  }
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+int launch_move_batch(const BatchDesc *d, const int *map, int nblocks, int reset_slot, hipStream_t s) {
+  if (nblocks <= 0) return LBMIB_OK;
+  k_move_batch<<<nblocks, kMoveThreads, 0, s>>>(d, map, reset_slot);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 

@@ -107,7 +107,45 @@ struct IBState {
 // k_fiber: one thread per node, kFiberThreads per block; each block leaves one
 // partial support box per sheet in IBState::box_part.
 constexpr int kFiberThreads = 64;
-inline int fiber_blocks(int nnodes) { return (nnodes + kFiberThreads - 1) / kFiberThreads; }
+__host__ __device__ inline int fiber_blocks(int nnodes) { return (nnodes + kFiberThreads - 1) / kFiberThreads; }
+// Grids of the other IB kernels for a sheet set of nnodes nodes (the kernels
+// stride over any further work).
+__host__ __device__ inline int gather_blocks(int nnodes) {
+  const int b = nnodes / 8;  // about one 4x4-bin tile per 16 nodes at unit spacing
+  return b < 16 ? 16 : (b > 4096 ? 4096 : b);
+}
+__host__ __device__ inline int band_u_blocks(int nnodes) {
+  const int b = (4 * nnodes + 255) / 256;  // about 4 support-box cells per node for a flat sheet
+  return b < 16 ? 16 : (b > 8192 ? 8192 : b);
+}
+__host__ __device__ inline int move_blocks(int nnodes) { return (nnodes + 3) / 4; }  // one wave per node
+
+// One instance of a batched ensemble, as the batched kernels read it.
+struct BatchDesc {
+  Geometry g;
+  IBState ib;
+  const void *f[2];         // population buffers (ping-pong)
+  void *F[2];               // force fields by step parity
+  const uint32_t *rowtab;   // row source table (row_source_table)
+  int nsheets, mcap;        // sheets; gather candidate capacity
+  int wpr;                  // waves per row of the fused launch (ceil(nx / 64))
+  int nrows;                // ny * nz
+  double u0[3], gz;         // init_gv velocity, gravity
+  alignas(16) unsigned char kc[128];  // KConsts<T> of the batch's precision
+};
+// Block map of a batched launch: one int per block, instance << kBatchShift |
+// block index within the instance.
+constexpr int kBatchShift = 20;
+int launch_collide_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int wpb, int cur, int own,
+                         int slot, int nt_mask, hipStream_t s);
+int launch_fiber_batch(const BatchDesc *d, const int *map, int nblocks, int slot, hipStream_t s);
+int launch_gather_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int slot, int mcap_max,
+                        hipStream_t s);
+int launch_band_u_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int cur_new, int slot,
+                        hipStream_t s);
+int launch_move_batch(const BatchDesc *d, const int *map, int nblocks, int reset_slot, hipStream_t s);
+int gather_mcap(int cap);
+int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[128]);
 
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 // Launch shape of the fused kernel (all forms compute identical bits).

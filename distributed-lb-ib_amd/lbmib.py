@@ -36,7 +36,10 @@ ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lb
                "lbmib_get_time", "lbmib_get_fibers", "lbmib_get_macroscopic", "lbmib_get_df",
                "lbmib_get_force", "lbmib_get_totals", "lbmib_get_band_box", "lbmib_time_steps",
                "lbmib_bytes_per_cell", "lbmib_probe_copy_bandwidth",
-               "lbmib_set_kernel_variant", "lbmib_set_overlap", "lbmib_timed_kernel_cells") + tuple("lbmib_" + n for n in CTX_CALLS)
+               "lbmib_set_kernel_variant", "lbmib_set_overlap", "lbmib_timed_kernel_cells",
+               "lbmib_batch_create", "lbmib_batch_destroy", "lbmib_batch_instance", "lbmib_batch_init",
+               "lbmib_batch_step", "lbmib_batch_set_tile", "lbmib_batch_cells", "lbmib_batch_status",
+               "lbmib_batch_time_steps") + tuple("lbmib_" + n for n in CTX_CALLS)
 
 
 class LbmibError(RuntimeError):
@@ -89,6 +92,16 @@ def load_library(path=LIB_PATH):
     L.lbmib_set_kernel_variant.argtypes = [vp, i, i, i]
     L.lbmib_set_overlap.argtypes = [vp, i]
     L.lbmib_timed_kernel_cells.argtypes = [vp, dp]
+    L.lbmib_batch_create.argtypes = [i, ctypes.POINTER(FluidParams), ip, ctypes.POINTER(SheetParams),
+                                     ctypes.POINTER(vp)]
+    L.lbmib_batch_destroy.argtypes = [vp]
+    L.lbmib_batch_instance.argtypes = [vp, i, ctypes.POINTER(vp)]
+    L.lbmib_batch_init.argtypes = [vp]
+    L.lbmib_batch_step.argtypes = [vp, i]
+    L.lbmib_batch_set_tile.argtypes = [vp, i, i]
+    L.lbmib_batch_cells.argtypes = [vp, dp, ip]
+    L.lbmib_batch_status.argtypes = [vp]
+    L.lbmib_batch_time_steps.argtypes = [vp, i, i, dp, dp]
     for n in CTX_CALLS:
         getattr(L, "lbmib_" + n).argtypes = [vp]
     for n in ALL_SYMBOLS:
@@ -163,6 +176,11 @@ class Sheet:
 
 # ------------------------------------------------------------------ engine
 
+def fluid_params(nx, ny, nz, bc, precision, tau, rho0, u0, quirks, g=0.0):
+    return FluidParams(nx, ny, nz, bc[0], bc[1], bc[2], PRECISIONS[precision], int(bool(quirks)),
+                       tau, rho0, (ctypes.c_double * 3)(*u0), g)
+
+
 class Engine:
     """One simulation on the GPU behind the C-ABI.
 
@@ -175,18 +193,34 @@ class Engine:
                  rho0=1.0, u0=(0.01, 0.0, 0.0), quirks=True, sheets=(), g=0.0, preshift=True,
                  preshift_all=True):
         self.L = load_library()
-        self.nx, self.ny, self.nz = nx, ny, nz
-        self.shape = (nz, ny, nx)
-        self.precision = precision
-        self.sheets = list(sheets)
-        fp = FluidParams(nx, ny, nz, bc[0], bc[1], bc[2], PRECISIONS[precision], int(bool(quirks)),
-                         tau, rho0, (ctypes.c_double * 3)(*u0), g)
+        self._owner = True
+        self._describe(nx, ny, nz, precision, sheets)
+        fp = fluid_params(nx, ny, nz, bc, precision, tau, rho0, u0, quirks, g)
         sp = (SheetParams * max(1, len(self.sheets)))()
         for k, s in enumerate(self.sheets):
             sp[k] = SheetParams(s.rows, s.cols, s.width, s.height, s.ks, s.kb)
         h = ctypes.c_void_p()
         self._check(self.L.lbmib_create(ctypes.byref(fp), len(self.sheets), sp, ctypes.byref(h)), "lbmib_create")
         self.h = h
+        self.upload_sheets(u0, preshift, preshift_all)
+
+    def _describe(self, nx, ny, nz, precision, sheets):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (nz, ny, nx)
+        self.precision = precision
+        self.sheets = list(sheets)
+
+    @classmethod
+    def _view(cls, L, handle, nx, ny, nz, precision, sheets):
+        """A non-owning view of a context that a batch owns."""
+        e = cls.__new__(cls)
+        e.L = L
+        e._owner = False
+        e._describe(nx, ny, nz, precision, sheets)
+        e.h = handle
+        return e
+
+    def upload_sheets(self, u0, preshift=True, preshift_all=True):
         for k, s in enumerate(self.sheets):
             xyz = s.xyz.copy()
             # init_gv shifts the sheet by u_l*dt before the first step
@@ -202,7 +236,8 @@ class Engine:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.lbmib_destroy(self.h)
+            if getattr(self, "_owner", True):
+                self.L.lbmib_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -314,3 +349,86 @@ def probe_copy_bandwidth(nbytes=4e9, reps=5):
     if rc != OK:
         raise LbmibError(rc, "probe_copy_bandwidth")
     return out.value
+
+
+class Batch:
+    """An ensemble of independent simulations stepped together: each kernel
+    of a step runs as one launch over all instances (lbmib_batch_*).
+
+    ``configs``: one dict of Engine keyword arguments per instance (nx, ny,
+    nz, bc, precision, tau, rho0, u0, quirks, sheets, g).  ``instances`` are
+    Engine views of the member contexts for readbacks and status.
+    """
+
+    def __init__(self, configs, preshift=True, preshift_all=True):
+        self.L = load_library()
+        n = len(configs)
+        fps = (FluidParams * n)()
+        nsh = (ctypes.c_int * n)()
+        all_sheets = [s for c in configs for s in c.get("sheets", ())]
+        sps = (SheetParams * max(1, len(all_sheets)))()
+        for i, c in enumerate(configs):
+            fps[i] = fluid_params(c["nx"], c["ny"], c["nz"], c.get("bc", (BC_INOUT, BC_WALL, BC_WALL)),
+                                  c.get("precision", "f32"), c.get("tau", 0.6), c.get("rho0", 1.0),
+                                  c.get("u0", (0.01, 0.0, 0.0)), c.get("quirks", True), c.get("g", 0.0))
+            nsh[i] = len(c.get("sheets", ()))
+        for k, s in enumerate(all_sheets):
+            sps[k] = SheetParams(s.rows, s.cols, s.width, s.height, s.ks, s.kb)
+        h = ctypes.c_void_p()
+        rc = self.L.lbmib_batch_create(n, fps, nsh, sps, ctypes.byref(h))
+        if rc != OK:
+            raise LbmibError(rc, "lbmib_batch_create")
+        self.h = h
+        self.instances = []
+        for i, c in enumerate(configs):
+            ch = ctypes.c_void_p()
+            self._check(self.L.lbmib_batch_instance(self.h, i, ctypes.byref(ch)), "batch_instance")
+            e = Engine._view(self.L, ch, c["nx"], c["ny"], c["nz"], c.get("precision", "f32"), c.get("sheets", ()))
+            e.upload_sheets(c.get("u0", (0.01, 0.0, 0.0)), preshift, preshift_all)
+            self.instances.append(e)
+
+    _check = staticmethod(Engine._check)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for e in self.instances:
+                e.h = None
+            self.L.lbmib_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self):
+        self._check(self.L.lbmib_batch_init(self.h), "batch_init")
+        return self
+
+    def step(self, n=1):
+        self._check(self.L.lbmib_batch_step(self.h, n), "batch_step")
+
+    def status(self):
+        return self.L.lbmib_batch_status(self.h)
+
+    def set_tile(self, waves_per_block, nontemporal=-1):
+        self._check(self.L.lbmib_batch_set_tile(self.h, waves_per_block, nontemporal), "batch_set_tile")
+
+    def cells(self):
+        c, m = ctypes.c_double(), ctypes.c_int()
+        self._check(self.L.lbmib_batch_cells(self.h, ctypes.byref(c), ctypes.byref(m)), "batch_cells")
+        return c.value, m.value
+
+    def time_steps(self, nwarm, nsteps, fused=True):
+        """(total ms of nsteps, mean ms of one batched fused launch or None)."""
+        t, f = ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.lbmib_batch_time_steps(self.h, nwarm, nsteps, ctypes.byref(t),
+                                                  ctypes.byref(f) if fused else None), "batch_time_steps")
+        return t.value, (f.value if fused else None)

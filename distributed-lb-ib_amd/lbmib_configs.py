@@ -68,6 +68,36 @@ def c3(precision="bf16", seed=3, quirks=True):
                 u0=(u_in, 0.0, 0.0), quirks=quirks, sheets=sheets), 2000
 
 
+C4_EDGES = (32, 48, 64, 96, 128)
+
+
+def c4(precision="f32", seed=4, instances=64, quirks=True):
+    """Batched ensemble: `instances` independent cubic grids with the edge
+    drawn from {32, 48, 64, 96, 128}, tau ~ U[0.55, 1.0], one sheet of 16-64
+    nodes per side (spacing 1, at most edge - 8 so it fits between the
+    walls), Ks = Kb ~ log-uniform [1e-4, 1e-2], centre x ~ U[edge/4, edge/2]
+    and y, z placed uniformly where the sheet fits, jitter 0.05; inlet/outlet
+    x (u_in 0.01), periodic y, walls z.  Every draw comes from SplitMix64(seed)
+    in instance order.  500 steps."""
+    u = uniform_stream(seed)
+    out = []
+    for i in range(instances):
+        edge = C4_EDGES[min(int(next(u) * len(C4_EDGES)), len(C4_EDGES) - 1)]
+        tau = 0.55 + 0.45 * next(u)
+        hi = min(64, edge - 8)
+        side = 16 + min(int(next(u) * (hi - 16 + 1)), hi - 16)
+        k = 10.0 ** (-4.0 + 2.0 * next(u))
+        half = (side - 1) / 2.0
+        xc = edge / 4.0 + (edge / 4.0) * next(u)
+        lo_c, hi_c = half + 4.0, edge - 1.0 - half - 4.0
+        yc = lo_c + (hi_c - lo_c) * next(u)
+        zc = lo_c + (hi_c - lo_c) * next(u)
+        sheet = Sheet.centred(side, side, 1.0, (xc, yc, zc), k, k, jitter=0.05, seed=seed * 1000 + i)
+        out.append(dict(nx=edge, ny=edge, nz=edge, bc=BASE_BC, precision=precision, tau=tau, rho0=1.0,
+                        u0=(0.01, 0.0, 0.0), quirks=quirks, sheets=[sheet]))
+    return out, 500
+
+
 def scaled(cfg, n, rows, centre_x):
     """A small variant of a workload with the same physics, for parity tests."""
     kw = dict(cfg)

@@ -174,6 +174,38 @@ int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes);
  * kernel's achieved bandwidth is compared against. */
 int lbmib_probe_copy_bandwidth(double bytes, int reps, double *gbps);
 
+/* ---- batched ensembles (not in the reference) ----
+ * n independent simulations -- each with its own grid, fluid constants and
+ * sheets -- stepped together: each kernel of a step (fiber forces,
+ * spreading, fused collide-stream, band velocity, move) runs as one launch
+ * over all instances, whose blocks map to (instance, block of that
+ * instance).  Each instance is an lbmib_ctx (lbmib_batch_instance) for
+ * positions, readbacks and status, and its results are those of lbmib_step
+ * on it alone, bit for bit.  All instances share the precision and need an
+ * inlet/outlet x axis and nx <= 1024.  Not in the reference: the ensemble
+ * form of the workload. */
+typedef struct lbmib_batch lbmib_batch;
+/* sheets: the sheets of every instance in order, nsheets[i] of instance i. */
+int lbmib_batch_create(int n, const lbmib_fluid_params *fluids, const int *nsheets,
+                       const lbmib_sheet_params *sheets, lbmib_batch **out);
+int lbmib_batch_destroy(lbmib_batch *b); /* destroys the instances too */
+int lbmib_batch_instance(lbmib_batch *b, int i, lbmib_ctx **ctx);
+/* the three init calls of every instance (after their fiber positions) */
+int lbmib_batch_init(lbmib_batch *b);
+int lbmib_batch_step(lbmib_batch *b, int nsteps);
+/* Tile shape of the batched fused launch: 64-lane waves per block (one row,
+ * or 64-cell chunk of a row, per wave; a block holds whole rows) and the
+ * nontemporal mask as in lbmib_set_kernel_variant.  bench.py autotunes it. */
+int lbmib_batch_set_tile(lbmib_batch *b, int waves_per_block, int nontemporal);
+/* Grid cells and fiber nodes over all instances. */
+int lbmib_batch_cells(lbmib_batch *b, double *cells, int *nodes);
+/* First non-zero status of the instances (synchronising). */
+int lbmib_batch_status(lbmib_batch *b);
+/* nwarm untimed batched steps, then nsteps timed with HIP events (total
+ * milliseconds), then, if ms_fused_mean is given, nsteps more with an event
+ * pair around each batched fused launch (its mean milliseconds). */
+int lbmib_batch_time_steps(lbmib_batch *b, int nwarm, int nsteps, double *ms_total, double *ms_fused_mean);
+
 #ifdef __cplusplus
 }
 #endif

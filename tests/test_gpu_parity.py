@@ -360,3 +360,66 @@ def test_cli_replays_reference_fixture_refA(tmp_path, fixtures_meta):
         assert np.array_equal(ij_got, ij_ref)
         assert np.abs(got[:, :3] - ref[:, :3]).max() <= 1e-12
         assert np.abs(got[:, 3:] - ref[:, 3:]).max() <= 1e-9 * max(np.abs(ref[:, 3:]).max(), 1e-12) + 1e-20
+
+
+# ------------------------------------------------------- batched ensembles
+
+def _ensemble(lbmib, precision):
+    """Five instances of uneven size: one wider than 64 cells (two waves per
+    row), one without a sheet, different tau and stiffness."""
+    out = []
+    for k, (n, rows, tau, kk) in enumerate(((24, 8, 0.6, 1e-3), (40, 12, 0.8, 3e-3), (66, 14, 0.55, 1e-4),
+                                            (32, 0, 0.7, 0.0), (48, 16, 0.95, 1e-2))):
+        sheets = []
+        if rows:
+            sheets = [lbmib.Sheet.centred(rows, rows, 1.0, (n / 3.0, n / 2.0 + 0.3, n / 2.0 - 0.2), kk, kk,
+                                          jitter=0.05, seed=20 + k)]
+        out.append(dict(nx=n, ny=n + 2, nz=n - 2, bc=(BC_INOUT, BC_PERIODIC, BC_WALL), precision=precision,
+                        tau=tau, rho0=1.0, u0=(0.01, 0.0, 0.0), quirks=True, sheets=sheets))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16", "f64"])
+def test_batch_matches_single_instances(lbmib, precision):
+    """Each member of a batched ensemble ends with the bits of the same
+    simulation stepped alone."""
+    cfgs = _ensemble(lbmib, precision)
+    with lbmib.Batch(cfgs) as b:
+        b.init()
+        b.step(12)
+        assert b.status() == 0
+        got = [(e.df(), e.fibers(0) if e.sheets else None) for e in b.instances]
+    for cfg, (df, fib) in zip(cfgs, got):
+        with lbmib.Engine(**cfg) as e:
+            e.init()
+            e.step(12)
+            assert e.status() == 0
+            assert np.array_equal(df, e.df())
+            if fib is not None:
+                assert np.array_equal(fib, e.fibers(0))
+
+
+def test_batch_tile_shapes_are_bitwise_identical(lbmib):
+    cfgs = _ensemble(lbmib, "f32")
+    outs = []
+    for wpb, nt in ((2, 3), (4, 1), (8, 0)):
+        with lbmib.Batch(cfgs) as b:
+            b.set_tile(wpb, nt)
+            b.init()
+            b.step(10)
+            assert b.status() == 0
+            outs.append([e.df() for e in b.instances])
+    for o in outs[1:]:
+        for a, r in zip(o, outs[0]):
+            assert np.array_equal(a, r)
+
+
+def test_batch_rejects_unsupported_members(lbmib):
+    cfgs = _ensemble(lbmib, "f32")
+    cfgs[1] = dict(cfgs[1], bc=(BC_WALL, BC_PERIODIC, BC_WALL))
+    with pytest.raises(lbmib.LbmibError):
+        lbmib.Batch(cfgs)
+    cfgs = _ensemble(lbmib, "f32")
+    cfgs[2] = dict(cfgs[2], precision="f64")
+    with pytest.raises(lbmib.LbmibError):
+        lbmib.Batch(cfgs)
