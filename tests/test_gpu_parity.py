@@ -423,3 +423,67 @@ def test_batch_rejects_unsupported_members(lbmib):
     cfgs[2] = dict(cfgs[2], precision="f64")
     with pytest.raises(lbmib.LbmibError):
         lbmib.Batch(cfgs)
+
+
+# -------------------------------------------- the benchmark workloads
+
+def test_c1_workload_f64_matches_oracle(configs):
+    """The CPU-runnable workload (64^3, 26x26 sheet, inlet/outlet x, periodic
+    y, walls z, tau 0.7) for its full 200 steps: the FP64 GPU path against
+    the FP64 oracle."""
+    import lbmib
+    kw, steps = configs.c1("f64")
+    o = oracle_for(kw)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(steps)
+        assert o.step(steps) == 0
+        assert e.status() == 0
+        rho, vel = e.macroscopic()
+        got = e.fibers(0)
+    ref = o.nodes(0)
+    print(f"c1 f64: velocity {rel(vel, o.vel()):.3e}, rho {rel(rho, o.rho()):.3e}, "
+          f"positions {np.abs(got[..., :3] - ref[..., :3]).max():.3e}")
+    assert rel(vel, o.vel()) < 1e-10
+    assert rel(rho, o.rho()) < 1e-13
+    assert np.abs(got[..., :3] - ref[..., :3]).max() < 1e-11
+
+
+def _run(lbmib, kw, steps):
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(steps)
+        assert e.status() == 0
+        _, vel = e.macroscopic()
+        return vel, [e.fibers(k)[..., :3] for k in range(len(kw["sheets"]))]
+
+
+def test_c2_workload_f32_within_1e5_of_f64(lbmib, configs):
+    """256^3 with the 104x104 sheet, 200 steps: FP32 storage against the FP64
+    path (itself pinned to the oracle) -- velocity error relative to max|u|
+    within 1e-5."""
+    kw64, _ = configs.c2("f64")
+    kw32, _ = configs.c2("f32")
+    v64, x64 = _run(lbmib, kw64, 200)
+    v32, x32 = _run(lbmib, kw32, 200)
+    err = rel(v32, v64)
+    print(f"c2 f32 vs f64: velocity {err:.3e}, positions {np.abs(x32[0] - x64[0]).max():.3e}")
+    assert err < 1e-5
+    assert np.abs(x32[0] - x64[0]).max() < 1e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("f16", 1e-3), ("bf16", 1e-2)])
+def test_c3_workload_16bit_vs_f64(lbmib, configs, precision, tol):
+    """512x256x256 with two 128x128 sheets, 100 steps: 16-bit storage against
+    the FP64 path (which runs the general per-cell kernel here: its arrays
+    pass the 4 GiB of the buffer-addressed path), per field."""
+    kw64, _ = configs.c3("f64")
+    kw16, _ = configs.c3(precision)
+    v64, x64 = _run(lbmib, kw64, 100)
+    v16, x16 = _run(lbmib, kw16, 100)
+    errs = [rel(v16[a], v64[a]) for a in range(3)]
+    err = rel(v16, v64)
+    dx = max(np.abs(a - b).max() for a, b in zip(x16, x64))
+    print(f"c3 {precision} vs f64: velocity {err:.3e} (per component {errs[0]:.2e} {errs[1]:.2e} {errs[2]:.2e}), "
+          f"positions {dx:.3e}")
+    assert err < tol
