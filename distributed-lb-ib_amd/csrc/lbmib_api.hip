@@ -171,17 +171,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   alloc(&ctx->d_tab, sizeof(double) * 19);
   alloc((void **)&ctx->d_rect, sizeof(int) * kRectBufInts);
   if (!rc) {
-    // kernel units: f (F64), f - w (F32), 2^8 (f - w) (16-bit storage)
-    double td[19];
-    float tf[19];
-    const double sc = (p.precision == LBMIB_F16 || p.precision == LBMIB_BF16) ? (double)kDevScale16 : 1.0;
-    for (int q = 0; q < 19; ++q) {
-      td[q] = c.tab_in[q];
-      tf[q] = (float)(sc * c.tab_in[q]);
-    }
-    if (hipMemcpy(ctx->d_tab, dev ? (const void *)tf : (const void *)td, dev ? sizeof(tf) : sizeof(td),
-                  hipMemcpyHostToDevice) != hipSuccess)
-      rc = LBMIB_ERR_HIP;
+    rc = upload_frozen_table(p.precision, c, ctx->d_tab);
     c.dtab = ctx->d_tab;
   }
   if (!rc && row_path_buffered(g, ctx->esz)) {

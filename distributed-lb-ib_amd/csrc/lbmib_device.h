@@ -27,7 +27,7 @@ struct Traits<LBMIB_F64> {
 template <>
 struct Traits<LBMIB_F32> {
   using S = float;
-  using T = float;
+  using T = double;  // FP32 storage, FP64 collision (no rounding drift beyond the stores)
   using FT = float;
   static constexpr bool dev = true;
   static constexpr float scale = 1.0f, inv_scale = 1.0f;

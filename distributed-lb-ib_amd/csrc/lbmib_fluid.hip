@@ -74,14 +74,18 @@ __device__ __forceinline__ double collide_exact(int q, double fq, double rho, do
   return fq * c.omc + c.om * eq + c.wb[kCLS[q]] * (a / c.cs2 + cu / c.cs4 * cf);
 }
 
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // c_q . (a, b, c) with only the non-zero lattice components (q is a
 // compile-time constant after unrolling).
-__device__ __forceinline__ float cdot(int q, float a, float b, float c) {
-  float r = 0.0f;
+template <class T>
+__device__ __forceinline__ T cdot(int q, T a, T b, T c) {
+  T r = 0;
   bool first = true;
   if (kCX[q]) { r = kCX[q] > 0 ? a : -a; first = false; }
-  if (kCY[q]) { const float t = kCY[q] > 0 ? b : -b; r = first ? t : r + t; first = false; }
-  if (kCZ[q]) { const float t = kCZ[q] > 0 ? c : -c; r = first ? t : r + t; }
+  if (kCY[q]) { const T t = kCY[q] > 0 ? b : -b; r = first ? t : r + t; first = false; }
+  if (kCZ[q]) { const T t = kCZ[q] > 0 ? c : -c; r = first ? t : r + t; }
   return r;
 }
 
@@ -110,7 +114,8 @@ __device__ __forceinline__ void lagged_moments(const typename Traits<P>::T v[19]
     uz = (s4 + 0.5 * (fpz + rho * c.g)) / s1;
   } else {
     // sums of h = s (f - w); rho = 1 + Sum(h)/s
-    float D = v[0], MX = v[1], MY = v[6], MZ = v[3];
+    using T = typename Traits<P>::T;
+    T D = v[0], MX = v[1], MY = v[6], MZ = v[3];
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
       D += v[q];
@@ -118,13 +123,13 @@ __device__ __forceinline__ void lagged_moments(const typename Traits<P>::T v[19]
       if (kCY[q] && q != 6) MY = kCY[q] > 0 ? MY + v[q] : MY - v[q];
       if (kCZ[q] && q != 3) MZ = kCZ[q] > 0 ? MZ + v[q] : MZ - v[q];
     }
-    constexpr float is = Traits<P>::inv_scale;
+    constexpr T is = (T)Traits<P>::inv_scale;
     d = D * is;
-    rho = 1.0f + d;
-    const float ir = __builtin_amdgcn_rcpf(rho);
-    ux = __builtin_fmaf(MX, is, 0.5f * fpx) * ir;
-    uy = __builtin_fmaf(MY, is, 0.5f * fpy) * ir;
-    uz = __builtin_fmaf(MZ, is, 0.5f * __builtin_fmaf(rho, c.g, fpz)) * ir;
+    rho = (T)1 + d;
+    const T ir = (T)1 / rho;  // correctly rounded
+    ux = fma_t(MX, is, (T)0.5 * fpx) * ir;
+    uy = fma_t(MY, is, (T)0.5 * fpy) * ir;
+    uz = fma_t(MZ, is, (T)0.5 * fma_t(rho, c.g, fpz)) * ir;
   }
 }
 
@@ -147,25 +152,26 @@ __device__ __forceinline__ void collide_cell(typename Traits<P>::T v[19], typena
     //   h*_q = omc h_q + E + O,  h*_opp = omc h_opp + E - O
     //   E = wa (d + rho (4.5 cu^2 - 1.5 u^2)) + wb (9 cu cF - 3 u.F)
     //   O = 3 wa rho cu + 3 wb cF
-    const float fzg = __builtin_fmaf(rho, c.g, fz);
-    const float usq15 = 1.5f * __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz));
-    const float uF3 = 3.0f * __builtin_fmaf(ux, fx, __builtin_fmaf(uy, fy, uz * fzg));
-    const float dr = __builtin_fmaf(-rho, usq15, d);  // d - 1.5 rho u^2
-    const float r45 = 4.5f * rho;
-    v[0] = __builtin_fmaf(c.omc, v[0], __builtin_fmaf(c.wa[0], dr, -c.wb[0] * uF3));
-    const float ar3[3] = {0.0f, 3.0f * c.wa[1] * rho, 3.0f * c.wa[2] * rho};
-    const float wb3[3] = {0.0f, 3.0f * c.wb[1], 3.0f * c.wb[2]};
+    using T = typename Traits<P>::T;
+    const T fzg = fma_t(rho, c.g, fz);
+    const T usq15 = (T)1.5 * fma_t(ux, ux, fma_t(uy, uy, uz * uz));
+    const T uF3 = (T)3 * fma_t(ux, fx, fma_t(uy, fy, uz * fzg));
+    const T dr = fma_t(-rho, usq15, d);  // d - 1.5 rho u^2
+    const T r45 = (T)4.5 * rho;
+    v[0] = fma_t(c.omc, v[0], fma_t(c.wa[0], dr, -c.wb[0] * uF3));
+    const T ar3[3] = {(T)0, (T)3 * c.wa[1] * rho, (T)3 * c.wa[2] * rho};
+    const T wb3[3] = {(T)0, (T)3 * c.wb[1], (T)3 * c.wb[2]};
 #pragma unroll
     for (int q = 1; q < 19; q += 2) {
       const int k = kCLS[q];
-      const float cu = cdot(q, ux, uy, uz);
-      const float cF = cdot(q, fx, fy, fzg);
-      const float base = __builtin_fmaf(r45, cu * cu, dr);
-      const float s9 = __builtin_fmaf(9.0f * cu, cF, -uF3);
-      const float E = __builtin_fmaf(c.wa[k], base, c.wb[k] * s9);
-      const float O = __builtin_fmaf(ar3[k], cu, wb3[k] * cF);
-      v[q] = __builtin_fmaf(c.omc, v[q], E + O);
-      v[q + 1] = __builtin_fmaf(c.omc, v[q + 1], E - O);
+      const T cu = cdot<T>(q, ux, uy, uz);
+      const T cF = cdot<T>(q, fx, fy, fzg);
+      const T base = fma_t(r45, cu * cu, dr);
+      const T s9 = fma_t((T)9 * cu, cF, -uF3);
+      const T E = fma_t(c.wa[k], base, c.wb[k] * s9);
+      const T O = fma_t(ar3[k], cu, wb3[k] * cF);
+      v[q] = fma_t(c.omc, v[q], E + O);
+      v[q + 1] = fma_t(c.omc, v[q + 1], E - O);
     }
   }
 }
@@ -451,8 +457,6 @@ __global__ void __launch_bounds__(1024)
                           const typename Traits<P>::FT *__restrict__ Fcur,
                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
                           int sel, int *__restrict__ rect_buf) {
-  using T = typename Traits<P>::T;
-  using S = typename Traits<P>::S;
   if (sel == kRowsBand && blockIdx.x == 0 && threadIdx.x == 0)
     store_band_rects(g, box_cur, box_prev, nsheets, rect_buf);
   if constexpr (!BUF) {
@@ -1232,6 +1236,17 @@ int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[128]
     const KConsts<typename Traits<P>::T> k = kconsts<P>(c);
     static_assert(sizeof(k) <= 128, "KConsts fits the batch descriptor");
     memcpy(out, &k, sizeof(k));
+  }));
+  return LBMIB_OK;
+}
+
+int upload_frozen_table(int prec, const FluidConsts<double> &c, void *d_tab) {
+  LBMIB_DISPATCH(prec, ({
+    using TT = typename Traits<P>::T;
+    TT t[19];
+    // kernel units: f (F64), scale * (f - w) (deviation forms; tab_in holds f - w)
+    for (int q = 0; q < 19; ++q) t[q] = (TT)((double)Traits<P>::scale * c.tab_in[q]);
+    if (hipMemcpy(d_tab, t, sizeof(t), hipMemcpyHostToDevice) != hipSuccess) return LBMIB_ERR_HIP;
   }));
   return LBMIB_OK;
 }

@@ -146,6 +146,9 @@ int launch_band_u_batch(int prec, const BatchDesc *d, const int *map, int nblock
 int launch_move_batch(const BatchDesc *d, const int *map, int nblocks, int reset_slot, hipStream_t s);
 int gather_mcap(int cap);
 int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[128]);
+// Copies tab_in, in the kernels' units and arithmetic type, to d_tab (room
+// for 19 doubles).
+int upload_frozen_table(int prec, const FluidConsts<double> &c, void *d_tab);
 
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 // Launch shape of the fused kernel (all forms compute identical bits).
