@@ -71,7 +71,7 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (ctx->stream_b) (void)hipStreamDestroy(ctx->stream_b);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
                   ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.box_part, ctx->ib.done, ctx->ib.ub, ctx->d_sheets,
-                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab, ctx->d_rect};
+                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab, ctx->d_rect, ctx->d_ref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -88,6 +88,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   if (p.bc_x < 0 || p.bc_x > 2 || p.bc_y < 1 || p.bc_y > 2 || p.bc_z < 1 || p.bc_z > 2) return LBMIB_ERR_ARG;
   if (!(p.tau > 0.5) || !(p.rho0 > 0)) return LBMIB_ERR_ARG;
   if (elem_size(p.precision) == 0) return LBMIB_ERR_ARG;
+  if (p.dev_ref != LBMIB_DEV_EQ0 && p.dev_ref != LBMIB_DEV_REST) return LBMIB_ERR_ARG;
   const long n = (long)p.nx * p.ny * p.nz;
   if (n > (1L << 31) - 1) return LBMIB_ERR_ARG;
 
@@ -136,10 +137,16 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   c.cs4 = pow(cs, 4.0);
   c.g = p.g;
   c.rho0 = p.rho0; c.u0x = p.u0[0]; c.u0y = p.u0[1]; c.u0z = p.u0[2];
-  const bool dev = p.precision != LBMIB_F64;
+  // the deviation forms store f - feq(rhor, ur): the initial flow, or the
+  // rest state (r = w)
+  const bool dev = p.precision != LBMIB_F64, eq0 = p.dev_ref == LBMIB_DEV_EQ0;
+  c.rhor = eq0 ? p.rho0 : 1.0;
+  c.urx = eq0 ? p.u0[0] : 0.0;  // the inflow component
   for (int q = 0; q < 19; ++q) {
+    // the inlet/outlet tables are the initial equilibrium feq(rho0, u0)
     const double feq = host_feq(q, p.rho0, p.u0[0], p.u0[1], p.u0[2]);
-    c.tab_in[q] = c.tab_out[q] = dev ? feq - 1.0 / kWDEN[q] : feq;
+    c.ref[q] = dev ? host_feq(q, c.rhor, c.urx, 0.0, 0.0) : 0.0;
+    c.tab_in[q] = c.tab_out[q] = feq - c.ref[q];
   }
 
   ctx->esz = elem_size(p.precision);
@@ -170,9 +177,11 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   alloc((void **)&ctx->ib.box, sizeof(int) * 2 * kMaxSheets * 6);
   alloc(&ctx->d_tab, sizeof(double) * 19);
   alloc((void **)&ctx->d_rect, sizeof(int) * kRectBufInts);
+  alloc((void **)&ctx->d_ref, sizeof(double) * 19);
   if (!rc) {
-    rc = upload_frozen_table(p.precision, c, ctx->d_tab);
+    rc = upload_fluid_tables(p.precision, c, ctx->d_tab, ctx->d_ref);
     c.dtab = ctx->d_tab;
+    c.dref = ctx->d_ref;
   }
   if (!rc && row_path_buffered(g, ctx->esz)) {
     std::vector<uint32_t> tab;

@@ -136,6 +136,7 @@ extern "C" int lbmib_batch_create(int n, const lbmib_fluid_params *fluids, const
     d.u0[1] = c->c.u0y;
     d.u0[2] = c->c.u0z;
     d.gz = c->c.g;
+    d.ref = c->d_ref;
     kconsts_bytes(b->prec, c->c, d.kc);
     b->wpb = std::max(b->wpb, d.wpr);
     b->mcap_max = std::max(b->mcap_max, d.mcap);

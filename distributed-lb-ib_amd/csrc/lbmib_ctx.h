@@ -35,6 +35,7 @@ struct lbmib_ctx {
   double *d_tot = nullptr;
   double *d_mat = nullptr;   // materialisation scratch (lazy)
   void *d_tab = nullptr;     // frozen-plane populations in kernel units (19 values)
+  double *d_ref = nullptr;   // reference populations of the deviation forms
   uint32_t *d_rowtab = nullptr;  // row source table of the buffer-addressed fused kernel
   size_t mat_bytes = 0;
   hipStream_t stream = nullptr;

@@ -15,7 +15,8 @@ typedef _Float16 half_t;
 template <int P>
 struct Traits;
 // S: stored type, T: arithmetic type, FT: force field type; dev: stored as
-// deviations f - w; scale: stored value = scale * (f - w).
+// deviations f - r from reference populations r; scale: stored value =
+// scale * (f - r).
 template <>
 struct Traits<LBMIB_F64> {
   using S = double;
@@ -53,7 +54,7 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_floa
 // round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
-// Stored value <-> kernel units (h = scale (f - w), or f for F64).
+// Stored value <-> kernel units (h = scale (f - r), or f for F64).
 template <int P>
 __device__ __forceinline__ typename Traits<P>::T from_bits(typename Traits<P>::S s) {
   if constexpr (P == LBMIB_F16) return (float)s;
@@ -119,7 +120,7 @@ __device__ __forceinline__ void bst_h(__amdgpu_buffer_rsrc_t r, uint32_t voff, u
   }
 }
 
-// Deviation g = f - w (or f for F64) of a stored population.
+// Deviation f - r (or f for F64) of a stored population.
 template <int P, bool NTL = false>
 __device__ __forceinline__ typename Traits<P>::T ld(const typename Traits<P>::S *p) {
   return ld_h<P, NTL>(p) * Traits<P>::inv_scale;
@@ -153,10 +154,13 @@ __device__ __forceinline__ bool in_any_box(const int *boxes, int nsheets, int x,
 // Wall axes: a source outside the active box is replaced by the opposite
 // population of the same node (half-way bounce-back, lbm_ib.c:2310-2423).
 // Periodic axes wrap.  Interior cells take the branch-free path.
-// RAW: values in kernel units (ld_h) instead of deviations (ld).
+// RAW: values in kernel units (ld_h) instead of deviations (ld).  b1, b2:
+// the deviation forms' bounce-back correction s (r_opp - r_q) = -c_x b_class
+// (kernel units), since a reflected population is stored relative to r_opp.
 template <int P, bool NTL = false, bool RAW = false>
 __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>::S *__restrict__ f, int x, int y,
-                                     int z, long c, typename Traits<P>::T v[19]) {
+                                     int z, long c, typename Traits<P>::T b1, typename Traits<P>::T b2,
+                                     typename Traits<P>::T v[19]) {
   const bool interior = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1] && z > g.lo[2] && z < g.hi[2];
   const long sx = 1, sy = g.nx, sz = (long)g.nx * g.ny;
   if (interior) {
@@ -180,6 +184,12 @@ __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>
     const long src = bb ? c + (long)kOPP[q] * g.pitch - (long)q * g.pitch
                         : (long)px + sy * py + sz * pz;
     v[q] = RAW ? ld_h<P, NTL>(f + q * g.pitch + src) : ld<P, NTL>(f + q * g.pitch + src);
+    if constexpr (Traits<P>::dev) {
+      if (kCX[q] != 0 && bb) {
+        const typename Traits<P>::T corr = -kCX[q] * (kCLS[q] == 1 ? b1 : b2);
+        v[q] += RAW ? corr : corr * Traits<P>::inv_scale;
+      }
+    }
   }
 }
 

@@ -52,7 +52,19 @@ static KConsts<typename Traits<P>::T> kconsts(const FluidConsts<double> &d) {
   c.u0x = (T)d.u0x;
   c.u0y = (T)d.u0y;
   c.u0z = (T)d.u0z;
+  c.rhor = (T)d.rhor;
+  c.d0 = (T)(d.rho0 - d.rhor);
+  c.mrx = (T)(d.rhor * d.urx);
+  const double u2 = d.urx * d.urx;
+  c.kr[0] = (T)(d.rhor * (-1.5 * u2));
+  c.kr[1] = (T)(d.rhor * (4.5 * u2 - 1.5 * u2));
+  for (int k = 0; k < 3; ++k) {
+    const double w = 1.0 / kWDEN_CLS[k];
+    c.orr[k] = Traits<P>::dev && k ? (T)(3.0 * (s * d.om * w) * d.rhor * d.urx) : (T)0;
+    c.br[k] = Traits<P>::dev ? (T)host_bounce_corr(d, k, s) : (T)0;
+  }
   c.tab = (const T *)d.dtab;
+  c.ref = d.dref;
   return c;
 }
 
@@ -123,11 +135,12 @@ __device__ __forceinline__ void lagged_moments(const typename Traits<P>::T v[19]
       if (kCY[q] && q != 6) MY = kCY[q] > 0 ? MY + v[q] : MY - v[q];
       if (kCZ[q] && q != 3) MZ = kCZ[q] > 0 ? MZ + v[q] : MZ - v[q];
     }
+    // h = s (f - r): rho = rhor + Sum(h)/s, rho u = rhor ur + Sum(c h)/s + F/2
     constexpr T is = (T)Traits<P>::inv_scale;
-    d = D * is;
-    rho = (T)1 + d;
+    d = D * is;  // rho - rhor
+    rho = c.rhor + d;
     const T ir = (T)1 / rho;  // correctly rounded
-    ux = fma_t(MX, is, (T)0.5 * fpx) * ir;
+    ux = fma_t(MX, is, fma_t((T)0.5, fpx, c.mrx)) * ir;
     uy = fma_t(MY, is, (T)0.5 * fpy) * ir;
     uz = fma_t(MZ, is, (T)0.5 * fma_t(rho, c.g, fpz)) * ir;
   }
@@ -144,21 +157,23 @@ __device__ __forceinline__ void collide_cell(typename Traits<P>::T v[19], typena
 #pragma unroll
     for (int q = 0; q < 19; ++q) v[q] = collide_exact(q, v[q], rho, ux, uy, uz, fx, fy, fz, c);
   } else {
-    // Deviation form on h_q = s (f_q - w_q), d = rho - 1:
-    //   h*_q = (1 - 1/tau) h_q + s/tau (feq_q - w_q) + s S_q
-    //   feq_q - w_q = w_q (d + rho (3 cu + 4.5 cu^2 - 1.5 u^2))
+    // Deviation form on h_q = s (f_q - r_q), r = feq(rhor, ur), d = rho - rhor:
+    //   h*_q = (1 - 1/tau) h_q + s/tau (feq_q - r_q) + s S_q
+    //   feq_q - r_q = w_q (rho (1 + 3 cu + 4.5 cu^2 - 1.5 u^2)
+    //                      - rhor (1 + 3 cur + 4.5 cur^2 - 1.5 ur^2))
     //   S_q = w_q (1 - 1/(2 tau)) (3 (c_q - u).F + 9 (c_q.u)(c_q.F))
     // Opposite directions share the even part E and flip the odd part O:
     //   h*_q = omc h_q + E + O,  h*_opp = omc h_opp + E - O
-    //   E = wa (d + rho (4.5 cu^2 - 1.5 u^2)) + wb (9 cu cF - 3 u.F)
-    //   O = 3 wa rho cu + 3 wb cF
+    //   E = wa (d - k_p + rho (4.5 cu^2 - 1.5 u^2)) + wb (9 cu cF - 3 u.F)
+    //   O = 3 wa rho cu - o_p + 3 wb cF
+    // with k_p = rhor (4.5 cur^2 - 1.5 ur^2), o_p = 3 wa rhor cur, which for
+    // ur = (urx, 0, 0) are kr[|c_x|] and c_x orr[class].
     using T = typename Traits<P>::T;
     const T fzg = fma_t(rho, c.g, fz);
     const T usq15 = (T)1.5 * fma_t(ux, ux, fma_t(uy, uy, uz * uz));
     const T uF3 = (T)3 * fma_t(ux, fx, fma_t(uy, fy, uz * fzg));
-    const T dr = fma_t(-rho, usq15, d);  // d - 1.5 rho u^2
     const T r45 = (T)4.5 * rho;
-    v[0] = fma_t(c.omc, v[0], fma_t(c.wa[0], dr, -c.wb[0] * uF3));
+    v[0] = fma_t(c.omc, v[0], fma_t(c.wa[0], fma_t(-rho, usq15, d - c.kr[0]), -c.wb[0] * uF3));
     const T ar3[3] = {(T)0, (T)3 * c.wa[1] * rho, (T)3 * c.wa[2] * rho};
     const T wb3[3] = {(T)0, (T)3 * c.wb[1], (T)3 * c.wb[2]};
 #pragma unroll
@@ -166,10 +181,11 @@ __device__ __forceinline__ void collide_cell(typename Traits<P>::T v[19], typena
       const int k = kCLS[q];
       const T cu = cdot<T>(q, ux, uy, uz);
       const T cF = cdot<T>(q, fx, fy, fzg);
-      const T base = fma_t(r45, cu * cu, dr);
+      const T base = fma_t(r45, cu * cu, fma_t(-rho, usq15, d - c.kr[kCX[q] != 0]));
       const T s9 = fma_t((T)9 * cu, cF, -uF3);
       const T E = fma_t(c.wa[k], base, c.wb[k] * s9);
-      const T O = fma_t(ar3[k], cu, wb3[k] * cF);
+      const T o_p = kCX[q] > 0 ? c.orr[k] : (kCX[q] < 0 ? -c.orr[k] : (T)0);
+      const T O = fma_t(ar3[k], cu, fma_t(wb3[k], cF, -o_p));
       v[q] = fma_t(c.omc, v[q], E + O);
       v[q + 1] = fma_t(c.omc, v[q + 1], E - O);
     }
@@ -202,10 +218,10 @@ __device__ __forceinline__ void arrive_general(const Geometry &g, const typename
 #pragma unroll
       for (int q = 0; q < 19; ++q) v[q] = ld_h<P>(fin + q * g.pitch + c0);
     }
-    rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
+    rho = c.rho0; d = c.d0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
     return;
   }
-  pull<P, NTL, true>(g, fin, x, y, z, c0, v);
+  pull<P, NTL, true>(g, fin, x, y, z, c0, c.br[1], c.br[2], v);
   T fpx = 0, fpy = 0, fpz = 0;
   if (in_any_box(box_prev, nsheets, x, y, z)) {
     fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
@@ -239,7 +255,8 @@ __device__ __forceinline__ void update_cell(const Geometry &g, const typename Tr
 // Byte offsets, in a population array, of the 19 sources of cell (0, y, z)
 // of every row (y, z) -- the pull of stream_distrfunc with the y/z wrap and
 // half-way bounce-back of the previous step -- for the buffer-addressed row
-// path; the lane adds x * sizeof(S).  Built once per context on the host.
+// path; the lane adds x * sizeof(S).  Entry 19 of a row is the mask of its
+// bounced-back directions (bit q).  Built once per context on the host.
 constexpr int kRowTabStride = 20;
 
 int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
@@ -249,6 +266,7 @@ int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
   for (long row = 0; row < nrows; ++row) {
     const int y = (int)(row % g.ny), z = (int)(row / g.ny);
     if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
+    uint32_t mask = 0;
     for (int q = 0; q < 19; ++q) {
       int py = y - kCY[q], pz = z - kCZ[q];
       bool bb = false;
@@ -260,7 +278,9 @@ int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
                              : (int64_t)(q * pitch_b + (uint64_t)(py + (long)g.ny * pz) * sy_b) - kCX[q] * (int64_t)es;
       if (off < 0 || off > 0xFFFFFFFFll) return LBMIB_ERR_ARG;
       tab[(size_t)row * kRowTabStride + q] = (uint32_t)off;
+      mask |= (uint32_t)bb << q;
     }
+    tab[(size_t)row * kRowTabStride + 19] = mask;
   }
   return LBMIB_OK;
 }
@@ -356,6 +376,22 @@ __device__ __forceinline__ bool next_row(const Geometry &g, int sel, const int *
   }
 }
 
+// Deviation forms: a population reflected at a y/z wall (bit q of the row's
+// mask) arrives as h* of the opposite direction, stored relative to r_opp;
+// add s (r_opp - r_q) = -c_x br (non-zero for c_x != 0 only).  The mask is wave-uniform; rows away from the walls
+// skip the loop.
+template <int P>
+__device__ __forceinline__ void bounce_fix(uint32_t mask, typename Traits<P>::T b1, typename Traits<P>::T b2,
+                                           typename Traits<P>::T v[19]) {
+  if constexpr (Traits<P>::dev) {
+    if (mask) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q)
+        if (kCX[q] != 0 && ((mask >> q) & 1u)) v[q] -= kCX[q] * (kCLS[q] == 1 ? b1 : b2);
+    }
+  }
+}
+
 // One lane of the buffer-addressed row path: the update of cell (x, row) by
 // a wave whose lanes hold consecutive x of the same row (SGPR row offsets).
 // Dead lanes of an active row store zeros (whole cache lines); x edges of a
@@ -394,7 +430,7 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
     if (frozen) {
 #pragma unroll
       for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
-      rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;
+      rho = c.rho0; d = c.d0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
     } else {
       // per-row source offsets (row_source_table): wrap and bounce-back
       // of the y/z axes and the x shift folded in
@@ -412,9 +448,10 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
         for (int q = 0; q < 19; ++q) so[q] = rt[q];
 #pragma unroll
         for (int q = 0; q < 19; ++q) v[q] = bld_h<P, NTL>(rin, vo, so[q]);
+        bounce_fix<P>(rt[19], c.br[1], c.br[2], v);
       }
       if (own) {
-        rho = c.rho0; d = c.rho0 - (T)1; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
+        rho = c.rho0; d = c.d0; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
       } else {
         T fpx = 0, fpy = 0, fpz = 0;
         if (in_any_box(box_prev, nsheets, x, y, z)) {
@@ -569,7 +606,7 @@ __global__ void __launch_bounds__(512)
         }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+          rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
       } else {
         const uint32_t *rt = rowtab + (long)row * kRowTabStride;
@@ -578,8 +615,10 @@ __global__ void __launch_bounds__(512)
         for (int q = 0; q < 19; ++q) so[q] = rt[q];
 #pragma unroll
         for (int q = 0; q < 19; ++q) bld_pair<P, NTL>(rin, vo, so[q], v[0][q], v[1][q]);
+        const uint32_t bbm = rt[19];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
+          bounce_fix<P>(bbm, c.br[1], c.br[2], v[k]);
           T fpx = 0, fpy = 0, fpz = 0;
           if (in_any_box(box_prev, nsheets, x0 + k, y, z)) {
             fpx = (T)Fprev[c0 + k]; fpy = (T)Fprev[g.n + c0 + k]; fpz = (T)Fprev[2 * g.n + c0 + k];
@@ -593,7 +632,7 @@ __global__ void __launch_bounds__(512)
           // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
 #pragma unroll
           for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
-          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+          rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
         T fx = 0, fy = 0, fz = 0;
         if (in_any_box(box_cur, nsheets, x0 + k, y, z)) {
@@ -662,7 +701,7 @@ __global__ void __launch_bounds__(512)
             v[k][q] = bld_h<P, NTL>(rin, vo, qb);
             qb += pitch_b;
           }
-          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+          rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
       } else {
         const uint32_t *rt = rowtab + (long)row0 * kRowTabStride;
@@ -677,6 +716,7 @@ __global__ void __launch_bounds__(512)
           for (int q = 0; q < 19; ++q) v[k][q] = bld_h<P, NTL>(rin, vo, so[k][q]);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
+          bounce_fix<P>(rt[k * kRowTabStride + 19], c.br[1], c.br[2], v[k]);
           const long ck = (long)x + (long)g.nx * (row0 + k);
           T fpx = 0, fpy = 0, fpz = 0;
           if (in_any_box(box_prev, nsheets, x, y0 + k, z)) {
@@ -692,7 +732,7 @@ __global__ void __launch_bounds__(512)
           // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
 #pragma unroll
           for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
-          rho[k] = c.rho0; d[k] = c.rho0 - (T)1; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+          rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
         T fx = 0, fy = 0, fz = 0;
         if (in_any_box(box_cur, nsheets, x, y0 + k, z)) {
@@ -893,9 +933,9 @@ __global__ void k_materialize(Geometry g, const typename Traits<P>::S *__restric
     } else if (own) {
       for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pitch + c0);
     } else {
-      pull<P>(g, f, x, y, z, c0, v);
+      pull<P>(g, f, x, y, z, c0, c.br[1], c.br[2], v);
     }
-    for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + 1.0 / kWDEN[q] : (double)v[q];
+    for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + c.ref[q] : (double)v[q];
   } else {
     for (int q = 0; q < 19; ++q) fv[q] = 0.0;
   }
@@ -952,10 +992,10 @@ __global__ void __launch_bounds__(kTotThreads)
       } else if (own) {
         for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pitch + c0);
       } else {
-        pull<P>(g, f, x, y, z, c0, v);
+        pull<P>(g, f, x, y, z, c0, c.br[1], c.br[2], v);
       }
       for (int q = 0; q < 19; ++q) {
-        const double fq = Traits<P>::dev ? (double)v[q] + 1.0 / kWDEN[q] : (double)v[q];
+        const double fq = Traits<P>::dev ? (double)v[q] + c.ref[q] : (double)v[q];
         acc[0] += fq;
         acc[1] += kCX[q] * fq;
         acc[2] += kCY[q] * fq;
@@ -1230,23 +1270,24 @@ int launch_totals(int prec, const Geometry &g, const void *f, int own_mode, cons
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
-int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[128]) {
-  memset(out, 0, 128);
+int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[256]) {
+  memset(out, 0, 256);
   LBMIB_DISPATCH(prec, ({
     const KConsts<typename Traits<P>::T> k = kconsts<P>(c);
-    static_assert(sizeof(k) <= 128, "KConsts fits the batch descriptor");
+    static_assert(sizeof(k) <= 256, "KConsts fits the batch descriptor");
     memcpy(out, &k, sizeof(k));
   }));
   return LBMIB_OK;
 }
 
-int upload_frozen_table(int prec, const FluidConsts<double> &c, void *d_tab) {
+int upload_fluid_tables(int prec, const FluidConsts<double> &c, void *d_tab, double *d_ref) {
   LBMIB_DISPATCH(prec, ({
     using TT = typename Traits<P>::T;
     TT t[19];
-    // kernel units: f (F64), scale * (f - w) (deviation forms; tab_in holds f - w)
-    for (int q = 0; q < 19; ++q) t[q] = (TT)((double)Traits<P>::scale * c.tab_in[q]);
+    // kernel units: f (F64), s (f - ref) (deviation forms; tab_in holds f - ref)
+    for (int q = 0; q < 19; ++q) t[q] = (TT)(Traits<P>::scale * c.tab_in[q]);
     if (hipMemcpy(d_tab, t, sizeof(t), hipMemcpyHostToDevice) != hipSuccess) return LBMIB_ERR_HIP;
+    if (hipMemcpy(d_ref, c.ref, sizeof(double) * 19, hipMemcpyHostToDevice) != hipSuccess) return LBMIB_ERR_HIP;
   }));
   return LBMIB_OK;
 }

@@ -374,7 +374,9 @@ __global__ void __launch_bounds__(kGatherThreads)
 template <int P>
 __device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g, const typename Traits<P>::S *__restrict__ f,
                                             const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y,
-                                            double u0z, double gz, int slot, int bid, int nblk) {
+                                            double u0z, double gz, const double *__restrict__ ref,
+                                            typename Traits<P>::T b1, typename Traits<P>::T b2, int slot, int bid,
+                                            int nblk) {
   const long tid = (long)bid * blockDim.x + threadIdx.x, stride = (long)nblk * blockDim.x;
   for (int s = 0; s < ib.nsheets; ++s) {
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
@@ -390,9 +392,9 @@ __device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g
       double ux = u0x, uy = u0y, uz = u0z;
       if (mom) {
         typename Traits<P>::T v[19];
-        pull<P>(g, f, x, y, z, c0, v);
+        pull<P>(g, f, x, y, z, c0, b1, b2, v);
         double fv[19];
-        for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + 1.0 / kWDEN[q] : (double)v[q];
+        for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + ref[q] : (double)v[q];
         double s1 = fv[0], s2 = 0, s3 = 0, s4 = 0;
         for (int q = 1; q < 19; ++q) {
           s1 += fv[q];
@@ -415,8 +417,9 @@ __device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g
 template <int P>
 __global__ void __launch_bounds__(256)
     k_band_u(IBState ib, Geometry g, const typename Traits<P>::S *__restrict__ f,
-             const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y, double u0z, double gz, int slot) {
-  band_u_body<P>(ib, g, f, F, u0x, u0y, u0z, gz, slot, blockIdx.x, gridDim.x);
+             const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y, double u0z, double gz,
+             const double *__restrict__ ref, typename Traits<P>::T b1, typename Traits<P>::T b2, int slot) {
+  band_u_body<P>(ib, g, f, F, u0x, u0y, u0z, gz, ref, b1, b2, slot, blockIdx.x, gridDim.x);
 }
 
 constexpr int kMoveThreads = 256;
@@ -513,7 +516,10 @@ __global__ void __launch_bounds__(256) k_band_u_batch(const BatchDesc *__restric
   batch_block(map, inst, lb);
   const BatchDesc &d = descs[inst];
   band_u_body<P>(d.ib, d.g, (const typename Traits<P>::S *)d.f[cur_new], (const typename Traits<P>::FT *)d.F[slot],
-                 d.u0[0], d.u0[1], d.u0[2], d.gz, slot, lb, band_u_blocks(d.ib.nnodes));
+                 d.u0[0], d.u0[1], d.u0[2], d.gz, d.ref,
+                 reinterpret_cast<const KConsts<typename Traits<P>::T> *>(d.kc)->br[1],
+                 reinterpret_cast<const KConsts<typename Traits<P>::T> *>(d.kc)->br[2], slot, lb,
+                 band_u_blocks(d.ib.nnodes));
 }
 
 __global__ void __launch_bounds__(kMoveThreads) k_move_batch(const BatchDesc *__restrict__ descs,
@@ -549,22 +555,26 @@ int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, c
   if (ib.nnodes <= 0) return LBMIB_OK;
   const int slot = 1 - reset_slot;  // the box of this step's spread
   const int ublocks = band_u_blocks(ib.nnodes);
+  // bounce-back correction of the deviation forms, scale 1 (F32) or 2^8
+  const double s16 = kDevScale16;
+  const double d1 = host_bounce_corr(c, 1, 1.0), d2 = host_bounce_corr(c, 2, 1.0);
+  const float h1 = (float)host_bounce_corr(c, 1, s16), h2 = (float)host_bounce_corr(c, 2, s16);
   switch (prec) {
     case LBMIB_F64:
       k_band_u<LBMIB_F64><<<ublocks, 256, 0, s>>>(ib, g, (const double *)f, (const double *)F, c.u0x, c.u0y, c.u0z,
-                                                  c.g, slot);
+                                                  c.g, c.dref, 0.0, 0.0, slot);
       break;
     case LBMIB_F32:
       k_band_u<LBMIB_F32><<<ublocks, 256, 0, s>>>(ib, g, (const float *)f, (const float *)F, c.u0x, c.u0y, c.u0z,
-                                                  c.g, slot);
+                                                  c.g, c.dref, d1, d2, slot);
       break;
     case LBMIB_F16:
       k_band_u<LBMIB_F16><<<ublocks, 256, 0, s>>>(ib, g, (const half_t *)f, (const float *)F, c.u0x, c.u0y, c.u0z,
-                                                  c.g, slot);
+                                                  c.g, c.dref, h1, h2, slot);
       break;
     case LBMIB_BF16:
       k_band_u<LBMIB_BF16><<<ublocks, 256, 0, s>>>(ib, g, (const uint16_t *)f, (const float *)F, c.u0x, c.u0y,
-                                                   c.u0z, c.g, slot);
+                                                   c.u0z, c.g, c.dref, h1, h2, slot);
       break;
     default:
       return LBMIB_ERR_ARG;

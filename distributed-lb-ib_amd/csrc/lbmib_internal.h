@@ -57,13 +57,21 @@ struct FluidConsts {
   T cs2, cs4;         // cs^2, cs^4 as the reference computes them
   T g;
   T rho0, u0x, u0y, u0z;
-  T tab_in[19], tab_out[19];  // f (F64) or f - w (deviation forms)
+  T rhor, urx;                // deviation forms: reference state, ref = feq(rhor, (urx, 0, 0))
+  T tab_in[19], tab_out[19];  // f (F64) or f - ref (deviation forms)
+  T ref[19];                  // deviation forms: reference populations; F64: 0
   const void *dtab;           // device copy of tab_in in kernel units (see KConsts::tab)
+  const double *dref;         // device copy of ref
 };
 
 // Constants of the fluid kernels, passed by value, in kernel units: the F64
-// form works on f; the deviation forms on h = s (f - w) with s = 1 (F32) or
-// 2^8 (16-bit storage), the stored value itself.
+// form works on f; the deviation forms on h = s (f - r) with s = 1 (F32) or
+// 2^8 (16-bit storage), the stored value itself, where r_q = feq_q(rhor, ur)
+// is a fixed reference equilibrium with ur = (urx, 0, 0) along the inflow
+// axis: by default the initial flow (rhor, urx) = (rho0, u0x), so the stored
+// values are the departure from it; (1, 0) gives r_q = w_q.  With ur along x
+// every reference term depends on c_x and the weight class only:
+//   r_q / w_q = rhor + kr[|c_x|] + c_x orr[class] / (s om w_q)
 template <class T>
 struct KConsts {
   T omc, om;          // 1 - 1/tau, 1/tau
@@ -71,8 +79,18 @@ struct KConsts {
   T wb[3];            // deviation forms: s w (1 - 1/(2 tau)); F64: dt (1 - 1/(2 tau)) / wden
   T cs2, cs4, g;
   T rho0, u0x, u0y, u0z;
+  T rhor, d0, mrx;    // reference density; rho0 - rhor; rhor urx
+  T kr[2];            // rhor (4.5 (c_x urx)^2 - 1.5 urx^2) for c_x = 0, +-1
+  T orr[3];           // per weight class: 3 wa rhor urx (odd part of s om r, c_x = +1)
+  T br[3];            // per weight class: 6 s w rhor urx; s (r_opp - r_q) = -c_x br
   const T *tab;       // [19] populations of the frozen inlet/outlet planes, kernel units
+  const double *ref;  // [19] reference populations r_q (deviation forms)
 };
+
+// The bounce-back correction constant br of weight class k (kernel units).
+inline double host_bounce_corr(const FluidConsts<double> &c, int k, double s) {
+  return k == 0 ? 0.0 : 6.0 * s * c.rhor * c.urx / (k == 1 ? 18.0 : 36.0);
+}
 
 struct Geometry {
   int nx, ny, nz;
@@ -131,7 +149,8 @@ struct BatchDesc {
   int wpr;                  // waves per row of the fused launch (ceil(nx / 64))
   int nrows;                // ny * nz
   double u0[3], gz;         // init_gv velocity, gravity
-  alignas(16) unsigned char kc[128];  // KConsts<T> of the batch's precision
+  const double *ref;        // reference populations of the deviation forms
+  alignas(16) unsigned char kc[256];  // KConsts<T> of the batch's precision
 };
 // Block map of a batched launch: one int per block, instance << kBatchShift |
 // block index within the instance.
@@ -145,10 +164,11 @@ int launch_band_u_batch(int prec, const BatchDesc *d, const int *map, int nblock
                         hipStream_t s);
 int launch_move_batch(const BatchDesc *d, const int *map, int nblocks, int reset_slot, hipStream_t s);
 int gather_mcap(int cap);
-int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[128]);
-// Copies tab_in, in the kernels' units and arithmetic type, to d_tab (room
-// for 19 doubles).
-int upload_frozen_table(int prec, const FluidConsts<double> &c, void *d_tab);
+int kconsts_bytes(int prec, const FluidConsts<double> &c, unsigned char out[256]);
+// Copies tab_in (19), the pair constants KConsts::pq (20) -- in the kernels'
+// units and arithmetic type -- and ref (19 doubles) to the device; d_tab has
+// room for 19 doubles, d_pq for 20, d_ref for 19.
+int upload_fluid_tables(int prec, const FluidConsts<double> &c, void *d_tab, double *d_ref);
 
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 // Launch shape of the fused kernel (all forms compute identical bits).

@@ -42,9 +42,13 @@ extern "C" {
 
 /* ---- population storage ---- */
 #define LBMIB_F64 0  /* f in FP64, FP64 arithmetic in the reference's operation order */
-#define LBMIB_F32 1  /* f - w_i in FP32, FP32 collision */
-#define LBMIB_F16 2  /* (f - w_i) * 2^8 in FP16, FP32 collision, FP64 reductions */
-#define LBMIB_BF16 3 /* (f - w_i) * 2^8 in BF16, FP32 collision, FP64 reductions */
+#define LBMIB_F32 1  /* f - r_i in FP32, FP64 collision */
+#define LBMIB_F16 2  /* (f - r_i) * 2^8 in FP16, FP32 collision, FP64 reductions */
+#define LBMIB_BF16 3 /* (f - r_i) * 2^8 in BF16, FP32 collision, FP64 reductions */
+
+/* Reference populations r_i of the deviation forms (lbmib_fluid_params.dev_ref) */
+#define LBMIB_DEV_EQ0 0  /* r_i = feq_i(rho0, u0), the initial flow (default) */
+#define LBMIB_DEV_REST 1 /* r_i = w_i, the rest state */
 
 typedef struct lbmib_ctx lbmib_ctx;
 
@@ -59,6 +63,7 @@ typedef struct {
   double rho0;
   double u0[3];        /* initial velocity; inlet/outlet tables are feq(rho0, u0) */
   double g;            /* gravity along z (lbm_ib.c:769 sets 0) */
+  int dev_ref;         /* LBMIB_DEV_EQ0 / LBMIB_DEV_REST; ignored by LBMIB_F64 */
 } lbmib_fluid_params;
 
 /* One fiber sheet (Fibersheet, lbm_ib.c:68-76): rows along y, cols along z.
