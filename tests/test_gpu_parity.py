@@ -7,9 +7,10 @@ Tolerances:
                  the device can differ by an ulp.
   F32            velocity <= 1e-5 relative to max|u| against the FP64 oracle.
   F16 / BF16     velocity <= 1e-3 / 1e-2 relative to max|u| (small grids);
-                 on the 512x256x256 workload against FP64, storing f - feq(rho0,
-                 u0x): relative L2 <= 1e-4 / 1e-3 and max <= 1e-3 / 1e-2;
-                 storing f - w: L2 <= 1e-3 / 1e-2 and max <= 3e-3 / 3e-2.
+                 on the 512x256x256 workload (2000 steps) against FP64,
+                 storing f - feq(rho0, u0x): relative L2 <= 1e-3 / 2e-2 and
+                 max <= 5e-3 / 1e-1; storing f - w: L2 <= 1e-2 / 5e-2 and
+                 max <= 3e-2 / 2e-1.
 """
 import os
 
@@ -493,13 +494,13 @@ def l2rel(a, b):
 
 
 def test_c2_workload_f32_within_1e5_of_f64(lbmib, configs):
-    """256^3 with the 104x104 sheet, 200 steps: FP32 storage (FP64
-    collision) against the FP64 path (itself pinned to the oracle) --
+    """256^3 with the 104x104 sheet for its full 1000 steps: FP32 storage
+    (FP64 collision) against the FP64 path (itself pinned to the oracle) --
     velocity error relative to max|u| within 1e-5."""
-    kw64, _ = configs.c2("f64")
+    kw64, steps = configs.c2("f64")
     kw32, _ = configs.c2("f32")
-    _, v64, x64 = _run(lbmib, kw64, 200)
-    _, v32, x32 = _run(lbmib, kw32, 200)
+    _, v64, x64 = _run(lbmib, kw64, steps)
+    _, v32, x32 = _run(lbmib, kw32, steps)
     err = rel(v32, v64)
     print(f"c2 f32 vs f64: velocity {err:.3e} (L2 {l2rel(v32, v64):.3e}), "
           f"positions {np.abs(x32[0] - x64[0]).max():.3e}")
@@ -509,29 +510,29 @@ def test_c2_workload_f32_within_1e5_of_f64(lbmib, configs):
 
 @pytest.fixture(scope="module")
 def c3_f64(lbmib, configs):
-    """The 512x256x256 two-sheet workload in FP64, 100 steps (the FP64 path
-    runs the general per-cell kernel here: its arrays pass the 4 GiB of the
-    buffer-addressed path)."""
-    kw64, _ = configs.c3("f64")
-    return _run(lbmib, kw64, 100)
+    """The 512x256x256 two-sheet workload in FP64 for its full 2000 steps
+    (the FP64 path runs the general per-cell kernel here: its arrays pass the
+    4 GiB of the buffer-addressed path)."""
+    kw64, steps = configs.c3("f64")
+    return _run(lbmib, kw64, steps)
 
 
 C3_TOL = {  # (precision, dev_ref): relative L2, max relative to the field's max
-    ("f16", "eq0"): (1e-4, 1e-3), ("bf16", "eq0"): (1e-3, 1e-2),
-    ("f16", "rest"): (1e-3, 3e-3), ("bf16", "rest"): (1e-2, 3e-2),
+    ("f16", "eq0"): (1e-3, 5e-3), ("bf16", "eq0"): (2e-2, 1e-1),
+    ("f16", "rest"): (1e-2, 3e-2), ("bf16", "rest"): (5e-2, 2e-1),
 }
 
 
 @pytest.mark.parametrize("precision,dev_ref", sorted(C3_TOL))
 def test_c3_workload_16bit_vs_f64(lbmib, configs, c3_f64, precision, dev_ref):
-    """512x256x256 with two 128x128 sheets, 100 steps: 16-bit storage of
+    """512x256x256 with two 128x128 sheets, 2000 steps: 16-bit storage of
     f - r (r = feq(rho0, u0x) or w) against the FP64 path, per field:
     relative L2 error and max error relative to the field's max."""
     tol_l2, tol_max = C3_TOL[precision, dev_ref]
     r64, v64, x64 = c3_f64
-    kw16, _ = configs.c3(precision)
+    kw16, steps = configs.c3(precision)
     kw16["dev_ref"] = dev_ref
-    r16, v16, x16 = _run(lbmib, kw16, 100)
+    r16, v16, x16 = _run(lbmib, kw16, steps)
     fields = {"ux": (v16[0], v64[0]), "uy": (v16[1], v64[1]), "uz": (v16[2], v64[2]),
               "rho-1": (r16 - 1.0, r64 - 1.0)}
     report = ", ".join(f"{k} L2 {l2rel(a, b):.2e} max {rel(a, b):.2e}" for k, (a, b) in fields.items())
