@@ -2,7 +2,8 @@
 
 Workload (default): 256^3 D3Q19 BGK with inlet/outlet x, periodic y, wall z,
 tau 0.6, u_in 0.01, one 104x104 fiber sheet (spacing 1, jitter 0.1) centred
-at (64, 128, 128), Ks = Kb = 1e-3, populations stored as FP32 deviations.
+at (64, 128, 128), Ks = Kb = 1e-3, populations stored in FP32 as deviations
+from the initial equilibrium feq(rho0, (u_in, 0, 0)), collision in FP64.
 A "step" is one full LBM-IB time step (fiber forces, spreading, fused
 collide-stream, band moments, interpolation and move) over the whole grid;
 by default steps are split so that the IB chain overlaps the fluid update
@@ -33,6 +34,18 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS + % HBM roofline, 256³ D3Q19+sheet; mixed/surrogate vs plain at stated err"
 
 
+# arithmetic type of the collision per storage form
+ARITH = {"f64": "f64", "f32": "f64", "f16": "f32", "bf16": "f32"}
+
+
+def storage_desc(precision, dev_ref):
+    if precision == "f64":
+        return "f64 populations f, f64 collision"
+    ref = "feq(rho0, (u0x, 0, 0))" if dev_ref == "eq0" else "w"
+    scale = "" if precision == "f32" else "2^8 * "
+    return f"{precision} deviations {scale}(f - {ref}), {ARITH[precision]} collision"
+
+
 def parse_args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -40,6 +53,8 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--precision", default="f32", choices=["f32", "f64", "f16", "bf16"])
     p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    p.add_argument("--dev-ref", default="eq0", choices=["eq0", "rest"],
+                   help="reference populations of the deviation forms: feq(rho0, u0x) or w")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", default="2,6", help="N1,N2 steps of the differenced CPU timing")
     p.add_argument("--variant", default="auto",
@@ -124,7 +139,6 @@ def cpu_reference_baseline(n, rows, steps_pair):
                       f"T_step = (T({n2}) - T({n1}))/{n2 - n1} = {t_step:.3f} s"}
 
 
-# (cells per thread, nontemporal mask: 1 stores / 2 loads, blocks per CU; 0 = one block per row)
 # (cells per thread: 1 one cell per lane, 2 x pairs; nontemporal mask: 1
 # stores, 2 loads; blocks per CU, 0 = one block per row).  The two-rows
 # shape (3) is never faster and is left out.
@@ -165,6 +179,8 @@ def run_batch(a, lbmib, lbmib_configs):
     kernel per step): total MLUPS over all instances, roofline of the
     batched fused launch, tile shape autotuned."""
     cfgs, _ = lbmib_configs.c4(precision=a.precision)
+    for c in cfgs:
+        c["dev_ref"] = a.dev_ref
     print(f"[bench] c4 ensemble: {len(cfgs)} instances, {a.precision}, warmup={a.warmup} steps={a.steps}",
           file=sys.stderr, flush=True)
     with lbmib.Batch(cfgs) as b:
@@ -199,7 +215,7 @@ def run_batch(a, lbmib, lbmib_configs):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": a.precision,
+        "dtype": ARITH[a.precision],
         "data": "synthetic",
         "config": {
             "workload": f"batched ensemble of {len(cfgs)} instances, cubic edges {edges} drawn from the seed, "
@@ -208,6 +224,7 @@ def run_batch(a, lbmib, lbmib_configs):
             "cells_total": cells,
             "fiber_nodes": nodes,
             "bytes_per_cell_update": bpc,
+            "storage": storage_desc(a.precision, a.dev_ref),
             "tile": {"waves_per_block": best[1], "nontemporal_mask": best[2]},
         },
         "roofline": {
@@ -258,6 +275,7 @@ def main():
         return
     cfg_fn = getattr(lbmib_configs, a.workload)
     kw, nominal_steps = cfg_fn(precision=a.precision)
+    kw["dev_ref"] = a.dev_ref
     n = kw["nx"] * kw["ny"] * kw["nz"]
     nodes = sum(s.rows * s.cols for s in kw["sheets"])
     print(f"[bench] {a.workload} {kw['nx']}x{kw['ny']}x{kw['nz']} {a.precision} nodes={nodes} "
@@ -310,7 +328,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": a.precision,
+        "dtype": ARITH[a.precision],
         "data": "synthetic",
         "config": {
             "workload": f"{kw['nx']}x{kw['ny']}x{kw['nz']} D3Q19 BGK, inlet/outlet x, periodic y, walls z, "
@@ -318,7 +336,7 @@ def main():
                         f"{kw['sheets'][0].rows}x{kw['sheets'][0].cols} nodes (Ks=Kb=1e-3, jitter 0.1)",
             "grid": [kw["nx"], kw["ny"], kw["nz"]],
             "fiber_nodes": nodes,
-            "storage": a.precision + (" deviations f-w" if a.precision != "f64" else ""),
+            "storage": storage_desc(a.precision, a.dev_ref),
             "bytes_per_cell_update": bpc,
             "kernel_variant": {"cells_per_thread": variant[0], "nontemporal_mask": variant[1],
                                "blocks_per_cu": variant[2]},
