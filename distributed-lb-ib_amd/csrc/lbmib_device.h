@@ -130,8 +130,29 @@ __device__ __forceinline__ void st(typename Traits<P>::S *p, typename Traits<P>:
   st_h<P, NT>(p, v * Traits<P>::scale);
 }
 
+// A load of memory that no store of the running kernel touches, through the
+// constant address space: a wave-uniform address then compiles to a scalar
+// load (as for kernel arguments), not to a vector load of one address.
+template <class X>
+__device__ __forceinline__ X ldc(const X *p) {
+  return *(const __attribute__((address_space(4))) X *)p;
+}
+// Copies a trivially copyable object from such memory into registers, one
+// dword at a time (sizeof(X) % 4 == 0).
+template <class X>
+__device__ __forceinline__ X ldc_obj(const X *p) {
+  static_assert(sizeof(X) % 4 == 0, "dword-sized object");
+  X out;
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(p);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(X) / 4); ++i) dst[i] = ldc(src + i);
+  return out;
+}
+
+// Support boxes are read-only in every kernel that tests them.
 __device__ __forceinline__ bool in_box6(const int *b, int x, int y, int z) {
-  return x >= b[0] && x <= b[1] && y >= b[2] && y <= b[3] && z >= b[4] && z <= b[5];
+  return x >= ldc(b) && x <= ldc(b + 1) && y >= ldc(b + 2) && y <= ldc(b + 3) && z >= ldc(b + 4) && z <= ldc(b + 5);
 }
 
 // Any box overlapping the x-run [x0, x1] of row (y, z).
@@ -139,7 +160,8 @@ __device__ __forceinline__ bool any_box_overlaps(const int *boxes, int nsheets, 
   bool in = false;
   for (int s = 0; s < nsheets; ++s) {
     const int *b = boxes + 6 * s;
-    in |= b[0] <= x1 && b[1] >= x0 && y >= b[2] && y <= b[3] && z >= b[4] && z <= b[5];
+    in |= ldc(b) <= x1 && ldc(b + 1) >= x0 && y >= ldc(b + 2) && y <= ldc(b + 3) && z >= ldc(b + 4) &&
+          z <= ldc(b + 5);
   }
   return in;
 }

@@ -213,7 +213,7 @@ __device__ __forceinline__ void arrive_general(const Geometry &g, const typename
     // keeps init_gv's rho/u (lbm_ib.c:837-849).
     if (frozen) {
 #pragma unroll
-      for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
+      for (int q = 0; q < 19; ++q) v[q] = ldc(c.tab + q);
     } else {
 #pragma unroll
       for (int q = 0; q < 19; ++q) v[q] = ld_h<P>(fin + q * g.pitch + c0);
@@ -347,12 +347,13 @@ __device__ __forceinline__ void store_band_rects(const Geometry &g, const int *b
   rect_buf[0] = n;
 }
 
+// (rect_buf is written by the band launch and only read afterwards.)
 __device__ __forceinline__ bool in_band_rects(const int *rect_buf, int y, int z) {
   bool in = false;
-  const int n = rect_buf[0];
+  const int n = ldc(rect_buf);
   for (int k = 0; k < n; ++k) {
     const int *r = rect_buf + 1 + 4 * k;
-    in |= y >= r[0] && y <= r[1] && z >= r[2] && z <= r[3];
+    in |= y >= ldc(r) && y <= ldc(r + 1) && z >= ldc(r + 2) && z <= ldc(r + 3);
   }
   return in;
 }
@@ -429,7 +430,7 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
     T d, rho, ux, uy, uz;
     if (frozen) {
 #pragma unroll
-      for (int q = 0; q < 19; ++q) v[q] = c.tab[q];
+      for (int q = 0; q < 19; ++q) v[q] = ldc(c.tab + q);
       rho = c.rho0; d = c.d0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
     } else {
       // per-row source offsets (row_source_table): wrap and bounce-back
@@ -445,10 +446,10 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
         const uint32_t *rt = rowtab + (long)row * kRowTabStride;
         uint32_t so[19];
 #pragma unroll
-        for (int q = 0; q < 19; ++q) so[q] = rt[q];
+        for (int q = 0; q < 19; ++q) so[q] = ldc(rt + q);
 #pragma unroll
         for (int q = 0; q < 19; ++q) v[q] = bld_h<P, NTL>(rin, vo, so[q]);
-        bounce_fix<P>(rt[19], c.br[1], c.br[2], v);
+        bounce_fix<P>(ldc(rt + 19), c.br[1], c.br[2], v);
       }
       if (own) {
         rho = c.rho0; d = c.d0; ux = c.u0x; uy = c.u0y; uz = c.u0z;  // lbm_ib.c:837-849
@@ -612,10 +613,10 @@ __global__ void __launch_bounds__(512)
         const uint32_t *rt = rowtab + (long)row * kRowTabStride;
         uint32_t so[19];
 #pragma unroll
-        for (int q = 0; q < 19; ++q) so[q] = rt[q];
+        for (int q = 0; q < 19; ++q) so[q] = ldc(rt + q);
 #pragma unroll
         for (int q = 0; q < 19; ++q) bld_pair<P, NTL>(rin, vo, so[q], v[0][q], v[1][q]);
-        const uint32_t bbm = rt[19];
+        const uint32_t bbm = ldc(rt + 19);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
           bounce_fix<P>(bbm, c.br[1], c.br[2], v[k]);
@@ -631,7 +632,7 @@ __global__ void __launch_bounds__(512)
         if (frozen[k]) {
           // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
 #pragma unroll
-          for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
+          for (int q = 0; q < 19; ++q) v[k][q] = ldc(c.tab + q);
           rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
         T fx = 0, fy = 0, fz = 0;
@@ -709,14 +710,14 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
         for (int k = 0; k < 2; ++k)
 #pragma unroll
-          for (int q = 0; q < 19; ++q) so[k][q] = rt[k * kRowTabStride + q];
+          for (int q = 0; q < 19; ++q) so[k][q] = ldc(rt + k * kRowTabStride + q);
 #pragma unroll
         for (int k = 0; k < 2; ++k)
 #pragma unroll
           for (int q = 0; q < 19; ++q) v[k][q] = bld_h<P, NTL>(rin, vo, so[k][q]);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-          bounce_fix<P>(rt[k * kRowTabStride + 19], c.br[1], c.br[2], v[k]);
+          bounce_fix<P>(ldc(rt + k * kRowTabStride + 19), c.br[1], c.br[2], v[k]);
           const long ck = (long)x + (long)g.nx * (row0 + k);
           T fpx = 0, fpy = 0, fpz = 0;
           if (in_any_box(box_prev, nsheets, x, y0 + k, z)) {
@@ -731,7 +732,7 @@ __global__ void __launch_bounds__(512)
         if (frozen) {
           // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
 #pragma unroll
-          for (int q = 0; q < 19; ++q) v[k][q] = c.tab[q];
+          for (int q = 0; q < 19; ++q) v[k][q] = ldc(c.tab + q);
           rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
         }
         T fx = 0, fy = 0, fz = 0;
@@ -768,18 +769,22 @@ __global__ void __launch_bounds__(512)
   using FT = typename Traits<P>::FT;
   const int m = map[blockIdx.x];
   const int inst = m >> kBatchShift, lb = m & ((1 << kBatchShift) - 1);
-  const BatchDesc &d = descs[inst];
+  // The descriptor is read-only during the launch: its wave-uniform fields
+  // come in with scalar loads, as kernel arguments do.
+  const BatchDesc *dp = descs + inst;
+  const int wpr = ldc(&dp->wpr), nrows = ldc(&dp->nrows);
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-  const int rpb = wpb / d.wpr;
-  if (wave >= rpb * d.wpr) return;
-  const int row = lb * rpb + wave / d.wpr;
-  const int x = (wave % d.wpr) * 64 + lane;
-  if (row >= d.nrows || x >= d.g.nx) return;
-  const KConsts<T> &c = *reinterpret_cast<const KConsts<T> *>(d.kc);
-  const int *box = d.ib.box;
-  row_cell<P, NT, NTL>(d.g, make_rsrc(d.f[cur]), make_rsrc(d.f[1 - cur]), own, c, d.nsheets,
-                       box + slot * kMaxSheets * 6, box + (1 - slot) * kMaxSheets * 6, (const FT *)d.F[slot],
-                       (const FT *)d.F[1 - slot], d.rowtab, row, x);
+  const int rpb = wpb / wpr;
+  if (wave >= rpb * wpr) return;
+  const int row = lb * rpb + wave / wpr;
+  const int x = (wave % wpr) * 64 + lane;
+  const Geometry g = ldc_obj(&dp->g);
+  if (row >= nrows || x >= g.nx) return;
+  const KConsts<T> c = ldc_obj(reinterpret_cast<const KConsts<T> *>(dp->kc));
+  const int *box = ldc(&dp->ib.box);
+  row_cell<P, NT, NTL>(g, make_rsrc(ldc(&dp->f[cur])), make_rsrc(ldc(&dp->f[1 - cur])), own, c, ldc(&dp->nsheets),
+                       box + slot * kMaxSheets * 6, box + (1 - slot) * kMaxSheets * 6,
+                       (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab), row, x);
 }
 
 // The x-edge cells (x = lo, hi) of every active row when x is a wall or a
