@@ -60,7 +60,8 @@ def parse_args():
     p.add_argument("--variant", default="auto",
                    help="fused-kernel launch shape 'cells,ntmask,blocks_per_cu' or 'auto' (autotune)")
     p.add_argument("--overlap", type=int, default=-1, choices=[-1, 0, 1],
-                   help="split steps so the IB chain overlaps the bulk fluid update: -1 library default, 0 off, 1 on")
+                   help="overlap the IB kernels with the fluid update (split steps; for c4, two instance "
+                        "groups): -1 library default, 0 off, 1 on")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -184,6 +185,8 @@ def run_batch(a, lbmib, lbmib_configs):
     print(f"[bench] c4 ensemble: {len(cfgs)} instances, {a.precision}, warmup={a.warmup} steps={a.steps}",
           file=sys.stderr, flush=True)
     with lbmib.Batch(cfgs) as b:
+        if a.overlap >= 0:
+            b.set_overlap(a.overlap)
         b.init()
         cells, nodes = b.cells()
         best = None
@@ -226,6 +229,7 @@ def run_batch(a, lbmib, lbmib_configs):
             "bytes_per_cell_update": bpc,
             "storage": storage_desc(a.precision, a.dev_ref),
             "tile": {"waves_per_block": best[1], "nontemporal_mask": best[2]},
+            "overlapping_groups": a.overlap != 0,
         },
         "roofline": {
             "bound": "hbm",

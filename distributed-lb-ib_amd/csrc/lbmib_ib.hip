@@ -492,12 +492,15 @@ __device__ __forceinline__ void batch_block(const int *map, int &inst, int &lb) 
   lb = m & ((1 << kBatchShift) - 1);
 }
 
+// The descriptor is read-only during a launch: its IB state and geometry are
+// copied into registers with scalar loads (as kernel arguments would be).
 __global__ void __launch_bounds__(kFiberThreads) k_fiber_batch(const BatchDesc *__restrict__ descs,
                                                                const int *__restrict__ map, int slot) {
   int inst, lb;
   batch_block(map, inst, lb);
-  const BatchDesc &d = descs[inst];
-  fiber_body(d.ib, d.g, FK_ALL, slot, lb, fiber_blocks(d.ib.nnodes));
+  const BatchDesc *dp = descs + inst;
+  const IBState ib = ldc_obj(&dp->ib);
+  fiber_body(ib, ldc_obj(&dp->g), FK_ALL, slot, lb, fiber_blocks(ib.nnodes));
 }
 
 template <class FT>
@@ -505,29 +508,33 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather_batch(const BatchDesc
                                                                  const int *__restrict__ map, int slot) {
   int inst, lb;
   batch_block(map, inst, lb);
-  const BatchDesc &d = descs[inst];
-  gather_body<FT>(d.ib, d.g, slot, (FT *)d.F[slot], d.mcap, lb, gather_blocks(d.ib.nnodes));
+  const BatchDesc *dp = descs + inst;
+  const IBState ib = ldc_obj(&dp->ib);
+  gather_body<FT>(ib, ldc_obj(&dp->g), slot, (FT *)ldc(&dp->F[slot]), ldc(&dp->mcap), lb, gather_blocks(ib.nnodes));
 }
 
 template <int P>
 __global__ void __launch_bounds__(256) k_band_u_batch(const BatchDesc *__restrict__ descs,
                                                       const int *__restrict__ map, int cur_new, int slot) {
+  using T = typename Traits<P>::T;
   int inst, lb;
   batch_block(map, inst, lb);
-  const BatchDesc &d = descs[inst];
-  band_u_body<P>(d.ib, d.g, (const typename Traits<P>::S *)d.f[cur_new], (const typename Traits<P>::FT *)d.F[slot],
-                 d.u0[0], d.u0[1], d.u0[2], d.gz, d.ref,
-                 reinterpret_cast<const KConsts<typename Traits<P>::T> *>(d.kc)->br[1],
-                 reinterpret_cast<const KConsts<typename Traits<P>::T> *>(d.kc)->br[2], slot, lb,
-                 band_u_blocks(d.ib.nnodes));
+  const BatchDesc *dp = descs + inst;
+  const IBState ib = ldc_obj(&dp->ib);
+  const KConsts<T> *kc = reinterpret_cast<const KConsts<T> *>(dp->kc);
+  band_u_body<P>(ib, ldc_obj(&dp->g), (const typename Traits<P>::S *)ldc(&dp->f[cur_new]),
+                 (const typename Traits<P>::FT *)ldc(&dp->F[slot]), ldc(&dp->u0[0]), ldc(&dp->u0[1]),
+                 ldc(&dp->u0[2]), ldc(&dp->gz), ldc(&dp->ref), ldc(&kc->br[1]), ldc(&kc->br[2]), slot, lb,
+                 band_u_blocks(ib.nnodes));
 }
 
 __global__ void __launch_bounds__(kMoveThreads) k_move_batch(const BatchDesc *__restrict__ descs,
                                                              const int *__restrict__ map, int reset_slot) {
   int inst, lb;
   batch_block(map, inst, lb);
-  const BatchDesc &d = descs[inst];
-  move_body(d.ib, d.g, reset_slot, lb, move_blocks(d.ib.nnodes));
+  const BatchDesc *dp = descs + inst;
+  const IBState ib = ldc_obj(&dp->ib);
+  move_body(ib, ldc_obj(&dp->g), reset_slot, lb, move_blocks(ib.nnodes));
 }
 
 // ------------------------------------------------------------- launchers

@@ -202,13 +202,19 @@ int lbmib_batch_step(lbmib_batch *b, int nsteps);
  * or 64-cell chunk of a row, per wave; a block holds whole rows) and the
  * nontemporal mask as in lbmib_set_kernel_variant.  bench.py autotunes it. */
 int lbmib_batch_set_tile(lbmib_batch *b, int waves_per_block, int nontemporal);
+/* on = 1 (default): two groups of instances of about equal cell count, each
+ * on its own stream, whose fused launches alternate so that one group's IB
+ * kernels run beside the other group's fluid update; on = 0: one group.
+ * Results are the same bits either way. */
+int lbmib_batch_set_overlap(lbmib_batch *b, int on);
 /* Grid cells and fiber nodes over all instances. */
 int lbmib_batch_cells(lbmib_batch *b, double *cells, int *nodes);
 /* First non-zero status of the instances (synchronising). */
 int lbmib_batch_status(lbmib_batch *b);
 /* nwarm untimed batched steps, then nsteps timed with HIP events (total
  * milliseconds), then, if ms_fused_mean is given, nsteps more with an event
- * pair around each batched fused launch (its mean milliseconds). */
+ * pair around each batched fused launch (the mean per step of their sum over
+ * the groups, in milliseconds). */
 int lbmib_batch_time_steps(lbmib_batch *b, int nwarm, int nsteps, double *ms_total, double *ms_fused_mean);
 
 #ifdef __cplusplus

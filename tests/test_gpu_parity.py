@@ -410,14 +410,18 @@ def _ensemble(lbmib, precision):
     return out
 
 
+@pytest.mark.parametrize("overlap", [1, 0])
 @pytest.mark.parametrize("precision", ["f32", "bf16", "f64"])
-def test_batch_matches_single_instances(lbmib, precision):
+def test_batch_matches_single_instances(lbmib, precision, overlap):
     """Each member of a batched ensemble ends with the bits of the same
-    simulation stepped alone."""
+    simulation stepped alone, with the instances in two overlapping groups
+    (the default) or in one."""
     cfgs = _ensemble(lbmib, precision)
     with lbmib.Batch(cfgs) as b:
+        b.set_overlap(overlap)
         b.init()
-        b.step(12)
+        b.step(5)
+        b.step(7)
         assert b.status() == 0
         got = [(e.df(), e.fibers(0) if e.sheets else None) for e in b.instances]
     for cfg, (df, fib) in zip(cfgs, got):
