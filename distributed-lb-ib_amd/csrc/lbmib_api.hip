@@ -177,6 +177,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   alloc((void **)&ctx->ib.box, sizeof(int) * 2 * kMaxSheets * 6);
   alloc(&ctx->d_tab, sizeof(double) * 19);
   alloc((void **)&ctx->d_rect, sizeof(int) * kRectBufInts);
+  if (!rc && hipMemset(ctx->d_rect, 0, sizeof(int) * kRectBufInts) != hipSuccess) rc = LBMIB_ERR_HIP;
   alloc((void **)&ctx->d_ref, sizeof(double) * 19);
   if (!rc) {
     rc = upload_fluid_tables(p.precision, c, ctx->d_tab, ctx->d_ref);
@@ -442,16 +443,16 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
 
 // ------------------------------------------------------------ split steps
 //
-// With sheets present, a split step updates the band rows -- every (y, z)
-// row near the sheets' support boxes of this and the previous step, all x --
-// first, then the remaining rows (lbmib_fluid.hip, "split steps"):
+// With sheets present, a split step updates the band -- the cells of the
+// sheets' support boxes of this and the previous step, widened by one cell --
+// first, then every other cell (lbmib_fluid.hip, "split steps"):
 //   stream   : wait IB(t) | band(t) | bulk(t)
 //   stream_b : wait band(t) | band velocity + move(t), fiber forces and
 //              spreading of t+1 -> IB(t+1)
 // The band velocity reads the new populations only on support cells and
 // their neighbours, which the band holds, so the IB chain of the next step
-// runs while the bulk rows are updated.  Results are bit-identical to
-// unsplit steps: every cell does the same arithmetic.
+// runs while the bulk is updated.  Results are bit-identical to unsplit
+// steps: every cell does the same arithmetic.
 
 // Automatic mode splits whenever there are sheets and the buffer-addressed
 // row kernels apply (measured at 256^3 with a 104x104 sheet: 4% faster in
@@ -461,14 +462,14 @@ static bool split_enabled(const lbmib_ctx *ctx) {
   return ctx->overlap_req != 0;
 }
 
-// Grid of the band update: about the rows the sheets' support spans, from
-// the uploaded positions (the kernel strides over any further rows).
+// Grid of the band update (256 threads a block): about one thread per band
+// cell, from the sheets' y/z extent at upload and a support about 8 cells
+// thick in x (the kernel strides over any further cells).
 static int band_blocks(const lbmib_ctx *ctx) {
-  double rows = 64;
+  double cells = 0;
   for (int s = 0; s < ctx->nsheets; ++s)
-    rows += (ctx->hyz[s][1] - ctx->hyz[s][0] + 10.0) * (ctx->hyz[s][3] - ctx->hyz[s][2] + 10.0);
-  const double nrows = (double)ctx->g.ny * ctx->g.nz;
-  return (int)std::min(rows * 1.25, nrows);
+    cells += 8.0 * (ctx->hyz[s][1] - ctx->hyz[s][0] + 8.0) * (ctx->hyz[s][3] - ctx->hyz[s][2] + 8.0);
+  return (int)std::min(std::max(cells / 256.0, 64.0), 16.0 * ctx->ncu);
 }
 
 static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, bool more) {

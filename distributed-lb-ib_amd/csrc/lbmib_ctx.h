@@ -53,7 +53,7 @@ struct lbmib_ctx {
   int split_run = 0;              // steps taken in the current run of split steps
   int ib_ready = 0;               // fiber forces and spreading of the current step done (stream_b)
   int *d_rect = nullptr;          // band rectangles of the last band update
-  double hyz[kMaxSheets][4];      // host y/z extent of each sheet's nodes (from the uploaded positions)
+  double hyz[kMaxSheets][4] = {};      // host y/z extent of each sheet's nodes (from the uploaded positions)
   std::vector<hipEvent_t> evs;
 };
 

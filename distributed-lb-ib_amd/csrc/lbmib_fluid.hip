@@ -287,94 +287,77 @@ int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
 
 // ---------------------------------------------------------- split steps
 //
-// A split step (lbmib_set_overlap) updates the rows (y, z) near the sheets
-// first -- the band -- and the remaining rows -- the bulk -- afterwards, so
-// that the next step's IB chain, which needs only the band's new
-// populations, overlaps the bulk.  The band holds, per sheet, the (y, z)
-// rectangle spanned by the support boxes of this and the previous step (all
-// cells with a force in either), widened by one row on each side for the
-// pulls of the band velocity; on a periodic axis a rectangle touching an end
-// takes the whole axis.  The band kernel's first block stores the
-// rectangles in rect_buf ({n, (y0, y1, z0, z1) x n}) for the bulk kernel,
-// which runs while the next step's IB chain rewrites the box slots.
+// A split step (lbmib_set_overlap) first updates the band -- per sheet, the
+// box spanned by the support boxes of this and the previous step (all cells
+// with a force in either), widened by one cell on each side for the pulls of
+// the band velocity -- and then the bulk, every other cell, so that the next
+// step's IB chain, which needs only the band's new populations, overlaps the
+// bulk.  The band is a few thousand cells per sheet; k_collide_band updates
+// them cell by cell by the general rules, and its first thread stores the
+// boxes in rect_buf ({n, (x0, x1, y0, y1, z0, z1) x n}) for the bulk launch,
+// which skips those cells and runs while the next step's IB chain rewrites the
+// support-box slots and the force field (the bulk reads neither: its cells
+// have no force).  A support box moves by less than a cell per step, so the
+// next step's box lies inside this step's band box.
 enum { kRowsAll = 0, kRowsBand = 1, kRowsBulk = 2 };
 
-__device__ __forceinline__ bool band_rect(const Geometry &g, const int *box_cur, const int *box_prev, int s,
-                                          int r[4]) {
+__device__ __forceinline__ bool band_box3(const Geometry &g, const int *box_cur, const int *box_prev, int s,
+                                          int r[6]) {
   const int *b1 = box_cur + 6 * s, *b0 = box_prev + 6 * s;
-  const bool v1 = b1[0] <= b1[1], v0 = b0[0] <= b0[1];
+  const bool v1 = ldc(b1) <= ldc(b1 + 1), v0 = ldc(b0) <= ldc(b0 + 1);
   if (!v1 && !v0) return false;
-  int y0 = INT_MAX, y1 = INT_MIN, z0 = INT_MAX, z1 = INT_MIN;
-  if (v1) { y0 = min(y0, b1[2]); y1 = max(y1, b1[3]); z0 = min(z0, b1[4]); z1 = max(z1, b1[5]); }
-  if (v0) { y0 = min(y0, b0[2]); y1 = max(y1, b0[3]); z0 = min(z0, b0[4]); z1 = max(z1, b0[5]); }
-  y0 -= 1; y1 += 1; z0 -= 1; z1 += 1;
-  if (g.bc[1] == LBMIB_BC_PERIODIC && (y0 < 0 || y1 >= g.ny)) { y0 = 0; y1 = g.ny - 1; }
-  if (g.bc[2] == LBMIB_BC_PERIODIC && (z0 < 0 || z1 >= g.nz)) { z0 = 0; z1 = g.nz - 1; }
-  r[0] = max(y0, 0); r[1] = min(y1, g.ny - 1); r[2] = max(z0, 0); r[3] = min(z1, g.nz - 1);
-  return true;
+  const int dims[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int lo = INT_MAX, hi = INT_MIN;
+    if (v1) { lo = min(lo, ldc(b1 + 2 * a)); hi = max(hi, ldc(b1 + 2 * a + 1)); }
+    if (v0) { lo = min(lo, ldc(b0 + 2 * a)); hi = max(hi, ldc(b0 + 2 * a + 1)); }
+    lo -= 1;
+    hi += 1;
+    // whole x pairs (even x0, odd x1), so that no lane of the two-cell row
+    // kernel straddles the band
+    if (a == 0) { lo -= lo & 1; hi += 1 - (hi & 1); }
+    // on a periodic axis a box touching an end takes the whole axis
+    if (g.bc[a] == LBMIB_BC_PERIODIC && (lo < 0 || hi >= dims[a])) { lo = 0; hi = dims[a] - 1; }
+    r[2 * a] = max(lo, 0);
+    r[2 * a + 1] = min(hi, dims[a] - 1);
+  }
+  return r[0] <= r[1] && r[2] <= r[3] && r[4] <= r[5];
 }
 
-// Row index of band item `it` (rectangles in sheet order, y fastest); -1 for
-// a row an earlier rectangle already holds, -2 past the last item.
-__device__ __forceinline__ int band_row_at(const Geometry &g, const int *box_cur, const int *box_prev, int nsheets,
-                                           long it) {
-  for (int s = 0; s < nsheets; ++s) {
-    int r[4];
-    if (!band_rect(g, box_cur, box_prev, s, r)) continue;
-    const long h = r[1] - r[0] + 1, area = h * (r[3] - r[2] + 1);
-    if (it < area) {
-      const int y = r[0] + (int)(it % h), z = r[2] + (int)(it / h);
-      for (int t = 0; t < s; ++t) {
-        int q[4];
-        if (band_rect(g, box_cur, box_prev, t, q) && y >= q[0] && y <= q[1] && z >= q[2] && z <= q[3]) return -1;
-      }
-      return y + g.ny * z;
-    }
-    it -= area;
-  }
-  return -2;
-}
-
-__device__ __forceinline__ void store_band_rects(const Geometry &g, const int *box_cur, const int *box_prev,
-                                                 int nsheets, int *rect_buf) {
-  int n = 0;
-  for (int s = 0; s < nsheets; ++s) {
-    int r[4];
-    if (!band_rect(g, box_cur, box_prev, s, r)) continue;
-    for (int k = 0; k < 4; ++k) rect_buf[1 + 4 * n + k] = r[k];
-    ++n;
-  }
-  rect_buf[0] = n;
+__device__ __forceinline__ bool in_box3(const int *r, int x, int y, int z) {
+  return x >= r[0] && x <= r[1] && y >= r[2] && y <= r[3] && z >= r[4] && z <= r[5];
 }
 
 // (rect_buf is written by the band launch and only read afterwards.)
-__device__ __forceinline__ bool in_band_rects(const int *rect_buf, int y, int z) {
+__device__ __forceinline__ bool in_band_boxes(const int *rect_buf, int x, int y, int z) {
   bool in = false;
-  const int n = ldc(rect_buf);
+  const int n = min(ldc(rect_buf), kMaxSheets);
   for (int k = 0; k < n; ++k) {
-    const int *r = rect_buf + 1 + 4 * k;
-    in |= y >= ldc(r) && y <= ldc(r + 1) && z >= ldc(r + 2) && z <= ldc(r + 3);
+    const int *r = rect_buf + 1 + 6 * k;
+    in |= x >= ldc(r) && x <= ldc(r + 1) && y >= ldc(r + 2) && y <= ldc(r + 3) && z >= ldc(r + 4) &&
+          z <= ldc(r + 5);
   }
   return in;
 }
 
-// Next row of this block's walk over the rows of `sel` (all, band, bulk);
-// false when the walk is over.  `it` starts at blockIdx.x.
-__device__ __forceinline__ bool next_row(const Geometry &g, int sel, const int *box_cur, const int *box_prev,
-                                         int nsheets, const int *rect_buf, long &it, int &row) {
-  const long nrows = (long)g.ny * g.nz;
-  for (;; it += gridDim.x) {
-    if (sel == kRowsBand) {
-      row = band_row_at(g, box_cur, box_prev, nsheets, it);
-      if (row == -2) return false;
-      if (row < 0) continue;
-      return true;
-    }
-    if (it >= nrows) return false;
-    row = (int)it;
-    if (sel == kRowsBulk && in_band_rects(rect_buf, row % g.ny, row / g.ny)) continue;
-    return true;
+// Whether row (y, z) meets any band box (wave-uniform pre-test of the bulk).
+__device__ __forceinline__ bool row_meets_band(const int *rect_buf, int y, int z) {
+  bool in = false;
+  const int n = min(ldc(rect_buf), kMaxSheets);
+  for (int k = 0; k < n; ++k) {
+    const int *r = rect_buf + 1 + 6 * k;
+    in |= y >= ldc(r + 2) && y <= ldc(r + 3) && z >= ldc(r + 4) && z <= ldc(r + 5);
   }
+  return in;
+}
+
+// Next row of this block's walk over all rows; false when the walk is over.
+// `it` starts at blockIdx.x.
+__device__ __forceinline__ bool next_row(const Geometry &g, long it, int &row) {
+  if (it >= (long)g.ny * g.nz) return false;
+  row = (int)it;
+  return true;
 }
 
 // Deviation forms: a population reflected at a y/z wall (bit q of the row's
@@ -403,12 +386,15 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
                                          const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                                          const typename Traits<P>::FT *__restrict__ Fcur,
                                          const typename Traits<P>::FT *__restrict__ Fprev,
-                                         const uint32_t *__restrict__ rowtab, int row, int x) {
+                                         const uint32_t *__restrict__ rowtab, int row, int x,
+                                         const int *__restrict__ band = nullptr) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   constexpr uint32_t es = sizeof(S);
   const uint32_t pitch_b = (uint32_t)g.pitch * es;
   const uint32_t sy_b = (uint32_t)g.nx * es;
+  // bulk of a split step: the band launch has stored the band's cells
+  if (band && row_meets_band(band, row % g.ny, row / g.ny) && in_band_boxes(band, x, row % g.ny, row / g.ny)) return;
   const bool edge = x == g.lo[0] || x == g.hi[0];
   // frozen inlet/outlet planes; x edges of wall/periodic x go to k_collide_xedges
   const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
@@ -494,15 +480,14 @@ __global__ void __launch_bounds__(1024)
                           int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                           const typename Traits<P>::FT *__restrict__ Fcur,
                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
-                          int sel, int *__restrict__ rect_buf) {
-  if (sel == kRowsBand && blockIdx.x == 0 && threadIdx.x == 0)
-    store_band_rects(g, box_cur, box_prev, nsheets, rect_buf);
+                          const int *__restrict__ band) {
   if constexpr (!BUF) {
     int row;
-    for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x) {
+    for (long it = blockIdx.x; next_row(g, it, row); it += gridDim.x) {
       const int y = row % g.ny, z = row / g.ny;
       for (int x = threadIdx.x; x < g.nx; x += blockDim.x)
-        update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
+        if (!band || !in_band_boxes(band, x, y, z))
+          update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
     }
   } else {
     // one cell per thread: blockDim.x >= nx (checked by the launcher)
@@ -510,8 +495,8 @@ __global__ void __launch_bounds__(1024)
     const int x = threadIdx.x;
     if (x >= g.nx) return;
     int row;
-    for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x)
-      row_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, row, x);
+    for (long it = blockIdx.x; next_row(g, it, row); it += gridDim.x)
+      row_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, row, x, band);
   }
 }
 
@@ -570,15 +555,13 @@ __global__ void __launch_bounds__(512)
                            int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                            const typename Traits<P>::FT *__restrict__ Fcur,
                            const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
-                           int sel, int *__restrict__ rect_buf) {
+                           const int *__restrict__ band) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
   constexpr uint32_t es = sizeof(S);
   const uint32_t pitch_b = (uint32_t)g.pitch * es;
   const uint32_t sy_b = (uint32_t)g.nx * es;
-  if (sel == kRowsBand && blockIdx.x == 0 && threadIdx.x == 0)
-    store_band_rects(g, box_cur, box_prev, nsheets, rect_buf);
   const int x0 = 2 * (int)threadIdx.x;
   if (x0 >= g.nx) return;
   // dead pairs store zeros so that the row's cache lines are written whole
@@ -587,9 +570,12 @@ __global__ void __launch_bounds__(512)
   const bool frozen[2] = {inout_x && x0 == g.lo[0], inout_x && x0 + 1 == g.hi[0]};
   const uint32_t vo = (uint32_t)x0 * es;
   int row;
-  for (long it = blockIdx.x; next_row(g, sel, box_cur, box_prev, nsheets, rect_buf, it, row); it += gridDim.x) {
+  for (long it = blockIdx.x; next_row(g, it, row); it += gridDim.x) {
     const int y = row % g.ny, z = row / g.ny;
     if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
+    // bulk of a split step: the band launch stores the band's cells (whole
+    // x pairs)
+    if (band && row_meets_band(band, y, z) && in_band_boxes(band, x0, y, z)) continue;
     const uint32_t row_b = (uint32_t)row * sy_b;
     const long c0 = (long)x0 + (long)g.nx * row;
     T v[2][19];
@@ -787,6 +773,54 @@ __global__ void __launch_bounds__(512)
                        (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab), row, x);
 }
 
+// The band of a split step: every cell of the band boxes by the general
+// rules, each once (a cell in several boxes goes with the first).  Threads
+// stride over the cells of all boxes; the first thread stores the boxes for
+// the bulk launch.
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(256)
+    k_collide_band(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                   typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c, int nsheets,
+                   const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                   const typename Traits<P>::FT *__restrict__ Fcur, const typename Traits<P>::FT *__restrict__ Fprev,
+                   int *__restrict__ rect_buf) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid == 0) {
+    int n = 0;
+    for (int s = 0; s < nsheets; ++s) {
+      int r[6];
+      if (!band_box3(g, box_cur, box_prev, s, r)) continue;
+      for (int k = 0; k < 6; ++k) rect_buf[1 + 6 * n + k] = r[k];
+      ++n;
+    }
+    rect_buf[0] = n;
+  }
+  long total = 0;
+  for (int s = 0; s < nsheets; ++s) {
+    int r[6];
+    if (band_box3(g, box_cur, box_prev, s, r)) total += (long)(r[1] - r[0] + 1) * (r[3] - r[2] + 1) * (r[5] - r[4] + 1);
+  }
+  for (long it = tid; it < total; it += (long)gridDim.x * blockDim.x) {
+    long rest = it;
+    int s = 0, r[6];
+    for (; s < nsheets; ++s) {
+      if (!band_box3(g, box_cur, box_prev, s, r)) continue;
+      const long vol = (long)(r[1] - r[0] + 1) * (r[3] - r[2] + 1) * (r[5] - r[4] + 1);
+      if (rest < vol) break;
+      rest -= vol;
+    }
+    if (s == nsheets) break;  // not reached: it < total
+    const int nx = r[1] - r[0] + 1, ny = r[3] - r[2] + 1;
+    const int x = r[0] + (int)(rest % nx), y = r[2] + (int)((rest / nx) % ny), z = r[4] + (int)(rest / ((long)nx * ny));
+    bool dup = false;
+    for (int t = 0; t < s; ++t) {
+      int q[6];
+      if (band_box3(g, box_cur, box_prev, t, q) && in_box3(q, x, y, z)) dup = true;
+    }
+    if (!dup) update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
+  }
+}
+
 // The x-edge cells (x = lo, hi) of every active row when x is a wall or a
 // periodic axis, by the general rules; completes the buffer-addressed rows.
 template <int P, bool NT, bool NTL>
@@ -795,13 +829,13 @@ __global__ void __launch_bounds__(256)
                      typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
                      int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                      const typename Traits<P>::FT *__restrict__ Fcur,
-                     const typename Traits<P>::FT *__restrict__ Fprev, int sel, const int *__restrict__ rect_buf) {
+                     const typename Traits<P>::FT *__restrict__ Fprev, const int *__restrict__ band) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 2L * g.ny * g.nz) return;
   const long row = e >> 1;
   const int x = (e & 1) ? g.hi[0] : g.lo[0];
   if ((e & 1) && g.lo[0] == g.hi[0]) return;  // one-cell-wide axis: done by the even item
-  if (sel != kRowsAll && in_band_rects(rect_buf, (int)(row % g.ny), (int)(row / g.ny)) != (sel == kRowsBand)) return;
+  if (band && in_band_boxes(band, x, (int)(row % g.ny), (int)(row / g.ny))) return;  // stored by the band launch
   update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, (int)(row % g.ny),
                           (int)(row / g.ny));
 }
@@ -1105,8 +1139,6 @@ template <int P, bool NT, bool NTL>
 static void launch_rows2(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                          const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
-  const int sel = kRowsAll;  // whole-grid steps only
-  int *rect_buf = nullptr;
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const int npairs = g.ny * g.nz / 2;
@@ -1114,11 +1146,11 @@ static void launch_rows2(int nblocks, const Geometry &g, const void *fin, void *
   const dim3 gr(nblocks < npairs ? nblocks : npairs);
   k_collide_stream_rows2<P, NT, NTL><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
                                                       box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab);
-  if (g.bc[0] != LBMIB_BC_INOUT) {
+  if (g.bc[0] != LBMIB_BC_INOUT) {  // whole-grid steps only
     const long items = 2L * g.ny * g.nz;
     k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
         g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
-        (const FT *)Fprev, sel, rect_buf);
+        (const FT *)Fprev, nullptr);
   }
 }
 
@@ -1133,7 +1165,7 @@ bool pair_path_ok(const Geometry &g) {
 template <int P, bool NT, bool NTL>
 static void launch_pairs(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, int sel, int *rect_buf,
+                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, const int *band,
                          hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
@@ -1142,19 +1174,19 @@ static void launch_pairs(int nblocks, const Geometry &g, const void *fin, void *
   const dim3 gr(nblocks < nrows ? nblocks : nrows);
   k_collide_stream_pairs<P, NT, NTL><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
                                                       box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab,
-                                                      sel, rect_buf);
+                                                      band);
   if (g.bc[0] != LBMIB_BC_INOUT) {
     const long items = 2L * g.ny * g.nz;
     k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
         g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
-        (const FT *)Fprev, sel, rect_buf);
+        (const FT *)Fprev, band);
   }
 }
 
 template <int P, bool NT, bool NTL>
 static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                         const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                        const void *Fcur, const void *Fprev, const uint32_t *rowtab, int sel, int *rect_buf,
+                        const void *Fcur, const void *Fprev, const uint32_t *rowtab, const int *band,
                         hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
@@ -1164,17 +1196,17 @@ static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *f
   if (rowtab && row_path_buffered(g, sizeof(S))) {
     k_collide_stream_rows<P, NT, NTL, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
                                                              box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev,
-                                                             rowtab, sel, rect_buf);
+                                                             rowtab, band);
     if (g.bc[0] != LBMIB_BC_INOUT) {
       const long items = 2L * g.ny * g.nz;
       k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
           g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur,
-          (const FT *)Fprev, sel, rect_buf);
+          (const FT *)Fprev, band);
     }
   } else
     k_collide_stream_rows<P, NT, NTL, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c),
                                                               nsheets, box_cur, box_prev, (const FT *)Fcur,
-                                                              (const FT *)Fprev, nullptr, sel, rect_buf);
+                                                              (const FT *)Fprev, nullptr, band);
 }
 
 template <int P, bool NT>
@@ -1191,20 +1223,35 @@ static void launch_any(int v, const Geometry &g, const void *fin, void *fout, in
     if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   }
   launch_rows<P, NT, false>(g.ny * g.nz, g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, nullptr,
-                            kRowsAll, nullptr, s);
+                            nullptr, s);
 }
 
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
                           hipStream_t s, int sel, int *rect_buf, int band_blocks) {
-  if (sel != kRowsAll && !rowtab) return LBMIB_ERR_ARG;
+  if (sel != kRowsAll && (!rowtab || !rect_buf)) return LBMIB_ERR_ARG;
   // nontemporal stores and loads by default: the populations are touched
   // once per step, and keeping them out of the caches was fastest measured
   const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
   const int ntl = kv.nt_mask < 0 ? 1 : ((kv.nt_mask >> 1) & 1);
   // automatic shape: x pairs for the 16-bit forms (4-byte lanes), one cell
   // per lane otherwise -- the fastest measured at 256^3
+  if (sel == kRowsBand) {
+    // the band of a split step, cell by cell (band_blocks blocks of 256)
+    LBMIB_DISPATCH(prec, ({
+      using S = typename Traits<P>::S;
+      using FT = typename Traits<P>::FT;
+      const dim3 gr((unsigned)(band_blocks > 0 ? band_blocks : 1));
+      if (nt && ntl) k_collide_band<P, true, true><<<gr, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rect_buf);
+      else if (nt) k_collide_band<P, true, false><<<gr, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rect_buf);
+      else if (ntl) k_collide_band<P, false, true><<<gr, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rect_buf);
+      else k_collide_band<P, false, false><<<gr, 256, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, kconsts<P>(c), nsheets, box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rect_buf);
+    }));
+    return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+  }
+  // the bulk of a split step skips the band's cells
+  const int *band = sel == kRowsBulk ? rect_buf : nullptr;
   int cpt = kv.cells_per_thread;
   if (cpt <= 0) cpt = ((prec == LBMIB_F16 || prec == LBMIB_BF16) && rowtab && pair_path_ok(g)) ? 2 : 1;
   if (cpt == 3 && sel != kRowsAll) cpt = 1;  // split steps use the row and pair kernels
@@ -1219,23 +1266,23 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
   if (cpt == 2 && rowtab && pair_path_ok(g)) {
-    const int nb = sel == kRowsBand ? band_blocks : (kv.blocks > 0 ? kv.blocks : g.ny * g.nz);
+    const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
-      else if (nt) launch_pairs<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
-      else if (ntl) launch_pairs<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
-      else launch_pairs<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      else if (nt) launch_pairs<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      else if (ntl) launch_pairs<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      else launch_pairs<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
   const int v = sel != kRowsAll ? 1 : collide_vec_width(prec, g.nx, cpt);
   if (v == 1) {
-    const int nb = sel == kRowsBand ? band_blocks : (kv.blocks > 0 ? kv.blocks : g.ny * g.nz);
+    const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
-      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
-      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
-      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, sel, rect_buf, s);
+      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }

@@ -186,7 +186,7 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
                           hipStream_t s, int sel = 0, int *rect_buf = nullptr, int band_blocks = 0);
-constexpr int kRectBufInts = 1 + 4 * kMaxSheets;
+constexpr int kRectBufInts = 1 + 6 * kMaxSheets;
 bool row_path_buffered(const Geometry &g, size_t es);
 bool pair_path_ok(const Geometry &g);
 bool rows2_path_ok(const Geometry &g);
