@@ -66,7 +66,7 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->evs) (void)hipEventDestroy(e);
   if (ctx->stream_b) (void)hipStreamSynchronize(ctx->stream_b);
-  for (hipEvent_t e : {ctx->evIB, ctx->evBand, ctx->evJoin})
+  for (hipEvent_t e : {ctx->evFib, ctx->evBulk, ctx->evJoin})
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream_b) (void)hipStreamDestroy(ctx->stream_b);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
@@ -159,7 +159,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   };
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
   if (!rc && hipStreamCreateWithFlags(&ctx->stream_b, hipStreamNonBlocking) != hipSuccess) rc = LBMIB_ERR_HIP;
-  for (hipEvent_t *e : {&ctx->evIB, &ctx->evBand, &ctx->evJoin})
+  for (hipEvent_t *e : {&ctx->evFib, &ctx->evBulk, &ctx->evJoin})
     if (!rc && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = LBMIB_ERR_HIP;
   {
     int dev = 0;
@@ -176,8 +176,9 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   alloc((void **)&ctx->d_tot, sizeof(double) * 4);
   alloc((void **)&ctx->ib.box, sizeof(int) * 2 * kMaxSheets * 6);
   alloc(&ctx->d_tab, sizeof(double) * 19);
-  alloc((void **)&ctx->d_rect, sizeof(int) * kRectBufInts);
-  if (!rc && hipMemset(ctx->d_rect, 0, sizeof(int) * kRectBufInts) != hipSuccess) rc = LBMIB_ERR_HIP;
+  alloc((void **)&ctx->d_rect, sizeof(int) * 2 * kRectBufInts);
+  if (!rc && hipMemset(ctx->d_rect, 0, sizeof(int) * 2 * kRectBufInts) != hipSuccess) rc = LBMIB_ERR_HIP;
+  ctx->ib.band = ctx->d_rect;
   alloc((void **)&ctx->d_ref, sizeof(double) * 19);
   if (!rc) {
     rc = upload_fluid_tables(p.precision, c, ctx->d_tab, ctx->d_ref);
@@ -444,19 +445,22 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
 // ------------------------------------------------------------ split steps
 //
 // With sheets present, a split step updates the band -- the cells of the
-// sheets' support boxes of this and the previous step, widened by one cell --
-// first, then every other cell (lbmib_fluid.hip, "split steps"):
-//   stream   : wait IB(t) | band(t) | bulk(t)
-//   stream_b : wait band(t) | band velocity + move(t), fiber forces and
-//              spreading of t+1 -> IB(t+1)
-// The band velocity reads the new populations only on support cells and
-// their neighbours, which the band holds, so the IB chain of the next step
-// runs while the bulk is updated.  Results are bit-identical to unsplit
-// steps: every cell does the same arithmetic.
+// sheets' support boxes of this and the previous step, widened by one cell,
+// whose boxes the step's fiber kernel stores -- and the bulk, every other
+// cell, in two launches that read the same populations and write disjoint
+// cells (lbmib_fluid.hip, "split steps"):
+//   stream   : wait fiber(t) | bulk(t)
+//   stream_b : fiber(t), spreading(t) | wait bulk(t-1) | band(t) | band
+//              velocity + move(t) | fiber(t+1) | spreading(t+1)
+// The bulk needs no force, only the band boxes and the previous step's
+// populations (the previous band precedes fiber(t) on stream_b); the band
+// velocity reads the new populations only on support cells and their
+// neighbours, which the band holds.  So the main stream runs bulk after bulk
+// while the band and the whole IB chain run beside it.  Results are
+// bit-identical to unsplit steps: every cell does the same arithmetic.
 
 // Automatic mode splits whenever there are sheets and the buffer-addressed
-// row kernels apply (measured at 256^3 with a 104x104 sheet: 4% faster in
-// FP32, 2% with 16-bit storage, than unsplit steps).
+// row kernels apply.
 static bool split_enabled(const lbmib_ctx *ctx) {
   if (ctx->nnodes == 0 || !ctx->d_rowtab) return false;
   return ctx->overlap_req != 0;
@@ -475,10 +479,13 @@ static int band_blocks(const lbmib_ctx *ctx) {
 static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, bool more) {
   const int s = slot_now(ctx), prec = ctx->fp.precision;
   const int *box = ctx->ib.box;
+  int *band = ctx->d_rect + s * kRectBufInts;  // written by fiber(t)
   if (ctx->split_run == 0) {
-    // stream_b starts after everything already queued on the main stream
+    // stream_b starts after everything already queued on the main stream,
+    // which also stands for the previous step's bulk
     HIPCHK(hipEventRecord(ctx->evJoin, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evJoin, 0));
+    HIPCHK(hipEventRecord(ctx->evBulk, ctx->stream));
   }
   if (!ctx->ib_ready) {
     if (!ctx->ib_clean) {
@@ -486,34 +493,36 @@ static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, bool more) {
       ctx->ib_clean = 1;
     }
     RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, ctx->stream_b));
+    HIPCHK(hipEventRecord(ctx->evFib, ctx->stream_b));
     RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream_b));
-    HIPCHK(hipEventRecord(ctx->evIB, ctx->stream_b));
   }
   ctx->spread_slot = s;
-  // main stream: the band rows after this step's spreading, then the bulk
-  // rows (no force there)
-  HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evIB, 0));
+  // main stream: the bulk, after fiber(t) (band boxes; the previous band
+  // precedes it on stream_b); no sheets: its cells carry no force
+  HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evFib, 0));
   if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
-  RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
-                             box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
-                             ctx->d_rowtab, ctx->kv, ctx->stream, 1, ctx->d_rect, band_blocks(ctx)));
-  HIPCHK(hipEventRecord(ctx->evBand, ctx->stream));
   RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, 0,
                              box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
-                             ctx->d_rowtab, ctx->kv, ctx->stream, 2, ctx->d_rect, 0));
+                             ctx->d_rowtab, ctx->kv, ctx->stream, 2, band, 0));
   if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
-  // second stream: band velocity and move, then the next step's fiber
-  // forces and spreading (slot 1 - s, reset by the move)
-  HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evBand, 0));
+  // second stream: the band after this step's spreading (stream order) and
+  // the previous bulk, which read the buffer the band writes
+  HIPCHK(hipStreamWaitEvent(ctx->stream_b, ctx->evBulk, 0));
+  HIPCHK(hipEventRecord(ctx->evBulk, ctx->stream));
+  RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
+                             box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
+                             ctx->d_rowtab, ctx->kv, ctx->stream_b, 1, band, band_blocks(ctx)));
+  // band velocity and move, then the next step's fiber forces and spreading
+  // (slot 1 - s, reset by the move)
   RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream_b));
   ctx->ib_clean = 1;
   ctx->ib_ready = 0;
   if (more) {
     RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, 1 - s, ctx->stream_b));
+    HIPCHK(hipEventRecord(ctx->evFib, ctx->stream_b));
     RCHK(launch_gather(prec, ctx->ib, ctx->g, 1 - s, ctx->F[1 - s], ctx->stream_b));
     ctx->ib_ready = 1;
   }
-  HIPCHK(hipEventRecord(ctx->evIB, ctx->stream_b));
   ctx->cur = 1 - ctx->cur;
   ctx->own = 0;
   ctx->pending = 0;

@@ -49,10 +49,10 @@ struct lbmib_ctx {
   // `stream_b`, overlapping the bulk
   int overlap_req = -1;           // -1 automatic (on with sheets), 0 off, 1 on
   hipStream_t stream_b = nullptr;
-  hipEvent_t evIB = nullptr, evBand = nullptr, evJoin = nullptr;
+  hipEvent_t evFib = nullptr, evBulk = nullptr, evJoin = nullptr;  // split steps
   int split_run = 0;              // steps taken in the current run of split steps
   int ib_ready = 0;               // fiber forces and spreading of the current step done (stream_b)
-  int *d_rect = nullptr;          // band rectangles of the last band update
+  int *d_rect = nullptr;          // [2][kRectBufInts] band boxes of split steps by step parity (IBState::band)
   double hyz[kMaxSheets][4] = {};      // host y/z extent of each sheet's nodes (from the uploaded positions)
   std::vector<hipEvent_t> evs;
 };

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 #include "lbmib_internal.h"
@@ -164,6 +165,36 @@ __device__ __forceinline__ bool any_box_overlaps(const int *boxes, int nsheets, 
           z <= ldc(b + 5);
   }
   return in;
+}
+
+// Band box of a split step for one sheet (see the split steps in
+// lbmib_fluid.hip): the union of its support boxes b1 (this step) and b0
+// (the previous step), widened by one cell on each side, in whole x pairs;
+// on a periodic axis a box touching an end takes the whole axis; clamped to
+// the grid.  False when neither box is valid.
+__device__ __forceinline__ bool band_box_of(const Geometry &g, const int b1[6], const int b0[6], int r[6]) {
+  const bool v1 = b1[0] <= b1[1], v0 = b0[0] <= b0[1];
+  if (!v1 && !v0) return false;
+  const int dims[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int lo = INT_MAX, hi = INT_MIN;
+    if (v1) { lo = min(lo, b1[2 * a]); hi = max(hi, b1[2 * a + 1]); }
+    if (v0) { lo = min(lo, b0[2 * a]); hi = max(hi, b0[2 * a + 1]); }
+    lo -= 1;
+    hi += 1;
+    // whole x pairs (even x0, odd x1), so that no lane of the two-cell row
+    // kernel straddles the band
+    if (a == 0) { lo -= lo & 1; hi += 1 - (hi & 1); }
+    if (g.bc[a] == LBMIB_BC_PERIODIC && (lo < 0 || hi >= dims[a])) { lo = 0; hi = dims[a] - 1; }
+    r[2 * a] = max(lo, 0);
+    r[2 * a + 1] = min(hi, dims[a] - 1);
+  }
+  return r[0] <= r[1] && r[2] <= r[3] && r[4] <= r[5];
+}
+
+__device__ __forceinline__ bool in_box3(const int *r, int x, int y, int z) {
+  return x >= r[0] && x <= r[1] && y >= r[2] && y <= r[3] && z >= r[4] && z <= r[5];
 }
 
 __device__ __forceinline__ bool in_any_box(const int *boxes, int nsheets, int x, int y, int z) {

@@ -287,49 +287,21 @@ int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
 
 // ---------------------------------------------------------- split steps
 //
-// A split step (lbmib_set_overlap) first updates the band -- per sheet, the
-// box spanned by the support boxes of this and the previous step (all cells
-// with a force in either), widened by one cell on each side for the pulls of
-// the band velocity -- and then the bulk, every other cell, so that the next
-// step's IB chain, which needs only the band's new populations, overlaps the
-// bulk.  The band is a few thousand cells per sheet; k_collide_band updates
-// them cell by cell by the general rules, and its first thread stores the
-// boxes in rect_buf ({n, (x0, x1, y0, y1, z0, z1) x n}) for the bulk launch,
-// which skips those cells and runs while the next step's IB chain rewrites the
-// support-box slots and the force field (the bulk reads neither: its cells
-// have no force).  A support box moves by less than a cell per step, so the
-// next step's box lies inside this step's band box.
+// A split step (lbmib_set_overlap) updates the band -- per sheet, the box
+// spanned by the support boxes of this and the previous step (all cells with
+// a force in either), widened by one cell on each side for the pulls of the
+// band velocity (band_box_of) -- and the bulk, every other cell, in two
+// launches that read the same populations and write disjoint cells, so they
+// may run side by side.  The step's fiber kernel stores the band boxes
+// ({n, (x0, x1, y0, y1, z0, z1) x n}, one buffer per step parity);
+// k_collide_band updates the band's cells one by one by the general rules,
+// and the bulk row kernels skip them.  The bulk reads neither the force
+// field nor the support boxes (its cells have no force), so the IB chain may
+// rewrite them while it runs.  A support box moves by less than a cell per
+// step, so the next step's box lies inside this step's band box.
 enum { kRowsAll = 0, kRowsBand = 1, kRowsBulk = 2 };
 
-__device__ __forceinline__ bool band_box3(const Geometry &g, const int *box_cur, const int *box_prev, int s,
-                                          int r[6]) {
-  const int *b1 = box_cur + 6 * s, *b0 = box_prev + 6 * s;
-  const bool v1 = ldc(b1) <= ldc(b1 + 1), v0 = ldc(b0) <= ldc(b0 + 1);
-  if (!v1 && !v0) return false;
-  const int dims[3] = {g.nx, g.ny, g.nz};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int lo = INT_MAX, hi = INT_MIN;
-    if (v1) { lo = min(lo, ldc(b1 + 2 * a)); hi = max(hi, ldc(b1 + 2 * a + 1)); }
-    if (v0) { lo = min(lo, ldc(b0 + 2 * a)); hi = max(hi, ldc(b0 + 2 * a + 1)); }
-    lo -= 1;
-    hi += 1;
-    // whole x pairs (even x0, odd x1), so that no lane of the two-cell row
-    // kernel straddles the band
-    if (a == 0) { lo -= lo & 1; hi += 1 - (hi & 1); }
-    // on a periodic axis a box touching an end takes the whole axis
-    if (g.bc[a] == LBMIB_BC_PERIODIC && (lo < 0 || hi >= dims[a])) { lo = 0; hi = dims[a] - 1; }
-    r[2 * a] = max(lo, 0);
-    r[2 * a + 1] = min(hi, dims[a] - 1);
-  }
-  return r[0] <= r[1] && r[2] <= r[3] && r[4] <= r[5];
-}
-
-__device__ __forceinline__ bool in_box3(const int *r, int x, int y, int z) {
-  return x >= r[0] && x <= r[1] && y >= r[2] && y <= r[3] && z >= r[4] && z <= r[5];
-}
-
-// (rect_buf is written by the band launch and only read afterwards.)
+// (the band boxes are written by the step's fiber kernel, before both launches)
 __device__ __forceinline__ bool in_band_boxes(const int *rect_buf, int x, int y, int z) {
   bool in = false;
   const int n = min(ldc(rect_buf), kMaxSheets);
@@ -773,49 +745,40 @@ __global__ void __launch_bounds__(512)
                        (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab), row, x);
 }
 
-// The band of a split step: every cell of the band boxes by the general
-// rules, each once (a cell in several boxes goes with the first).  Threads
-// stride over the cells of all boxes; the first thread stores the boxes for
-// the bulk launch.
+// The band of a split step: every cell of the band boxes (written by the
+// step's fiber kernel) by the general rules, each once (a cell in several
+// boxes goes with the first).  Threads stride over the cells of all boxes.
 template <int P, bool NT, bool NTL>
 __global__ void __launch_bounds__(256)
     k_collide_band(Geometry g, const typename Traits<P>::S *__restrict__ fin,
                    typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c, int nsheets,
                    const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                    const typename Traits<P>::FT *__restrict__ Fcur, const typename Traits<P>::FT *__restrict__ Fprev,
-                   int *__restrict__ rect_buf) {
-  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid == 0) {
-    int n = 0;
-    for (int s = 0; s < nsheets; ++s) {
-      int r[6];
-      if (!band_box3(g, box_cur, box_prev, s, r)) continue;
-      for (int k = 0; k < 6; ++k) rect_buf[1 + 6 * n + k] = r[k];
-      ++n;
-    }
-    rect_buf[0] = n;
-  }
+                   const int *__restrict__ band) {
+  const int nb = min(ldc(band), kMaxSheets);
   long total = 0;
-  for (int s = 0; s < nsheets; ++s) {
-    int r[6];
-    if (band_box3(g, box_cur, box_prev, s, r)) total += (long)(r[1] - r[0] + 1) * (r[3] - r[2] + 1) * (r[5] - r[4] + 1);
+  for (int k = 0; k < nb; ++k) {
+    const int *r = band + 1 + 6 * k;
+    total += (long)(ldc(r + 1) - ldc(r) + 1) * (ldc(r + 3) - ldc(r + 2) + 1) * (ldc(r + 5) - ldc(r + 4) + 1);
   }
-  for (long it = tid; it < total; it += (long)gridDim.x * blockDim.x) {
+  for (long it = (long)blockIdx.x * blockDim.x + threadIdx.x; it < total; it += (long)gridDim.x * blockDim.x) {
     long rest = it;
-    int s = 0, r[6];
-    for (; s < nsheets; ++s) {
-      if (!band_box3(g, box_cur, box_prev, s, r)) continue;
+    int k = 0;
+    int r[6];
+    for (; k < nb; ++k) {
+      for (int a = 0; a < 6; ++a) r[a] = ldc(band + 1 + 6 * k + a);
       const long vol = (long)(r[1] - r[0] + 1) * (r[3] - r[2] + 1) * (r[5] - r[4] + 1);
       if (rest < vol) break;
       rest -= vol;
     }
-    if (s == nsheets) break;  // not reached: it < total
-    const int nx = r[1] - r[0] + 1, ny = r[3] - r[2] + 1;
-    const int x = r[0] + (int)(rest % nx), y = r[2] + (int)((rest / nx) % ny), z = r[4] + (int)(rest / ((long)nx * ny));
+    if (k == nb) break;  // not reached: it < total
+    const int w = r[1] - r[0] + 1, h = r[3] - r[2] + 1;
+    const int x = r[0] + (int)(rest % w), y = r[2] + (int)((rest / w) % h), z = r[4] + (int)(rest / ((long)w * h));
     bool dup = false;
-    for (int t = 0; t < s; ++t) {
+    for (int t = 0; t < k; ++t) {
       int q[6];
-      if (band_box3(g, box_cur, box_prev, t, q) && in_box3(q, x, y, z)) dup = true;
+      for (int a = 0; a < 6; ++a) q[a] = ldc(band + 1 + 6 * t + a);
+      dup |= in_box3(q, x, y, z);
     }
     if (!dup) update_cell<P, NT, NTL>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, x, y, z);
   }

@@ -235,6 +235,21 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
   }
   __syncthreads();
   for (int k = threadIdx.x; k < nk; k += blockDim.x) ib.box[slot * kMaxSheets * 6 + k] = sbox[k];
+  // the band boxes of this step for split steps: this step's boxes and the
+  // previous step's (the other slot, not written by this launch)
+  if (threadIdx.x == 0 && ib.band) {
+    int *rb = ib.band + slot * kRectBufInts;
+    const int *prev = ib.box + (1 - slot) * kMaxSheets * 6;
+    int n = 0;
+    for (int s = 0; s < ib.nsheets; ++s) {
+      int b1[6], b0[6], r[6];
+      for (int k = 0; k < 6; ++k) { b1[k] = sbox[s * 6 + k]; b0[k] = prev[s * 6 + k]; }
+      if (!band_box_of(g, b1, b0, r)) continue;
+      for (int k = 0; k < 6; ++k) rb[1 + 6 * n + k] = r[k];
+      ++n;
+    }
+    rb[0] = n;
+  }
   if (threadIdx.x == 0) atomicExch(ib.done, 0u);
 }
 
@@ -254,8 +269,8 @@ constexpr int kCandBins = kTileBins + 1;  // candidate bins per axis
 __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int &s, int &ty, int &tz) {
   for (s = 0; s < ib.nsheets; ++s) {
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
-    if (b[0] > b[1]) continue;  // empty (all nodes invalid)
-    const int ty0 = b[2] / kTile, ty1 = b[3] / kTile, tz0 = b[4] / kTile, tz1 = b[5] / kTile;
+    if (ldc(b) > ldc(b + 1)) continue;  // empty (all nodes invalid)
+    const int ty0 = ldc(b + 2) / kTile, ty1 = ldc(b + 3) / kTile, tz0 = ldc(b + 4) / kTile, tz1 = ldc(b + 5) / kTile;
     const int nt = (ty1 - ty0 + 1) * (tz1 - tz0 + 1);
     if (w < nt) {
       ty = ty0 + w / (tz1 - tz0 + 1);
@@ -280,30 +295,43 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
   int *raw = reinterpret_cast<int *>(se + kGatherStage * 3);
   int *ids = raw + mcap;
   int *sx = ids + mcap, *sy = sx + mcap, *sz = sy + mcap;
+  __shared__ int s_tot;
   const int M = ib.nnodes;
   const double cf = 1.0 / (64.0 * 1.0 * 1.0 * 1.0);  // lbm_ib.c:1357
   for (int w = bid;; w += nblk) {
     int s, ty, tz;
     if (!tile_of(ib, slot, w, s, ty, tz)) break;
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
+    const int x0 = b[0], x1 = b[1];
     // a node with support start j0 reaches rows j0..j0+3, so tile rows
     // kTile*ty .. kTile*ty+kTile-1 need j0 in [kTile*ty-3, kTile*ty+kTile-1]:
-    // bins kTileBins*ty-1 .. kTileBins*ty+kTileBins-1.
+    // bins kTileBins*ty-1 .. kTileBins*ty+kTileBins-1.  Of those, only the
+    // nodes whose x support [i0, i0+3] meets the box's x range reach a cell
+    // of the tile (the others, e.g. of another sheet in the same (y, z)
+    // bins, add nothing); they are gathered in any order and rank-sorted
+    // below, so each cell still sums its contributions in node order.
     const int by0 = kTileBins * ty - 1, bz0 = kTileBins * tz - 1;
-    int tot = 0;
+    if (threadIdx.x == 0) s_tot = 0;
+    __syncthreads();
+    int seen = 0;
     for (int k = 0; k < kCandBins * kCandBins; ++k) {
       const int by = by0 + k / kCandBins, bz = bz0 + k % kCandBins;
       if (by < 0 || bz < 0 || by >= ib.nby || bz >= ib.nbz) continue;
       const int bin = by * ib.nbz + bz;
-      int c = min(ib.bin_count[bin], ib.cap);
-      if (tot + c > mcap) {
+      int c = min(ldc(ib.bin_count + bin), ib.cap);
+      if (seen + c > mcap) {
         if (threadIdx.x == 0) atomicOr(ib.err, DERR_BIN_OVERFLOW);
-        c = mcap - tot;
+        c = mcap - seen;
       }
-      for (int e = threadIdx.x; e < c; e += blockDim.x) raw[tot + e] = ib.bin_list[(long)bin * ib.cap + e];
-      tot += c;
+      for (int e = threadIdx.x; e < c; e += blockDim.x) {
+        const int v = ib.bin_list[(long)bin * ib.cap + e];
+        const int i0 = ib.start[v];
+        if (i0 <= x1 && i0 + 3 >= x0) raw[atomicAdd(&s_tot, 1)] = v;
+      }
+      seen += c;
     }
     __syncthreads();
+    const int tot = s_tot;
     // rank sort (node ids are distinct): ascending (sheet, row, col) order
     const bool staged = tot <= kGatherStage;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -320,7 +348,6 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
       }
     }
     __syncthreads();
-    const int x0 = b[0], x1 = b[1];
     const int y0 = max(kTile * ty, b[2]), y1 = min(kTile * ty + kTile - 1, b[3]);
     const int z0 = max(kTile * tz, b[4]), z1 = min(kTile * tz + kTile - 1, b[5]);
     const int nxs = x1 - x0 + 1, nys = y1 - y0 + 1, nzs = z1 - z0 + 1;

@@ -118,6 +118,7 @@ struct IBState {
   int *err;
   int quirks;
   double *ub;        // [3][n] fluid velocity on the support boxes (k_band_u -> k_move)
+  int *band;         // [2][kRectBufInts] band boxes of split steps by step parity (written by k_fiber)
 };
 
 // host-side launchers (defined in lbmib_fluid.hip / lbmib_ib.hip)
