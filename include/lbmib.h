@@ -119,15 +119,17 @@ int lbmib_replace_old_DF(lbmib_ctx *ctx);             /* lbm_ib.c:2909-2968: buf
 /* ---- whole steps: the do_thread loop body, lbm_ib.c:3115-3563 ---- */
 int lbmib_step(lbmib_ctx *ctx, int nsteps);
 /* Split steps for lbmib_step / lbmib_time_steps (not in the reference): with
- * sheets present, each step first updates the rows (y, z) near the sheets'
- * support, then the remaining rows, while a second stream moves the sheets
- * and computes the next step's fiber forces and spreading -- the IB chain
- * overlaps the bulk of the fluid update.  Results are bit-identical to
- * unsplit steps.  mode: -1 automatic (the default: split when there are
- * sheets and the buffer-addressed row kernels apply), 0 never, 1 always. */
+ * sheets present, each step updates the cells near the sheets' support (the
+ * band) on a second stream, which then moves the sheets and computes the
+ * next step's fiber forces and spreading, while the main stream updates all
+ * other cells -- the IB chain overlaps the bulk of the fluid update.
+ * Results are bit-identical to unsplit steps.  mode: -1 automatic (the
+ * default: split when there are sheets and the buffer-addressed row kernels
+ * apply), 0 never, 1 always. */
 int lbmib_set_overlap(lbmib_ctx *ctx, int mode);
-/* Cells updated by the kernels lbmib_time_steps times per step (all cells:
- * split steps time their band and bulk updates together). */
+/* Cells updated by the kernels lbmib_time_steps times per step (all cells;
+ * with split steps the timed launch is the bulk, all cells but the band's,
+ * under 1% of a large grid). */
 int lbmib_timed_kernel_cells(lbmib_ctx *ctx, double *cells);
 
 /* ---- queries (synchronising) ---- */
