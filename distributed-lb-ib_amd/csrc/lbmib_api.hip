@@ -70,7 +70,7 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream_b) (void)hipStreamDestroy(ctx->stream_b);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
-                  ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.box_part, ctx->ib.done, ctx->ib.ub, ctx->d_sheets,
+                  ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.done, ctx->ib.ub, ctx->d_sheets,
                   ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab, ctx->d_rect, ctx->d_ref};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -219,7 +219,6 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
     alloc((void **)&ib.bin_count, sizeof(int) * (size_t)ib.nby * ib.nbz);
     alloc((void **)&ib.bin_list, sizeof(int) * (size_t)ib.nby * ib.nbz * cap);
     alloc((void **)&ctx->d_sheets, sizeof(SheetDesc) * kMaxSheets);
-    alloc((void **)&ib.box_part, sizeof(int) * kMaxSheets * 6 * (size_t)fiber_blocks((int)M));
     alloc((void **)&ib.ub, sizeof(double) * 3 * n);
     alloc((void **)&ib.done, sizeof(unsigned));
   }

@@ -170,10 +170,10 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
     }
   }
   if (!(mask & FK_PREP)) return;  // uniform over the grid
-  // Support box of each sheet without contended atomics: wave min/max, LDS
-  // combine per block, one partial per block in global memory, and the last
-  // block to finish reduces the partials (min/max are order-independent, so
-  // the box is deterministic).
+  // Support box of each sheet: wave min/max, LDS combine per block, then one
+  // global atomic min/max per block and coordinate into the box slot (reset
+  // to empty by the previous move); min/max are order-independent, so the
+  // box is deterministic.  The last block to finish derives the band boxes.
   __shared__ int sbox[kMaxSheets * 6];
   __shared__ int s_last;
   for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) sbox[t] = (t & 1) ? INT_MIN : INT_MAX;
@@ -200,41 +200,24 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) ib.box_part[bid * kMaxSheets * 6 + t] = sbox[t];
+  int *box = ib.box + slot * kMaxSheets * 6;
+  const int nk = ib.nsheets * 6;
+  for (int t = threadIdx.x; t < nk; t += blockDim.x) {
+    if (t & 1) {
+      if (sbox[t] != INT_MIN) atomicMax(box + t, sbox[t]);
+    } else if (sbox[t] != INT_MAX) {
+      atomicMin(box + t, sbox[t]);
+    }
+  }
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == nblk - 1;
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  // min/max over every block's partial, all threads at once: eight
-  // independent loads per thread in flight, combined with LDS atomics
-  for (int t = threadIdx.x; t < kMaxSheets * 6; t += blockDim.x) sbox[t] = (t & 1) ? INT_MIN : INT_MAX;
+  for (int k = threadIdx.x; k < nk; k += blockDim.x)
+    sbox[k] = __hip_atomic_load(box + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  const int nk = ib.nsheets * 6;
-  const int items = (int)nblk * nk;
-  for (int base = 0; base < items; base += 8 * blockDim.x) {
-    int v[8], kk[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int it = base + u * blockDim.x + threadIdx.x;
-      kk[u] = -1;
-      v[u] = 0;
-      if (it < items) {
-        const int b = it / nk, k = it % nk;
-        kk[u] = k;
-        v[u] = __hip_atomic_load(ib.box_part + b * kMaxSheets * 6 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (kk[u] < 0) continue;
-      if (kk[u] & 1) atomicMax(&sbox[kk[u]], v[u]);
-      else atomicMin(&sbox[kk[u]], v[u]);
-    }
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < nk; k += blockDim.x) ib.box[slot * kMaxSheets * 6 + k] = sbox[k];
   // the band boxes of this step for split steps: this step's boxes and the
   // previous step's (the other slot, not written by this launch)
   if (threadIdx.x == 0 && ib.band) {

@@ -113,7 +113,6 @@ struct IBState {
   int *bin_list;     // [nbins][cap]
   int nby, nbz, cap;
   int *box;          // [2][kMaxSheets][6]
-  int *box_part;     // [fiber blocks][kMaxSheets][6] per-block partial boxes
   unsigned *done;    // blocks of k_fiber finished (reset by the last one)
   int *err;
   int quirks;
@@ -123,8 +122,8 @@ struct IBState {
 
 // host-side launchers (defined in lbmib_fluid.hip / lbmib_ib.hip)
 
-// k_fiber: one thread per node, kFiberThreads per block; each block leaves one
-// partial support box per sheet in IBState::box_part.
+// k_fiber: one thread per node, kFiberThreads per block; each block folds its
+// nodes' support boxes into the box slot with atomic min/max.
 constexpr int kFiberThreads = 64;
 __host__ __device__ inline int fiber_blocks(int nnodes) { return (nnodes + kFiberThreads - 1) / kFiberThreads; }
 // Grids of the other IB kernels for a sheet set of nnodes nodes (the kernels
