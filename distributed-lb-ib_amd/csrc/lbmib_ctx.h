@@ -44,9 +44,9 @@ struct lbmib_ctx {
   int latched = LBMIB_OK;
   KernelVariant kv = {0, -1, 0};  // fused-kernel launch shape
   int ncu = 0;                    // compute units of the device
-  // split steps (lbmib_set_overlap): band rows, then bulk rows on `stream`;
-  // band velocity, move and the next step's fiber forces and spreading on
-  // `stream_b`, overlapping the bulk
+  // split steps (lbmib_set_overlap): the bulk on `stream`; the band, band
+  // velocity, move and the next step's fiber forces and spreading on
+  // `stream_b`, beside the bulk
   int overlap_req = -1;           // -1 automatic (on with sheets), 0 off, 1 on
   hipStream_t stream_b = nullptr;
   hipEvent_t evFib = nullptr, evBulk = nullptr, evJoin = nullptr;  // split steps

@@ -179,9 +179,10 @@ struct KernelVariant {
 };
 
 // rowtab: the row source table (row_source_table) on the device, or null for
-// the general path.  sel: which rows to update -- 0 all, 1 the band of a
-// split step (grid of band_blocks blocks; stores the band rectangles in
-// rect_buf), 2 the rest (reads rect_buf); 1 and 2 need rowtab.
+// the general path.  sel: which cells to update -- 0 all, 1 the band of a
+// split step (the cells of the band boxes in rect_buf, which the step's
+// fiber kernel wrote; a grid of band_blocks blocks of 256), 2 all others;
+// 1 and 2 need rowtab and rect_buf.
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
