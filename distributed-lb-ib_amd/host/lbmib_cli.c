@@ -9,7 +9,9 @@
  * physics the reference hard-codes:
  *   -tau T -u_in U -ks K -kb K -bc X Y Z (INOUT|WALL|PERIODIC)
  *   -precision f64|f32|f16|bf16  -jitter A -seed S  -no_quirks  -shift_all
- *   -dump_fiber -dump_fluid  -quiet
+ *   -dump_fiber -dump_fluid -dump_fv  -quiet
+ * (the reference's dumps are compile-time switches: DUMP_VERIFY and USE_FV,
+ * lbm_ib.c:30, :3504-3538).
  * Prints "TOTAL TIME TAKEN IN Seconds:%f" like the reference, then MLUPS.
  */
 #define _POSIX_C_SOURCE 200809L
@@ -123,6 +125,7 @@ int main(int argc, char *argv[]) {
     else if (!strcmp(a, "-shift_all")) { gv->shift_all_sheets = 1; }
     else if (!strcmp(a, "-dump_fiber")) dumps |= 1;
     else if (!strcmp(a, "-dump_fluid")) dumps |= 2;
+    else if (!strcmp(a, "-dump_fv")) dumps |= 4;
     else if (!strcmp(a, "-quiet")) gv->quiet = 1;
     else { fprintf(stderr, "error: unknown flag \"%s\"\n", a); return 2; }
   }

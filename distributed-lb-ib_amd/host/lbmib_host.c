@@ -379,12 +379,158 @@ void save_fluid_sub_grid(GV gv, int start_x, int start_y, int start_z, int end_x
   fclose(f);
 }
 
+/* Host mirror value helpers: the fluid arrays are x-fastest planes. */
+static size_t mirror_cell(const Fluidgrid *g, int X, int Y, int Z) {
+  return (size_t)X + (size_t)g->x_dim * ((size_t)Y + (size_t)g->y_dim * Z);
+}
+
+/* Whole cubes BI..BI_end x BJ..BJ_end x BK..BK_end (closed), every node of
+ * each with its cube-local coordinates and 19 populations, to stdout, in the
+ * layout of print_fluid_sub_grid / print_fluid_cube (lbm_ib.c:234-330;
+ * `tag` is the per-node line of each). */
+static void print_cubes(GV gv, int BI0, int BJ0, int BK0, int BI1, int BJ1, int BK1, int cs, const char *tag) {
+  Fluidgrid *g = gv->fluid_grid;
+  const size_t n = (size_t)g->x_dim * g->y_dim * g->z_dim;
+  if (cs <= 0) cs = 1;
+  printf("(BI,BJ,BK): {vel_x, vel_y, vel_z} || {G0, DF1, DF2}|| rho || {ElasticF_x, y, z}\n");
+  for (int BI = BI0; BI <= BI1; ++BI)
+    for (int BJ = BJ0; BJ <= BJ1; ++BJ)
+      for (int BK = BK0; BK <= BK1; ++BK)
+        for (int li = 0; li < cs; li++)
+          for (int lj = 0; lj < cs; lj++)
+            for (int lk = 0; lk < cs; lk++) {
+              printf(tag, BI, BJ, BK);
+              const int X = BI * cs + li, Y = BJ * cs + lj, Z = BK * cs + lk;
+              if (X >= g->x_dim || Y >= g->y_dim || Z >= g->z_dim) continue;
+              const size_t c = mirror_cell(g, X, Y, Z);
+              for (int q = 0; q < 19; ++q)
+                printf("(%d, %d, %d, %d):{%.12f, %.12f, %.12f} || {%.12f, %.12f, %.12f} || %.12f || {%.12f, %.12f, %.12f}\n",
+                       li, lj, lk, q, g->vel[c], g->vel[n + c], g->vel[2 * n + c], g->dfeq ? g->dfeq[q * n + c] : 0.0,
+                       g->df[q * n + c], g->df[q * n + c], g->rho[c], g->force[c], g->force[n + c],
+                       g->force[2 * n + c]);
+            }
+  printf("\n");
+}
+
+void print_fluid_sub_grid(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y, int end_z,
+                          int cube_size) {
+  const int cs = cube_size > 0 ? cube_size : 1;
+  print_cubes(gv, start_x / cs, start_y / cs, start_z / cs, end_x / cs, end_y / cs, end_z / cs, cs,
+              "For cube <%d, %d, %d>\n");
+}
+
+void print_fluid_cube(GV gv, int BI_start, int BJ_start, int BK_start, int BI_end, int BJ_end, int BK_end,
+                      int cube_size) {
+  print_cubes(gv, BI_start, BJ_start, BK_start, BI_end, BJ_end, BK_end, cube_size, "FOr cube <%d,%d,%d>\n");
+}
+
+void save_fluid_sub_grid_Vel(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y, int end_z,
+                             char fName[]) {
+  /* "u,v,w" lines over [start, end) with X slowest and Z fastest,
+   * lbm_ib.c:395-432 (there the cube counts are left unset, :400; here the
+   * mirror is indexed directly) */
+  FILE *f = fopen(fName, "w");
+  if (!f) return;
+  Fluidgrid *g = gv->fluid_grid;
+  const size_t n = (size_t)g->x_dim * g->y_dim * g->z_dim;
+  for (int X = start_x; X < end_x; X++)
+    for (int Y = start_y; Y < end_y; Y++)
+      for (int Z = start_z; Z < end_z; Z++) {
+        const size_t c = mirror_cell(g, X, Y, Z);
+        fprintf(f, "%f,%f,%f\n", g->vel[c], g->vel[n + c], g->vel[2 * n + c]);
+      }
+  fclose(f);
+}
+
+void save_fluid_sub_grid_vel_use_fv(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y,
+                                    int end_z, char fName[]) {
+  /* FieldView function file, lbm_ib.c:436-507: "nx,ny,nz,3", then all u,
+   * all v, all w over [start, end) with X fastest, each value "%f," */
+  FILE *f = fopen(fName, "w");
+  if (!f) return;
+  Fluidgrid *g = gv->fluid_grid;
+  const size_t n = (size_t)g->x_dim * g->y_dim * g->z_dim;
+  fprintf(f, "%d,%d,%d,3\n", end_x - start_x, end_y - start_y, end_z - start_z);
+  for (int comp = 0; comp < 3; ++comp)
+    for (int Z = start_z; Z < end_z; Z++)
+      for (int Y = start_y; Y < end_y; Y++)
+        for (int X = start_x; X < end_x; X++) fprintf(f, "%f,", g->vel[comp * n + mirror_cell(g, X, Y, Z)]);
+  fclose(f);
+}
+
+void save_fluid_sub_grid_cord_use_fv(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y,
+                                     int end_z, char fName[]) {
+  /* FieldView grid file, lbm_ib.c:510-548: "nx,ny,nz", then the X, Y and Z
+   * coordinates of every node over [start, end) with X fastest */
+  (void)gv;
+  FILE *f = fopen(fName, "wb");
+  if (!f) return;
+  fprintf(f, "%d,%d,%d\n", end_x - start_x, end_y - start_y, end_z - start_z);
+  for (int axis = 0; axis < 3; ++axis)
+    for (int Z = start_z; Z < end_z; Z++)
+      for (int Y = start_y; Y < end_y; Y++)
+        for (int X = start_x; X < end_x; X++) fprintf(f, "%d,", axis == 0 ? X : (axis == 1 ? Y : Z));
+  fclose(f);
+}
+
+void print_fiber_sub_grid(GV gv, int start_y, int start_z, int end_y, int end_z) {
+  /* sheet 0, rows start_y..end_y and columns start_z..end_z (closed), to
+   * stdout, lbm_ib.c:551-572 */
+  Fiber *fa = gv->fiber_shape->sheets[0].fibers;
+  printf("(i,j): {cord_x, cord_y, cord_z} || SF_x,SF_y, SF_z || BF_X, BF_y, BF_z\n");
+  for (int i = start_y; i <= end_y; ++i) {
+    for (int j = start_z; j <= end_z; ++j) {
+      Fibernode *nd = fa[i].nodes + j;
+      printf("(%d, %d):{%f, %f, %f} || %.12f, %.12f, %.12f || %.12f, %.12f, %.12f\n", i, j, nd->x, nd->y, nd->z,
+             nd->stretch_force_x, nd->stretch_force_y, nd->stretch_force_z, nd->bend_force_x, nd->bend_force_y,
+             nd->bend_force_z);
+    }
+    printf("\n");
+  }
+}
+
+void save_fiber_sub_grid_cord_ef_use_fv(GV gv, int start_y, int start_z, int end_y, int end_z, char grid_fname[],
+                                        char elastic_force_fname[]) {
+  /* FieldView grid ("rows,cols,1") and function ("rows,cols,1,3") files of
+   * sheet 0: all x then y then z (resp. elastic force components) with the
+   * row index fastest, lbm_ib.c:650-692 */
+  FILE *gf = fopen(grid_fname, "w");
+  FILE *ef = fopen(elastic_force_fname, "w");
+  if (!gf || !ef) {
+    if (gf) fclose(gf);
+    if (ef) fclose(ef);
+    return;
+  }
+  Fiber *fa = gv->fiber_shape->sheets[0].fibers;
+  fprintf(gf, "%d,%d,1\n", end_y - start_y + 1, end_z - start_z + 1);
+  fprintf(ef, "%d,%d,1,3\n", end_y - start_y + 1, end_z - start_z + 1);
+  for (int comp = 0; comp < 3; ++comp)
+    for (int j = start_z; j <= end_z; ++j)
+      for (int i = start_y; i <= end_y; ++i) {
+        Fibernode *nd = fa[i].nodes + j;
+        fprintf(gf, "%f,", comp == 0 ? nd->x : (comp == 1 ? nd->y : nd->z));
+        fprintf(ef, "%f,", comp == 0 ? nd->elastic_force_x : (comp == 1 ? nd->elastic_force_y : nd->elastic_force_z));
+      }
+  fclose(gf);
+  fclose(ef);
+}
+
 /* --------------------------------------------------------------- the loop */
 
 void *do_thread(void *v) {
   LV lv = (LV)v;
   GV gv = lv->gv;
-  char fname[128];
+  char fname[128], fname1[128];
+  if ((gv->dump_mask & 4) && lv->tid == 0) {
+    /* FieldView variable names and fluid grid, written once (lbm_ib.c:3100-3112) */
+    FILE *nf = fopen("FV_nameFluid.nam", "w");
+    if (nf) { fprintf(nf, "u\nv\nw"); fclose(nf); }
+    nf = fopen("FV_nameFiber.nam", "w");
+    if (nf) { fprintf(nf, "Afx\nAfy\nAfz"); fclose(nf); }
+    Fluidgrid *g = gv->fluid_grid;
+    snprintf(fname, sizeof fname, "FV_gridFluid.xyz");
+    save_fluid_sub_grid_cord_use_fv(gv, 0, 0, 0, g->x_dim, g->y_dim, g->z_dim, fname);
+  }
   while (gv->time <= gv->TIME_STOP) {
     const int dumping = gv->dump_mask && gv->dump > 0 && ((gv->time % gv->dump == 0) || gv->time == 1);
     compute_bendingforce(lv);
@@ -401,7 +547,7 @@ void *do_thread(void *v) {
     replace_old_DF(lv);
     /* periodicBC(lv) is not called by the reference either (lbm_ib.c:3484) */
     if (dumping && lv->tid == 0) {
-      lbmib_sync_to_host(gv, (gv->dump_mask & 2) != 0);
+      lbmib_sync_to_host(gv, (gv->dump_mask & 6) != 0);
       if (gv->dump_mask & 2) {
         snprintf(fname, sizeof fname, "Fluid%d_dump_step%d.dat", 0, gv->time);
         save_fluid_sub_grid(gv, 19, 20, 10, 22, 25, 33, fname);  /* the box of lbm_ib.c:3511 */
@@ -410,6 +556,15 @@ void *do_thread(void *v) {
         snprintf(fname, sizeof fname, "Fiber%d_dump_step%d.bin", 1, gv->time);
         Fibersheet *sh = gv->fiber_shape->sheets;
         save_fiber_sub_grid_binary(gv, 0, 0, sh->num_rows - 1, sh->num_cols - 1, fname);
+      }
+      if (gv->dump_mask & 4) {  /* FieldView, lbm_ib.c:3514-3517, :3531-3535 */
+        Fluidgrid *g = gv->fluid_grid;
+        snprintf(fname, sizeof fname, "FV_fluid_vel_%d.f", gv->time);
+        save_fluid_sub_grid_vel_use_fv(gv, 0, 0, 0, g->x_dim, g->y_dim, g->z_dim, fname);
+        Fibersheet *sh = gv->fiber_shape->sheets;
+        snprintf(fname, sizeof fname, "FV_gridFiber%d.xyz", gv->time);
+        snprintf(fname1, sizeof fname1, "FV_fiber_elastic_force_%d.f", gv->time);
+        save_fiber_sub_grid_cord_ef_use_fv(gv, 0, 0, sh->num_rows - 1, sh->num_cols - 1, fname, fname1);
       }
     }
     if (lv->tid == 0 && !gv->quiet) {

@@ -85,7 +85,7 @@ typedef struct gv_t {
   int shift_all_sheets;
   int last_error;
   int quiet;
-  int dump_mask;      /* do_thread dumps: 1 binary fiber, 2 fluid cubes */
+  int dump_mask;      /* do_thread dumps: 1 binary fiber, 2 fluid cubes, 4 FieldView */
 } *GV;
 
 /* LV, lbm_ib.c:164-167 */
@@ -136,18 +136,33 @@ void replace_old_DF(LV lv);
  * every gv->dump steps when dumps are enabled) */
 void *do_thread(void *v);
 
-/* dump writers, lbm_ib.c:332-392, :560-646 */
+/* dump writers, lbm_ib.c:234-692 (fluid values from the host mirror of
+ * lbmib_sync_to_host(gv, 1)) */
+void print_fluid_sub_grid(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y, int end_z,
+                          int cube_size);                                  /* lbm_ib.c:234-289 */
+void print_fluid_cube(GV gv, int BI_start, int BJ_start, int BK_start, int BI_end, int BJ_end, int BK_end,
+                      int cube_size);                                      /* lbm_ib.c:292-329 */
+void save_fluid_sub_grid(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y, int end_z,
+                         char fName[]);                                    /* lbm_ib.c:332-392 */
+void save_fluid_sub_grid_Vel(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y, int end_z,
+                             char fName[]);                                /* lbm_ib.c:395-432 */
+void save_fluid_sub_grid_vel_use_fv(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y,
+                                    int end_z, char fName[]);              /* lbm_ib.c:436-507 */
+void save_fluid_sub_grid_cord_use_fv(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y,
+                                     int end_z, char fName[]);             /* lbm_ib.c:510-548 */
+void print_fiber_sub_grid(GV gv, int start_y, int start_z, int end_y, int end_z); /* lbm_ib.c:551-572 */
 void save_fiber_sub_grid(GV gv, int start_y, int start_z, int end_y, int end_z, char fName[]);
 void save_fiber_sub_grid_binary(GV gv, int start_y, int start_z, int end_y, int end_z, char fName[]);
-void save_fluid_sub_grid(GV gv, int start_x, int start_y, int start_z, int end_x, int end_y, int end_z,
-                         char fName[]);
+void save_fiber_sub_grid_cord_ef_use_fv(GV gv, int start_y, int start_z, int end_y, int end_z, char grid_fname[],
+                                        char elastic_force_fname[]);       /* lbm_ib.c:650-692 */
 double get_cur_time(void);
 
 /* host mirror and status */
 int lbmib_sync_to_host(GV gv, int with_fluid);
 int lbmib_host_status(GV gv);
 void lbmib_host_free(GV gv);
-/* dump switches for do_thread: 1 = binary fiber dump, 2 = fluid cube dump */
+/* dump switches for do_thread: 1 = binary fiber dump, 2 = fluid cube dump,
+ * 4 = FieldView files (the reference's USE_FV, lbm_ib.c:30) */
 void lbmib_host_set_dumps(GV gv, int mask);
 
 #ifdef __cplusplus

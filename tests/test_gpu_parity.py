@@ -420,6 +420,46 @@ def test_cli_replays_reference_fixture_refA(tmp_path, fixtures_meta):
         assert np.abs(got[:, 3:] - ref[:, 3:]).max() <= 1e-9 * max(np.abs(ref[:, 3:]).max(), 1e-12) + 1e-20
 
 
+def test_cli_fieldview_dumps(tmp_path, fixtures_meta):
+    """The refA run with the FieldView dumps on (the reference's USE_FV
+    files, lbm_ib.c:3100-3112 and :3514-3535): names and grid files once;
+    fluid velocity and fiber grid / elastic force files at step 1 and every
+    dump steps.  The fluid velocity at step 20 matches the compiled
+    reference's own fluid dump on its box (to the %f of the format), and the
+    fiber grid matches the binary fiber dump of the same run."""
+    import subprocess
+    meta = fixtures_meta["refA"]
+    n, rows, cols = meta["grid"], meta["rows"], meta["cols"]
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "distributed-lb-ib_amd", "lbmib")
+    out = subprocess.run([os.path.abspath(exe)] + meta["args"] + ["-dump_fv", "-dump_fiber", "-quiet"],
+                         cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-400:]
+    assert open(tmp_path / "FV_nameFluid.nam").read() == "u\nv\nw"
+    assert open(tmp_path / "FV_nameFiber.nam").read() == "Afx\nAfy\nAfz"
+    head, body = open(tmp_path / "FV_gridFluid.xyz").read().split("\n", 1)
+    assert head == f"{n},{n},{n}" and body.count(",") == 3 * n ** 3
+
+    def values(text):
+        return np.array([float(v) for v in text.strip().rstrip(",").split(",")])
+
+    for step in meta["fiber_steps"]:
+        head, body = open(tmp_path / f"FV_fluid_vel_{step}.f").read().split("\n", 1)
+        assert head == f"{n},{n},{n},3"
+        vel = values(body).reshape(3, n, n, n)  # [comp, z, y, x]: x fastest
+        head, body = open(tmp_path / f"FV_gridFiber{step}.xyz").read().split("\n", 1)
+        assert head == f"{rows},{cols},1"
+        grid = values(body).reshape(3, cols, rows)  # [comp, col, row]: row fastest
+        _, rec = read_fiber_dump(os.path.join(tmp_path, f"Fiber1_dump_step{step}.bin"))
+        pos = rec[:, :3].reshape(rows, cols, 3)
+        assert np.abs(grid.transpose(2, 1, 0) - pos).max() <= 5e-7
+        head, body = open(tmp_path / f"FV_fiber_elastic_force_{step}.f").read().split("\n", 1)
+        assert head == f"{rows},{cols},1,3"
+        if step == meta["fluid_steps"][-1]:
+            fx = np.load(os.path.join(GOLDEN, f"refA_fluid_step{step}.npz"))
+            X, Y, Z = fx["xyz"][:, 0], fx["xyz"][:, 1], fx["xyz"][:, 2]
+            assert np.abs(vel[:, Z, Y, X].T - fx["vel"]).max() <= 5e-7
+
+
 # ------------------------------------------------------- batched ensembles
 
 def _ensemble(lbmib, precision):
