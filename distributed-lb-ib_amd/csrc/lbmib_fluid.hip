@@ -221,11 +221,16 @@ __device__ __forceinline__ void arrive_general(const Geometry &g, const typename
     rho = c.rho0; d = c.d0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
     return;
   }
-  pull<P, NTL, true>(g, fin, x, y, z, c0, c.br[1], c.br[2], v);
+  // the force is loaded before the box test (the field exists whenever
+  // there are sheets), so that its loads overlap the pulls; it counts only
+  // inside a support box
   T fpx = 0, fpy = 0, fpz = 0;
-  if (in_any_box(box_prev, nsheets, x, y, z)) {
-    fpx = (T)Fprev[c0]; fpy = (T)Fprev[g.n + c0]; fpz = (T)Fprev[2 * g.n + c0];
+  if (nsheets > 0) {
+    const T lx = (T)Fprev[c0], ly = (T)Fprev[g.n + c0], lz = (T)Fprev[2 * g.n + c0];
+    const bool in = in_any_box(box_prev, nsheets, x, y, z);
+    fpx = in ? lx : (T)0; fpy = in ? ly : (T)0; fpz = in ? lz : (T)0;
   }
+  pull<P, NTL, true>(g, fin, x, y, z, c0, c.br[1], c.br[2], v);
   lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
 }
 
@@ -242,11 +247,13 @@ __device__ __forceinline__ void update_cell(const Geometry &g, const typename Tr
   const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
   T v[19];
   T d, rho, ux, uy, uz;
-  arrive_general<P, NTL>(g, fin, own, c, nsheets, box_prev, Fprev, x, y, z, c0, v, d, rho, ux, uy, uz);
   T fx = 0, fy = 0, fz = 0;
-  if (in_any_box(box_cur, nsheets, x, y, z)) {
-    fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
+  if (nsheets > 0) {  // loaded early, as in arrive_general
+    const T lx = (T)Fcur[c0], ly = (T)Fcur[g.n + c0], lz = (T)Fcur[2 * g.n + c0];
+    const bool in = in_any_box(box_cur, nsheets, x, y, z);
+    fx = in ? lx : (T)0; fy = in ? ly : (T)0; fz = in ? lz : (T)0;
   }
+  arrive_general<P, NTL>(g, fin, own, c, nsheets, box_prev, Fprev, x, y, z, c0, v, d, rho, ux, uy, uz);
   collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
 #pragma unroll
   for (int q = 0; q < 19; ++q) st_h<P, NT>(fout + q * g.pitch + c0, v[q]);
@@ -772,8 +779,9 @@ __global__ void __launch_bounds__(256)
       rest -= vol;
     }
     if (k == nb) break;  // not reached: it < total
-    const int w = r[1] - r[0] + 1, h = r[3] - r[2] + 1;
-    const int x = r[0] + (int)(rest % w), y = r[2] + (int)((rest / w) % h), z = r[4] + (int)(rest / ((long)w * h));
+    // within one box (a part of the grid: under 2^31 cells) in 32 bits
+    const int w = r[1] - r[0] + 1, h = r[3] - r[2] + 1, e = (int)rest;
+    const int x = r[0] + e % w, y = r[2] + (e / w) % h, z = r[4] + e / (w * h);
     bool dup = false;
     for (int t = 0; t < k; ++t) {
       int q[6];
