@@ -6,8 +6,8 @@ at (64, 128, 128), Ks = Kb = 1e-3, populations stored in FP32 as deviations
 from the initial equilibrium feq(rho0, (u_in, 0, 0)), collision in FP64.
 A "step" is one full LBM-IB time step (fiber forces, spreading, fused
 collide-stream, band moments, interpolation and move) over the whole grid;
-by default steps are split so that the IB chain overlaps the fluid update
-(bit-identical to unsplit steps).
+steps are split so that the IB chain overlaps the fluid update, or not,
+whichever the autotune times faster (both give identical bits).
 
 Prints one JSON line: MLUPS over all nx*ny*nz cells, the HBM roofline of the
 fused collide-stream kernel (algorithmic 2*19*4 = 152 B per cell, its mean
@@ -61,7 +61,7 @@ def parse_args():
                    help="fused-kernel launch shape 'cells,ntmask,blocks_per_cu' or 'auto' (autotune)")
     p.add_argument("--overlap", type=int, default=-1, choices=[-1, 0, 1],
                    help="overlap the IB kernels with the fluid update (split steps; for c4, two instance "
-                        "groups): -1 library default, 0 off, 1 on")
+                        "groups): -1 autotuned (c4: on), 0 off, 1 on")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -151,25 +151,53 @@ VARIANTS = {"f64": [(1, 3, 0), (1, 1, 0), (2, 3, 0)],
 
 def autotune(e, a, lbmib):
     """Picks the fused-kernel variant (cells per thread, nontemporal
-    stores, blocks per CU) with the shortest whole step, timing a few steps of
-    each on the live engine; all variants compute identical bits."""
+    stores, blocks per CU) and, unless --overlap fixes it, split or unsplit
+    steps, with the shortest whole step, timing a few steps of each on the
+    live engine; every choice computes identical bits.  Returns (variant,
+    overlap mode)."""
+    modes = [a.overlap] if a.overlap >= 0 else [1, 0]
     if a.variant != "auto":
-        v = tuple(int(x) for x in a.variant.split(","))
-        e.set_kernel_variant(*v)
-        return v
-    best = None
-    for v in VARIANTS[a.precision]:
-        try:
-            e.set_kernel_variant(*v)
-        except lbmib.LbmibError:
-            continue
-        ms, _ = e.time_steps(5, 30)
-        ms /= 30
-        print(f"[autotune] variant {v}: {ms:.4f} ms/step", file=sys.stderr, flush=True)
-        if best is None or ms < best[0]:
-            best = (ms, v)
-    e.set_kernel_variant(*best[1])
-    return best[1]
+        variants = [tuple(int(x) for x in a.variant.split(","))]
+    else:
+        variants = VARIANTS[a.precision]
+    if len(variants) == 1 and len(modes) == 1:
+        e.set_overlap(modes[0])
+        e.set_kernel_variant(*variants[0])
+        return variants[0], modes[0]
+    best = {}
+    for mode in modes:
+        e.set_overlap(mode)
+        for v in variants:
+            try:
+                e.set_kernel_variant(*v)
+            except lbmib.LbmibError:
+                continue
+            ms, _ = e.time_steps(5, 30)
+            ms /= 30
+            print(f"[autotune] variant {v} split {mode}: {ms:.4f} ms/step", file=sys.stderr, flush=True)
+            if mode not in best or ms < best[mode][0]:
+                best[mode] = (ms, v)
+    # the modes are compared on the best variant of each over longer runs,
+    # alternating, so that neither gains from running later
+    pick = None
+    if len(best) > 1:
+        tmin = {}
+        for _ in range(2):
+            for mode, (_, v) in best.items():
+                e.set_overlap(mode)
+                e.set_kernel_variant(*v)
+                ms, _ = e.time_steps(10, 200)
+                ms /= 200
+                print(f"[autotune] split {mode} with {v}: {ms:.4f} ms/step over 200 steps", file=sys.stderr,
+                      flush=True)
+                tmin[mode] = min(ms, tmin.get(mode, ms))
+        best = {m: (tmin[m], v) for m, (_, v) in best.items()}
+    for mode, (ms, v) in best.items():
+        if pick is None or ms < pick[0]:
+            pick = (ms, v, mode)
+    e.set_overlap(pick[2])
+    e.set_kernel_variant(*pick[1])
+    return pick[1], pick[2]
 
 
 BATCH_TILES = [(2, 3), (4, 3), (8, 3), (2, 1), (4, 1)]  # (waves per block, nontemporal mask)
@@ -285,9 +313,8 @@ def main():
     print(f"[bench] {a.workload} {kw['nx']}x{kw['ny']}x{kw['nz']} {a.precision} nodes={nodes} "
           f"warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
     with lbmib.Engine(**kw) as e:
-        e.set_overlap(a.overlap)
         e.init()
-        variant = autotune(e, a, lbmib)
+        variant, overlap = autotune(e, a, lbmib)
         ms_total, _ = e.time_steps(a.warmup, a.steps)
         status = e.status()
         # the fused kernel's own launch time, whole grid per launch: a pass of
@@ -295,7 +322,7 @@ def main():
         # beside the IB chain)
         e.set_overlap(0)
         _, ms_main = e.time_steps(5, min(a.steps, 100))
-        e.set_overlap(a.overlap)
+        e.set_overlap(overlap)
         mass = e.totals()[0]
         bpc = e.bytes_per_cell()
         kernel_cells = e.timed_kernel_cells()
@@ -344,7 +371,7 @@ def main():
             "bytes_per_cell_update": bpc,
             "kernel_variant": {"cells_per_thread": variant[0], "nontemporal_mask": variant[1],
                                "blocks_per_cu": variant[2]},
-            "split_steps": a.overlap != 0,
+            "split_steps": overlap != 0,
         },
         "roofline": {
             "bound": "hbm",

@@ -491,7 +491,7 @@ static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, bool more) {
       RCHK(launch_ib_reset(ctx->ib, 1, 1 << s, ctx->stream_b));
       ctx->ib_clean = 1;
     }
-    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, ctx->stream_b));
+    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, s, ctx->stream_b, true));
     HIPCHK(hipEventRecord(ctx->evFib, ctx->stream_b));
     RCHK(launch_gather(prec, ctx->ib, ctx->g, s, ctx->F[s], ctx->stream_b));
   }
@@ -517,7 +517,7 @@ static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, bool more) {
   ctx->ib_clean = 1;
   ctx->ib_ready = 0;
   if (more) {
-    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, 1 - s, ctx->stream_b));
+    RCHK(launch_fiber(ctx->ib, ctx->g, FK_ALL, 1 - s, ctx->stream_b, true));
     HIPCHK(hipEventRecord(ctx->evFib, ctx->stream_b));
     RCHK(launch_gather(prec, ctx->ib, ctx->g, 1 - s, ctx->F[1 - s], ctx->stream_b));
     ctx->ib_ready = 1;
@@ -786,31 +786,37 @@ extern "C" int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes) {
 
 typedef unsigned int probe_u32x4 __attribute__((ext_vector_type(4)));
 
+// Each block copies 4 x 256 consecutive 16-byte elements; every lane issues
+// its four loads before its stores, so that four reads per lane are in flight.
 __global__ void __launch_bounds__(256) k_copy16(const probe_u32x4 *__restrict__ a, probe_u32x4 *__restrict__ b,
                                                 long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    __builtin_nontemporal_store(a[i], b + i);
+  const long i0 = (long)blockIdx.x * 1024 + threadIdx.x;
+  probe_u32x4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + 256 * k < n) v[k] = __builtin_nontemporal_load(a + i0 + 256 * k);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + 256 * k < n) __builtin_nontemporal_store(v[k], b + i0 + 256 * k);
 }
 
 extern "C" int lbmib_probe_copy_bandwidth(double bytes, int reps, double *gbps) {
   if (!gbps || bytes < 1e6 || reps < 1) return LBMIB_ERR_ARG;
   const long n = (long)(bytes / 2 / sizeof(probe_u32x4));
+  if ((n + 1023) / 1024 > 0x7fffffffL) return LBMIB_ERR_ARG;
   probe_u32x4 *a = nullptr, *b = nullptr;
   if (hipMalloc((void **)&a, n * sizeof(probe_u32x4)) != hipSuccess) return LBMIB_ERR_ALLOC;
   if (hipMalloc((void **)&b, n * sizeof(probe_u32x4)) != hipSuccess) { (void)hipFree(a); return LBMIB_ERR_ALLOC; }
   int rc = LBMIB_OK;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = LBMIB_ERR_HIP;
-  hipDeviceProp_t prop;
-  int dev = 0;
-  if (!rc && (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)) rc = LBMIB_ERR_HIP;
   double best = 0;
   if (!rc) {
     (void)hipMemset(a, 1, n * sizeof(probe_u32x4));
-    const int blocks = prop.multiProcessorCount * 16;
+    const long blocks = (n + 1023) / 1024;
     for (int r = 0; r < reps + 1 && !rc; ++r) {
       (void)hipEventRecord(e0, 0);
-      k_copy16<<<blocks, 256, 0, 0>>>(a, b, n);
+      k_copy16<<<(unsigned)blocks, 256, 0, 0>>>(a, b, n);
       (void)hipEventRecord(e1, 0);
       if (hipEventSynchronize(e1) != hipSuccess) { rc = LBMIB_ERR_HIP; break; }
       float ms = 0;

@@ -193,6 +193,7 @@ extern "C" int lbmib_batch_create(int n, const lbmib_fluid_params *fluids, const
     memset(&d, 0, sizeof(d));
     d.g = c->g;
     d.ib = c->ib;
+    d.ib.band = nullptr;  // no split steps in a batch
     d.f[0] = c->f[0];
     d.f[1] = c->f[1];
     d.F[0] = c->F[0];

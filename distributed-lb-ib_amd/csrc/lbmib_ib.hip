@@ -209,6 +209,7 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
       atomicMin(box + t, sbox[t]);
     }
   }
+  if (!ib.band) return;  // no band boxes wanted (unsplit steps)
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == nblk - 1;
@@ -220,7 +221,7 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
   __syncthreads();
   // the band boxes of this step for split steps: this step's boxes and the
   // previous step's (the other slot, not written by this launch)
-  if (threadIdx.x == 0 && ib.band) {
+  if (threadIdx.x == 0) {
     int *rb = ib.band + slot * kRectBufInts;
     const int *prev = ib.box + (1 - slot) * kMaxSheets * 6;
     int n = 0;
@@ -265,19 +266,26 @@ __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int 
   return false;
 }
 
-constexpr int kGatherMaxCand = 1536;   // ids, sorted ids, 3 starts: 30 KB of LDS
+constexpr int kGatherMaxCand = 1536;   // raw ids, sorted ids, starts: 36 KB of LDS
 constexpr int kGatherStage = 256;      // candidates whose delta factors / forces are staged: 30 KB
+
+// LDS of a gather block for mcap candidates (the layout of gather_body).
+static size_t gather_shm_bytes(int mcap) {
+  return sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * (6 * (size_t)mcap + 4);
+}
 
 template <class FT>
 __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g, int slot, FT *__restrict__ F, int mcap,
                                             int bid, int nblk) {
-  // LDS: raw candidate ids, then the sorted ids and their support starts
-  extern __shared__ double smem_d[];
+  // LDS: delta factors and forces of the staged candidates, the sorted
+  // candidates' support starts (x, y, z, -), the raw candidate ids (padded
+  // to a multiple of 4 for the 16-byte reads of the rank sort), sorted ids
+  extern __shared__ __attribute__((aligned(16))) double smem_d[];
   double *sw = smem_d;                      // [kGatherStage][12] delta factors
   double *se = sw + kGatherStage * 12;      // [kGatherStage][3] elastic force
-  int *raw = reinterpret_cast<int *>(se + kGatherStage * 3);
-  int *ids = raw + mcap;
-  int *sx = ids + mcap, *sy = sx + mcap, *sz = sy + mcap;
+  int4 *sst = reinterpret_cast<int4 *>(se + kGatherStage * 3);
+  int *raw = reinterpret_cast<int *>(sst + mcap);
+  int *ids = raw + mcap + 4;
   __shared__ int s_tot;
   const int M = ib.nnodes;
   const double cf = 1.0 / (64.0 * 1.0 * 1.0 * 1.0);  // lbm_ib.c:1357
@@ -285,7 +293,7 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
     int s, ty, tz;
     if (!tile_of(ib, slot, w, s, ty, tz)) break;
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
-    const int x0 = b[0], x1 = b[1];
+    const int x0 = b[0], x1 = b[1], bybox0 = b[2], bybox1 = b[3], bzbox0 = b[4], bzbox1 = b[5];
     // a node with support start j0 reaches rows j0..j0+3, so tile rows
     // kTile*ty .. kTile*ty+kTile-1 need j0 in [kTile*ty-3, kTile*ty+kTile-1]:
     // bins kTileBins*ty-1 .. kTileBins*ty+kTileBins-1.  Of those, only the
@@ -293,66 +301,128 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
     // of the tile (the others, e.g. of another sheet in the same (y, z)
     // bins, add nothing); they are gathered in any order and rank-sorted
     // below, so each cell still sums its contributions in node order.
+    // The bins' counts are loaded together, then every lane takes candidates
+    // of the concatenated bin lists, so that the list reads of all bins are
+    // in flight at once.
     const int by0 = kTileBins * ty - 1, bz0 = kTileBins * tz - 1;
-    if (threadIdx.x == 0) s_tot = 0;
-    __syncthreads();
-    int seen = 0;
-    for (int k = 0; k < kCandBins * kCandBins; ++k) {
+    constexpr int kNB = kCandBins * kCandBins;
+    int bins[kNB], cnt[kNB];
+#pragma unroll
+    for (int k = 0; k < kNB; ++k) {
       const int by = by0 + k / kCandBins, bz = bz0 + k % kCandBins;
-      if (by < 0 || bz < 0 || by >= ib.nby || bz >= ib.nbz) continue;
-      const int bin = by * ib.nbz + bz;
-      int c = min(ldc(ib.bin_count + bin), ib.cap);
+      const bool in = by >= 0 && bz >= 0 && by < ib.nby && bz < ib.nbz;
+      bins[k] = in ? by * ib.nbz + bz : 0;
+      cnt[k] = in ? ldc(ib.bin_count + bins[k]) : 0;
+    }
+    int seen = 0;
+#pragma unroll
+    for (int k = 0; k < kNB; ++k) {
+      int c = min(cnt[k], ib.cap);
       if (seen + c > mcap) {
         if (threadIdx.x == 0) atomicOr(ib.err, DERR_BIN_OVERFLOW);
         c = mcap - seen;
       }
-      for (int e = threadIdx.x; e < c; e += blockDim.x) {
-        const int v = ib.bin_list[(long)bin * ib.cap + e];
-        const int i0 = ib.start[v];
-        if (i0 <= x1 && i0 + 3 >= x0) raw[atomicAdd(&s_tot, 1)] = v;
-      }
+      cnt[k] = seen;  // becomes the bin's offset in the concatenation
       seen += c;
     }
+    if (threadIdx.x == 0) s_tot = 0;
     __syncthreads();
-    const int tot = s_tot;
-    // rank sort (node ids are distinct): ascending (sheet, row, col) order
+    for (int e = threadIdx.x; e < seen; e += blockDim.x) {
+      int bin = bins[0], off = 0;  // the last bin whose offset is <= e (empty bins share the next offset)
+#pragma unroll
+      for (int j = 1; j < kNB; ++j)
+        if (e >= cnt[j]) { bin = bins[j]; off = cnt[j]; }
+      const int v = ib.bin_list[(long)bin * ib.cap + (e - off)];
+      const int i0 = ib.start[v];
+      if (i0 <= x1 && i0 + 3 >= x0) raw[atomicAdd(&s_tot, 1)] = v;
+    }
+    __syncthreads();
+    const int tot = s_tot, tot4 = (tot + 3) & ~3;
     const bool staged = tot <= kGatherStage;
+    if (threadIdx.x < tot4 - tot) {
+      // padding up to a multiple of 4: an id above every id for the rank
+      // sort; a support start far from every cell and zero factors and
+      // forces for the unrolled sum
+      const int k = tot + threadIdx.x;
+      raw[k] = INT_MAX;
+      if (staged) {
+        sst[k] = make_int4(-(1 << 28), -(1 << 28), -(1 << 28), 0);
+        for (int a = 0; a < 12; ++a) sw[k * 12 + a] = 0.0;
+        for (int a = 0; a < 3; ++a) se[k * 3 + a] = 0.0;
+      }
+    }
+    __syncthreads();
+    // rank sort (node ids are distinct): ascending (sheet, row, col) order
+    const int4 *raw4 = reinterpret_cast<const int4 *>(raw);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
       const int v = raw[e];
       int r = 0;
-      for (int k = 0; k < tot; ++k) r += raw[k] < v;
+      for (int k = 0; k < tot4 / 4; ++k) {
+        const int4 q = raw4[k];
+        r += (q.x < v) + (q.y < v) + (q.z < v) + (q.w < v);
+      }
       ids[r] = v;
-      sx[r] = ib.start[v];
-      sy[r] = ib.start[M + v];
-      sz[r] = ib.start[2 * M + v];
+      sst[r] = make_int4(ib.start[v], ib.start[M + v], ib.start[2 * M + v], 0);
       if (staged) {
         for (int a = 0; a < 12; ++a) sw[r * 12 + a] = ib.wgt[a * (long)M + v];
         for (int a = 0; a < 3; ++a) se[r * 3 + a] = ib.frc[(NF_EX + a) * (long)M + v];
       }
     }
     __syncthreads();
-    const int y0 = max(kTile * ty, b[2]), y1 = min(kTile * ty + kTile - 1, b[3]);
-    const int z0 = max(kTile * tz, b[4]), z1 = min(kTile * tz + kTile - 1, b[5]);
+    const int y0 = max(kTile * ty, bybox0), y1 = min(kTile * ty + kTile - 1, bybox1);
+    const int z0 = max(kTile * tz, bzbox0), z1 = min(kTile * tz + kTile - 1, bzbox1);
     const int nxs = x1 - x0 + 1, nys = y1 - y0 + 1, nzs = z1 - z0 + 1;
     const int ncell = nxs * nys * nzs;
     for (int e = threadIdx.x; e < ncell; e += blockDim.x) {
       const int x = x0 + e % nxs, y = y0 + (e / nxs) % nys, z = z0 + e / (nxs * nys);
       double ax = 0.0, ay = 0.0, az = 0.0;
-      for (int k = 0; k < tot; ++k) {
-        const int ix = x - sx[k], iy = y - sy[k], iz = z - sz[k];
-        if ((unsigned)ix > 3u || (unsigned)iy > 3u || (unsigned)iz > 3u) continue;
-        double wgt, ex, ey, ez;
-        if (staged) {
-          wgt = cf * sw[k * 12 + ix] * sw[k * 12 + 4 + iy] * sw[k * 12 + 8 + iz];
-          ex = se[k * 3]; ey = se[k * 3 + 1]; ez = se[k * 3 + 2];
-        } else {
-          const int n = ids[k];
-          wgt = cf * ib.wgt[ix * (long)M + n] * ib.wgt[(4 + iy) * (long)M + n] * ib.wgt[(8 + iz) * (long)M + n];
-          ex = ib.frc[NF_EX * (long)M + n]; ey = ib.frc[NF_EY * (long)M + n]; ez = ib.frc[NF_EZ * (long)M + n];
+      if (staged) {
+        // branch-free over the candidates, four at a time, so that their LDS
+        // reads are all in flight together (the empty asm keeps the force
+        // reads unconditional): a candidate whose support misses the cell
+        // adds fma(0, w, a) = a (w is a finite factor read at an in-range
+        // index; a sum that starts at +0 is never -0), so each cell still
+        // adds exactly its own contributions in node order
+        for (int k = 0; k < tot4; k += 4) {
+          int4 st[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) st[u] = sst[k + u];
+          bool hit[4];
+          double wa[4], wb[4], wc[4], ex[4], ey[4], ez[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int ix = x - st[u].x, iy = y - st[u].y, iz = z - st[u].z;
+            hit[u] = (unsigned)ix <= 3u && (unsigned)iy <= 3u && (unsigned)iz <= 3u;
+            const double *w = sw + (k + u) * 12;
+            wa[u] = w[ix & 3];
+            wb[u] = w[4 + (iy & 3)];
+            wc[u] = w[8 + (iz & 3)];
+            ex[u] = se[(k + u) * 3];
+            ey[u] = se[(k + u) * 3 + 1];
+            ez[u] = se[(k + u) * 3 + 2];
+          }
+          asm volatile("" : "+v"(ex[0]), "+v"(ey[0]), "+v"(ez[0]), "+v"(ex[1]), "+v"(ey[1]), "+v"(ez[1]),
+                       "+v"(ex[2]), "+v"(ey[2]), "+v"(ez[2]), "+v"(ex[3]), "+v"(ey[3]), "+v"(ez[3]));
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double wgt = cf * wa[u] * wb[u] * wc[u];
+            ax = fma(hit[u] ? ex[u] : 0.0, wgt, ax);
+            ay = fma(hit[u] ? ey[u] : 0.0, wgt, ay);
+            az = fma(hit[u] ? ez[u] : 0.0, wgt, az);
+          }
         }
-        ax += ex * wgt;
-        ay += ey * wgt;
-        az += ez * wgt;
+      } else {
+        for (int k = 0; k < tot; ++k) {
+          const int4 st = sst[k];
+          const int ix = x - st.x, iy = y - st.y, iz = z - st.z;
+          if ((unsigned)ix > 3u || (unsigned)iy > 3u || (unsigned)iz > 3u) continue;
+          const int n = ids[k];
+          const double wgt =
+              cf * ib.wgt[ix * (long)M + n] * ib.wgt[(4 + iy) * (long)M + n] * ib.wgt[(8 + iz) * (long)M + n];
+          ax = fma(ib.frc[NF_EX * (long)M + n], wgt, ax);
+          ay = fma(ib.frc[NF_EY * (long)M + n], wgt, ay);
+          az = fma(ib.frc[NF_EZ * (long)M + n], wgt, az);
+        }
       }
       const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
       F[c0] = (FT)ax;
@@ -551,16 +621,18 @@ __global__ void __launch_bounds__(kMoveThreads) k_move_batch(const BatchDesc *__
 
 static inline int grid_for(long n, int b) { return (int)((n + b - 1) / b); }
 
-int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s) {
+int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s, bool band) {
   if (ib.nnodes <= 0) return LBMIB_OK;
-  k_fiber<<<fiber_blocks(ib.nnodes), kFiberThreads, 0, s>>>(ib, g, mask, slot);
+  IBState k = ib;
+  if (!band) k.band = nullptr;
+  k_fiber<<<fiber_blocks(ib.nnodes), kFiberThreads, 0, s>>>(k, g, mask, slot);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
   const int mcap = gather_mcap(ib.cap);
-  const size_t shm = sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * 5 * (size_t)mcap;
+  const size_t shm = gather_shm_bytes(mcap);
   const int blocks = gather_blocks(ib.nnodes);
   if (prec == LBMIB_F64) k_gather<double><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (double *)F, mcap);
   else k_gather<float><<<blocks, kGatherThreads, shm, s>>>(ib, g, slot, (float *)F, mcap);
@@ -617,7 +689,7 @@ int launch_fiber_batch(const BatchDesc *d, const int *map, int nblocks, int slot
 int launch_gather_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int slot, int mcap_max,
                         hipStream_t s) {
   if (nblocks <= 0) return LBMIB_OK;
-  const size_t shm = sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * 5 * (size_t)mcap_max;
+  const size_t shm = gather_shm_bytes(mcap_max);
   if (prec == LBMIB_F64) k_gather_batch<double><<<nblocks, kGatherThreads, shm, s>>>(d, map, slot);
   else k_gather_batch<float><<<nblocks, kGatherThreads, shm, s>>>(d, map, slot);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;

@@ -133,7 +133,9 @@ __host__ __device__ inline int gather_blocks(int nnodes) {
   return b < 16 ? 16 : (b > 4096 ? 4096 : b);
 }
 __host__ __device__ inline int band_u_blocks(int nnodes) {
-  const int b = (4 * nnodes + 255) / 256;  // about 4 support-box cells per node for a flat sheet
+  // a flat rows x cols sheet at unit spacing has a 5 x (rows+4) x (cols+4)
+  // support box: at most 7 cells per node from 20 x 20 nodes up, so one pass
+  const int b = (7 * nnodes + 255) / 256;
   return b < 16 ? 16 : (b > 8192 ? 8192 : b);
 }
 __host__ __device__ inline int move_blocks(int nnodes) { return (nnodes + 3) / 4; }  // one wave per node
@@ -200,7 +202,9 @@ int launch_totals(int prec, const Geometry &g, const void *f, int own_mode, cons
                   double *partials, int nblocks_max, double *out4, hipStream_t s);
 int totals_blocks(const Geometry &g);
 
-int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s);
+// band: also derive the band boxes of split steps (IBState::band) in the
+// launch's last block
+int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s, bool band = false);
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s);
 int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
                 const FluidConsts<double> &c, int reset_slot, hipStream_t s);
