@@ -217,7 +217,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
     alloc((void **)&ib.start, sizeof(int) * 3 * M);
     alloc((void **)&ib.wgt, sizeof(double) * 12 * M);
     alloc((void **)&ib.bin_count, sizeof(int) * (size_t)ib.nby * ib.nbz);
-    alloc((void **)&ib.bin_list, sizeof(int) * (size_t)ib.nby * ib.nbz * cap);
+    alloc((void **)&ib.bin_list, sizeof(int4) * (size_t)ib.nby * ib.nbz * cap);
     alloc((void **)&ctx->d_sheets, sizeof(SheetDesc) * kMaxSheets);
     alloc((void **)&ib.ub, sizeof(double) * 3 * n);
     alloc((void **)&ib.done, sizeof(unsigned));

@@ -164,7 +164,7 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
         }
         const int b = (st[1] >> 2) * ib.nbz + (st[2] >> 2);
         const int slot_in_bin = atomicAdd(&ib.bin_count[b], 1);
-        if (slot_in_bin < ib.cap) ib.bin_list[(long)b * ib.cap + slot_in_bin] = n;
+        if (slot_in_bin < ib.cap) ib.bin_list[(long)b * ib.cap + slot_in_bin] = make_int4(n, st[0], st[1], st[2]);
         else atomicOr(ib.err, DERR_BIN_OVERFLOW);
       }
     }
@@ -266,25 +266,27 @@ __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int 
   return false;
 }
 
-constexpr int kGatherMaxCand = 1536;   // raw ids, sorted ids, starts: 36 KB of LDS
+constexpr int kGatherMaxCand = 1536;   // raw and sorted ids and starts: 60 KB of LDS
 constexpr int kGatherStage = 256;      // candidates whose delta factors / forces are staged: 30 KB
 
 // LDS of a gather block for mcap candidates (the layout of gather_body).
 static size_t gather_shm_bytes(int mcap) {
-  return sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * (6 * (size_t)mcap + 4);
+  return sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * (10 * (size_t)mcap + 4);
 }
 
 template <class FT>
 __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g, int slot, FT *__restrict__ F, int mcap,
                                             int bid, int nblk) {
-  // LDS: delta factors and forces of the staged candidates, the sorted
-  // candidates' support starts (x, y, z, -), the raw candidate ids (padded
-  // to a multiple of 4 for the 16-byte reads of the rank sort), sorted ids
+  // LDS: delta factors and forces of the staged candidates, the sorted and
+  // the raw candidates' support starts (x, y, z, -), the raw candidate ids
+  // (padded to a multiple of 4 for the 16-byte reads of the rank sort),
+  // sorted ids
   extern __shared__ __attribute__((aligned(16))) double smem_d[];
   double *sw = smem_d;                      // [kGatherStage][12] delta factors
   double *se = sw + kGatherStage * 12;      // [kGatherStage][3] elastic force
   int4 *sst = reinterpret_cast<int4 *>(se + kGatherStage * 3);
-  int *raw = reinterpret_cast<int *>(sst + mcap);
+  int4 *rst = sst + mcap;  // support starts of the raw candidates
+  int *raw = reinterpret_cast<int *>(rst + mcap);
   int *ids = raw + mcap + 4;
   __shared__ int s_tot;
   const int M = ib.nnodes;
@@ -332,9 +334,12 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
 #pragma unroll
       for (int j = 1; j < kNB; ++j)
         if (e >= cnt[j]) { bin = bins[j]; off = cnt[j]; }
-      const int v = ib.bin_list[(long)bin * ib.cap + (e - off)];
-      const int i0 = ib.start[v];
-      if (i0 <= x1 && i0 + 3 >= x0) raw[atomicAdd(&s_tot, 1)] = v;
+      const int4 q = ib.bin_list[(long)bin * ib.cap + (e - off)];  // id, support start
+      if (q.y <= x1 && q.y + 3 >= x0) {
+        const int k = atomicAdd(&s_tot, 1);
+        raw[k] = q.x;
+        rst[k] = make_int4(q.y, q.z, q.w, 0);
+      }
     }
     __syncthreads();
     const int tot = s_tot, tot4 = (tot + 3) & ~3;
@@ -362,7 +367,7 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
         r += (q.x < v) + (q.y < v) + (q.z < v) + (q.w < v);
       }
       ids[r] = v;
-      sst[r] = make_int4(ib.start[v], ib.start[M + v], ib.start[2 * M + v], 0);
+      sst[r] = rst[e];
       if (staged) {
         for (int a = 0; a < 12; ++a) sw[r * 12 + a] = ib.wgt[a * (long)M + v];
         for (int a = 0; a < 3; ++a) se[r * 3 + a] = ib.frc[(NF_EX + a) * (long)M + v];
@@ -509,17 +514,24 @@ __device__ __forceinline__ void move_body(const IBState &ib, const Geometry &g, 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = bid * (kMoveThreads / 64) + wv;
   const int M = ib.nnodes;
-  const bool live = n < M && ib.start[n] >= 0;
+  // the node's support start, delta factors and (lanes 0..2) position are
+  // read together, at a clamped index for the waves past the last node;
+  // only the u reads depend on them
+  const int nn = min(n, M - 1);
+  const int i2 = lane & 3, j2 = (lane >> 2) & 3, k2 = lane >> 4;
+  const int sx = ib.start[nn], sy = ib.start[M + nn], sz = ib.start[2 * M + nn];
+  const double wx = ib.wgt[i2 * (long)M + nn], wy = ib.wgt[(4 + j2) * (long)M + nn],
+               wz = ib.wgt[(8 + k2) * (long)M + nn];
+  const double X = ib.pos[(long)min(lane, 2) * M + nn];
+  const bool live = n < M && sx >= 0;  // start -1: support off the grid (error latched)
   if (live) {
     // lanes run along x (consecutive addresses); the product goes to its
     // slot in the reference's (i2, j2, k2) loop order
-    const int i2 = lane & 3, j2 = (lane >> 2) & 3, k2 = lane >> 4;
     const int slot_ref = i2 * 16 + j2 * 4 + k2;
-    const int x = ib.start[n] + i2, y = ib.start[M + n] + j2, z = ib.start[2 * M + n] + k2;
+    const int x = sx + i2, y = sy + j2, z = sz + k2;
     const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
     const double ux = ib.ub[c0], uy = ib.ub[g.n + c0], uz = ib.ub[2 * g.n + c0];
     const double c_64 = 1.0 / (6.4e1);
-    const double wx = ib.wgt[i2 * (long)M + n], wy = ib.wgt[(4 + j2) * (long)M + n], wz = ib.wgt[(8 + k2) * (long)M + n];
     prod[wv][0][slot_ref] = ux * wx * wy * wz * c_64;
     prod[wv][1][slot_ref] = uy * wx * wy * wz * c_64;
     prod[wv][2][slot_ref] = uz * wx * wy * wz * c_64;
@@ -528,7 +540,7 @@ __device__ __forceinline__ void move_body(const IBState &ib, const Geometry &g, 
   if (live && lane < 3) {
     double acc = 0;
     for (int l = 0; l < 64; ++l) acc += prod[wv][lane][l];
-    ib.pos[(long)lane * M + n] += 1 * acc;  // X += dt * u(X), lbm_ib.c:2697-2699
+    ib.pos[(long)lane * M + n] = X + 1 * acc;  // X += dt * u(X), lbm_ib.c:2697-2699
   }
   // clear the bins for the next spread and the box slot it will fill
   const int tid = bid * blockDim.x + threadIdx.x;

@@ -110,7 +110,7 @@ struct IBState {
   int *start;        // [3][M]; start[0] < 0 marks an invalid node
   double *wgt;       // [12][M]: 1+cos for x(4), y(4), z(4)
   int *bin_count;    // [nbins]
-  int *bin_list;     // [nbins][cap]
+  int4 *bin_list;    // [nbins][cap] (node id, support start x, y, z)
   int nby, nbz, cap;
   int *box;          // [2][kMaxSheets][6]
   unsigned *done;    // blocks of k_fiber finished (reset by the last one)
