@@ -1,11 +1,11 @@
 // lbmib_ib.hip -- immersed-boundary kernels for gfx950.
 //
 // Per step (the order of do_thread, lbm_ib.c:3124-3412):
-//   k_fiber        one thread per fiber node: bending, stretching and elastic
-//                  force (FP64), then the node's 4-point-delta support start
-//                  floor(X-2)+1 and its twelve 1+cos(pi r/2) factors, its
-//                  (y,z) bin and the sheet's support box.  One launch covers
-//                  every sheet.
+//   k_fiber        four lanes per fiber node: bending, stretching and
+//                  elastic force (FP64) on one, the node's 4-point-delta
+//                  support start floor(X-2)+1 and 1+cos(pi r/2) factors on
+//                  one lane per axis; its (y,z) bin and the sheet's support
+//                  box.  One launch covers every sheet.
 //   k_gather       cell-owned force spreading: each workgroup owns a 4x4
 //                  (y,z) column tile of a sheet's support box, loads the
 //                  candidate node ids of the 2x2 bins that can reach it into
@@ -88,9 +88,16 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// Four lanes per node (a quad of consecutive lanes): lane 0 of the quad
+// computes the node forces, lane 1 + d the support start and the four delta
+// factors along axis d, so that the FP64 force and cosine chains run side
+// by side.  The quad exchanges the starts by lane shuffles for the node's
+// validity, bin entry and support box.  Every value is the same expression
+// as for one lane per node.
 __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g, int mask, int slot, int bid,
                                            int nblk) {
-  const int n = bid * blockDim.x + threadIdx.x;
+  const int t = bid * blockDim.x + threadIdx.x;
+  const int n = t >> 2, role = t & 3;
   const bool live = n < ib.nnodes;
   const int M = ib.nnodes;
   const double *__restrict__ pos = ib.pos;
@@ -98,8 +105,8 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
   int s = 0;
   bool ok = false;
   int st[3] = {INT_MAX, INT_MAX, INT_MAX};
-  if (live) {
-    s = find_sheet(ib, n);
+  if (live) s = find_sheet(ib, n);
+  if (live && role == 0 && (mask & (FK_BEND | FK_STRETCH | FK_ELASTIC))) {
     const SheetDesc sd = ib.sheets[s];
     const int k = n - sd.off, i = k / sd.cols, j = k % sd.cols;
     const bool quirk = ib.quirks != 0;
@@ -140,33 +147,44 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
         frc[(NF_EX + d) * (long)M + n] = sv + bv;
       }
     }
-    if (mask & FK_PREP) {
-      const double X[3] = {Wr[0][2], Wr[1][2], Wr[2][2]};
-      const int dims[3] = {g.nx, g.ny, g.nz};
-      ok = true;
-      for (int d = 0; d < 3; ++d) {
-        st[d] = (int)(floor(X[d] / 1.0 - 2) + 1);  // lbm_ib.c:1424-1429
-        ok &= st[d] >= 0 && st[d] + 3 < dims[d];
-      }
-      if (!ok) {
-        // the reference exits here (lbm_ib.c:1458-1466); latch the error and
-        // leave the node out of spreading and moving.
+  }
+  if (mask & FK_PREP) {
+    // lane 1 + d: support start floor(X - 2) + 1 along axis d
+    // (lbm_ib.c:1424-1429) and whether it lies on the grid
+    const int d = role - 1;
+    int sdv = INT_MAX, okd = 1;
+    double Xd = 0.0;
+    if (live && role > 0) {
+      Xd = pos[(long)d * M + n];
+      const int dim = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+      sdv = (int)(floor(Xd / 1.0 - 2) + 1);
+      okd = sdv >= 0 && sdv + 3 < dim;
+    }
+    const int q0 = (threadIdx.x & 63) & ~3;  // the quad's lane 0 in the wave
+    for (int a = 0; a < 3; ++a) st[a] = __shfl(sdv, q0 + 1 + a, 64);
+    const int okq = __shfl(okd, q0 + 1, 64) & __shfl(okd, q0 + 2, 64) & __shfl(okd, q0 + 3, 64);
+    ok = live && okq;
+    if (live && !ok) {
+      // the reference exits here (lbm_ib.c:1458-1466); latch the error and
+      // leave the node out of spreading and moving.
+      if (role == 0) {
         atomicOr(ib.err, DERR_FIBER_OOB);
         ib.start[n] = -1;
-      } else {
-        const double cpi = kRefPi / (2.0 * 1.0);
-        for (int d = 0; d < 3; ++d) {
-          ib.start[d * (long)M + n] = st[d];
-          for (int a = 0; a < 4; ++a) {
-            const double r = 1.0 * (st[d] + a) - X[d];
-            ib.wgt[(4 * d + a) * (long)M + n] = 1.0e0 + cos(cpi * r);
-          }
-        }
-        const int b = (st[1] >> 2) * ib.nbz + (st[2] >> 2);
-        const int slot_in_bin = atomicAdd(&ib.bin_count[b], 1);
-        if (slot_in_bin < ib.cap) ib.bin_list[(long)b * ib.cap + slot_in_bin] = make_int4(n, st[0], st[1], st[2]);
-        else atomicOr(ib.err, DERR_BIN_OVERFLOW);
       }
+    } else if (live && role > 0) {
+      // 1 + cos(pi r / 2) at r = (start + a) - X, a = 0..3, each cosine
+      // evaluated as the reference does (lbm_ib.c:1456, :2670-2673)
+      const double cpi = kRefPi / (2.0 * 1.0);
+      ib.start[d * (long)M + n] = sdv;
+      for (int a = 0; a < 4; ++a) {
+        const double r = 1.0 * (sdv + a) - Xd;
+        ib.wgt[(4 * d + a) * (long)M + n] = 1.0e0 + cos(cpi * r);
+      }
+    } else if (live) {
+      const int b = (st[1] >> 2) * ib.nbz + (st[2] >> 2);
+      const int slot_in_bin = atomicAdd(&ib.bin_count[b], 1);
+      if (slot_in_bin < ib.cap) ib.bin_list[(long)b * ib.cap + slot_in_bin] = make_int4(n, st[0], st[1], st[2]);
+      else atomicOr(ib.err, DERR_BIN_OVERFLOW);
     }
   }
   if (!(mask & FK_PREP)) return;  // uniform over the grid

@@ -122,10 +122,10 @@ struct IBState {
 
 // host-side launchers (defined in lbmib_fluid.hip / lbmib_ib.hip)
 
-// k_fiber: one thread per node, kFiberThreads per block; each block folds its
+// k_fiber: four lanes per node, kFiberThreads per block; each block folds its
 // nodes' support boxes into the box slot with atomic min/max.
-constexpr int kFiberThreads = 64;
-__host__ __device__ inline int fiber_blocks(int nnodes) { return (nnodes + kFiberThreads - 1) / kFiberThreads; }
+constexpr int kFiberThreads = 256;  // four lanes per node: 64 nodes per block
+__host__ __device__ inline int fiber_blocks(int nnodes) { return (4 * nnodes + kFiberThreads - 1) / kFiberThreads; }
 // Grids of the other IB kernels for a sheet set of nnodes nodes (the kernels
 // stride over any further work).
 __host__ __device__ inline int gather_blocks(int nnodes) {
