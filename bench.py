@@ -53,6 +53,8 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--precision", default="f32", choices=["f32", "f64", "f16", "bf16"])
     p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    p.add_argument("--layout", default=None, choices=["planes", "rows"],
+                   help="population layout in HBM (default: the library's)")
     p.add_argument("--dev-ref", default="eq0", choices=["eq0", "rest"],
                    help="reference populations of the deviation forms: feq(rho0, u0x) or w")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -210,6 +212,7 @@ def run_batch(a, lbmib, lbmib_configs):
     cfgs, _ = lbmib_configs.c4(precision=a.precision)
     for c in cfgs:
         c["dev_ref"] = a.dev_ref
+        c["layout"] = a.layout or lbmib.DEFAULT_LAYOUT
     print(f"[bench] c4 ensemble: {len(cfgs)} instances, {a.precision}, warmup={a.warmup} steps={a.steps}",
           file=sys.stderr, flush=True)
     with lbmib.Batch(cfgs) as b:
@@ -256,6 +259,7 @@ def run_batch(a, lbmib, lbmib_configs):
             "fiber_nodes": nodes,
             "bytes_per_cell_update": bpc,
             "storage": storage_desc(a.precision, a.dev_ref),
+            "layout": a.layout or lbmib.DEFAULT_LAYOUT,
             "tile": {"waves_per_block": best[1], "nontemporal_mask": best[2]},
             "overlapping_groups": a.overlap != 0,
         },
@@ -308,6 +312,7 @@ def main():
     cfg_fn = getattr(lbmib_configs, a.workload)
     kw, nominal_steps = cfg_fn(precision=a.precision)
     kw["dev_ref"] = a.dev_ref
+    kw["layout"] = a.layout or lbmib.DEFAULT_LAYOUT
     n = kw["nx"] * kw["ny"] * kw["nz"]
     nodes = sum(s.rows * s.cols for s in kw["sheets"])
     print(f"[bench] {a.workload} {kw['nx']}x{kw['ny']}x{kw['nz']} {a.precision} nodes={nodes} "
@@ -368,6 +373,7 @@ def main():
             "grid": [kw["nx"], kw["ny"], kw["nz"]],
             "fiber_nodes": nodes,
             "storage": storage_desc(a.precision, a.dev_ref),
+            "layout": a.layout or lbmib.DEFAULT_LAYOUT,
             "bytes_per_cell_update": bpc,
             "kernel_variant": {"cells_per_thread": variant[0], "nontemporal_mask": variant[1],
                                "blocks_per_cu": variant[2]},

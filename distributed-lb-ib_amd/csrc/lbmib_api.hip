@@ -2,7 +2,7 @@
 //
 // One context owns one device, one HIP stream (default priority) and all
 // device buffers of a simulation:
-//   f[2]      population ping-pong, SoA, 19 planes of `pitch` elements
+//   f[2]      population ping-pong, g.pelems elements each (planes or rows layout)
 //   F[2]      force field ping-pong (step parity), dense [3][n], valid only
 //             inside that step's support boxes
 //   node arrays (FP64 positions/forces, support starts, delta factors)
@@ -89,6 +89,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   if (!(p.tau > 0.5) || !(p.rho0 > 0)) return LBMIB_ERR_ARG;
   if (elem_size(p.precision) == 0) return LBMIB_ERR_ARG;
   if (p.dev_ref != LBMIB_DEV_EQ0 && p.dev_ref != LBMIB_DEV_REST) return LBMIB_ERR_ARG;
+  if (p.layout != LBMIB_LAYOUT_PLANES && p.layout != LBMIB_LAYOUT_ROWS) return LBMIB_ERR_ARG;
   const long n = (long)p.nx * p.ny * p.nz;
   if (n > (1L << 31) - 1) return LBMIB_ERR_ARG;
 
@@ -121,7 +122,17 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   if (const char *e = getenv("LBMIB_PLANE_PAD")) pad = atol(e);
   if (pad < 0) pad = 0;
   pad = (pad + 127) / 128 * 128;
-  g.pitch = (n + 127) / 128 * 128 + pad;
+  // the pad keeps the x-shifted reads of the last row's edge cells inside
+  // the array
+  if (p.layout == LBMIB_LAYOUT_ROWS) {
+    g.pq = p.nx;
+    g.prow = 19L * p.nx;
+    g.pelems = 19 * n + pad;
+  } else {
+    g.pq = (n + 127) / 128 * 128 + pad;
+    g.prow = p.nx;
+    g.pelems = 19 * g.pq;
+  }
 
   // BGK/Guo constants as init_gv and compute_eqlbrmdistrfuncDF1 form them
   FluidConsts<double> &c = ctx->c;
@@ -167,7 +178,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
                 hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess))
       rc = LBMIB_ERR_HIP;
   }
-  const size_t fbytes = (size_t)19 * g.pitch * ctx->esz;
+  const size_t fbytes = (size_t)g.pelems * ctx->esz;
   alloc(&ctx->f[0], fbytes);
   alloc(&ctx->f[1], fbytes);
   alloc((void **)&ctx->d_err, sizeof(int));

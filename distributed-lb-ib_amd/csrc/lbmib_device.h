@@ -212,15 +212,16 @@ __device__ __forceinline__ bool in_any_box(const int *boxes, int nsheets, int x,
 // (kernel units), since a reflected population is stored relative to r_opp.
 template <int P, bool NTL = false, bool RAW = false>
 __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>::S *__restrict__ f, int x, int y,
-                                     int z, long c, typename Traits<P>::T b1, typename Traits<P>::T b2,
+                                     int z, typename Traits<P>::T b1, typename Traits<P>::T b2,
                                      typename Traits<P>::T v[19]) {
   const bool interior = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1] && z > g.lo[2] && z < g.hi[2];
-  const long sx = 1, sy = g.nx, sz = (long)g.nx * g.ny;
+  const long c = pop_index(g, x, y, z);
+  const long sx = 1, sy = g.prow, sz = g.prow * g.ny;
   if (interior) {
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       const long off = kCX[q] * sx + kCY[q] * sy + kCZ[q] * sz;
-      v[q] = RAW ? ld_h<P, NTL>(f + q * g.pitch + c - off) : ld<P, NTL>(f + q * g.pitch + c - off);
+      v[q] = RAW ? ld_h<P, NTL>(f + q * g.pq + c - off) : ld<P, NTL>(f + q * g.pq + c - off);
     }
     return;
   }
@@ -234,9 +235,9 @@ __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>
     else bb |= (py < g.lo[1] || py > g.hi[1]);
     if (g.bc[2] == LBMIB_BC_PERIODIC) pz = pz < 0 ? pz + g.nz : (pz >= g.nz ? pz - g.nz : pz);
     else bb |= (pz < g.lo[2] || pz > g.hi[2]);
-    const long src = bb ? c + (long)kOPP[q] * g.pitch - (long)q * g.pitch
+    const long src = bb ? c + (long)kOPP[q] * g.pq - (long)q * g.pq
                         : (long)px + sy * py + sz * pz;
-    v[q] = RAW ? ld_h<P, NTL>(f + q * g.pitch + src) : ld<P, NTL>(f + q * g.pitch + src);
+    v[q] = RAW ? ld_h<P, NTL>(f + q * g.pq + src) : ld<P, NTL>(f + q * g.pq + src);
     if constexpr (Traits<P>::dev) {
       if (kCX[q] != 0 && bb) {
         const typename Traits<P>::T corr = -kCX[q] * (kCLS[q] == 1 ? b1 : b2);

@@ -1,7 +1,12 @@
 // lbmib_fluid.hip -- fused LBM kernels for gfx950.
 //
-// Layout in HBM: structure-of-arrays, one plane per population q, x fastest:
-//   f[q * pitch + x + nx * (y + ny * z)]
+// Layout in HBM: population q of cell (x, y, z) at f[q * pq + x + prow *
+// (y + ny * z)], x fastest, in one of two layouts (Geometry::pq, prow):
+//   planes  one plane of the whole grid per population (pq >= nx ny nz,
+//           prow = nx): f[q][z][y][x]
+//   rows    the 19 populations of each x row side by side (pq = nx,
+//           prow = 19 nx): f[z][y][q][x], so that the loads and stores of a
+//           row's update fall in a few contiguous blocks of memory
 // with two such buffers (ping-pong).  The stored state between steps is the
 // post-collision population f* of the last step.  One launch of
 // k_collide_stream per step pulls f*(x - c_q) from the previous step's buffer
@@ -216,7 +221,7 @@ __device__ __forceinline__ void arrive_general(const Geometry &g, const typename
       for (int q = 0; q < 19; ++q) v[q] = ldc(c.tab + q);
     } else {
 #pragma unroll
-      for (int q = 0; q < 19; ++q) v[q] = ld_h<P>(fin + q * g.pitch + c0);
+      for (int q = 0; q < 19; ++q) v[q] = ld_h<P>(fin + q * g.pq + pop_index(g, x, y, z));
     }
     rho = c.rho0; d = c.d0; ux = c.u0x; uy = c.u0y; uz = c.u0z;
     return;
@@ -230,7 +235,7 @@ __device__ __forceinline__ void arrive_general(const Geometry &g, const typename
     const bool in = in_any_box(box_prev, nsheets, x, y, z);
     fpx = in ? lx : (T)0; fpy = in ? ly : (T)0; fpz = in ? lz : (T)0;
   }
-  pull<P, NTL, true>(g, fin, x, y, z, c0, c.br[1], c.br[2], v);
+  pull<P, NTL, true>(g, fin, x, y, z, c.br[1], c.br[2], v);
   lagged_moments<P>(v, fpx, fpy, fpz, c, d, rho, ux, uy, uz);
 }
 
@@ -255,8 +260,9 @@ __device__ __forceinline__ void update_cell(const Geometry &g, const typename Tr
   }
   arrive_general<P, NTL>(g, fin, own, c, nsheets, box_prev, Fprev, x, y, z, c0, v, d, rho, ux, uy, uz);
   collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
+  const long p0 = pop_index(g, x, y, z);
 #pragma unroll
-  for (int q = 0; q < 19; ++q) st_h<P, NT>(fout + q * g.pitch + c0, v[q]);
+  for (int q = 0; q < 19; ++q) st_h<P, NT>(fout + q * g.pq + p0, v[q]);
 }
 
 // Byte offsets, in a population array, of the 19 sources of cell (0, y, z)
@@ -269,7 +275,7 @@ constexpr int kRowTabStride = 20;
 int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab) {
   const long nrows = (long)g.ny * g.nz;
   tab.assign((size_t)nrows * kRowTabStride, 0u);
-  const uint64_t pitch_b = (uint64_t)g.pitch * es, sy_b = (uint64_t)g.nx * es;
+  const uint64_t pitch_b = (uint64_t)g.pq * es, sy_b = (uint64_t)g.prow * es;
   for (long row = 0; row < nrows; ++row) {
     const int y = (int)(row % g.ny), z = (int)(row / g.ny);
     if (y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;  // dead row
@@ -370,8 +376,8 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   constexpr uint32_t es = sizeof(S);
-  const uint32_t pitch_b = (uint32_t)g.pitch * es;
-  const uint32_t sy_b = (uint32_t)g.nx * es;
+  const uint32_t pitch_b = (uint32_t)g.pq * es;   // between the populations of a cell
+  const uint32_t sy_b = (uint32_t)g.prow * es;    // between rows (y, z) and (y + 1, z)
   // bulk of a split step: the band launch has stored the band's cells
   if (band && row_meets_band(band, row % g.ny, row / g.ny) && in_band_boxes(band, x, row % g.ny, row / g.ny)) return;
   const bool edge = x == g.lo[0] || x == g.hi[0];
@@ -443,7 +449,7 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
 // Row form: blocks walk whole (y, z) rows with a stride (one block per row
 // when the grid has as many blocks as rows).
 //
-// BUF (buffer addressing, needs 19 * pitch * sizeof(S) < 2^32): for each
+// BUF (buffer addressing, needs a population array under 2^32 bytes): for each
 // population the source row -- wrapped on periodic axes, or the own row's
 // opposite population where the source lies beyond a y/z wall (half-way
 // bounce-back) -- is the same for the whole row, so its byte offset is a
@@ -539,8 +545,8 @@ __global__ void __launch_bounds__(512)
   using S = typename Traits<P>::S;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
   constexpr uint32_t es = sizeof(S);
-  const uint32_t pitch_b = (uint32_t)g.pitch * es;
-  const uint32_t sy_b = (uint32_t)g.nx * es;
+  const uint32_t pitch_b = (uint32_t)g.pq * es;   // between the populations of a cell
+  const uint32_t sy_b = (uint32_t)g.prow * es;    // between rows (y, z) and (y + 1, z)
   const int x0 = 2 * (int)threadIdx.x;
   if (x0 >= g.nx) return;
   // dead pairs store zeros so that the row's cache lines are written whole
@@ -633,8 +639,8 @@ __global__ void __launch_bounds__(512)
   const int npairs = g.ny * g.nz / 2;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
   constexpr uint32_t es = sizeof(S);
-  const uint32_t pitch_b = (uint32_t)g.pitch * es;
-  const uint32_t sy_b = (uint32_t)g.nx * es;
+  const uint32_t pitch_b = (uint32_t)g.pq * es;   // between the populations of a cell
+  const uint32_t sy_b = (uint32_t)g.prow * es;    // between rows (y, z) and (y + 1, z)
   const int x = threadIdx.x;
   if (x >= g.nx) return;
   const bool edge = x == g.lo[0] || x == g.hi[0];
@@ -871,11 +877,12 @@ __global__ void __launch_bounds__(256)
     return;
   }
   const long c0 = (long)x0 + (long)g.nx * ((long)y + (long)g.ny * z);
-  const long sy = g.nx, sz = (long)g.nx * g.ny;
+  const long p0 = pop_index(g, x0, y, z);
+  const long sy = g.prow, sz = g.prow * g.ny;
   T v[V][19];
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const S *p = fin + q * g.pitch + c0 - (kCX[q] + kCY[q] * sy + kCZ[q] * sz);
+    const S *p = fin + q * g.pq + p0 - (kCX[q] + kCY[q] * sy + kCZ[q] * sz);
     VecT<S, V> t;
     if (kCX[q] == 0) t = *reinterpret_cast<const VecT<S, V> *>(p);
     else __builtin_memcpy(&t, p, sizeof(t));
@@ -903,7 +910,7 @@ __global__ void __launch_bounds__(256)
     VecT<S, V> t;
 #pragma unroll
     for (int k = 0; k < V; ++k) t.a[k] = to_bits<P>(v[k][q]);
-    store_vec<NT>(reinterpret_cast<VecT<S, V> *>(fout + q * g.pitch + c0), t);
+    store_vec<NT>(reinterpret_cast<VecT<S, V> *>(fout + q * g.pq + p0), t);
   }
 }
 
@@ -914,10 +921,10 @@ __global__ void k_fill_init(Geometry g, typename Traits<P>::S *__restrict__ f, K
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, z = blockIdx.z;
   if (x >= g.nx) return;
-  const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
+  const long p0 = pop_index(g, x, y, z);
   const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
 #pragma unroll
-  for (int q = 0; q < 19; ++q) st_h<P>(f + q * g.pitch + c0, act ? c.tab[q] : (typename Traits<P>::T)0);
+  for (int q = 0; q < 19; ++q) st_h<P>(f + q * g.pq + p0, act ? c.tab[q] : (typename Traits<P>::T)0);
 }
 
 // Materialises df2 (= df1 after the swap), rho and vel of the last completed
@@ -941,9 +948,9 @@ __global__ void k_materialize(Geometry g, const typename Traits<P>::S *__restric
     if (frozen) {
       for (int q = 0; q < 19; ++q) v[q] = c.tab[q] * Traits<P>::inv_scale;
     } else if (own) {
-      for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pitch + c0);
+      for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pq + pop_index(g, x, y, z));
     } else {
-      pull<P>(g, f, x, y, z, c0, c.br[1], c.br[2], v);
+      pull<P>(g, f, x, y, z, c.br[1], c.br[2], v);
     }
     for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + c.ref[q] : (double)v[q];
   } else {
@@ -1000,9 +1007,9 @@ __global__ void __launch_bounds__(kTotThreads)
       if (frozen) {
         for (int q = 0; q < 19; ++q) v[q] = c.tab[q] * Traits<P>::inv_scale;
       } else if (own) {
-        for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pitch + c0);
+        for (int q = 0; q < 19; ++q) v[q] = ld<P>(f + q * g.pq + pop_index(g, x, y, z));
       } else {
-        pull<P>(g, f, x, y, z, c0, c.br[1], c.br[2], v);
+        pull<P>(g, f, x, y, z, c.br[1], c.br[2], v);
       }
       for (int q = 0; q < 19; ++q) {
         const double fq = Traits<P>::dev ? (double)v[q] + c.ref[q] : (double)v[q];
@@ -1103,7 +1110,7 @@ static void launch_vec(const Geometry &g, const void *fin, void *fout, int own, 
 // Whether the buffer-addressed row path applies: every byte offset of a
 // population array fits 32 bits and one block covers a row.
 bool row_path_buffered(const Geometry &g, size_t es) {
-  return 19ull * (unsigned long long)g.pitch * es <= 0xFFFFFFFFull && (int)row_block(g).x >= g.nx;
+  return (unsigned long long)g.pelems * es <= 0xFFFFFFFFull && (int)row_block(g).x >= g.nx;
 }
 
 template <int P, bool NT, bool NTL>

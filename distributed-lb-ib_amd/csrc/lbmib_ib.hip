@@ -495,7 +495,7 @@ __device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g
       double ux = u0x, uy = u0y, uz = u0z;
       if (mom) {
         typename Traits<P>::T v[19];
-        pull<P>(g, f, x, y, z, c0, b1, b2, v);
+        pull<P>(g, f, x, y, z, b1, b2, v);
         double fv[19];
         for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + ref[q] : (double)v[q];
         double s1 = fv[0], s2 = 0, s3 = 0, s4 = 0;

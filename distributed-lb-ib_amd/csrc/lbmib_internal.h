@@ -98,8 +98,17 @@ struct Geometry {
   int lo[3], hi[3];    // active box per axis
   int mlo[3], mhi[3];  // moment box per axis
   long n;              // cells
-  long pitch;          // elements between population planes
+  // population arrays: population q of cell (x, y, z) is element
+  // q * pq + x + prow * (y + ny * z) (pop_index); pelems elements in all
+  //   planes (LBMIB_LAYOUT_PLANES): pq = plane pitch (>= n), prow = nx
+  //   rows (LBMIB_LAYOUT_ROWS):     pq = nx, prow = 19 nx
+  long pq, prow, pelems;
 };
+
+// Element of population 0 of cell (x, y, z) in a population array.
+__host__ __device__ inline long pop_index(const Geometry &g, int x, int y, int z) {
+  return (long)x + g.prow * ((long)y + (long)g.ny * z);
+}
 
 // IB device state pointers, passed by value to the IB kernels.
 struct IBState {

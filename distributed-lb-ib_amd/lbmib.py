@@ -54,7 +54,8 @@ class FluidParams(ctypes.Structure):
                 ("bc_x", ctypes.c_int), ("bc_y", ctypes.c_int), ("bc_z", ctypes.c_int),
                 ("precision", ctypes.c_int), ("quirks", ctypes.c_int),
                 ("tau", ctypes.c_double), ("rho0", ctypes.c_double),
-                ("u0", ctypes.c_double * 3), ("g", ctypes.c_double), ("dev_ref", ctypes.c_int)]
+                ("u0", ctypes.c_double * 3), ("g", ctypes.c_double), ("dev_ref", ctypes.c_int),
+                ("layout", ctypes.c_int)]
 
 
 class SheetParams(ctypes.Structure):
@@ -179,11 +180,14 @@ class Sheet:
 # ------------------------------------------------------------------ engine
 
 DEV_REFS = {"eq0": 0, "rest": 1}
+# population layout in HBM: "rows" f[z][y][q][x], "planes" f[q][z][y][x]
+LAYOUTS = {"rows": 0, "planes": 1}
+DEFAULT_LAYOUT = "rows"
 
 
-def fluid_params(nx, ny, nz, bc, precision, tau, rho0, u0, quirks, g=0.0, dev_ref="eq0"):
+def fluid_params(nx, ny, nz, bc, precision, tau, rho0, u0, quirks, g=0.0, dev_ref="eq0", layout=DEFAULT_LAYOUT):
     return FluidParams(nx, ny, nz, bc[0], bc[1], bc[2], PRECISIONS[precision], int(bool(quirks)),
-                       tau, rho0, (ctypes.c_double * 3)(*u0), g, DEV_REFS[dev_ref])
+                       tau, rho0, (ctypes.c_double * 3)(*u0), g, DEV_REFS[dev_ref], LAYOUTS[layout])
 
 
 class Engine:
@@ -191,17 +195,18 @@ class Engine:
 
     ``precision``: "f64", "f32", "f16" or "bf16".  ``bc``: per-axis boundary
     kinds (x may be BC_INOUT).  ``dev_ref``: what the deviation forms store
-    populations relative to, "eq0" (feq(rho0, u0)) or "rest" (w).  Field
-    arrays returned by the getters are indexed [..., z, y, x].
+    populations relative to, "eq0" (feq(rho0, u0)) or "rest" (w).
+    ``layout``: population layout in HBM, "planes" or "rows" (same results).
+    Field arrays returned by the getters are indexed [..., z, y, x].
     """
 
     def __init__(self, nx, ny, nz, *, bc=(BC_INOUT, BC_WALL, BC_WALL), precision="f32", tau=0.6,
                  rho0=1.0, u0=(0.01, 0.0, 0.0), quirks=True, sheets=(), g=0.0, preshift=True,
-                 preshift_all=True, dev_ref="eq0"):
+                 preshift_all=True, dev_ref="eq0", layout=DEFAULT_LAYOUT):
         self.L = load_library()
         self._owner = True
         self._describe(nx, ny, nz, precision, sheets)
-        fp = fluid_params(nx, ny, nz, bc, precision, tau, rho0, u0, quirks, g, dev_ref)
+        fp = fluid_params(nx, ny, nz, bc, precision, tau, rho0, u0, quirks, g, dev_ref, layout)
         sp = (SheetParams * max(1, len(self.sheets)))()
         for k, s in enumerate(self.sheets):
             sp[k] = SheetParams(s.rows, s.cols, s.width, s.height, s.ks, s.kb)
@@ -377,7 +382,7 @@ class Batch:
             fps[i] = fluid_params(c["nx"], c["ny"], c["nz"], c.get("bc", (BC_INOUT, BC_WALL, BC_WALL)),
                                   c.get("precision", "f32"), c.get("tau", 0.6), c.get("rho0", 1.0),
                                   c.get("u0", (0.01, 0.0, 0.0)), c.get("quirks", True), c.get("g", 0.0),
-                                  c.get("dev_ref", "eq0"))
+                                  c.get("dev_ref", "eq0"), c.get("layout", DEFAULT_LAYOUT))
             nsh[i] = len(c.get("sheets", ()))
         for k, s in enumerate(all_sheets):
             sps[k] = SheetParams(s.rows, s.cols, s.width, s.height, s.ks, s.kb)

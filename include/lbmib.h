@@ -50,6 +50,12 @@ extern "C" {
 #define LBMIB_DEV_EQ0 0  /* r_i = feq_i(rho0, u0), the initial flow (default) */
 #define LBMIB_DEV_REST 1 /* r_i = w_i, the rest state */
 
+/* Population layout in HBM (lbmib_fluid_params.layout); results are
+ * identical in both.  The rows layout keeps the loads and stores of a row's
+ * update in a few contiguous blocks and streams ~3% faster. */
+#define LBMIB_LAYOUT_ROWS 0   /* f[z][y][q][x]: the 19 populations of each x row side by side (default) */
+#define LBMIB_LAYOUT_PLANES 1 /* f[q][z][y][x]: 19 planes of the whole grid */
+
 typedef struct lbmib_ctx lbmib_ctx;
 
 /* Fluid configuration.  The reference hard-codes these in init_gv
@@ -64,6 +70,7 @@ typedef struct {
   double u0[3];        /* initial velocity; inlet/outlet tables are feq(rho0, u0) */
   double g;            /* gravity along z (lbm_ib.c:769 sets 0) */
   int dev_ref;         /* LBMIB_DEV_EQ0 / LBMIB_DEV_REST; ignored by LBMIB_F64 */
+  int layout;          /* LBMIB_LAYOUT_ROWS (0) / LBMIB_LAYOUT_PLANES */
 } lbmib_fluid_params;
 
 /* One fiber sheet (Fibersheet, lbm_ib.c:68-76): rows along y, cols along z.

@@ -290,6 +290,66 @@ def test_row_and_pair_paths_agree_on_every_boundary(lbmib, bc, precision):
             assert np.array_equal(a, b)
 
 
+LAYOUT_SHAPES = {  # launch shapes (cells per thread, nontemporal mask, blocks per CU) per precision
+    "f64": [(1, 3, 0), (2, 3, 0), (3, 3, 0), (1, 3, 4)],
+    "f32": [(1, 3, 0), (2, 3, 0), (3, 3, 0), (4, 1, 0), (1, 3, 4)],
+    "f16": [(1, 3, 0), (2, 3, 0), (3, 3, 0), (8, 1, 0), (2, 3, 4)],
+    "bf16": [(1, 3, 0), (2, 3, 0), (3, 3, 0), (4, 1, 0), (2, 3, 4)],
+}
+
+
+@pytest.mark.parametrize("bc", [(BC_INOUT, BC_PERIODIC, BC_WALL), (BC_WALL, BC_PERIODIC, BC_WALL),
+                                (BC_PERIODIC, BC_WALL, BC_PERIODIC)])
+@pytest.mark.parametrize("precision", sorted(LAYOUT_SHAPES))
+def test_layouts_are_bitwise_identical(lbmib, bc, precision):
+    """The rows layout (the 19 populations of each x row side by side) gives
+    the same bits as the planes layout, for every launch shape, unsplit and
+    split steps, and the materialised fields and force."""
+    kw = small_ib_config(lbmib, precision, bc=bc, n=(64, 30, 34))
+    ref = None
+    ran = {"planes": 0, "rows": 0}
+    for layout in ("planes", "rows"):
+        runs = [(v, 0) for v in LAYOUT_SHAPES[precision]] + [(LAYOUT_SHAPES[precision][0], 1)]
+        for v, split in runs:
+            with lbmib.Engine(layout=layout, **kw) as e:
+                try:
+                    e.set_kernel_variant(*v)
+                except lbmib.LbmibError:
+                    continue  # a shape this grid and boundary set do not take (same for both layouts)
+                e.set_overlap(split)
+                e.init()
+                e.step(14)
+                assert e.status() == 0
+                rho, vel = e.macroscopic()
+                out = (e.df(), e.fibers(0), e.force(), rho, vel)
+            ran[layout] += 1
+            if ref is None:
+                ref = out
+                continue
+            for a, b in zip(out, ref):
+                assert np.array_equal(a, b), (layout, v, split)
+    assert ran["planes"] == ran["rows"] >= 3, ran
+
+
+def test_batch_mixes_layouts(lbmib):
+    """A batch whose instances use either layout evolves each instance as it
+    evolves alone."""
+    kw = small_ib_config(lbmib, "f32", n=(62, 30, 34))
+    cfgs = [dict(kw, layout="rows"), dict(kw, layout="planes")]
+    with lbmib.Batch(cfgs) as b:
+        b.init()
+        b.step(12)
+        assert b.status() == 0
+        got = [(b.instances[i].df(), b.instances[i].fibers(0)) for i in range(2)]
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(12)
+        ref = (e.df(), e.fibers(0))
+    for g in got:
+        for a, r in zip(g, ref):
+            assert np.array_equal(a, r)
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
 def test_split_steps_are_bitwise_identical(lbmib, precision):
     """lbmib_step with the IB chain on a second stream (bulk / band split)
