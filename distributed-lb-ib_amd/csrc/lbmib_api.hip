@@ -212,12 +212,15 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   ib.quirks = p.quirks;
   ib.nby = (p.ny + 3) / 4;
   ib.nbz = (p.nz + 3) / 4;
-  // bin capacity: a 4x4 column holds about (4/ds1+2)(4/ds2+2) nodes per
-  // sheet; keep a factor two of headroom for deformation.
+  // bin capacity: a 4x4 column holds at most (4/ds1+1)(4/ds2+1) nodes of a
+  // flat sheet; (4/ds1+2)(4/ds2+2) per sheet with a factor 1.5 of headroom
+  // for deformation (unit spacing: 64 for one sheet, 2.6x the flat maximum).
+  // Overflow is reported (LBMIB_ERR_BIN_OVERFLOW).  The spreading kernel's
+  // LDS grows with it, and with it fewer tiles are resident per CU.
   double dens = 0;
   for (int s = 0; s < nsheets; ++s) dens += (4.0 / ctx->sd[s].ds1 + 2.0) * (4.0 / ctx->sd[s].ds2 + 2.0);
   int cap = 32;
-  while (cap < 2 * dens && cap < 1024) cap <<= 1;
+  while (cap < 1.5 * dens && cap < 1024) cap <<= 1;
   ib.cap = cap;
   if (ctx->nnodes > 0) {
     const size_t M = ctx->nnodes;

@@ -284,28 +284,27 @@ __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int 
   return false;
 }
 
-constexpr int kGatherMaxCand = 1536;   // raw and sorted ids and starts: 60 KB of LDS
-constexpr int kGatherStage = 256;      // candidates whose delta factors / forces are staged: 30 KB
+constexpr int kGatherMaxCand = 1536;   // raw ids, raw and sorted starts: 54 KB of LDS
+constexpr int kGatherStage = 128;      // candidates whose delta factors / forces are staged: 15 KB
 
 // LDS of a gather block for mcap candidates (the layout of gather_body).
 static size_t gather_shm_bytes(int mcap) {
-  return sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * (10 * (size_t)mcap + 4);
+  return sizeof(double) * 15 * (size_t)kGatherStage + sizeof(int) * (9 * (size_t)mcap + 4);
 }
 
 template <class FT>
 __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g, int slot, FT *__restrict__ F, int mcap,
                                             int bid, int nblk) {
-  // LDS: delta factors and forces of the staged candidates, the sorted and
-  // the raw candidates' support starts (x, y, z, -), the raw candidate ids
-  // (padded to a multiple of 4 for the 16-byte reads of the rank sort),
-  // sorted ids
+  // LDS: delta factors and forces of the staged candidates, the sorted
+  // candidates' support starts and ids (x, y, z, id), the raw candidates'
+  // starts and the raw candidate ids (padded to a multiple of 4 for the
+  // 16-byte reads of the rank sort)
   extern __shared__ __attribute__((aligned(16))) double smem_d[];
   double *sw = smem_d;                      // [kGatherStage][12] delta factors
   double *se = sw + kGatherStage * 12;      // [kGatherStage][3] elastic force
   int4 *sst = reinterpret_cast<int4 *>(se + kGatherStage * 3);
   int4 *rst = sst + mcap;  // support starts of the raw candidates
   int *raw = reinterpret_cast<int *>(rst + mcap);
-  int *ids = raw + mcap + 4;
   __shared__ int s_tot;
   const int M = ib.nnodes;
   const double cf = 1.0 / (64.0 * 1.0 * 1.0 * 1.0);  // lbm_ib.c:1357
@@ -356,7 +355,7 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
       if (q.y <= x1 && q.y + 3 >= x0) {
         const int k = atomicAdd(&s_tot, 1);
         raw[k] = q.x;
-        rst[k] = make_int4(q.y, q.z, q.w, 0);
+        rst[k] = make_int4(q.y, q.z, q.w, q.x);
       }
     }
     __syncthreads();
@@ -384,8 +383,7 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
         const int4 q = raw4[k];
         r += (q.x < v) + (q.y < v) + (q.z < v) + (q.w < v);
       }
-      ids[r] = v;
-      sst[r] = rst[e];
+      sst[r] = rst[e];  // starts and id
       if (staged) {
         for (int a = 0; a < 12; ++a) sw[r * 12 + a] = ib.wgt[a * (long)M + v];
         for (int a = 0; a < 3; ++a) se[r * 3 + a] = ib.frc[(NF_EX + a) * (long)M + v];
@@ -439,7 +437,7 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
           const int4 st = sst[k];
           const int ix = x - st.x, iy = y - st.y, iz = z - st.z;
           if ((unsigned)ix > 3u || (unsigned)iy > 3u || (unsigned)iz > 3u) continue;
-          const int n = ids[k];
+          const int n = st.w;
           const double wgt =
               cf * ib.wgt[ix * (long)M + n] * ib.wgt[(4 + iy) * (long)M + n] * ib.wgt[(8 + iz) * (long)M + n];
           ax = fma(ib.frc[NF_EX * (long)M + n], wgt, ax);
@@ -481,13 +479,27 @@ __device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g
                                             typename Traits<P>::T b1, typename Traits<P>::T b2, int slot, int bid,
                                             int nblk) {
   const long tid = (long)bid * blockDim.x + threadIdx.x, stride = (long)nblk * blockDim.x;
+  // the cells of all sheets' boxes in one index range, so that one pass of
+  // the grid covers every sheet (a cell in two boxes gets the same value
+  // twice)
+  long tot = 0;
   for (int s = 0; s < ib.nsheets; ++s) {
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
-    const int x0 = b[0], y0 = b[2], z0 = b[4];
-    if (x0 > b[1]) continue;  // no valid node
-    const int nxs = b[1] - x0 + 1, nys = b[3] - y0 + 1;
-    const long cnt = (long)nxs * nys * (b[5] - z0 + 1);
-    for (long e = tid; e < cnt; e += stride) {
+    if (b[0] <= b[1]) tot += (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
+  }
+  for (long e0 = tid; e0 < tot; e0 += stride) {
+    long e = e0;
+    int s = 0;
+    for (;; ++s) {
+      const int *b = ib.box + (slot * kMaxSheets + s) * 6;
+      const long cnt = b[0] <= b[1] ? (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1) : 0;
+      if (e < cnt || s == ib.nsheets - 1) break;
+      e -= cnt;
+    }
+    {
+      const int *b = ib.box + (slot * kMaxSheets + s) * 6;
+      const int x0 = b[0], y0 = b[2], z0 = b[4];
+      const int nxs = b[1] - x0 + 1, nys = b[3] - y0 + 1;
       const int x = x0 + (int)(e % nxs), y = y0 + (int)((e / nxs) % nys), z = z0 + (int)(e / ((long)nxs * nys));
       const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
       const bool mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] &&
