@@ -284,7 +284,7 @@ __device__ __forceinline__ bool tile_of(const IBState &ib, int slot, int w, int 
   return false;
 }
 
-constexpr int kGatherMaxCand = 1536;   // raw ids, raw and sorted starts: 54 KB of LDS
+constexpr int kGatherMaxCand = 384;    // candidates whose support meets the tile: 13.5 KB of LDS
 constexpr int kGatherStage = 128;      // candidates whose delta factors / forces are staged: 15 KB
 
 // LDS of a gather block for mcap candidates (the layout of gather_body).
@@ -336,11 +336,7 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
     int seen = 0;
 #pragma unroll
     for (int k = 0; k < kNB; ++k) {
-      int c = min(cnt[k], ib.cap);
-      if (seen + c > mcap) {
-        if (threadIdx.x == 0) atomicOr(ib.err, DERR_BIN_OVERFLOW);
-        c = mcap - seen;
-      }
+      const int c = min(cnt[k], ib.cap);
       cnt[k] = seen;  // becomes the bin's offset in the concatenation
       seen += c;
     }
@@ -354,12 +350,15 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
       const int4 q = ib.bin_list[(long)bin * ib.cap + (e - off)];  // id, support start
       if (q.y <= x1 && q.y + 3 >= x0) {
         const int k = atomicAdd(&s_tot, 1);
-        raw[k] = q.x;
-        rst[k] = make_int4(q.y, q.z, q.w, q.x);
+        if (k < mcap) {  // LDS holds up to mcap candidates; more is reported
+          raw[k] = q.x;
+          rst[k] = make_int4(q.y, q.z, q.w, q.x);
+        }
       }
     }
     __syncthreads();
-    const int tot = s_tot, tot4 = (tot + 3) & ~3;
+    if (threadIdx.x == 0 && s_tot > mcap) atomicOr(ib.err, DERR_BIN_OVERFLOW);
+    const int tot = min(s_tot, mcap), tot4 = (tot + 3) & ~3;
     const bool staged = tot <= kGatherStage;
     if (threadIdx.x < tot4 - tot) {
       // padding up to a multiple of 4: an id above every id for the rank
