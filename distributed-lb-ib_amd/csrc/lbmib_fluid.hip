@@ -1001,7 +1001,6 @@ __global__ void __launch_bounds__(kTotThreads)
     const int y = (int)(row % g.ny), z = (int)(row / g.ny);
     for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
       if (x < g.lo[0] || x > g.hi[0] || y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;
-      const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
       T v[19];
       const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
       if (frozen) {

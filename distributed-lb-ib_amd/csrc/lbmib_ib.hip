@@ -230,7 +230,7 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
   if (!ib.band) return;  // no band boxes wanted (unsplit steps)
   __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == nblk - 1;
+  if (threadIdx.x == 0) s_last = atomicAdd(ib.done, 1u) == (unsigned)(nblk - 1);
   __syncthreads();
   if (!s_last) return;
   __threadfence();
@@ -360,7 +360,7 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
     if (threadIdx.x == 0 && s_tot > mcap) atomicOr(ib.err, DERR_BIN_OVERFLOW);
     const int tot = min(s_tot, mcap), tot4 = (tot + 3) & ~3;
     const bool staged = tot <= kGatherStage;
-    if (threadIdx.x < tot4 - tot) {
+    if ((int)threadIdx.x < tot4 - tot) {
       // padding up to a multiple of 4: an id above every id for the rank
       // sort; a support start far from every cell and zero factors and
       // forces for the unrolled sum
