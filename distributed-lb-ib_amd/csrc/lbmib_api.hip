@@ -472,11 +472,13 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
 // while the band and the whole IB chain run beside it.  Results are
 // bit-identical to unsplit steps: every cell does the same arithmetic.
 
-// Automatic mode splits whenever there are sheets and the buffer-addressed
-// row kernels apply.
+// Split steps need sheets and the buffer-addressed row kernels, and run only
+// when asked for (mode 1): unsplit steps measured as fast or faster on every
+// workload once the IB chain was short, because the IB kernels run several
+// times slower beside a bulk launch that saturates HBM.
 static bool split_enabled(const lbmib_ctx *ctx) {
   if (ctx->nnodes == 0 || !ctx->d_rowtab) return false;
-  return ctx->overlap_req != 0;
+  return ctx->overlap_req == 1;
 }
 
 // Grid of the band update (256 threads a block): about one thread per band

@@ -47,7 +47,7 @@ struct lbmib_ctx {
   // split steps (lbmib_set_overlap): the bulk on `stream`; the band, band
   // velocity, move and the next step's fiber forces and spreading on
   // `stream_b`, beside the bulk
-  int overlap_req = -1;           // -1 automatic (on with sheets), 0 off, 1 on
+  int overlap_req = -1;           // -1 automatic (unsplit), 0 off, 1 on
   hipStream_t stream_b = nullptr;
   hipEvent_t evFib = nullptr, evBulk = nullptr, evJoin = nullptr;  // split steps
   int split_run = 0;              // steps taken in the current run of split steps

@@ -337,7 +337,8 @@ class Engine:
                     "set_kernel_variant")
 
     def set_overlap(self, mode):
-        """-1 automatic, 0 never split steps, 1 always (see lbmib_set_overlap)."""
+        """-1 automatic (unsplit), 0 never split steps, 1 always (see
+        lbmib_set_overlap)."""
         self._check(self.L.lbmib_set_overlap(self.h, mode), "set_overlap")
 
     def timed_kernel_cells(self):

@@ -131,8 +131,8 @@ int lbmib_step(lbmib_ctx *ctx, int nsteps);
  * next step's fiber forces and spreading, while the main stream updates all
  * other cells -- the IB chain overlaps the bulk of the fluid update.
  * Results are bit-identical to unsplit steps.  mode: -1 automatic (the
- * default: split when there are sheets and the buffer-addressed row kernels
- * apply), 0 never, 1 always. */
+ * default: unsplit, which measured as fast or faster on every workload here),
+ * 0 never, 1 always (with sheets and the buffer-addressed row kernels). */
 int lbmib_set_overlap(lbmib_ctx *ctx, int mode);
 /* Cells updated by the kernels lbmib_time_steps times per step (all cells;
  * with split steps the timed launch is the bulk, all cells but the band's,
