@@ -317,9 +317,16 @@ def main():
     nodes = sum(s.rows * s.cols for s in kw["sheets"])
     print(f"[bench] {a.workload} {kw['nx']}x{kw['ny']}x{kw['nz']} {a.precision} nodes={nodes} "
           f"warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    # the launch shape and step mode are tuned on one engine; the timed steps
+    # start from the initial state on a fresh one, so that the sheet has not
+    # drifted through the tuning steps (a small grid's sheet would leave it)
     with lbmib.Engine(**kw) as e:
         e.init()
         variant, overlap = autotune(e, a, lbmib)
+    with lbmib.Engine(**kw) as e:
+        e.set_overlap(overlap)
+        e.set_kernel_variant(*variant)
+        e.init()
         ms_total, _ = e.time_steps(a.warmup, a.steps)
         status = e.status()
         # the fused kernel's own launch time, whole grid per launch: a pass of
