@@ -71,7 +71,8 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   if (ctx->stream_b) (void)hipStreamDestroy(ctx->stream_b);
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
                   ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.done, ctx->ib.ub, ctx->d_sheets,
-                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab, ctx->d_rect, ctx->d_ref};
+                  ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab, ctx->d_rect, ctx->d_ref,
+                  ctx->f_spare, ctx->pos_save, ctx->F_save, ctx->box_save, ctx->d_raster};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -120,10 +121,11 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
   g.n = n;
   long pad = 256;
   if (const char *e = getenv("LBMIB_PLANE_PAD")) pad = atol(e);
-  if (pad < 0) pad = 0;
-  pad = (pad + 127) / 128 * 128;
   // the pad keeps the x-shifted reads of the last row's edge cells inside
-  // the array
+  // the array (the pair kernel on a periodic x axis reads one element past
+  // the last row), and the buffer descriptors do not clamp: never below 128
+  if (pad < 128) pad = 128;
+  pad = (pad + 127) / 128 * 128;
   if (p.layout == LBMIB_LAYOUT_ROWS) {
     g.pq = p.nx;
     g.prow = 19L * p.nx;

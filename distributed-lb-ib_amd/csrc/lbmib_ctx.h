@@ -55,5 +55,18 @@ struct lbmib_ctx {
   int *d_rect = nullptr;          // [2][kRectBufInts] band boxes of split steps by step parity (IBState::band)
   double hyz[kMaxSheets][4] = {};      // host y/z extent of each sheet's nodes (from the uploaded positions)
   std::vector<hipEvent_t> evs;
+  // surrogate jumps (lbmib_jump.hip), allocated on first use: the population
+  // buffer a jump writes into (afterwards the pre-jump checkpoint), the
+  // pre-jump node positions, the previous step's force slot inside its boxes
+  // and those boxes, and the sheet raster's fixed-point accumulators
+  void *f_spare = nullptr;
+  double *pos_save = nullptr;
+  void *F_save = nullptr;
+  int *box_save = nullptr;
+  unsigned long long *d_raster = nullptr;
+  long raster_cells = 0;
+  int jump_active = 0;      // a jump awaits lbmib_jump_commit / lbmib_jump_revert
+  int jump_time = 0;        // time and spread slot before the jump
+  int jump_spread_slot = -1;
 };
 

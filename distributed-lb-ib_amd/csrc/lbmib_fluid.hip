@@ -22,10 +22,11 @@
 //   F64       f itself; arithmetic in the reference's operation order with no
 //             FMA contraction (the library is built with -ffp-contract=off),
 //             so results track the FP64 oracle to a few ulp.
-//   F32       g = f - w_q in FP32; collision in FP32 on deviations, which
-//             keeps the small velocity (|u| ~ 1e-2) out of the rounding of
-//             the O(0.3) rest population.
-//   F16/BF16  g * 2^8 in 16 bits; collision in FP32.
+//   F32       h = f - r_q in FP32, r_q = feq_q(rho0, (u0x, 0, 0)) by default
+//             (or w_q, LBMIB_DEV_REST); collision in FP64 on the stored
+//             deviations, which keeps the small velocity (|u| ~ 1e-2) out of
+//             the rounding of the O(0.3) rest population.
+//   F16/BF16  h = 2^8 (f - r_q) in 16 bits; collision in FP32.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>

@@ -28,7 +28,7 @@ constexpr int kWDEN_CLS[3] = {3, 18, 36};
 constexpr double kRefPi = 3.14159265358979;
 
 constexpr int kMaxSheets = 16;
-constexpr int kDevScale16 = 256;  // 16-bit deviation storage holds (f - w) * 2^8
+constexpr int kDevScale16 = 256;  // 16-bit deviation storage holds (f - r) * 2^8
 
 // Fibernode-equivalent SoA offsets inside the node force array.
 enum { NF_BX = 0, NF_BY, NF_BZ, NF_SX, NF_SY, NF_SZ, NF_EX, NF_EY, NF_EZ, NF_COUNT };

@@ -1,0 +1,888 @@
+// lbmib_surr.hip -- kernels of the learned surrogate (include/lbmib_surrogate.h):
+// 3x3x3 convolutions on channels-last BF16 tensors with BF16 MFMA and FP32
+// accumulation, their data and weight gradients, 2x pooling / upsampling, the
+// loss, the optimiser and the packing of field planes.
+//
+// Convolutions are implicit GEMMs over voxels v, output channels n and the
+// reduction index k = 27 taps x Cin input channels (tap-major, channel
+// fastest, the weight layout [Cout][27][Cin]):
+//   y[v][n] = sum_k W[n][k] X(v, k),   X(v, t*Cin + ci) = x[v + o(t)][ci]
+// with o(t) the neighbour offset of tap t and zero outside the volume.  One
+// v_mfma_f32_16x16x32_bf16 multiplies a 16 x 32 weight fragment (A: row n =
+// lane & 15, k = 8 (lane >> 4) .. + 8) by a 32 x 16 input fragment (B:
+// column v = lane & 15, the same 8 k), so that every lane fetches 8
+// consecutive channels of one voxel (16 bytes) and of one weight row; the
+// 16 x 16 result holds, per lane, 4 consecutive output channels of one voxel
+// (rows 4 (lane >> 4) + r), stored as one 8-byte write.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+
+#include "../../include/lbmib.h"
+#include "../../include/lbmib_surrogate.h"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+#define HIPCHK(x)                                   \
+  do {                                              \
+    if ((x) != hipSuccess) return LBMIB_ERR_HIP;    \
+  } while (0)
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+// round to nearest even (finite inputs; NaN stays NaN)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t *p) {
+  return __builtin_bit_cast(bf16x8, *(const u32x4 *)p);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u}); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__host__ __device__ inline int ilog2(int v) {
+  int r = 0;
+  while ((1 << r) < v) ++r;
+  return r;
+}
+
+// ---------------------------------------------------------------- forward
+//
+// Block: 4 waves stacked along the voxels; a wave owns WM x 16 voxels and
+// WN x 16 output channels (WM x WN accumulator fragments).  The reduction
+// runs over chunks of 32 k; the fragments of chunk c + 1 are loaded while
+// chunk c multiplies.  Operands come straight from global memory (L1/L2):
+// every voxel's 8-channel pieces are 16-byte aligned loads.
+template <int WM, int WN>
+__global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
+                                                  const float *__restrict__ bias, const uint16_t *__restrict__ mask,
+                                                  uint16_t *__restrict__ y, int D, int H, int W, long M, int Cin,
+                                                  int lgCin, int Cout, int K, int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const long mw = ((long)blockIdx.x * 4 + wave) * (WM * 16);
+  const int n0 = blockIdx.y * (WN * 16);
+  const long HW = (long)H * W;
+  int vz[WM], vy[WM], vx[WM];
+  bool vin[WM];
+#pragma unroll
+  for (int i = 0; i < WM; ++i) {
+    const long m = mw + i * 16 + col;
+    vin[i] = m < M;
+    long r = vin[i] ? m : 0;
+    vx[i] = (int)(r % W);
+    r /= W;
+    vy[i] = (int)(r % H);
+    r /= H;
+    vz[i] = (int)(r % D);
+  }
+  f32x4 acc[WN][WM];
+#pragma unroll
+  for (int j = 0; j < WN; ++j)
+#pragma unroll
+    for (int i = 0; i < WM; ++i) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int nch = (K + 31) >> 5;
+  bf16x8 a[WN], b[WM];
+  auto load = [&](int c) {
+    const int k = (c << 5) + (kq << 3);
+    const bool kin = k < K;
+    const int tap = k >> lgCin, ci = k & (Cin - 1);
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int n = n0 + j * 16 + col;
+      a[j] = (kin && n < Cout) ? ld8(w + (long)n * K + k) : zero8();
+    }
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      const int zz = vz[i] + dz, yy = vy[i] + dy, xx = vx[i] + dx;
+      const bool ok = kin && vin[i] && (unsigned)zz < (unsigned)D && (unsigned)yy < (unsigned)H &&
+                      (unsigned)xx < (unsigned)W;
+      const long m = mw + i * 16 + col;
+      b[i] = ok ? ld8(x + (m + dz * HW + (long)dy * W + dx) * Cin + ci) : zero8();
+    }
+  };
+  load(0);
+  for (int c = 0; c < nch; ++c) {
+    bf16x8 ca[WN], cb[WM];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) ca[j] = a[j];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) cb[i] = b[i];
+    if (c + 1 < nch) load(c + 1);
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
+      for (int i = 0; i < WM; ++i) acc[j][i] = mfma16(ca[j], cb[i], acc[j][i]);
+  }
+  // epilogue: lane holds channels n0 + 16 j + 4 kq + r of voxel mw + 16 i + col
+#pragma unroll
+  for (int j = 0; j < WN; ++j) {
+    const int n = n0 + j * 16 + 4 * kq;
+    if (n >= Cout) continue;
+    float bs[4] = {0.f, 0.f, 0.f, 0.f};
+    if (bias)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) bs[r] = bias[n + r];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      if (!vin[i]) continue;
+      const long m = mw + i * 16 + col;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        v[r] = acc[j][i][r] + bs[r];
+        if (relu) v[r] = fmaxf(v[r], 0.f);
+      }
+      if (mask) {
+        const u32x2 mk = *(const u32x2 *)(mask + m * Cout + n);
+        const uint16_t mb[4] = {(uint16_t)(mk[0] & 0xffff), (uint16_t)(mk[0] >> 16), (uint16_t)(mk[1] & 0xffff),
+                                (uint16_t)(mk[1] >> 16)};
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (!(bf2f(mb[r]) > 0.f)) v[r] = 0.f;
+      }
+      u32x2 o;
+      o[0] = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+      o[1] = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+      *(u32x2 *)(y + m * Cout + n) = o;
+    }
+  }
+}
+
+template <int WM, int WN>
+int launch_fwd(const void *x, const void *w, const float *bias, const void *mask, void *y, int D, int H, int W, long M,
+               int Cin, int Cout, int relu, hipStream_t s) {
+  const int K = 27 * Cin;
+  const long bm = 4L * WM * 16;
+  dim3 grid((unsigned)((M + bm - 1) / bm), (unsigned)((Cout + WN * 16 - 1) / (WN * 16)));
+  k_conv_fwd<WM, WN><<<grid, 256, 0, s>>>((const uint16_t *)x, (const uint16_t *)w, bias, (const uint16_t *)mask,
+                                           (uint16_t *)y, D, H, W, M, Cin, ilog2(Cin), Cout, K, relu);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+bool shape_ok(int B, int D, int H, int W, int Cin, int Cout) {
+  if (B < 1 || D < 1 || H < 1 || W < 1) return false;
+  if (Cin < 16 || Cin > 256 || (Cin & (Cin - 1))) return false;
+  if (Cout < 16 || Cout > 512 || Cout % 16) return false;
+  const long M = (long)B * D * H * W;
+  return M * (long)(Cin > Cout ? Cin : Cout) < (1L << 40);
+}
+
+int conv_fwd(const void *x, const void *w, const float *bias, const void *mask, void *y, int B, int D, int H, int W,
+             int Cin, int Cout, int relu, hipStream_t s) {
+  if (!x || !w || !y || !shape_ok(B, D, H, W, Cin, Cout)) return LBMIB_ERR_ARG;
+  const long M = (long)B * D * H * W;
+  // wave tiles: 4 x 16 voxels when the grid is large, fewer when the
+  // voxel count alone cannot fill the device; 64 channels per block
+  // (32 / 16 for the narrow layers)
+  const int wn = Cout % 64 == 0 ? 4 : (Cout % 32 == 0 ? 2 : 1);
+  const long blocks64 = (M + 255) / 256 * (Cout / (16 * wn));
+  if (blocks64 >= 512) {
+    if (wn == 4) return launch_fwd<4, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+    if (wn == 2) return launch_fwd<4, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+    return launch_fwd<4, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+  }
+  if (wn == 4) return launch_fwd<1, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+  if (wn == 2) return launch_fwd<1, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+  return launch_fwd<1, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+}
+
+// wt[ci][26 - t][co] = w[co][t][ci]
+__global__ void k_flip(const uint16_t *__restrict__ w, uint16_t *__restrict__ wt, int Cin, int Cout) {
+  const long n = (long)Cout * 27 * Cin;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int ci = (int)(e % Cin);
+    const long r = e / Cin;
+    const int t = (int)(r % 27), co = (int)(r / 27);
+    wt[((long)ci * 27 + (26 - t)) * Cout + co] = w[e];
+  }
+}
+
+// ----------------------------------------------------------- weight grad
+//
+// dw[co][t][ci] = sum_v dy[v][co] x[v + o(t)][ci]: per tap a GEMM whose
+// reduction runs over voxels.  A block owns a 32 x 32 (co, ci) tile (16 for
+// 16-channel tensors) and all 27 taps, spread over its 4 waves (taps
+// w, w + 4, ...: 7 accumulator tiles per wave), and a contiguous range of
+// 32-voxel chunks (the last one zero-filled past the end).  Per chunk it stages dy[32 voxels][co tile] and, for
+// every tap, x[neighbour][ci tile] in LDS with natural (channel-fastest)
+// 16-byte pieces; the MFMA operands need 8 consecutive voxels per lane, so
+// lanes gather them with 2-byte LDS reads.  A row pitch of 36 elements
+// (72 bytes) puts the four 8-voxel groups of a fragment on distinct banks.
+constexpr int kWgPitch = 36;
+constexpr int kWgChunk = 32;
+
+__global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy,
+                                                    float *__restrict__ part, int D, int H, int W, long M, int Cin,
+                                                    int Cout, int ci_tiles, int chunks_per_split, long nchunks) {
+  __shared__ uint16_t sx[27 * kWgChunk * kWgPitch];
+  __shared__ uint16_t sdy[kWgChunk * kWgPitch];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int ci0 = (blockIdx.x % ci_tiles) * 32, co0 = (blockIdx.x / ci_tiles) * 32;
+  const int nci = Cin - ci0 >= 32 ? 2 : 1;  // 16-channel fragments in the tile
+  const int nco = Cout - co0 >= 32 ? 2 : 1;
+  const long HW = (long)H * W;
+  const int K = 27 * Cin;
+  f32x4 acc[7][2][2];
+#pragma unroll
+  for (int tt = 0; tt < 7; ++tt)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[tt][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const long c_begin = (long)blockIdx.y * chunks_per_split;
+  long c_end = c_begin + chunks_per_split;
+  if (c_end > nchunks) c_end = nchunks;
+  const int xpieces = nci * 2;  // 8-channel pieces per voxel in the ci tile
+  const int npx = 27 * kWgChunk * xpieces;
+  const int npy = kWgChunk * nco * 2;
+  for (long c = c_begin; c < c_end; ++c) {
+    const long v0 = c * kWgChunk;
+    for (int e = tid; e < npx; e += 256) {
+      const int q = e % xpieces;
+      const int r = e / xpieces;
+      const int p = r % kWgChunk, t = r / kWgChunk;
+      const long v = v0 + p;
+      if (v >= M) {  // the last chunk's voxels past the end add nothing
+        uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
+        *(u32x2 *)dst = u32x2{0u, 0u};
+        *(u32x2 *)(dst + 4) = u32x2{0u, 0u};
+        continue;
+      }
+      long rr = v;
+      const int vx = (int)(rr % W);
+      rr /= W;
+      const int vy = (int)(rr % H);
+      rr /= H;
+      const int vz = (int)(rr % D);
+      const int dz = t / 9 - 1, dyy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      u32x4 val = {0u, 0u, 0u, 0u};
+      if ((unsigned)(vz + dz) < (unsigned)D && (unsigned)(vy + dyy) < (unsigned)H && (unsigned)(vx + dx) < (unsigned)W)
+        val = *(const u32x4 *)(x + (v + dz * HW + (long)dyy * W + dx) * Cin + ci0 + q * 8);
+      uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
+      *(u32x2 *)dst = u32x2{val[0], val[1]};
+      *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
+    }
+    for (int e = tid; e < npy; e += 256) {
+      const int q = e % (nco * 2), p = e / (nco * 2);
+      u32x4 val = {0u, 0u, 0u, 0u};
+      if (v0 + p < M) val = *(const u32x4 *)(dy + (v0 + p) * Cout + co0 + q * 8);
+      uint16_t *dst = sdy + p * kWgPitch + q * 8;
+      *(u32x2 *)dst = u32x2{val[0], val[1]};
+      *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
+    }
+    __syncthreads();
+    // A: dy^T fragments (row co = 16 a + col, voxels 8 kq .. + 8)
+    bf16x8 fa[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      uint16_t h[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) h[e] = sdy[(kq * 8 + e) * kWgPitch + a * 16 + col];
+      u32x4 pk;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pk[e] = (uint32_t)h[2 * e] | ((uint32_t)h[2 * e + 1] << 16);
+      fa[a] = __builtin_bit_cast(bf16x8, pk);
+    }
+#pragma unroll
+    for (int tt = 0; tt < 7; ++tt) {
+      const int t = wave + 4 * tt;
+      if (t >= 27) break;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (b >= nci) break;
+        uint16_t h[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) h[e] = sx[(t * kWgChunk + kq * 8 + e) * kWgPitch + b * 16 + col];
+        u32x4 pk;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) pk[e] = (uint32_t)h[2 * e] | ((uint32_t)h[2 * e + 1] << 16);
+        const bf16x8 fb = __builtin_bit_cast(bf16x8, pk);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          if (a >= nco) break;
+          acc[tt][a][b] = mfma16(fa[a], fb, acc[tt][a][b]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // partials: lane holds (co = co0 + 16 a + 4 kq + r, ci = ci0 + 16 b + col)
+  float *dst = part + (long)blockIdx.y * Cout * K;
+#pragma unroll
+  for (int tt = 0; tt < 7; ++tt) {
+    const int t = wave + 4 * tt;
+    if (t >= 27) break;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      if (a >= nco) break;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (b >= nci) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int co = co0 + a * 16 + 4 * kq + r, ci = ci0 + b * 16 + col;
+          dst[(long)co * K + t * Cin + ci] = acc[tt][a][b][r];
+        }
+      }
+    }
+  }
+}
+
+// out[e] = sum over s < ns of part[s][e], in order of s
+__global__ void k_sum_parts(const float *__restrict__ part, float *__restrict__ out, long n, int ns) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int k = 0; k < ns; ++k) s += part[(long)k * n + e];
+    out[e] = s;
+  }
+}
+
+// bias gradient partials: part[blk][co] = sum of dy[v][co] over the block's
+// voxel range (256 threads: 16 channel lanes x 16 voxel lanes)
+__global__ void __launch_bounds__(256) k_colsum(const uint16_t *__restrict__ dy, float *__restrict__ part, long M,
+                                                int C, long per_block) {
+  __shared__ float red[16][17];
+  const int cl = threadIdx.x & 15, vl = threadIdx.x >> 4;
+  const long v0 = (long)blockIdx.x * per_block;
+  long v1 = v0 + per_block;
+  if (v1 > M) v1 = M;
+  for (int c0 = 0; c0 < C; c0 += 16) {
+    float s = 0.f;
+    for (long v = v0 + vl; v < v1; v += 16) s += bf2f(dy[v * C + c0 + cl]);
+    red[vl][cl] = s;
+    __syncthreads();
+    if (vl == 0) {
+      float t = 0.f;
+      for (int k = 0; k < 16; ++k) t += red[k][cl];
+      part[(long)blockIdx.x * C + c0 + cl] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Split of the voxel reduction: chunks per split, chosen from the shape only
+// (so every entry point that shares it gives the same bits), capped so that
+// the partials fit in `cap_bytes`.
+struct WgradPlan {
+  int ci_tiles, co_tiles, splits, cps;
+  long nchunks;
+  size_t part_bytes;
+};
+
+WgradPlan wgrad_plan(long M, int Cin, int Cout, size_t cap_bytes) {
+  WgradPlan p;
+  p.ci_tiles = (Cin + 31) / 32;
+  p.co_tiles = (Cout + 31) / 32;
+  p.nchunks = (M + kWgChunk - 1) / kWgChunk;
+  const int tiles = p.ci_tiles * p.co_tiles;
+  const size_t per_split = (size_t)Cout * 27 * Cin * sizeof(float);
+  long want = (1024 + tiles - 1) / tiles;  // about 1024 blocks
+  if (want > p.nchunks) want = p.nchunks;
+  long cap = (long)(cap_bytes / per_split);
+  if (want > cap) want = cap;
+  if (want < 1) want = 1;
+  p.cps = (int)((p.nchunks + want - 1) / want);
+  p.splits = (int)((p.nchunks + p.cps - 1) / p.cps);
+  p.part_bytes = per_split * p.splits;
+  return p;
+}
+
+constexpr int kColsumBlocks = 256;
+
+int wgrad_with(const void *x, const void *dy, float *dw, float *db, int B, int D, int H, int W, int Cin, int Cout,
+               void *ws, size_t ws_bytes, hipStream_t s) {
+  const long M = (long)B * D * H * W;
+  const WgradPlan p = wgrad_plan(M, Cin, Cout, ws_bytes);
+  if (p.part_bytes > ws_bytes || p.nchunks < 1) return LBMIB_ERR_ARG;
+  float *part = (float *)ws;
+  k_conv_wgrad<<<dim3(p.ci_tiles * p.co_tiles, p.splits), 256, 0, s>>>(
+      (const uint16_t *)x, (const uint16_t *)dy, part, D, H, W, M, Cin, Cout, p.ci_tiles, p.cps, p.nchunks);
+  HIPCHK(hipGetLastError());
+  const long n = (long)Cout * 27 * Cin;
+  k_sum_parts<<<(unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096), 256, 0, s>>>(part, dw, n, p.splits);
+  HIPCHK(hipGetLastError());
+  if (db) {
+    if ((size_t)kColsumBlocks * Cout * sizeof(float) > ws_bytes) return LBMIB_ERR_ARG;
+    const long per = (M + kColsumBlocks - 1) / kColsumBlocks;
+    const int nb = (int)((M + per - 1) / per);
+    k_colsum<<<nb, 256, 0, s>>>((const uint16_t *)dy, part, M, Cout, per);
+    HIPCHK(hipGetLastError());
+    k_sum_parts<<<1, 256, 0, s>>>(part, db, Cout, nb);
+    HIPCHK(hipGetLastError());
+  }
+  return LBMIB_OK;
+}
+
+// The library's workspace: one buffer per device, allocated on first use.
+constexpr int kMaxDevices = 64;
+void *g_ws[kMaxDevices] = {};
+std::mutex g_ws_mu;
+
+int workspace(void **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return LBMIB_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  if (!g_ws[dev]) {
+    void *p = nullptr;
+    if (hipMalloc(&p, LBMIB_SG_WORKSPACE_BYTES) != hipSuccess) return LBMIB_ERR_ALLOC;
+    g_ws[dev] = p;
+  }
+  *out = g_ws[dev];
+  return LBMIB_OK;
+}
+
+// ------------------------------------------------------------ elementwise
+//
+// One thread per (voxel, 8-channel piece); C is a multiple of 16.
+
+__global__ void k_pool2(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, int D, int H, int W, int C, long n) {
+  const int P = C / 8;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(e % P);
+    long r = e / P;
+    const int ox = (int)(r % W);
+    r /= W;
+    const int oy = (int)(r % H);
+    r /= H;
+    const int oz = (int)(r % D);
+    const long b = r / D;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const long iz = 2 * oz + (c >> 2), iy = 2 * oy + ((c >> 1) & 1), ix = 2 * ox + (c & 1);
+      const long v = ((b * 2 * D + iz) * 2 * H + iy) * 2 * W + ix;
+      const u32x4 val = *(const u32x4 *)(x + v * C + q * 8);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        acc[2 * k] += bf2f((uint16_t)(val[k] & 0xffff));
+        acc[2 * k + 1] += bf2f((uint16_t)(val[k] >> 16));
+      }
+    }
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = (uint32_t)f2bf(acc[2 * k] * 0.125f) | ((uint32_t)f2bf(acc[2 * k + 1] * 0.125f) << 16);
+    *(u32x4 *)(y + e * 8) = o;
+  }
+}
+
+__device__ __forceinline__ void unpack8(u32x4 v, float f[8]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f[2 * k] = bf2f((uint16_t)(v[k] & 0xffff));
+    f[2 * k + 1] = bf2f((uint16_t)(v[k] >> 16));
+  }
+}
+__device__ __forceinline__ u32x4 pack8(const float f[8]) {
+  u32x4 o;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+  return o;
+}
+
+// voxel index of the parent (D, H, W fine dims) of fine voxel v
+__device__ __forceinline__ long parent_of(long v, int D, int H, int W) {
+  const int x = (int)(v % W);
+  long r = v / W;
+  const int y = (int)(r % H);
+  r /= H;
+  const int z = (int)(r % D);
+  const long b = r / D;
+  return ((b * (D / 2) + z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2;
+}
+
+__global__ void k_up2_add(const uint16_t *__restrict__ c, const uint16_t *__restrict__ e, uint16_t *__restrict__ s,
+                          int D, int H, int W, int C, long n) {
+  const int P = C / 8;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(i % P);
+    const long v = i / P;
+    const long u = parent_of(v, D, H, W);
+    float a[8], b[8];
+    unpack8(*(const u32x4 *)(c + u * C + q * 8), a);
+    if (e) {
+      unpack8(*(const u32x4 *)(e + i * 8), b);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] += b[k];
+    }
+    *(u32x4 *)(s + i * 8) = pack8(a);
+  }
+}
+
+__global__ void k_pool2_bwd(const uint16_t *__restrict__ a, const uint16_t *__restrict__ g,
+                            const uint16_t *__restrict__ mask, uint16_t *__restrict__ out, int D, int H, int W, int C,
+                            long n) {
+  const int P = C / 8;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(i % P);
+    const long v = i / P;
+    const long u = parent_of(v, D, H, W);
+    float f[8], t[8];
+    unpack8(*(const u32x4 *)(g + u * C + q * 8), f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] *= 0.125f;
+    if (a) {
+      unpack8(*(const u32x4 *)(a + i * 8), t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) f[k] += t[k];
+    }
+    if (mask) {
+      unpack8(*(const u32x4 *)(mask + i * 8), t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (!(t[k] > 0.f)) f[k] = 0.f;
+    }
+    *(u32x4 *)(out + i * 8) = pack8(f);
+  }
+}
+
+__global__ void k_up2_bwd(const uint16_t *__restrict__ ds, const uint16_t *__restrict__ mask,
+                          uint16_t *__restrict__ out, int D, int H, int W, int C, long n) {
+  const int P = C / 8;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(i % P);
+    long r = i / P;
+    const int ox = (int)(r % W);
+    r /= W;
+    const int oy = (int)(r % H);
+    r /= H;
+    const int oz = (int)(r % D);
+    const long b = r / D;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, t[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const long iz = 2 * oz + (c >> 2), iy = 2 * oy + ((c >> 1) & 1), ix = 2 * ox + (c & 1);
+      const long v = ((b * 2 * D + iz) * 2 * H + iy) * 2 * W + ix;
+      unpack8(*(const u32x4 *)(ds + v * C + q * 8), t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += t[k];
+    }
+    if (mask) {
+      unpack8(*(const u32x4 *)(mask + i * 8), t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (!(t[k] > 0.f)) acc[k] = 0.f;
+    }
+    *(u32x4 *)(out + i * 8) = pack8(acc);
+  }
+}
+
+// loss partials (per block) and the gradient
+__global__ void __launch_bounds__(256) k_mse(const uint16_t *__restrict__ out, const float *__restrict__ tgt,
+                                             uint16_t *__restrict__ dout, float *__restrict__ part, long M, int C,
+                                             int nl, float gscale) {
+  __shared__ float red[256];
+  float s = 0.f;
+  const long n = M * C;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const long v = e / C;
+    float gval = 0.f;
+    if (c < nl) {
+      const float d = bf2f(out[e]) - tgt[v * nl + c];
+      s += d * d;
+      gval = gscale * d;
+    }
+    dout[e] = f2bf(gval);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_final_sum(const float *__restrict__ part, int n, float scale, float *__restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    float s = 0.f;
+    for (int k = 0; k < n; ++k) s += part[k];
+    *out = s * scale;
+  }
+}
+
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       uint16_t *__restrict__ w16, long n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const float gr = g[e];
+    const float mm = b1 * m[e] + (1.f - b1) * gr;
+    const float vv = b2 * v[e] + (1.f - b2) * gr * gr;
+    m[e] = mm;
+    v[e] = vv;
+    const float pn = p[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    p[e] = pn;
+    if (w16) w16[e] = f2bf(pn);
+  }
+}
+
+__global__ void k_cast(const float *__restrict__ src, uint16_t *__restrict__ dst, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    dst[e] = f2bf(src[e]);
+}
+
+struct Norm16 {
+  float a[16], b[16];
+};
+
+__global__ void k_pack(const float *__restrict__ f, int nf, Norm16 nm, int nconst, uint16_t *__restrict__ x, long M,
+                       int C) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < M; v += (long)gridDim.x * blockDim.x) {
+    for (int c0 = 0; c0 < C; c0 += 8) {
+      float t[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int c = c0 + k;
+        float val = 0.f;
+        if (c < nf)
+          val = (f[(long)c * M + v] - nm.a[c]) * nm.b[c];
+        else if (c < nf + nconst)
+          val = nm.a[c];
+        t[k] = val;
+      }
+      *(u32x4 *)(x + v * C + c0) = pack8(t);
+    }
+  }
+}
+
+__global__ void k_target(const float *__restrict__ f0, const float *__restrict__ f1, int nl, Norm16 nm,
+                         float *__restrict__ tgt, long M) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < M; v += (long)gridDim.x * blockDim.x)
+    for (int c = 0; c < nl; ++c) {
+      const long i = (long)c * M + v;
+      tgt[v * nl + c] = ((f1[i] - f0[i]) - nm.a[c]) * nm.b[c];
+    }
+}
+
+__global__ void k_unpack(const uint16_t *__restrict__ out, int C, int nl, Norm16 nm, float *__restrict__ d, long M) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < M; v += (long)gridDim.x * blockDim.x)
+    for (int c = 0; c < nl; ++c) d[(long)c * M + v] = bf2f(out[v * C + c]) * nm.b[c] + nm.a[c];
+}
+
+__global__ void __launch_bounds__(256) k_sqdiff(const float *__restrict__ a, const float *__restrict__ b, long n,
+                                                double *__restrict__ part) {
+  __shared__ double r0[256], r1[256];
+  double s0 = 0.0, s1 = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const double av = a[e];
+    if (b) {
+      const double bv = b[e];
+      s0 += (av - bv) * (av - bv);
+      s1 += bv * bv;
+    } else {
+      s0 += av * av;
+    }
+  }
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r0[threadIdx.x] += r0[threadIdx.x + w];
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0[0];
+    part[2 * blockIdx.x + 1] = r1[0];
+  }
+}
+
+__global__ void k_sqdiff_final(const double *__restrict__ part, int nb, double *__restrict__ out2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < nb; ++k) {
+      s0 += part[2 * k];
+      s1 += part[2 * k + 1];
+    }
+    out2[0] = s0;
+    out2[1] = s1;
+  }
+}
+
+unsigned grid_for(long n, int per_block = 256, long cap = 8192) {
+  long g = (n + per_block - 1) / per_block;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+bool norm_from(const double *a, const double *b, int n, Norm16 &nm) {
+  if (n > 16) return false;
+  for (int k = 0; k < 16; ++k) {
+    nm.a[k] = (k < n && a) ? (float)a[k] : 0.f;
+    nm.b[k] = (k < n && b) ? (float)b[k] : 0.f;
+  }
+  return true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+
+extern "C" int lbmib_sg_conv3d_fwd(const void *x, const void *w, const float *bias, const void *mask, void *y, int B,
+                                   int D, int H, int W, int Cin, int Cout, int relu, void *stream) {
+  return conv_fwd(x, w, bias, mask, y, B, D, H, W, Cin, Cout, relu, (hipStream_t)stream);
+}
+
+extern "C" int lbmib_sg_conv3d_bwd_data(const void *dy, const void *w, void *wt, const void *mask, void *dx, int B,
+                                        int D, int H, int W, int Cin, int Cout, void *stream) {
+  if (!dy || !w || !wt || !dx) return LBMIB_ERR_ARG;
+  // the data gradient convolves dy (Cout channels) with wt into Cin channels
+  if (!shape_ok(B, D, H, W, Cout, Cin) || Cin % 16) return LBMIB_ERR_ARG;
+  if (Cout & (Cout - 1)) return LBMIB_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const long n = (long)Cout * 27 * Cin;
+  k_flip<<<grid_for(n), 256, 0, s>>>((const uint16_t *)w, (uint16_t *)wt, Cin, Cout);
+  HIPCHK(hipGetLastError());
+  return conv_fwd(dy, wt, nullptr, mask, dx, B, D, H, W, Cout, Cin, 0, s);
+}
+
+extern "C" int lbmib_sg_conv3d_bwd_weight(const void *x, const void *dy, float *dw, float *db, int B, int D, int H,
+                                          int W, int Cin, int Cout, void *stream) {
+  if (!x || !dy || !dw || !shape_ok(B, D, H, W, Cin, Cout)) return LBMIB_ERR_ARG;
+  void *ws = nullptr;
+  const int rc = workspace(&ws);
+  if (rc) return rc;
+  return wgrad_with(x, dy, dw, db, B, D, H, W, Cin, Cout, ws, LBMIB_SG_WORKSPACE_BYTES, (hipStream_t)stream);
+}
+
+extern "C" int lbmib_sg_pool2(const void *x, void *y, int B, int D, int H, int W, int C, void *stream) {
+  if (!x || !y || B < 1 || D < 1 || H < 1 || W < 1 || C < 8 || C % 8) return LBMIB_ERR_ARG;
+  const long n = (long)B * D * H * W * (C / 8);
+  k_pool2<<<grid_for(n), 256, 0, (hipStream_t)stream>>>((const uint16_t *)x, (uint16_t *)y, D, H, W, C, n);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_up2_add(const void *c, const void *e, void *s, int B, int D, int H, int W, int C,
+                                void *stream) {
+  if (!c || !s || B < 1 || D < 2 || H < 2 || W < 2 || (D | H | W) & 1 || C < 8 || C % 8) return LBMIB_ERR_ARG;
+  const long n = (long)B * D * H * W * (C / 8);
+  k_up2_add<<<grid_for(n), 256, 0, (hipStream_t)stream>>>((const uint16_t *)c, (const uint16_t *)e, (uint16_t *)s, D,
+                                                          H, W, C, n);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_pool2_bwd(const void *a, const void *g, const void *mask, void *out, int B, int D, int H, int W,
+                                  int C, void *stream) {
+  if (!g || !out || B < 1 || D < 2 || H < 2 || W < 2 || (D | H | W) & 1 || C < 8 || C % 8) return LBMIB_ERR_ARG;
+  const long n = (long)B * D * H * W * (C / 8);
+  k_pool2_bwd<<<grid_for(n), 256, 0, (hipStream_t)stream>>>((const uint16_t *)a, (const uint16_t *)g,
+                                                            (const uint16_t *)mask, (uint16_t *)out, D, H, W, C, n);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_up2_bwd(const void *ds, const void *mask, void *out, int B, int D, int H, int W, int C,
+                                void *stream) {
+  if (!ds || !out || B < 1 || D < 1 || H < 1 || W < 1 || C < 8 || C % 8) return LBMIB_ERR_ARG;
+  const long n = (long)B * D * H * W * (C / 8);
+  k_up2_bwd<<<grid_for(n), 256, 0, (hipStream_t)stream>>>((const uint16_t *)ds, (const uint16_t *)mask,
+                                                          (uint16_t *)out, D, H, W, C, n);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_mse(const void *out, const float *tgt, void *dout, float *loss, long M, int C, int nl,
+                            void *stream) {
+  if (!out || !tgt || !dout || !loss || M < 1 || C < 1 || nl < 1 || nl > C) return LBMIB_ERR_ARG;
+  void *ws = nullptr;
+  const int rc = workspace(&ws);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned nb = grid_for(M * C, 256, 1024);
+  const float denom = (float)((double)M * nl);
+  k_mse<<<nb, 256, 0, s>>>((const uint16_t *)out, tgt, (uint16_t *)dout, (float *)ws, M, C, nl, 2.0f / denom);
+  HIPCHK(hipGetLastError());
+  k_final_sum<<<1, 64, 0, s>>>((const float *)ws, (int)nb, 1.0f / denom, loss);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_adam(float *p, const float *g, float *m, float *v, void *w16, long n, float lr, float beta1,
+                             float beta2, float eps, int t, void *stream) {
+  if (!p || !g || !m || !v || n < 1 || t < 1) return LBMIB_ERR_ARG;
+  const float c1 = 1.f - powf(beta1, (float)t), c2 = 1.f - powf(beta2, (float)t);
+  k_adam<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(p, g, m, v, (uint16_t *)w16, n, lr, beta1, beta2, eps, c1, c2);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_cast_bf16(const float *src, void *dst, long n, void *stream) {
+  if (!src || !dst || n < 1) return LBMIB_ERR_ARG;
+  k_cast<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(src, (uint16_t *)dst, n);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_pack(const float *f, int nf, const double *mean, const double *istd, int nconst,
+                             const double *cval, void *x, long M, int C, void *stream) {
+  if (!x || M < 1 || C % 8 || nf < 0 || nconst < 0 || nf + nconst > C || nf + nconst > 16 || (nf && !f))
+    return LBMIB_ERR_ARG;
+  Norm16 nm;
+  for (int k = 0; k < 16; ++k) nm.a[k] = nm.b[k] = 0.f;
+  for (int k = 0; k < nf; ++k) {
+    nm.a[k] = mean ? (float)mean[k] : 0.f;
+    nm.b[k] = istd ? (float)istd[k] : 1.f;
+  }
+  for (int k = 0; k < nconst; ++k) nm.a[nf + k] = cval ? (float)cval[k] : 0.f;
+  k_pack<<<grid_for(M), 256, 0, (hipStream_t)stream>>>(f, nf, nm, nconst, (uint16_t *)x, M, C);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_target(const float *f0, const float *f1, int nl, const double *mean, const double *istd,
+                               float *tgt, long M, void *stream) {
+  Norm16 nm;
+  if (!f0 || !f1 || !tgt || nl < 1 || M < 1 || !norm_from(mean, istd, nl, nm)) return LBMIB_ERR_ARG;
+  k_target<<<grid_for(M), 256, 0, (hipStream_t)stream>>>(f0, f1, nl, nm, tgt, M);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_unpack(const void *out, int C, int nl, const double *mean, const double *std, float *d, long M,
+                               void *stream) {
+  Norm16 nm;
+  if (!out || !d || nl < 1 || nl > C || M < 1 || !norm_from(mean, std, nl, nm)) return LBMIB_ERR_ARG;
+  k_unpack<<<grid_for(M), 256, 0, (hipStream_t)stream>>>((const uint16_t *)out, C, nl, nm, d, M);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_sg_sqdiff(const float *a, const float *b, long n, double *out2, void *stream) {
+  if (!a || !out2 || n < 1) return LBMIB_ERR_ARG;
+  void *ws = nullptr;
+  const int rc = workspace(&ws);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned nb = grid_for(n, 256, 1024);
+  k_sqdiff<<<nb, 256, 0, s>>>(a, b, n, (double *)ws);
+  HIPCHK(hipGetLastError());
+  k_sqdiff_final<<<1, 64, 0, s>>>((const double *)ws, (int)nb, out2);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}

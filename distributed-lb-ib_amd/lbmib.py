@@ -40,7 +40,8 @@ ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lb
                "lbmib_batch_create", "lbmib_batch_destroy", "lbmib_batch_instance", "lbmib_batch_init",
                "lbmib_batch_step", "lbmib_batch_set_tile", "lbmib_batch_set_overlap", "lbmib_batch_cells",
                "lbmib_batch_status",
-               "lbmib_batch_time_steps") + tuple("lbmib_" + n for n in CTX_CALLS)
+               "lbmib_batch_time_steps", "lbmib_coarse_moments", "lbmib_coarse_sheets", "lbmib_jump",
+               "lbmib_jump_commit", "lbmib_jump_revert", "lbmib_get_stream") + tuple("lbmib_" + n for n in CTX_CALLS)
 
 
 class LbmibError(RuntimeError):
@@ -105,6 +106,12 @@ def load_library(path=LIB_PATH):
     L.lbmib_batch_cells.argtypes = [vp, dp, ip]
     L.lbmib_batch_status.argtypes = [vp]
     L.lbmib_batch_time_steps.argtypes = [vp, i, i, dp, dp]
+    L.lbmib_coarse_moments.argtypes = [vp, i, vp]
+    L.lbmib_coarse_sheets.argtypes = [vp, i, vp]
+    L.lbmib_jump.argtypes = [vp, i, vp, vp, i]
+    L.lbmib_jump_commit.argtypes = [vp]
+    L.lbmib_jump_revert.argtypes = [vp]
+    L.lbmib_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
     for n in CTX_CALLS:
         getattr(L, "lbmib_" + n).argtypes = [vp]
     for n in ALL_SYMBOLS:
@@ -345,6 +352,31 @@ class Engine:
         out = ctypes.c_double()
         self._check(self.L.lbmib_timed_kernel_cells(self.h, ctypes.byref(out)), "timed_kernel_cells")
         return out.value
+
+    # ------------------------------------------------ surrogate coupling
+    # d_* arguments are device addresses (ints, e.g. a torch tensor's
+    # data_ptr()) of FP32 planes [c][nz/f][ny/f][nx/f]
+    def coarse_moments(self, factor, d_out):
+        self._check(self.L.lbmib_coarse_moments(self.h, factor, ctypes.c_void_p(d_out)), "coarse_moments")
+
+    def coarse_sheets(self, factor, d_out):
+        self._check(self.L.lbmib_coarse_sheets(self.h, factor, ctypes.c_void_p(d_out)), "coarse_sheets")
+
+    def jump(self, factor, d_coarse, d_delta, ksteps):
+        self._check(self.L.lbmib_jump(self.h, factor, ctypes.c_void_p(d_coarse), ctypes.c_void_p(d_delta), ksteps),
+                    "jump")
+
+    def stream_handle(self):
+        """The context's HIP stream as an int (for torch.cuda.ExternalStream)."""
+        h = ctypes.c_void_p()
+        self._check(self.L.lbmib_get_stream(self.h, ctypes.byref(h)), "get_stream")
+        return h.value or 0
+
+    def jump_commit(self):
+        self._check(self.L.lbmib_jump_commit(self.h), "jump_commit")
+
+    def jump_revert(self):
+        self._check(self.L.lbmib_jump_revert(self.h), "jump_revert")
 
     def bytes_per_cell(self):
         b = ctypes.c_double()

@@ -1,0 +1,209 @@
+"""Surrogate-accelerated stepping of a running simulation: jumps of K steps
+predicted by the encoder-decoder (surrogate.Net), each re-verified by the
+plain solver, with fallback to plain steps when the verification fails.
+
+Not in the reference: cube_pthread/lbm_ib.c runs every step of its do_thread
+loop (lbm_ib.c:3115-3563).  One jump cycle here:
+  1. the coarse state at t: rho/u of compute_rho_and_u (lbm_ib.c:2440-2538)
+     averaged over f^3 blocks and the sheet raster (node count, elastic
+     force sums) -- lbmib_coarse_moments / lbmib_coarse_sheets;
+  2. the network predicts the change of the four fluid planes over K steps;
+  3. lbmib_jump shifts every cell's equilibrium by the interpolated change,
+     advects the sheet, keeps the pre-jump state aside and sets t += K;
+  4. re-verification: R plain steps (they count as simulated steps); the
+     coarse velocity change over them, per step, divided by the jump's own
+     mean change per step, is the verification ratio.  A state the plain
+     dynamics accept changes at about the rate the prediction implies; a
+     jump that leaves it inconsistent (pressure waves, a broken
+     non-equilibrium part) changes much faster.  Ratio <= tol: commit;
+     otherwise lbmib_jump_revert and K + R plain steps from the checkpoint,
+     which reproduce the plain trajectory bit for bit.
+
+Every kernel runs on the engine's stream; PyTorch only allocates memory.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+import lbmib
+import surrogate as sg
+
+NF = 8        # feature planes: rho, ux, uy, uz, node count, elastic force x, y, z
+NOUT = 4      # predicted planes: the change of rho, ux, uy, uz
+NCONST = 2    # constant channels: viscosity and inflow speed
+CIN = 16      # network input channels (NF + NCONST, zero padded)
+
+
+def const_channels(tau, u_in):
+    """Viscosity (tau - 1/2)/3 and inflow speed, centred and scaled to O(1)."""
+    nu = (tau - 0.5) / 3.0
+    return [(nu - 0.05) / 0.03, (u_in - 0.01) / 0.004]
+
+
+class Norm:
+    """Per-plane shift and scale of the inputs and of the predicted change."""
+
+    def __init__(self, in_mean, in_std, out_mean, out_std):
+        self.in_mean = [float(v) for v in in_mean]
+        self.in_std = [float(v) for v in in_std]
+        self.out_mean = [float(v) for v in out_mean]
+        self.out_std = [float(v) for v in out_std]
+
+    def arrays(self):
+        return {"in_mean": np.array(self.in_mean), "in_std": np.array(self.in_std),
+                "out_mean": np.array(self.out_mean), "out_std": np.array(self.out_std)}
+
+    @classmethod
+    def from_arrays(cls, a):
+        return cls(a["in_mean"], a["in_std"], a["out_mean"], a["out_std"])
+
+
+def load_model(path):
+    """Net and Norm from an .npz written by surrogate_train (no pickle)."""
+    with np.load(path, allow_pickle=False) as z:
+        st = {k: z[k] for k in z.files}
+    chans = tuple(int(c) for c in st.pop("chans"))
+    factor = int(st.pop("factor"))
+    ksteps = int(st.pop("ksteps"))
+    norm = Norm.from_arrays(st)
+    net = sg.Net(cin=CIN, cout=16, nout=NOUT, chans=chans)
+    net.load_state(st)
+    tol = float(st["verify_tol"]) if "verify_tol" in st else None
+    return net, norm, factor, ksteps, tol
+
+
+def save_model(path, net, norm, factor, ksteps, extra=None):
+    st = net.state()
+    st.update(norm.arrays())
+    st["chans"] = np.array(net.chans)
+    st["factor"] = np.array(factor)
+    st["ksteps"] = np.array(ksteps)
+    for k, v in (extra or {}).items():
+        st[k] = np.asarray(v)
+    np.savez(path, **st)
+
+
+class Coarse:
+    """Device feature planes [NF][nc] of one engine at coarsening factor f."""
+
+    def __init__(self, engine, factor):
+        self.e, self.f = engine, factor
+        self.dims = (engine.nz // factor, engine.ny // factor, engine.nx // factor)
+        self.nc = int(np.prod(self.dims))
+        self.feat = torch.empty((NF, self.nc), dtype=torch.float32, device="cuda")
+
+    def read(self, out=None):
+        """Fills out (default: self.feat) with the engine's current state."""
+        t = self.feat if out is None else out
+        self.e.coarse_moments(self.f, t[0].data_ptr())
+        self.e.coarse_sheets(self.f, t[4].data_ptr())
+        return t
+
+
+def pack_input(feat, norm, consts, x, nc, stream):
+    """Network input [.., CIN] (BF16) of feature planes feat [NF][nc]."""
+    L = sg._bind()
+    istd = [1.0 / s for s in norm.in_std]
+    sg.check(L.lbmib_sg_pack(ctypes.c_void_p(feat.data_ptr()), NF, sg.dvec(norm.in_mean), sg.dvec(istd), NCONST,
+                             sg.dvec(consts), ctypes.c_void_p(x.data_ptr()), nc, CIN, stream), "pack")
+
+
+class JumpRunner:
+    """Steps one engine with surrogate jumps (see the module docstring).
+
+    tol: acceptance bound of the verification ratio; 0 rejects every jump
+    (all plain steps, bit-identical to lbmib_step)."""
+
+    def __init__(self, engine, net, norm, tau, u_in, factor=4, ksteps=50, verify_steps=1, tol=4.0):
+        self.e, self.net, self.norm = engine, net, norm
+        self.K, self.R, self.tol = ksteps, verify_steps, tol
+        self.consts = const_channels(tau, u_in)
+        self.stream_handle = engine.stream_handle()
+        self.stream = ctypes.c_void_p(self.stream_handle)
+        self.ext = torch.cuda.ExternalStream(self.stream_handle)
+        self.co = Coarse(engine, factor)
+        nz, ny, nx = self.co.dims
+        self.x = torch.empty((1, nz, ny, nx, CIN), dtype=torch.bfloat16, device="cuda")
+        self.delta = torch.empty((NOUT, self.co.nc), dtype=torch.float32, device="cuda")
+        self.after = torch.empty((NF, self.co.nc), dtype=torch.float32, device="cuda")
+        self.sums = torch.zeros(4, dtype=torch.float64, device="cuda")
+        self.stats = {"jumps": 0, "accepted": 0, "rejected": 0, "plain_steps": 0, "ratios": []}
+        self.have_feat = False
+
+    def predict(self):
+        """delta = the network's change of (rho, ux, uy, uz) from self.co.feat."""
+        L = sg._bind()
+        with torch.cuda.stream(self.ext):
+            pack_input(self.co.feat, self.norm, self.consts, self.x, self.co.nc, self.stream)
+            out = self.net.forward(self.x)
+            sg.check(L.lbmib_sg_unpack(ctypes.c_void_p(out.data_ptr()), 16, NOUT, sg.dvec(self.norm.out_mean),
+                                       sg.dvec(self.norm.out_std), ctypes.c_void_p(self.delta.data_ptr()), self.co.nc,
+                                       self.stream), "unpack")
+        return self.delta
+
+    def _sqdiff(self, a, b, out):
+        L = sg._bind()
+        sg.check(L.lbmib_sg_sqdiff(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(0 if b is None else b.data_ptr()),
+                                   a.numel(), ctypes.c_void_p(out.data_ptr()), self.stream), "sqdiff")
+
+    def cycle(self, predictor=None):
+        """One jump of K steps plus R verification steps (K + R simulated
+        steps), or K + R plain steps after a rejected jump.  predictor: a
+        function filling self.delta instead of the network (None: the
+        network).  Returns True if the jump was accepted."""
+        with torch.cuda.stream(self.ext):
+            return self._cycle(predictor)
+
+    def _cycle(self, predictor):
+        e = self.e
+        if not self.have_feat:
+            self.co.read()
+            self.have_feat = True
+        if predictor is None:
+            self.predict()
+        else:
+            predictor(self)
+        e.jump(self.co.f, self.co.feat.data_ptr(), self.delta.data_ptr(), self.K)
+        self.co.read(self.after)                      # coarse state just after the jump
+        e.step(self.R)
+        self.co.read()                                # after R plain steps: the next cycle's input
+        # verification ratio: |du over R steps| / R  against  |du of the jump| / K
+        self._sqdiff(self.co.feat[1:4], self.after[1:4], self.sums[0:2])
+        self._sqdiff(self.delta[1:4], None, self.sums[2:4])
+        s = self.sums.cpu().numpy()                   # on the engine's stream: waits for it
+        step_rate = np.sqrt(s[0]) / self.R
+        jump_rate = np.sqrt(s[2]) / self.K
+        ratio = float(step_rate / max(jump_rate, 1e-30))
+        self.stats["jumps"] += 1
+        self.stats["ratios"].append(ratio)
+        if np.isfinite(ratio) and ratio <= self.tol and e.status() == 0:
+            e.jump_commit()
+            self.stats["accepted"] += 1
+            return True
+        e.jump_revert()
+        e.step(self.K + self.R)
+        self.stats["rejected"] += 1
+        self.stats["plain_steps"] += self.K + self.R
+        self.co.read()
+        return False
+
+    def run(self, nsteps, predictor=None):
+        """Advances nsteps simulated steps: whole jump cycles, then plain steps
+        for the remainder."""
+        done = 0
+        while done + self.K + self.R <= nsteps:
+            self.cycle(predictor)
+            done += self.K + self.R
+        if done < nsteps:
+            self.e.step(nsteps - done)
+            self.stats["plain_steps"] += nsteps - done
+            self.have_feat = False
+        return self.stats
+
+
+def zero_predictor(runner):
+    """The method without the network: no change of the coarse state over
+    the jump (the sheet still moves with the coarse velocity).  Called inside
+    JumpRunner.cycle, on the engine's stream."""
+    runner.delta.zero_()

@@ -226,6 +226,40 @@ int lbmib_batch_status(lbmib_batch *b);
  * the groups, in milliseconds). */
 int lbmib_batch_time_steps(lbmib_batch *b, int nwarm, int nsteps, double *ms_total, double *ms_fused_mean);
 
+/* ---- coupling of the learned surrogate (not in the reference) ----
+ * The reference steps every time step (do_thread, lbm_ib.c:3115-3563).  These
+ * calls let a caller read the coarse-grained state, replace K steps by a
+ * predicted change of it (include/lbmib_surrogate.h trains and runs the
+ * network), and undo the jump when the plain solver's re-verification
+ * rejects it.  Coarse fields are device FP32 planes [c][nz/f][ny/f][nx/f];
+ * f in {1, 2, 4}; nx a multiple of 64, ny and nz of 4. */
+/* c = 0..3: rho, ux, uy, uz of lbmib_get_macroscopic (compute_rho_and_u,
+ * lbm_ib.c:2440-2538) averaged over each f^3 block. */
+int lbmib_coarse_moments(lbmib_ctx *ctx, int factor, float *d_out);
+/* c = 0..3: number of fiber nodes in each coarse cell and the sums of their
+ * elastic forces (compute_elasticforce, lbm_ib.c:1285-1318); deterministic. */
+int lbmib_coarse_sheets(lbmib_ctx *ctx, int factor, float *d_out);
+/* Advances the simulation by ksteps (even, >= 2) steps without stepping:
+ * d_coarse is lbmib_coarse_moments of the current state, d_delta the
+ * predicted change of those four planes over ksteps.  Every updated cell
+ * keeps its non-equilibrium part and takes the equilibrium of its moments
+ * plus the change interpolated trilinearly at the cell; fiber nodes move by
+ * ksteps times the mean of the coarse velocity before and after the jump at
+ * the node.  The pre-jump state is kept until lbmib_jump_commit or
+ * lbmib_jump_revert; steps may run in between (the re-verification).  Needs
+ * unsplit steps and at least one completed step. */
+int lbmib_jump(lbmib_ctx *ctx, int factor, const float *d_coarse, const float *d_delta, int ksteps);
+/* The context's HIP stream (hipStream_t as void *), on which every entry
+ * point enqueues; work of other libraries placed on it is ordered with the
+ * simulation's (e.g. the surrogate's kernels between lbmib_coarse_moments
+ * and lbmib_jump). */
+int lbmib_get_stream(lbmib_ctx *ctx, void **stream);
+/* Accepts the last jump (drops its checkpoint). */
+int lbmib_jump_commit(lbmib_ctx *ctx);
+/* Restores the state and time of before the last jump, whatever steps ran
+ * since; the next lbmib_step continues the plain trajectory bit for bit. */
+int lbmib_jump_revert(lbmib_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,3 +38,16 @@ def test_host_layer_exports_reference_names():
     assert len(names) > 20
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_surrogate_header_symbols_are_exported():
+    names = declared("lbmib_surrogate.h")
+    assert len(names) >= 14
+    import surrogate
+    assert set(surrogate.SG_SYMBOLS) == set(names)
+    path = os.path.join(PKG_DIR, "liblbmib.so")
+    if not os.path.exists(path):
+        pytest.fail("liblbmib.so not built")
+    lib = ctypes.CDLL(path)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing

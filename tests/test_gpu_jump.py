@@ -1,0 +1,198 @@
+"""Coupling of the surrogate to a running simulation (csrc/lbmib_jump.hip,
+distributed-lb-ib_amd/surrogate_jump.py), through the C-ABI.
+
+- lbmib_coarse_moments: FP64 engine against the block average of the FP64
+  CPU oracle's rho/u (lbm_ib.c:2440-2538), <= 1e-6 relative (FP32 output of
+  FP64 sums), and bit-consistent with lbmib_get_macroscopic's fields;
+- lbmib_coarse_sheets: node counts exact, force sums <= 1e-9 relative, against
+  a numpy raster of the downloaded nodes;
+- lbmib_jump: with a zero change the populations stay bit for bit and the
+  nodes move by K times the interpolated coarse velocity (<= 1e-12); with a
+  smooth random change the post-stream populations match the numpy
+  restatement (oracle/surrogate_oracle.jump_pulled) to 1e-13;
+- lbmib_jump_revert: after a jump and several steps, revert plus plain steps
+  give the plain trajectory bit for bit (F64, F32, BF16, with a sheet);
+- the jump loop with a tolerance of 0 rejects every jump and reproduces
+  lbmib_step bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import surrogate_oracle as so
+from oracle.oracle import BC_INOUT, BC_PERIODIC, BC_WALL, Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_cfg(lbmib, precision, n=(64, 32, 32), rows=12, seed=3):
+    nx, ny, nz = n
+    sheet = lbmib.Sheet.centred(rows, rows, 1.0, (18.0, ny / 2.0, nz / 2.0), 2e-3, 2e-3, jitter=0.08, seed=seed)
+    return dict(nx=nx, ny=ny, nz=nz, bc=(BC_INOUT, BC_PERIODIC, BC_WALL), precision=precision, tau=0.7, rho0=1.0,
+                u0=(0.02, 0.0, 0.0), quirks=True, sheets=[sheet])
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to("cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy().astype(np.float64)
+
+
+def coarse_of(e, f):
+    out = torch.empty((4, e.nz // f, e.ny // f, e.nx // f), dtype=torch.float32, device="cuda")
+    e.coarse_moments(f, out.data_ptr())
+    e.sync()
+    return host(out)
+
+
+@pytest.mark.parametrize("factor", [2, 4])
+def test_coarse_moments_match_oracle_block_average(lbmib, factor):
+    kw = small_cfg(lbmib, "f64")
+    o = Oracle(kw["nx"], kw["ny"], kw["nz"], kw["bc"], kw["tau"], kw["rho0"], kw["u0"], True)
+    s = kw["sheets"][0]
+    o.add_sheet(s.rows, s.cols, s.width, s.height, s.ks, s.kb, s.xyz)
+    o.init()
+    o.preshift(all_sheets=True)
+    assert o.step(12) == 0
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(12)
+        c = coarse_of(e, factor)
+        rho, vel = e.macroscopic()
+    ref = [so.block_average(o.rho(), factor)] + [so.block_average(o.vel()[a], factor) for a in range(3)]
+    for a in range(4):
+        assert np.abs(c[a] - ref[a]).max() <= 1e-6 * np.abs(ref[a]).max(), a
+    # the same fields lbmib_get_macroscopic returns, averaged
+    np.testing.assert_allclose(c[0], so.block_average(rho, factor), rtol=1e-6)
+
+
+def test_coarse_sheets_raster(lbmib):
+    kw = small_cfg(lbmib, "f32")
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(5)
+        out = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.coarse_sheets(4, out.data_ptr())
+        e.sync()
+        got = host(out)
+        nodes = e.fibers(0).reshape(-1, 12)
+    ref = so.raster(nodes[:, :3], nodes[:, 9:12], 4, (8, 8, 16))
+    np.testing.assert_array_equal(got[0], ref[0])
+    assert got[0].sum() == nodes.shape[0]
+    assert np.abs(got[1:] - ref[1:]).max() <= 1e-9 * np.abs(ref[1:]).max() + 1e-12
+
+
+def test_zero_jump_keeps_populations_and_advects_sheet(lbmib):
+    kw = small_cfg(lbmib, "f64")
+    K = 10
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(6)
+        df0 = e.df()
+        x0 = e.fibers(0)[..., :3].reshape(-1, 3)
+        cm = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.coarse_moments(4, cm.data_ptr())
+        zero = torch.zeros((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        t0 = e.time
+        e.jump(4, cm.data_ptr(), zero.data_ptr(), K)
+        assert e.time == t0 + K
+        np.testing.assert_array_equal(e.df(), df0)
+        x1 = e.fibers(0)[..., :3].reshape(-1, 3)
+        c = host(cm)
+        want = x0 + K * np.stack([so.trilinear_at(c[1 + a], 4, x0) for a in range(3)], 1)
+        np.testing.assert_allclose(x1, want, rtol=0, atol=1e-12)
+        e.jump_commit()
+        e.step(2)
+        assert e.status() == 0
+
+
+def test_jump_shift_matches_restatement(lbmib):
+    """A smooth random change on an F64 state without sheets: the post-stream
+    populations after the jump, away from the boundaries, against
+    oracle/surrogate_oracle.jump_pulled."""
+    kw = small_cfg(lbmib, "f64")
+    kw["sheets"] = []
+    kw["bc"] = (BC_PERIODIC, BC_PERIODIC, BC_PERIODIC)
+    rng = np.random.default_rng(11)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(3)
+        f2 = e.df()
+        cm = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.coarse_moments(4, cm.data_ptr())
+        delta = (rng.standard_normal((4, 8, 8, 16)) * np.array([1e-3, 1e-3, 5e-4, 5e-4])[:, None, None, None])
+        delta = delta.astype(np.float32).astype(np.float64)
+        d_dev = dev(delta)
+        e.jump(4, cm.data_ptr(), d_dev.data_ptr(), 4)
+        got = e.df()
+    want = so.jump_pulled(f2, delta, 4)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
+def test_revert_gives_plain_trajectory(lbmib, precision):
+    kw = small_cfg(lbmib, precision)
+    rng = np.random.default_rng(5)
+    delta = dev(rng.standard_normal((4, 8, 8, 16)) * 1e-3)
+    with lbmib.Engine(**kw) as a, lbmib.Engine(**kw) as b:
+        a.init()
+        b.init()
+        a.step(7)
+        b.step(7)
+        cm = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        a.coarse_moments(4, cm.data_ptr())
+        a.jump(4, cm.data_ptr(), delta.data_ptr(), 20)
+        a.step(3)        # verification steps overwrite both force slots
+        a.jump_revert()
+        assert a.time == b.time
+        a.step(9)
+        b.step(9)
+        assert a.status() == 0 and b.status() == 0
+        np.testing.assert_array_equal(a.df(), b.df())
+        np.testing.assert_array_equal(a.fibers(0), b.fibers(0))
+        np.testing.assert_array_equal(a.force(), b.force())
+
+
+def test_rejecting_every_jump_reproduces_plain_steps(lbmib):
+    import surrogate as sg
+    import surrogate_jump as sj
+    kw = small_cfg(lbmib, "f32")
+    net = sg.Net(cin=sj.CIN, cout=16, nout=sj.NOUT, chans=(32, 64, 128, 256), seed=1)
+    norm = sj.Norm([1.0] + [0.0] * 7, [1e-3] * 8, [0.0] * 4, [1e-4] * 4)
+    with lbmib.Engine(**kw) as a, lbmib.Engine(**kw) as b:
+        a.init()
+        b.init()
+        a.step(2)
+        b.step(2)
+        r = sj.JumpRunner(a, net, norm, kw["tau"], kw["u0"][0], factor=4, ksteps=10, verify_steps=2, tol=0.0)
+        st = r.run(30)
+        b.step(30)
+        assert st["jumps"] == 2 and st["rejected"] == 2 and st["accepted"] == 0
+        assert a.time == b.time
+        np.testing.assert_array_equal(a.df(), b.df())
+        np.testing.assert_array_equal(a.fibers(0), b.fibers(0))
+
+
+def test_coupling_rejects_bad_shapes_and_states(lbmib):
+    kw = small_cfg(lbmib, "f32", n=(40, 32, 32))
+    buf = torch.zeros((4, 1024), dtype=torch.float32, device="cuda")
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        with pytest.raises(lbmib.LbmibError):      # nx not a multiple of 64
+            e.coarse_moments(4, buf.data_ptr())
+    kw = small_cfg(lbmib, "f32")
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        cm = torch.zeros((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        with pytest.raises(lbmib.LbmibError):      # no completed step yet
+            e.jump(4, cm.data_ptr(), cm.data_ptr(), 10)
+        e.step(1)
+        with pytest.raises(lbmib.LbmibError):      # odd step count
+            e.jump(4, cm.data_ptr(), cm.data_ptr(), 9)
+        with pytest.raises(lbmib.LbmibError):      # nothing to revert
+            e.jump_revert()
+        with pytest.raises(lbmib.LbmibError):      # factor 3
+            e.coarse_moments(3, cm.data_ptr())
