@@ -4,17 +4,18 @@
  * Accepts the reference's flags with the reference's meaning (main,
  * lbm_ib.c:3611-3955), including its x/z swap of -fluid_grid_xyz
  * (lbm_ib.c:3792-3794, the grid's x is the third number) and its sanity
- * checks (lbm_ib.c:3815-3833).  -thread_per_task_xyz is still checked but
- * the work runs on the GPU.  Extensions (not in the reference) set the
+ * checks (lbm_ib.c:3815-3833).  -thread_per_task_xyz starts P*Q*R threads
+ * on do_thread as the reference does; thread 0 enqueues the GPU work.  Extensions (not in the reference) set the
  * physics the reference hard-codes:
  *   -tau T -u_in U -ks K -kb K -bc X Y Z (INOUT|WALL|PERIODIC)
  *   -precision f64|f32|f16|bf16  -jitter A -seed S  -no_quirks  -shift_all
- *   -dump_fiber -dump_fluid -dump_fv  -quiet
+ *   -dump_fiber -dump_fluid -dump_fv -dump_fiber_text  -quiet
  * (the reference's dumps are compile-time switches: DUMP_VERIFY and USE_FV,
  * lbm_ib.c:30, :3504-3538).
  * Prints "TOTAL TIME TAKEN IN Seconds:%f" like the reference, then MLUPS.
  */
 #define _POSIX_C_SOURCE 200809L
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -126,6 +127,7 @@ int main(int argc, char *argv[]) {
     else if (!strcmp(a, "-dump_fiber")) dumps |= 1;
     else if (!strcmp(a, "-dump_fluid")) dumps |= 2;
     else if (!strcmp(a, "-dump_fv")) dumps |= 4;
+    else if (!strcmp(a, "-dump_fiber_text")) dumps |= 8;
     else if (!strcmp(a, "-quiet")) gv->quiet = 1;
     else { fprintf(stderr, "error: unknown flag \"%s\"\n", a); return 2; }
   }
@@ -191,12 +193,36 @@ int main(int argc, char *argv[]) {
   init_df_inout(gv);
   lbmib_host_set_dumps(gv, dumps);
 
-  struct lv_t lv0 = {0, gv};
+  /* P*Q*R threads run do_thread with a barrier after every routine, as the
+   * reference's main() starts them (lbm_ib.c:3844-3867, :3910-3928); thread 0
+   * enqueues the device work */
+  gv->lock_Fluid = (pthread_mutex_t *)malloc(sizeof(*gv->lock_Fluid) * total_threads);
+  for (int i = 0; i < total_threads; ++i)
+    if (pthread_mutex_init(&gv->lock_Fluid[i], NULL)) { fprintf(stderr, "Unable to initialize a mutex\n"); return 1; }
+  if (pthread_barrier_init(&gv->barr, NULL, total_threads)) { fprintf(stderr, "Could not create a barrier\n"); return 1; }
+  pthread_t *threads = (pthread_t *)malloc(sizeof(pthread_t) * total_threads);
+  struct lv_t *lvs = (struct lv_t *)malloc(sizeof(struct lv_t) * total_threads);
+  gv->threads = threads;
   lbmib_sync_to_host(gv, 0);
   double t0 = get_cur_time();
-  do_thread(&lv0);
+  for (int i = 0; i < total_threads; ++i) {
+    lvs[i].tid = i;
+    lvs[i].gv = gv;
+    if (pthread_create(threads + i, NULL, do_thread, lvs + i)) {
+      perror("Thread creation do_thread\n");
+      return 1;
+    }
+  }
+  for (int i = 0; i < total_threads; ++i) pthread_join(threads[i], NULL);
   int rc = lbmib_host_status(gv);
   double t1 = get_cur_time();
+  pthread_barrier_destroy(&gv->barr);
+  for (int i = 0; i < total_threads; ++i) pthread_mutex_destroy(&gv->lock_Fluid[i]);
+  free(gv->lock_Fluid);
+  gv->lock_Fluid = NULL;
+  free(threads);
+  free(lvs);
+  gv->threads = NULL;
   printf("***********IB Simulation on MI355X (LBM-IB engine) ends************\n\n\n");
   if (rc) {
     fprintf(stderr, "error: device status %d\n", rc);

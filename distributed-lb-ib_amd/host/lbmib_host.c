@@ -20,6 +20,8 @@ static const int HCX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -
 static const int HCY[19] = {0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, 1, -1, -1, 1, 1, -1};
 static const int HCZ[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0};
 
+static double host_feq(int q, double rho, double vx, double vy, double vz);
+
 static void note(GV gv, int rc) {
   if (rc && !gv->last_error) gv->last_error = rc;
 }
@@ -90,13 +92,32 @@ void gen_fiber_sheet_jittered(GV gv, int s, double w, double h, int num_cols, in
 }
 
 Fluidgrid *gen_fluid_grid(GV gv, int dim_x, int dim_y, int dim_z, int cube_size) {
-  /* lbm_ib.c:924-958; the populations are allocated on the device by
-   * init_gv, the host mirror on demand by lbmib_sync_to_host */
-  (void)cube_size;
+  /* lbm_ib.c:924-958: the grid, its inlet/outlet surfaces and the cube
+   * descriptors; the populations are allocated on the device by init_gv,
+   * the host mirror on demand by lbmib_sync_to_host */
   Fluidgrid *g = (Fluidgrid *)calloc(1, sizeof(Fluidgrid));
   g->x_dim = dim_x;
   g->y_dim = dim_y;
   g->z_dim = dim_z;
+  g->inlet = (Fluidsurface *)calloc(1, sizeof(Fluidsurface));
+  g->outlet = (Fluidsurface *)calloc(1, sizeof(Fluidsurface));
+  g->inlet->nodes = (Fluidnode *)calloc((size_t)dim_y * dim_z, sizeof(Fluidnode));
+  g->outlet->nodes = (Fluidnode *)calloc((size_t)dim_y * dim_z, sizeof(Fluidnode));
+  if (cube_size > 0) {
+    int ncx = dim_x / cube_size, ncy = dim_y / cube_size, ncz = dim_z / cube_size;
+    g->sub_fluid_grid = (Sub_Fluidgrid *)calloc((size_t)ncx * ncy * ncz, sizeof(Sub_Fluidgrid));
+    for (int bi = 0; bi < ncx; ++bi)
+      for (int bj = 0; bj < ncy; ++bj)
+        for (int bk = 0; bk < ncz; ++bk) {
+          Sub_Fluidgrid *c = g->sub_fluid_grid + ((size_t)bi * ncy + bj) * ncz + bk;
+          c->sub_coordx_start = bi * cube_size; c->sub_coordx_end = (bi + 1) * cube_size - 1;
+          c->sub_coordy_start = bj * cube_size; c->sub_coordy_end = (bj + 1) * cube_size - 1;
+          c->sub_coordz_start = bk * cube_size; c->sub_coordz_end = (bk + 1) * cube_size - 1;
+          c->nodes = NULL;
+          c->isboundary = bi == 0 || bj == 0 || bk == 0 || bi == ncx - 1 || bj == ncy - 1 || bk == ncz - 1;
+          c->grid_dim_x = c->grid_dim_y = c->grid_dim_z = cube_size;
+        }
+  }
   gv->fluid_grid = g;
   return g;
 }
@@ -196,9 +217,18 @@ void lbmib_host_free(GV gv) {
   if (gv->dev) lbmib_destroy(gv->dev);
   gv->dev = NULL;
   if (gv->fluid_grid) {
+    /* every pointer is cleared after its free, so a caller that frees
+     * sub_fluid_grid itself afterwards, as the reference's main() does
+     * (lbm_ib.c:3949), frees NULL */
     Fluidgrid *g = gv->fluid_grid;
     free(g->df); free(g->dfeq); free(g->rho); free(g->vel); free(g->force);
     g->df = g->dfeq = g->rho = g->vel = g->force = NULL;
+    Fluidsurface *sf[2] = {g->inlet, g->outlet};
+    for (int k = 0; k < 2; ++k)
+      if (sf[k]) { free(sf[k]->nodes); free(sf[k]); }
+    g->inlet = g->outlet = NULL;
+    free(g->sub_fluid_grid);
+    g->sub_fluid_grid = NULL;
   }
 }
 
@@ -248,7 +278,22 @@ void replace_old_DF(LV lv) { TID0(lv); note(gv, lbmib_replace_old_DF(gv->dev)); 
 
 void init_eqlbrmdistrfuncDF0(GV gv) { if (gv->dev) note(gv, lbmib_init_eqlbrmdistrfuncDF0(gv->dev)); }
 void init_df1(GV gv) { if (gv->dev) note(gv, lbmib_init_df1(gv->dev)); }
-void init_df_inout(GV gv) { if (gv->dev) note(gv, lbmib_init_df_inout(gv->dev)); }
+void init_df_inout(GV gv) {
+  if (!gv->dev) return;
+  note(gv, lbmib_init_df_inout(gv->dev));
+  /* the host tables of lbm_ib.c:1610-1670: the initial equilibrium at the
+   * inlet and (sampled at x = N-1) the outlet, both feq(rho_l, (u_l, 0, 0)) */
+  Fluidgrid *g = gv->fluid_grid;
+  if (!g || !g->inlet || !g->outlet || !g->inlet->nodes || !g->outlet->nodes) return;
+  double tab[19];
+  for (int q = 0; q < 19; ++q) tab[q] = host_feq(q, gv->rho_l, gv->u_l, 0.0, 0.0);
+  size_t m = (size_t)g->y_dim * g->z_dim;
+  for (size_t k = 0; k < m; ++k)
+    for (int q = 0; q < 19; ++q) {
+      g->inlet->nodes[k].df_inout[0][q] = tab[q];
+      g->outlet->nodes[k].df_inout[1][q] = tab[q];
+    }
+}
 
 /* ----------------------------------------------------------- host mirror */
 
@@ -517,6 +562,12 @@ void save_fiber_sub_grid_cord_ef_use_fv(GV gv, int start_y, int start_z, int end
 
 /* --------------------------------------------------------------- the loop */
 
+/* The barrier after every routine of lbm_ib.c:3124-3563, when several
+ * threads run the loop; a single caller needs none. */
+static void step_barrier(GV gv) {
+  if (gv->total_threads > 1) pthread_barrier_wait(&gv->barr);
+}
+
 void *do_thread(void *v) {
   LV lv = (LV)v;
   GV gv = lv->gv;
@@ -531,20 +582,35 @@ void *do_thread(void *v) {
     snprintf(fname, sizeof fname, "FV_gridFluid.xyz");
     save_fluid_sub_grid_cord_use_fv(gv, 0, 0, 0, g->x_dim, g->y_dim, g->z_dim, fname);
   }
+  step_barrier(gv);
   while (gv->time <= gv->TIME_STOP) {
     const int dumping = gv->dump_mask && gv->dump > 0 && ((gv->time % gv->dump == 0) || gv->time == 1);
-    compute_bendingforce(lv);
-    compute_stretchingforce(lv);
-    compute_elasticforce(lv);
-    get_influentialdomain_fluid_grid_and_SpreadForce(lv);
+    /* the collision of this step uses the equilibrium of the rho/u that the
+     * previous step's compute_rho_and_u left (with the previous step's force,
+     * lbm_ib.c:2521-2523, :1731ff): captured before this step's spreading */
     if (dumping && (gv->dump_mask & 2) && lv->tid == 0) capture_dfeq(gv);
+    compute_bendingforce(lv);
+    step_barrier(gv);
+    compute_stretchingforce(lv);
+    step_barrier(gv);
+    compute_elasticforce(lv);
+    step_barrier(gv);
+    get_influentialdomain_fluid_grid_and_SpreadForce(lv);
+    step_barrier(gv);
     compute_eqlbrmdistrfuncDF1(lv);
+    step_barrier(gv);
     stream_distrfunc(lv);
+    step_barrier(gv);
     bounceback_rigidwalls(lv);
+    step_barrier(gv);
     compute_rho_and_u(lv);
+    step_barrier(gv);
     moveFiberSheet(lv);
+    step_barrier(gv);
     copy_inout_to_df2(lv);
+    step_barrier(gv);
     replace_old_DF(lv);
+    step_barrier(gv);
     /* periodicBC(lv) is not called by the reference either (lbm_ib.c:3484) */
     if (dumping && lv->tid == 0) {
       lbmib_sync_to_host(gv, (gv->dump_mask & 6) != 0);
@@ -557,6 +623,11 @@ void *do_thread(void *v) {
         Fibersheet *sh = gv->fiber_shape->sheets;
         save_fiber_sub_grid_binary(gv, 0, 0, sh->num_rows - 1, sh->num_cols - 1, fname);
       }
+      if (gv->dump_mask & 8) {  /* the text fiber dump, lbm_ib.c:3529-3532 (USE_CATALYST) */
+        snprintf(fname, sizeof fname, "Fiber%d_dump_step%d.dat", 1, gv->time);
+        Fibersheet *sh = gv->fiber_shape->sheets;
+        save_fiber_sub_grid(gv, 0, 0, sh->num_rows - 1, sh->num_cols - 1, fname);
+      }
       if (gv->dump_mask & 4) {  /* FieldView, lbm_ib.c:3514-3517, :3531-3535 */
         Fluidgrid *g = gv->fluid_grid;
         snprintf(fname, sizeof fname, "FV_fluid_vel_%d.f", gv->time);
@@ -567,12 +638,17 @@ void *do_thread(void *v) {
         save_fiber_sub_grid_cord_ef_use_fv(gv, 0, 0, sh->num_rows - 1, sh->num_cols - 1, fname, fname1);
       }
     }
+    step_barrier(gv);
     if (lv->tid == 0 && !gv->quiet) {
       printf("End of time step %d\n", gv->time);
       fflush(stdout);
     }
-    if (gv->last_error) break;
+    /* every thread reads the error thread 0 noted before the last barrier */
+    const int stop = gv->last_error != 0;
+    step_barrier(gv);
     if (lv->tid == 0) gv->time += gv->dt;
+    step_barrier(gv);
+    if (stop) break;
   }
   if (lv->tid == 0) lbmib_host_status(gv);
   return NULL;

@@ -18,6 +18,9 @@
 #ifndef LBMIB_HOST_H
 #define LBMIB_HOST_H
 
+#include <pthread.h>
+#include <stdbool.h>
+
 #include "lbmib.h"
 
 #ifdef __cplusplus
@@ -50,11 +53,50 @@ typedef struct fiber_shape_t {
   int num_sheets;
 } Fibershape;
 
-/* Fluidgrid, lbm_ib.c:124-132.  The populations live on the device; the
- * host keeps the dimensions and, after lbmib_sync_to_host(), a mirror of the
- * post-stream state in x-fastest arrays (index x + x_dim*(y + y_dim*z)). */
+/* Fluidnode, lbm_ib.c:88-97 (824 B, same field order): the element type of
+ * the inlet/outlet surfaces.  The engine keeps the fluid state on the
+ * device; init_df_inout fills df_inout of both surfaces on the host as the
+ * reference does (the equilibrium feq(rho_l, (u_l, 0, 0)), lbm_ib.c:1610-1670). */
+typedef struct fluid_node_t {
+  double vel_x, vel_y, vel_z;
+  double rho, pressure;
+  double df1[19];
+  double df2[19];
+  double dfeq[19];
+  double df_inout[2][19];
+  double elastic_force_x, elastic_force_y, elastic_force_z;
+} Fluidnode;
+
+/* Fluidsurface, lbm_ib.c:100-103: y_dim * z_dim nodes, index y * z_dim + z */
+typedef struct fluid_surface_t {
+  Fluidnode *nodes;
+} Fluidsurface;
+
+/* Sub_Fluidgrid, lbm_ib.c:108-119: one cube_size^3 cube of the grid.  Here
+ * the cube descriptors carry the coordinates of the reference's cube
+ * decomposition (for callers that partition host work with cube2thread);
+ * nodes stays NULL: the populations live on the device. */
+typedef struct sub_fluid_grid_t {
+  int sub_coordx_start, sub_coordx_end;
+  int sub_coordy_start, sub_coordy_end;
+  int sub_coordz_start, sub_coordz_end;
+  Fluidnode *nodes;
+  bool isboundary;
+  int grid_dim_x;
+  int grid_dim_y;
+  int grid_dim_z;
+} Sub_Fluidgrid;
+
+/* Fluidgrid, lbm_ib.c:124-132 (the reference's fields first).  The
+ * populations live on the device; the host keeps the dimensions, the
+ * inlet/outlet surfaces, the cube descriptors and, after
+ * lbmib_sync_to_host(), a mirror of the post-stream state in x-fastest
+ * arrays (index x + x_dim*(y + y_dim*z)). */
 typedef struct fluid_grid_t {
   int x_dim, y_dim, z_dim;
+  Fluidsurface *inlet;
+  Fluidsurface *outlet;
+  Sub_Fluidgrid *sub_fluid_grid;  /* (x_dim/cube)(y_dim/cube)(z_dim/cube) cubes, BI*ncy*ncz + BJ*ncz + BK */
   double *df;     /* [19][n]: df1 == df2 after replace_old_DF */
   double *dfeq;   /* [19][n]: equilibrium of the last collision */
   double *rho;    /* [n] */
@@ -72,10 +114,13 @@ typedef struct gv_t {
   int dt, time, TIME_STOP, dump;
   double c[19][3];
   int ib, ie, je, jb, ke, kb;
+  pthread_t *threads;            /* set by the caller, as the reference's main() does */
+  pthread_mutex_t *lock_Fluid;   /* set by the caller; the engine takes no host locks */
   int total_threads;
   int cube_size;
   int num_cubes_x, num_cubes_y, num_cubes_z;
   int P, Q, R;
+  pthread_barrier_t barr;        /* do_thread waits on it after every routine when total_threads > 1 */
   /* -- MI355X engine -- */
   lbmib_ctx *dev;
   int custom_physics; /* 1: init_gv keeps tau,u_l,rho_l,Ks_l,Kb_l and the fields below */
@@ -85,7 +130,7 @@ typedef struct gv_t {
   int shift_all_sheets;
   int last_error;
   int quiet;
-  int dump_mask;      /* do_thread dumps: 1 binary fiber, 2 fluid cubes, 4 FieldView */
+  int dump_mask;      /* do_thread dumps: 1 binary fiber, 2 fluid cubes, 4 FieldView, 8 text fiber */
 } *GV;
 
 /* LV, lbm_ib.c:164-167 */
@@ -132,8 +177,12 @@ void copy_inout_to_df2(LV lv);
 void periodicBC(LV lv);
 void replace_old_DF(LV lv);
 
-/* the step loop, lbm_ib.c:3087-3570 (single caller; dumps at time==1 and
- * every gv->dump steps when dumps are enabled) */
+/* the step loop, lbm_ib.c:3087-3570: called by all total_threads threads
+ * of the caller (each with its own LV) or by one; with total_threads > 1 every
+ * thread waits on gv->barr (initialised by the caller for total_threads)
+ * after each routine, as the reference does; thread 0 enqueues the device
+ * work, writes the dumps (at time == 1 and every gv->dump steps when dumps
+ * are enabled) and advances gv->time */
 void *do_thread(void *v);
 
 /* dump writers, lbm_ib.c:234-692 (fluid values from the host mirror of
@@ -162,6 +211,7 @@ int lbmib_sync_to_host(GV gv, int with_fluid);
 int lbmib_host_status(GV gv);
 void lbmib_host_free(GV gv);
 /* dump switches for do_thread: 1 = binary fiber dump, 2 = fluid cube dump,
+ * 8 = text fiber dump (lbm_ib.c:560-601),
  * 4 = FieldView files (the reference's USE_FV, lbm_ib.c:30) */
 void lbmib_host_set_dumps(GV gv, int mask);
 

@@ -13,6 +13,7 @@ Tolerances:
                  max <= 3e-2 / 2e-1.
 """
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -476,6 +477,85 @@ def test_cli_replays_reference_fixture_refA(tmp_path, fixtures_meta):
         ij_got, got = read_fiber_dump(os.path.join(tmp_path, f"Fiber1_dump_step{step}.bin"))
         ij_ref, ref = read_fiber_dump(os.path.join(GOLDEN, f"refA_fiber_step{step}.bin"))
         assert np.array_equal(ij_got, ij_ref)
+        assert np.abs(got[:, :3] - ref[:, :3]).max() <= 1e-12
+        assert np.abs(got[:, 3:] - ref[:, 3:]).max() <= 1e-9 * max(np.abs(ref[:, 3:]).max(), 1e-12) + 1e-20
+
+
+def test_threaded_reference_caller_replays_refA(tmp_path, fixtures_meta):
+    """A caller written against the reference's API with 2x2x2 pthreads, each
+    running do_thread with a barrier after every routine (lbm_ib.c:3115-3563,
+    main() at :3836-3955), writes the compiled reference's binary fiber dumps."""
+    import subprocess
+    from test_host_writers import build_pthread_caller
+    meta = fixtures_meta["refA"]
+    exe = build_pthread_caller(tmp_path)
+    x0, y0, z0 = meta["origin"]
+    args = [str(meta["grid"]), str(meta["steps"]), "10", "2", "2", "2", str(meta["rows"]), str(meta["cols"]),
+            str(meta["width"]), str(meta["height"]), str(x0), str(y0), str(z0)]
+    out = subprocess.run([exe] + args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(out.stdout[-600:], out.stderr[-400:])
+    assert out.returncode == 0
+    assert f"time {meta['steps'] + 1}, status 0" in out.stdout
+    for step in meta["fiber_steps"]:
+        ij_got, got = read_fiber_dump(os.path.join(tmp_path, f"Fiber1_dump_step{step}.bin"))
+        ij_ref, ref = read_fiber_dump(os.path.join(GOLDEN, f"refA_fiber_step{step}.bin"))
+        assert np.array_equal(ij_got, ij_ref)
+        assert np.abs(got[:, :3] - ref[:, :3]).max() <= 1e-12
+        assert np.abs(got[:, 3:] - ref[:, 3:]).max() <= 1e-9 * max(np.abs(ref[:, 3:]).max(), 1e-12) + 1e-20
+
+
+def test_cli_fluid_dump_matches_reference_dump(tmp_path, fixtures_meta):
+    """-dump_fluid writes the reference's %.24f fluid-cube text
+    (save_fluid_sub_grid, lbm_ib.c:332-392) on its box (lbm_ib.c:3511); at
+    step 20 of refA every column -- dfeq, df1, df2, rho, velocity, force --
+    matches the compiled reference's own Fluid0_dump_step20.dat
+    (tests/golden/refA_fluid_step20.npz), parsed by the same parser."""
+    import subprocess
+    sys.path.insert(0, GOLDEN)
+    from make_fixtures import parse_fluid
+    meta = fixtures_meta["refA"]
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "distributed-lb-ib_amd", "lbmib")
+    out = subprocess.run([os.path.abspath(exe)] + meta["args"] + ["-dump_fluid", "-quiet"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-400:]
+    step = meta["fluid_steps"][-1]
+    got = parse_fluid(os.path.join(tmp_path, f"Fluid0_dump_step{step}.dat"))
+    ref = np.load(os.path.join(GOLDEN, f"refA_fluid_step{step}.npz"))
+    np.testing.assert_array_equal(got["xyz"], ref["xyz"])
+    for k, tol in (("df1", 1e-12), ("df2", 1e-12), ("dfeq", 1e-12), ("rho", 1e-12), ("vel", 1e-10),
+                   ("force", 1e-9)):
+        scale = max(np.abs(ref[k]).max(), 1e-30)
+        err = np.abs(got[k] - ref[k]).max() / scale
+        print(f"fluid dump {k}: max relative difference {err:.2e}")
+        assert err <= tol, (k, err)
+
+
+def test_cli_fiber_text_dump_matches_reference(tmp_path, fixtures_meta):
+    """-dump_fiber_text writes the reference's %.24f fiber text
+    (save_fiber_sub_grid, lbm_ib.c:560-601); its values match the compiled
+    reference's binary fiber dumps of the same steps."""
+    import re
+    import subprocess
+    meta = fixtures_meta["refA"]
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "distributed-lb-ib_amd", "lbmib")
+    out = subprocess.run([os.path.abspath(exe)] + meta["args"] + ["-dump_fiber_text", "-quiet"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-400:]
+    line = re.compile(r"\(\s*(\d+),\s*(\d+)\):\{([^}]*)\} \|\| ([^|]*) \|\| ([^|]*) \|\| (.*)$")
+    for step in meta["fiber_steps"]:
+        text = open(os.path.join(tmp_path, f"Fiber1_dump_step{step}.dat")).read().splitlines()
+        assert text[0].startswith("Fiber_sheets[0] (i,j): {cord_x, cord_y, cord_z} || SF_x, SF_y, SF_z")
+        rows = []
+        for ln in text[1:]:
+            m = line.match(ln)
+            assert m, ln
+            vals = [float(v) for g in m.groups()[2:] for v in g.split(",")]
+            rows.append([int(m.group(1)), int(m.group(2))] + vals)
+        rows = np.array(rows)
+        ij_ref, ref = read_fiber_dump(os.path.join(GOLDEN, f"refA_fiber_step{step}.bin"))
+        assert np.array_equal(rows[:, :2].astype(int), ij_ref)
+        # text order: position, stretch, bend, elastic; binary: position, bend, stretch, elastic
+        got = np.concatenate([rows[:, 2:5], rows[:, 8:11], rows[:, 5:8], rows[:, 11:14]], axis=1)
         assert np.abs(got[:, :3] - ref[:, :3]).max() <= 1e-12
         assert np.abs(got[:, 3:] - ref[:, 3:]).max() <= 1e-9 * max(np.abs(ref[:, 3:]).max(), 1e-12) + 1e-20
 

@@ -9,9 +9,9 @@ Tolerances, written per element:
   FP32 weight gradients: |gpu - ref| <= 2^-18 * sum |terms| + 1e-30;
   pooling, upsampling and their backward: the same BF16 bound with the
       oracle's FP64 value;
-  the whole network (16^3 voxels, four levels), whose stored activations are
-      each rounded once on both sides: relative L2 <= 2e-2 on the output and
-      on every parameter gradient.
+  the whole network (16^3 voxels, four levels): the output and every
+      parameter gradient within 3x the relative L2 distance from exact FP64
+      math that the same BF16 rounding gives the oracle, plus 1e-3.
 """
 import numpy as np
 import pytest
@@ -248,24 +248,34 @@ def test_mse_adam_pack_unpack(sg):
 
 
 def test_network_forward_backward_matches_oracle(sg):
-    """The whole encoder-decoder on 16^3 voxels (levels 16, 8, 4, 2) against
-    the oracle network with the same BF16 rounding of stored tensors."""
+    """The whole encoder-decoder on 16^3 voxels (levels 16, 8, 4, 2): the
+    GPU's output and every parameter gradient against the oracle network in
+    exact FP64 (no rounding of stored tensors).  The GPU stores every
+    activation and gradient in BF16, so it cannot match exact math; the bound
+    is what the same rounding does to the oracle: the GPU's distance to the
+    exact result may be at most 3x the rounding oracle's own distance plus
+    1e-3 (relative L2).  An orchestration mistake -- a wrong mask, skip or
+    layer input -- puts the GPU far outside that."""
     net = sg.Net(cin=16, cout=16, nout=4, seed=5, out_scale=0.05)
     params = {k: (so.bf16_round(net.p[k].cpu().numpy()), net.b[k].cpu().numpy().astype(np.float64)) for k in net.p}
     rng = np.random.default_rng(6)
     x = so.bf16_round(rng.standard_normal((1, 16, 16, 16, 16)))
     out = host_bf16(net.forward(dev_bf16(x)))
-    ref, act = so.unet_forward(params, x, sg.CHANS)
-    relerr = np.linalg.norm(out - ref) / np.linalg.norm(ref)
-    print(f"network output relative L2 {relerr:.2e}")
-    assert relerr < 2e-2
+    exact, act_e = so.unet_forward(params, x, sg.CHANS, rounding=False)
+    rounded, act_r = so.unet_forward(params, x, sg.CHANS)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))  # noqa: E731
+    e_gpu, e_orc = rel(out, exact), rel(rounded, exact)
+    print(f"output: gpu {e_gpu:.2e}, rounding oracle {e_orc:.2e} (relative L2 to exact)")
+    assert e_gpu <= 3 * e_orc + 1e-3
     dout = so.bf16_round(rng.standard_normal(out.shape) * 0.01)
     net.backward(dev_bf16(dout))
-    grads = so.unet_backward(params, act, dout, sg.CHANS)
+    g_exact = so.unet_backward(params, act_e, dout, sg.CHANS, rounding=False)
+    g_round = so.unet_backward(params, act_r, dout, sg.CHANS)
     for k in net.p:
-        for gpu, r, what in ((host_f32(net.g[k]), grads[k][0], "w"), (host_f32(net.gb[k]), grads[k][1], "b")):
-            e = np.linalg.norm(gpu - r) / max(np.linalg.norm(r), 1e-30)
-            assert e < 2e-2, (k, what, e)
+        for i, (gpu, what) in enumerate(((host_f32(net.g[k]), "w"), (host_f32(net.gb[k]), "b"))):
+            e_gpu, e_orc = rel(gpu, g_exact[k][i]), rel(g_round[k][i], g_exact[k][i])
+            print(f"{k}.{what}: gpu {e_gpu:.2e}, rounding oracle {e_orc:.2e}")
+            assert e_gpu <= 3 * e_orc + 1e-3, (k, what, e_gpu, e_orc)
 
 
 def test_training_reduces_the_loss(sg):

@@ -80,3 +80,29 @@ def test_stdout_printers(run):
     # cube (0,0,0) of side 2 then cube (0,0,0)..(0,0,0): 8 nodes x (1 + 19) lines
     assert sum(1 for line in lines if line.startswith("For cube")) == 8
     assert lines[cube + 3].startswith("(0, 0, 0, 1):{0.000000000000,")
+
+
+def build_pthread_caller(d):
+    """Compiles tests/host_pthread_caller.c -- a caller written against the
+    reference's GV/LV API with its own main() that uses threads, lock_Fluid
+    and barr as lbm_ib.c:3836-3955 does -- against include/lbmib_host.h."""
+    exe = str(d / "host_pthread_caller")
+    subprocess.check_call(["gcc", "-O1", "-Wall", "-Werror", "-I", os.path.join(REPO, "include"), "-o", exe,
+                           os.path.join(REPO, "tests", "host_pthread_caller.c"), "-L", PKG, "-llbmib_host", "-llbmib",
+                           "-Wl,-rpath," + PKG, "-lpthread", "-lm"])
+    return exe
+
+
+def test_reference_style_threaded_caller_compiles(tmp_path):
+    exe = build_pthread_caller(tmp_path)
+    assert os.path.exists(exe)
+    # the structs keep the reference's field order for the fields it has
+    src = open(os.path.join(REPO, "include", "lbmib_host.h")).read()
+    gv = src[src.index("typedef struct gv_t"):]
+    order = [gv.index(f) for f in ("int ib, ie", "pthread_t *threads", "pthread_mutex_t *lock_Fluid",
+                                   "int total_threads", "int cube_size", "int P, Q, R", "pthread_barrier_t barr")]
+    assert order == sorted(order)
+    fg = src[src.index("typedef struct fluid_grid_t"):]
+    order = [fg.index(f) for f in ("int x_dim", "Fluidsurface *inlet", "Fluidsurface *outlet",
+                                   "Sub_Fluidgrid *sub_fluid_grid")]
+    assert order == sorted(order)

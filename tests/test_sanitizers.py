@@ -45,7 +45,7 @@ def test_host_layer_and_cli_under_sanitizers(tmp_path):
     exe = str(tmp_path / "lbmib_cli_san")
     subprocess.check_call(["gcc", *SAN, "-std=c11", "-I", os.path.join(REPO, "include"), "-o", exe,
                            os.path.join(PKG, "host", "lbmib_cli.c"), os.path.join(PKG, "host", "lbmib_host.c"),
-                           "-L", PKG, "-llbmib", "-Wl,-rpath," + PKG, "-lm"])
+                           "-L", PKG, "-llbmib", "-Wl,-rpath," + PKG, "-lpthread", "-lm"])
     full = ["-steps", "2", "-dump", "1000", "-fluid_grid_xyz", "32", "32", "32", "-cube_size", "4",
             "-thread_per_task_xyz", "2", "2", "2", "-num_fibersht", "1", "-fibersht_width_height", "7", "7",
             "-fibersht_row_clmn", "8", "8", "-fibersht_xyz_0", "8", "12", "12"]
