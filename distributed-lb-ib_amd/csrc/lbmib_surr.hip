@@ -437,6 +437,29 @@ constexpr int kMaxDevices = 64;
 void *g_ws[kMaxDevices] = {};
 std::mutex g_ws_mu;
 
+// Scratch of lbmib_sg_conv3d_bwd_weight_noalloc: a file-scope device array
+// that every GPU the library runs on holds until the process ends.  In host
+// code its name is not a device address; each device's own copy is reached
+// through the address hipGetSymbolAddress returns with that device current,
+// kept per device in g_scratch.
+static __device__ __align__(256) float scratch[LBMIB_SG_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+void *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(void **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return LBMIB_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LBMIB_ERR_HIP;
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return LBMIB_OK;
+}
+
 int workspace(void **out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -767,6 +790,18 @@ extern "C" int lbmib_sg_conv3d_bwd_weight(const void *x, const void *dy, float *
   const int rc = workspace(&ws);
   if (rc) return rc;
   return wgrad_with(x, dy, dw, db, B, D, H, W, Cin, Cout, ws, LBMIB_SG_WORKSPACE_BYTES, (hipStream_t)stream);
+}
+
+extern "C" int lbmib_sg_conv3d_bwd_weight_noalloc(const void *x, const void *dy, float *dw, float *db, int B, int D,
+                                                  int H, int W, int Cin, int Cout, void *stream) {
+  if (!x || !dy || !dw || !shape_ok(B, D, H, W, Cin, Cout)) return LBMIB_ERR_ARG;
+  void *ws = nullptr;
+  const int rc = scratch_ptr(&ws);
+  if (rc) return rc;
+  // the same split plan as lbmib_sg_conv3d_bwd_weight (same cap), so the
+  // partials and their fixed-order sums are the same bits; every partial is
+  // written before it is read, so nothing needs clearing between calls
+  return wgrad_with(x, dy, dw, db, B, D, H, W, Cin, Cout, ws, LBMIB_SG_NOALLOC_SCRATCH_BYTES, (hipStream_t)stream);
 }
 
 extern "C" int lbmib_sg_pool2(const void *x, void *y, int B, int D, int H, int W, int C, void *stream) {

@@ -32,6 +32,7 @@ def _bind():
     L.lbmib_sg_conv3d_fwd.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, i, vp]
     L.lbmib_sg_conv3d_bwd_data.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, vp]
     L.lbmib_sg_conv3d_bwd_weight.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, vp]
+    L.lbmib_sg_conv3d_bwd_weight_noalloc.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, vp]
     L.lbmib_sg_pool2.argtypes = [vp, vp, i, i, i, i, i, vp]
     L.lbmib_sg_up2_add.argtypes = [vp, vp, vp, i, i, i, i, i, vp]
     L.lbmib_sg_pool2_bwd.argtypes = [vp, vp, vp, vp, i, i, i, i, i, vp]
@@ -51,7 +52,8 @@ def _bind():
 
 SG_SYMBOLS = ("lbmib_sg_conv3d_fwd", "lbmib_sg_conv3d_bwd_data", "lbmib_sg_conv3d_bwd_weight", "lbmib_sg_pool2",
               "lbmib_sg_up2_add", "lbmib_sg_pool2_bwd", "lbmib_sg_up2_bwd", "lbmib_sg_mse", "lbmib_sg_adam",
-              "lbmib_sg_pack", "lbmib_sg_target", "lbmib_sg_unpack", "lbmib_sg_sqdiff", "lbmib_sg_cast_bf16")
+              "lbmib_sg_pack", "lbmib_sg_target", "lbmib_sg_unpack", "lbmib_sg_sqdiff", "lbmib_sg_cast_bf16",
+              "lbmib_sg_conv3d_bwd_weight_noalloc")
 
 
 def check(rc, where):

@@ -55,6 +55,20 @@ int lbmib_sg_conv3d_bwd_weight(const void *x, const void *dy, float *dw, float *
                                int Cin, int Cout, void *stream);
 #define LBMIB_SG_WORKSPACE_BYTES (32u << 20)
 
+/* The same weight and bias gradients, bit for bit, with no allocation of
+ * any kind.  No counterpart in the reference: this entry point is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight training loops, harnesses that forbid hipMalloc).  Its FP32 partials
+ * live in a fixed device array of LBMIB_SG_NOALLOC_SCRATCH_BYTES (32 MiB)
+ * that each GPU the library runs on holds until the process ends; shapes
+ * whose partials need more (Cout * 27 * Cin * 4 bytes above that) return
+ * LBMIB_ERR_ARG.  All calls on one GPU share that array: calls that overlap
+ * on one GPU (different streams without ordering) race.  The caller passes
+ * every input and output. */
+int lbmib_sg_conv3d_bwd_weight_noalloc(const void *x, const void *dy, float *dw, float *db, int B, int D, int H,
+                                       int W, int Cin, int Cout, void *stream);
+#define LBMIB_SG_NOALLOC_SCRATCH_BYTES (32u << 20)
+
 /* 2x2x2 average pooling: x [B][2D][2H][2W][C] -> y [B][D][H][W][C]. */
 int lbmib_sg_pool2(const void *x, void *y, int B, int D, int H, int W, int C, void *stream);
 /* Nearest 2x upsampling plus skip: s[v] = c[parent(v)] + e[v] (e nullable);
