@@ -66,11 +66,16 @@ __host__ __device__ inline int ilog2(int v) {
 // runs over chunks of 32 k; the fragments of chunk c + 1 are loaded while
 // chunk c multiplies.  Operands come straight from global memory (L1/L2):
 // every voxel's 8-channel pieces are 16-byte aligned loads.
-template <int WM, int WN>
+//
+// SPLIT: the reduction is split over gridDim.z (chunks_per_split chunks
+// each) and the block writes its FP32 partial sums to part[z][v][n] instead
+// of the epilogue; k_conv_reduce then sums the partials in order of z.
+template <int WM, int WN, bool SPLIT>
 __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
                                                   const float *__restrict__ bias, const uint16_t *__restrict__ mask,
                                                   uint16_t *__restrict__ y, int D, int H, int W, long M, int Cin,
-                                                  int lgCin, int Cout, int K, int relu) {
+                                                  int lgCin, int Cout, int K, int relu, float *__restrict__ part,
+                                                  int chunks_per_split) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
   const long mw = ((long)blockIdx.x * 4 + wave) * (WM * 16);
@@ -95,7 +100,11 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x
 #pragma unroll
     for (int i = 0; i < WM; ++i) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  const int nch = (K + 31) >> 5;
+  int c_begin = 0, c_end = (K + 31) >> 5;
+  if (SPLIT) {
+    c_begin = blockIdx.z * chunks_per_split;
+    c_end = min(c_end, c_begin + chunks_per_split);
+  }
   bf16x8 a[WN], b[WM];
   auto load = [&](int c) {
     const int k = (c << 5) + (kq << 3);
@@ -116,20 +125,34 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x
       b[i] = ok ? ld8(x + (m + dz * HW + (long)dy * W + dx) * Cin + ci) : zero8();
     }
   };
-  load(0);
-  for (int c = 0; c < nch; ++c) {
+  if (c_begin < c_end) load(c_begin);
+  for (int c = c_begin; c < c_end; ++c) {
     bf16x8 ca[WN], cb[WM];
 #pragma unroll
     for (int j = 0; j < WN; ++j) ca[j] = a[j];
 #pragma unroll
     for (int i = 0; i < WM; ++i) cb[i] = b[i];
-    if (c + 1 < nch) load(c + 1);
+    if (c + 1 < c_end) load(c + 1);
 #pragma unroll
     for (int j = 0; j < WN; ++j)
 #pragma unroll
       for (int i = 0; i < WM; ++i) acc[j][i] = mfma16(ca[j], cb[i], acc[j][i]);
   }
   // epilogue: lane holds channels n0 + 16 j + 4 kq + r of voxel mw + 16 i + col
+  if (SPLIT) {
+    float *dst = part + (long)blockIdx.z * M * Cout;
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int n = n0 + j * 16 + 4 * kq;
+      if (n >= Cout) continue;
+#pragma unroll
+      for (int i = 0; i < WM; ++i) {
+        if (!vin[i]) continue;
+        *(f32x4 *)(dst + (mw + i * 16 + col) * Cout + n) = acc[j][i];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < WN; ++j) {
     const int n = n0 + j * 16 + 4 * kq;
@@ -164,14 +187,67 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x
   }
 }
 
+// y[v][n] = act(sum over z < ns of part[z][v][n] + bias[n]) [* (mask > 0)]:
+// one thread per (voxel, 4 channels)
+__global__ void k_conv_reduce(const float *__restrict__ part, int ns, const float *__restrict__ bias,
+                              const uint16_t *__restrict__ mask, uint16_t *__restrict__ y, long M, int Cout, int relu) {
+  const long nq = M * (Cout / 4);
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nq; e += (long)gridDim.x * blockDim.x) {
+    const long off = e * 4;
+    f32x4 s = {0.f, 0.f, 0.f, 0.f};
+    for (int z = 0; z < ns; ++z) s += *(const f32x4 *)(part + (long)z * M * Cout + off);
+    const int n = (int)(off % Cout);
+    float v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = s[r] + (bias ? bias[n + r] : 0.f);
+      if (relu) v[r] = fmaxf(v[r], 0.f);
+    }
+    if (mask) {
+      const u32x2 mk = *(const u32x2 *)(mask + off);
+      const uint16_t mb[4] = {(uint16_t)(mk[0] & 0xffff), (uint16_t)(mk[0] >> 16), (uint16_t)(mk[1] & 0xffff),
+                              (uint16_t)(mk[1] >> 16)};
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (!(bf2f(mb[r]) > 0.f)) v[r] = 0.f;
+    }
+    u32x2 o;
+    o[0] = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+    o[1] = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+    *(u32x2 *)(y + off) = o;
+  }
+}
+
+int workspace(void **out);
+
+// ksplit > 1: split reduction through the library workspace
 template <int WM, int WN>
 int launch_fwd(const void *x, const void *w, const float *bias, const void *mask, void *y, int D, int H, int W, long M,
-               int Cin, int Cout, int relu, hipStream_t s) {
+               int Cin, int Cout, int relu, int ksplit, hipStream_t s) {
   const int K = 27 * Cin;
   const long bm = 4L * WM * 16;
+  const int nch = (K + 31) / 32;
   dim3 grid((unsigned)((M + bm - 1) / bm), (unsigned)((Cout + WN * 16 - 1) / (WN * 16)));
-  k_conv_fwd<WM, WN><<<grid, 256, 0, s>>>((const uint16_t *)x, (const uint16_t *)w, bias, (const uint16_t *)mask,
-                                           (uint16_t *)y, D, H, W, M, Cin, ilog2(Cin), Cout, K, relu);
+  if (ksplit <= 1) {
+    k_conv_fwd<WM, WN, false><<<grid, 256, 0, s>>>((const uint16_t *)x, (const uint16_t *)w, bias,
+                                                    (const uint16_t *)mask, (uint16_t *)y, D, H, W, M, Cin,
+                                                    ilog2(Cin), Cout, K, relu, nullptr, nch);
+    HIPCHK(hipGetLastError());
+    return LBMIB_OK;
+  }
+  void *ws = nullptr;
+  const int rc = workspace(&ws);
+  if (rc) return rc;
+  const int cps = (nch + ksplit - 1) / ksplit;
+  grid.z = (nch + cps - 1) / cps;
+  k_conv_fwd<WM, WN, true><<<grid, 256, 0, s>>>((const uint16_t *)x, (const uint16_t *)w, nullptr, nullptr, nullptr,
+                                                 D, H, W, M, Cin, ilog2(Cin), Cout, K, relu, (float *)ws, cps);
+  HIPCHK(hipGetLastError());
+  const long nq = M * (Cout / 4);
+  long nb = (nq + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  k_conv_reduce<<<(unsigned)nb, 256, 0, s>>>((const float *)ws, (int)grid.z, bias, (const uint16_t *)mask,
+                                              (uint16_t *)y, M, Cout, relu);
   HIPCHK(hipGetLastError());
   return LBMIB_OK;
 }
@@ -194,13 +270,22 @@ int conv_fwd(const void *x, const void *w, const float *bias, const void *mask, 
   const int wn = Cout % 64 == 0 ? 4 : (Cout % 32 == 0 ? 2 : 1);
   const long blocks64 = (M + 255) / 256 * (Cout / (16 * wn));
   if (blocks64 >= 512) {
-    if (wn == 4) return launch_fwd<4, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
-    if (wn == 2) return launch_fwd<4, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
-    return launch_fwd<4, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+    if (wn == 4) return launch_fwd<4, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
+    if (wn == 2) return launch_fwd<4, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
+    return launch_fwd<4, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
   }
-  if (wn == 4) return launch_fwd<1, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
-  if (wn == 2) return launch_fwd<1, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
-  return launch_fwd<1, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+  // small grids: split the reduction so that about 1024 blocks run, each
+  // with at least 4 chunks of 32, within the workspace
+  const long blocks = (M + 63) / 64 * (Cout / (16 * wn));
+  const int nch = (27 * Cin + 31) / 32;
+  long ks = (1024 + blocks - 1) / blocks;
+  if (ks > nch / 4) ks = nch / 4;
+  const long cap = (long)(LBMIB_SG_WORKSPACE_BYTES / ((size_t)M * Cout * sizeof(float)));
+  if (ks > cap) ks = cap;
+  if (ks < 1) ks = 1;
+  if (wn == 4) return launch_fwd<1, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, (int)ks, s);
+  if (wn == 2) return launch_fwd<1, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, (int)ks, s);
+  return launch_fwd<1, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, (int)ks, s);
 }
 
 // wt[ci][26 - t][co] = w[co][t][ci]

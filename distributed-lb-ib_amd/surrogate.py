@@ -278,15 +278,26 @@ class Net:
         return loss
 
     # ------------------------------------------------------------ weights
-    def state(self):
+    def state(self, bf16=False):
+        """Weights and biases as numpy arrays; bf16=True stores the weights as
+        the BF16 bit patterns (uint16) the convolutions read, half the size
+        and the same inference results."""
         st = {}
+        torch.cuda.synchronize()
         for name in self.p:
-            st[name + ".w"] = self.p[name].cpu().numpy()
+            if bf16:
+                st[name + ".w"] = self.w16[name].view(torch.int16).cpu().numpy().view(np.uint16)
+            else:
+                st[name + ".w"] = self.p[name].cpu().numpy()
             st[name + ".b"] = self.b[name].cpu().numpy()
         return st
 
     def load_state(self, st):
         for name in self.p:
-            self.p[name].copy_(torch.from_numpy(np.asarray(st[name + ".w"], dtype=np.float32)))
+            w = np.asarray(st[name + ".w"])
+            if w.dtype == np.uint16:  # BF16 bit patterns
+                w = (w.astype(np.uint32) << 16).view(np.float32)
+            self.p[name].copy_(torch.from_numpy(np.ascontiguousarray(w, dtype=np.float32)))
             self.b[name].copy_(torch.from_numpy(np.asarray(st[name + ".b"], dtype=np.float32)))
             self._cast(self.p[name], self.w16[name])
+        torch.cuda.synchronize()

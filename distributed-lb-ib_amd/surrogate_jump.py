@@ -74,14 +74,16 @@ def load_model(path):
 
 
 def save_model(path, net, norm, factor, ksteps, extra=None):
-    st = net.state()
+    """Weights as BF16 bit patterns plus the normalisation and the jump's
+    constants, in a compressed .npz (numpy arrays only)."""
+    st = net.state(bf16=True)
     st.update(norm.arrays())
     st["chans"] = np.array(net.chans)
     st["factor"] = np.array(factor)
     st["ksteps"] = np.array(ksteps)
     for k, v in (extra or {}).items():
         st[k] = np.asarray(v)
-    np.savez(path, **st)
+    np.savez_compressed(path, **st)
 
 
 class Coarse:
@@ -115,7 +117,7 @@ class JumpRunner:
     tol: acceptance bound of the verification ratio; 0 rejects every jump
     (all plain steps, bit-identical to lbmib_step)."""
 
-    def __init__(self, engine, net, norm, tau, u_in, factor=4, ksteps=50, verify_steps=1, tol=4.0):
+    def __init__(self, engine, net, norm, tau, u_in, factor=4, ksteps=50, verify_steps=1, tol=4.0, graph=True):
         self.e, self.net, self.norm = engine, net, norm
         self.K, self.R, self.tol = ksteps, verify_steps, tol
         self.consts = const_channels(tau, u_in)
@@ -130,16 +132,37 @@ class JumpRunner:
         self.sums = torch.zeros(4, dtype=torch.float64, device="cuda")
         self.stats = {"jumps": 0, "accepted": 0, "rejected": 0, "plain_steps": 0, "ratios": []}
         self.have_feat = False
+        self.use_graph, self.graph = graph, None
+
+    def _predict_kernels(self):
+        L = sg._bind()
+        pack_input(self.co.feat, self.norm, self.consts, self.x, self.co.nc, self.stream)
+        out = self.net.forward(self.x)
+        sg.check(L.lbmib_sg_unpack(ctypes.c_void_p(out.data_ptr()), 16, NOUT, sg.dvec(self.norm.out_mean),
+                                   sg.dvec(self.norm.out_std), ctypes.c_void_p(self.delta.data_ptr()), self.co.nc,
+                                   self.stream), "unpack")
 
     def predict(self):
-        """delta = the network's change of (rho, ux, uy, uz) from self.co.feat."""
-        L = sg._bind()
+        """delta = the network's change of (rho, ux, uy, uz) from self.co.feat.
+
+        The first call runs the kernels one by one (it also sizes the
+        network's buffers and the library workspace); with graph=True the
+        second call records the same launches -- packing, the convolutions,
+        unpacking, all on the engine's stream, reading and writing only
+        buffers that stay in place -- into a HIP graph, and every call from
+        then on replays it."""
         with torch.cuda.stream(self.ext):
-            pack_input(self.co.feat, self.norm, self.consts, self.x, self.co.nc, self.stream)
-            out = self.net.forward(self.x)
-            sg.check(L.lbmib_sg_unpack(ctypes.c_void_p(out.data_ptr()), 16, NOUT, sg.dvec(self.norm.out_mean),
-                                       sg.dvec(self.norm.out_std), ctypes.c_void_p(self.delta.data_ptr()), self.co.nc,
-                                       self.stream), "unpack")
+            if self.graph is not None:
+                self.graph.replay()
+            elif not self.use_graph or not getattr(self, "_eager_done", False):
+                self._predict_kernels()
+                self._eager_done = True
+            else:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=self.ext):
+                    self._predict_kernels()
+                self.graph = g
+                g.replay()
         return self.delta
 
     def _sqdiff(self, a, b, out):
@@ -207,3 +230,62 @@ def zero_predictor(runner):
     the jump (the sheet still moves with the coarse velocity).  Called inside
     JumpRunner.cycle, on the engine's stream."""
     runner.delta.zero_()
+
+
+def evaluate(net, norm, kw, warm, horizon, factor=4, ksteps=50, verify_steps=1, tol=4.0, baseline=True,
+             log=print):
+    """The 256^3 evaluation: from the same state after `warm` plain steps,
+    `horizon` more steps (a) plain, (b) with surrogate jumps, (c) with jumps
+    of the method without the network (zero change).  Returns timings
+    (wall clock around each leg, synchronised) and the velocity error of
+    (b) and (c) against (a): relative L2 over the whole field, and relative
+    to the departure of (a) from the uniform inflow."""
+    import time
+    out = {"warm": warm, "horizon": horizon, "ksteps": ksteps, "verify_steps": verify_steps, "tol": tol,
+           "factor": factor}
+    u_in = kw["u0"][0]
+
+    def fields(e):
+        rho, vel = e.macroscopic()
+        return vel
+
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(warm)
+        e.sync()
+        t0 = time.time()
+        e.step(horizon)
+        e.sync()
+        out["plain_s"] = time.time() - t0
+        assert e.status() == 0
+        ref = fields(e)
+        ref_x = e.fibers(0)[..., :3]
+    legs = [("surrogate", None)] + ([("no_network", zero_predictor)] if baseline else [])
+    for name, pred in legs:
+        with lbmib.Engine(**kw) as e:
+            e.init()
+            e.step(warm)
+            r = JumpRunner(e, net, norm, kw["tau"], u_in, factor, ksteps, verify_steps, tol)
+            if pred is None:
+                r.predict()   # first use allocates the network's buffers outside the timed leg
+            e.sync()
+            t0 = time.time()
+            st = r.run(horizon, pred)
+            e.sync()
+            dt = time.time() - t0
+            status = e.status()
+            vel = fields(e)
+            x = e.fibers(0)[..., :3]
+        pert = ref.copy()
+        pert[0] -= u_in
+        leg = {"seconds": dt, "status": status, "jumps": st["jumps"], "accepted": st["accepted"],
+               "rejected": st["rejected"], "plain_steps": st["plain_steps"],
+               "ratio_max": max(st["ratios"]) if st["ratios"] else None,
+               "ratio_median": float(np.median(st["ratios"])) if st["ratios"] else None,
+               "vel_rel_l2": float(np.linalg.norm(vel - ref) / np.linalg.norm(ref)),
+               "vel_rel_l2_perturbation": float(np.linalg.norm(vel - ref) / np.linalg.norm(pert)),
+               "fiber_pos_max_abs": float(np.abs(x - ref_x).max()),
+               "speedup": out["plain_s"] / dt}
+        out[name] = leg
+        log(f"[eval] {name}: {leg}")
+    return out

@@ -104,7 +104,14 @@ def norm_from(trajs, lag, nsample=64, seed=0):
     d = np.stack(ds, 1).reshape(sj.NOUT, -1)
     in_std = np.maximum(x.std(axis=1), 1e-12)
     out_std = np.maximum(d.std(axis=1), 1e-12)
-    return sj.Norm(x.mean(axis=1), in_std, d.mean(axis=1), out_std)
+    # the three velocity planes share one scale (the RMS of their spreads),
+    # in the input and in the change, so that the loss weighs velocity errors
+    # as the velocity L2 error does (a component that barely varies is not
+    # blown up to unit size)
+    in_std[1:4] = np.sqrt(np.mean(in_std[1:4] ** 2))
+    out_std[1:4] = np.sqrt(np.mean(out_std[1:4] ** 2))
+    out_mean = d.mean(axis=1)
+    return sj.Norm(x.mean(axis=1), in_std, out_mean, out_std)
 
 
 class Loader:

@@ -198,3 +198,34 @@ def test_coupling_rejects_bad_shapes_and_states(lbmib):
             e.jump_revert()
         with pytest.raises(lbmib.LbmibError):      # factor 3
             e.coarse_moments(3, cm.data_ptr())
+
+
+def test_prediction_graph_replays_match_eager(lbmib):
+    """The network's launches recorded in a HIP graph on the engine's stream:
+    every replay, with the engine's own steps and synchronisations between
+    replays, gives the same bits as the launches run one by one."""
+    import surrogate as sg
+    import surrogate_jump as sj
+    kw = small_cfg(lbmib, "f32")
+    net = sg.Net(cin=sj.CIN, cout=16, nout=sj.NOUT, seed=4, out_scale=0.05)
+    norm = sj.Norm([1.0, 0.02, 0, 0, 0, 0, 0, 0], [1e-3, 3e-3, 3e-3, 3e-3, 1.0, 1e-4, 1e-4, 1e-4], [0.0] * 4,
+                   [1e-4] * 4)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(3)
+        eager = sj.JumpRunner(e, net, norm, kw["tau"], kw["u0"][0], factor=4, ksteps=10, graph=False)
+        graphed = sj.JumpRunner(e, net, norm, kw["tau"], kw["u0"][0], factor=4, ksteps=10, graph=True)
+        for rep in range(4):
+            eager.co.read()
+            graphed.co.read()
+            eager.predict()
+            e.sync()                      # the engine's stream does not order with torch's
+            want = eager.delta.cpu().numpy().copy()
+            graphed.predict()
+            e.sync()
+            got = graphed.delta.cpu().numpy()
+            assert (graphed.graph is not None) == (rep >= 1)
+            np.testing.assert_array_equal(got, want)
+            e.step(2)
+            e.sync()
+

@@ -78,6 +78,7 @@ CONV_CASES = [
     (1, 4, 8, 8, 64, 128), (2, 4, 4, 4, 128, 128), (1, 4, 4, 8, 128, 256), (1, 2, 4, 4, 256, 256),
     (1, 4, 4, 4, 256, 128), (1, 8, 8, 8, 128, 64), (1, 8, 16, 8, 64, 32), (1, 8, 8, 8, 32, 16),
     (3, 5, 7, 3, 16, 48), (1, 64, 16, 16, 32, 32),
+    (1, 32, 64, 64, 32, 32),  # 131072 voxels: the unsplit four-fragment path
 ]
 
 
