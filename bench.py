@@ -13,7 +13,12 @@ Prints one JSON line: MLUPS over all nx*ny*nz cells, the HBM roofline of the
 fused collide-stream kernel (algorithmic 2*19*4 = 152 B per cell, its mean
 launch time over a pass of unsplit steps, HIP events on the engine's
 stream), and the reference's own pthread code timed on this machine's host
-cores on a bounded sample of the same grid.
+cores on a bounded sample of the same grid.  The default run adds, measured
+in the same run: a `mixed_precision` block (the 512x256x256 two-sheet
+workload in FP32 and the 16-bit forms: step times, ratio, velocity error
+against FP64 after its 2000 steps) and a `surrogate` block (surrogate jumps
+of 50 steps on the 256^3 workload against plain steps: simulated steps/s,
+speed-up and velocity error, and the method without the network).
 """
 import argparse
 import json
@@ -58,6 +63,8 @@ def parse_args():
     p.add_argument("--dev-ref", default="eq0", choices=["eq0", "rest"],
                    help="reference populations of the deviation forms: feq(rho0, u0x) or w")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the mixed-precision (C3) and surrogate blocks of the default run")
     p.add_argument("--cpu-steps", default="2,6", help="N1,N2 steps of the differenced CPU timing")
     p.add_argument("--variant", default="auto",
                    help="fused-kernel launch shape 'cells,ntmask,blocks_per_cu' or 'auto' (autotune)")
@@ -279,6 +286,79 @@ def run_batch(a, lbmib, lbmib_configs):
     print(json.dumps(out))
 
 
+def mixed_precision_block(lbmib, lbmib_configs, steps=200, warm=20, accuracy_steps=2000):
+    """The 512x256x256 two-sheet workload (C3) in FP32 and in the 16-bit
+    deviation forms, measured in this run: whole-step and fused-kernel times
+    over `steps` unsplit steps with two cells per lane, and the velocity error
+    of each 16-bit form against the FP64 path after the workload's
+    `accuracy_steps` steps (relative L2 over the field, and the maximum
+    relative to max|u|), as tests/test_gpu_parity.py::test_c3_workload_16bit_vs_f64
+    measures it."""
+    import numpy as np
+    out = {"workload": "512x256x256 D3Q19 BGK, two 128x128 sheets at x = 128 and 256, u_in from seed 3",
+           "storage": {p: storage_desc(p, "eq0") for p in ("f32", "f16", "bf16")}, "steps_timed": steps}
+    vel = {}
+    for prec in ("f64", "f32", "f16", "bf16"):
+        kw, nominal = lbmib_configs.c3(precision=prec)
+        kw["dev_ref"] = "eq0"
+        if prec != "f64":
+            with lbmib.Engine(**kw) as e:
+                e.set_overlap(0)
+                e.set_kernel_variant(2, 3, 0)  # two cells per lane, the shape the autotune picks here
+                e.init()
+                ms_total, ms_fused = e.time_steps(warm, steps)
+                out[prec] = {"ms_per_step": round(ms_total / steps, 5), "fused_kernel_ms": round(ms_fused, 5),
+                             "mlups": round(kw["nx"] * kw["ny"] * kw["nz"] / (ms_total / steps * 1e-3) / 1e6, 1)}
+                print(f"[mixed] {prec}: {ms_total / steps:.4f} ms/step, fused {ms_fused:.4f} ms", file=sys.stderr,
+                      flush=True)
+        # accuracy: a run of the workload's full length from the initial
+        # state (every launch shape gives the same bits)
+        with lbmib.Engine(**kw) as e:
+            e.init()
+            e.step(accuracy_steps)
+            if e.status() != 0:
+                raise lbmib.LbmibError(e.status(), f"c3 {prec}")
+            vel[prec] = e.macroscopic()[1]
+    v64 = vel["f64"]
+    for prec in ("f32", "f16", "bf16"):
+        d = vel[prec] - v64
+        out[prec]["vel_rel_l2_vs_f64"] = float(np.linalg.norm(d) / np.linalg.norm(v64))
+        out[prec]["vel_max_rel_vs_f64"] = float(np.abs(d).max() / np.abs(v64).max())
+    out["speedup_f16_vs_f32"] = round(out["f32"]["ms_per_step"] / out["f16"]["ms_per_step"], 4)
+    out["speedup_bf16_vs_f32"] = round(out["f32"]["ms_per_step"] / out["bf16"]["ms_per_step"], 4)
+    out["accuracy_steps"] = accuracy_steps
+    out["target"] = ">= 1.7x FP32 at <= 1e-3 relative velocity error (relative L2 over the field)"
+    out["meets_target"] = {p: bool(out[f"speedup_{p}_vs_f32"] >= 1.7 and out[p]["vel_rel_l2_vs_f64"] <= 1e-3)
+                           for p in ("f16", "bf16")}
+    return out
+
+
+def surrogate_block(lbmib_configs, warm=100, horizon=1020, tol=5.0):
+    """The learned surrogate on the 256^3 workload: from the state after
+    `warm` plain steps, `horizon` more steps plain, with surrogate jumps of K
+    steps (each re-verified by one plain step, falling back to plain steps
+    when the verification ratio exceeds `tol`), and with the method without
+    the network; wall clock around each leg, velocity error against the
+    plain run."""
+    import surrogate_jump as sj
+    path = os.path.join(PKG, "models", "surrogate_c5.npz")
+    if not os.path.exists(path):
+        return {"error": "no trained model at distributed-lb-ib_amd/models/surrogate_c5.npz"}
+    net, norm, factor, ksteps, _ = sj.load_model(path)
+    kw, _ = lbmib_configs.c2(precision="f32")
+    res = sj.evaluate(net, norm, kw, warm, horizon, factor, ksteps, 1, tol, baseline=True,
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    s = res["surrogate"]
+    res["model"] = "distributed-lb-ib_amd/models/surrogate_c5.npz"
+    res["plain_steps_per_s"] = round(horizon / res["plain_s"], 1)
+    res["surrogate_steps_per_s"] = round(horizon / s["seconds"], 1)
+    res["speedup"] = round(s["speedup"], 3)
+    res["vel_rel_l2"] = s["vel_rel_l2"]
+    res["target"] = ">= 10x plain simulated steps/s at <= 2% velocity L2 error, with fallback to the plain solver"
+    res["meets_target"] = bool(s["speedup"] >= 10.0 and s["vel_rel_l2"] <= 0.02)
+    return res
+
+
 def calibrate_copy():
     """A plain copy of 2^28 float32 (1 GiB read + 1 GiB written per launch),
     five launches: the counters it shows per dispatch calibrate the
@@ -351,6 +431,19 @@ def main():
     print(f"[bench] {ms_step:.4f} ms/step, {mlups:.1f} MLUPS, fused kernel {ms_main:.4f} ms "
           f"= {achieved:.1f} GB/s", file=sys.stderr, flush=True)
 
+    extra = {}
+    if a.workload == "c2" and a.precision == "f32" and not a.no_extras:
+        # the other headline forms, measured in the same run (GPU legs before
+        # the CPU baseline)
+        try:
+            extra["mixed_precision"] = mixed_precision_block(lbmib, lbmib_configs)
+        except Exception as ex:  # a report beside the measurement
+            extra["mixed_precision"] = {"error": str(ex)}
+        try:
+            extra["surrogate"] = surrogate_block(lbmib_configs)
+        except Exception as ex:
+            extra["surrogate"] = {"error": str(ex)}
+
     cpu = None
     if not a.no_cpu_baseline and a.workload == "c2":
         n1, n2 = (int(x) for x in a.cpu_steps.split(","))
@@ -402,6 +495,7 @@ def main():
         "mass_after": mass,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

@@ -75,6 +75,14 @@ def load_library(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is not built (run __graft_entry__.build() or make -C distributed-lb-ib_amd)")
+    # PyTorch's ROCm build carries its own HIP runtime under the same soname
+    # as the system one liblbmib.so links; loaded first, it serves both, so a
+    # process that also uses torch (the surrogate's device memory) sees one
+    # runtime.  Loaded after liblbmib.so, torch would find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)

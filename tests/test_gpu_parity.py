@@ -600,6 +600,67 @@ def test_cli_fieldview_dumps(tmp_path, fixtures_meta):
             assert np.abs(vel[:, Z, Y, X].T - fx["vel"]).max() <= 5e-7
 
 
+# ----------------------------------------------- oracle anchors at full size
+
+def test_c2_full_size_f64_matches_oracle(lbmib, configs):
+    """The 256^3 workload (104 x 104 sheet) in FP64 for 5 steps against the
+    CPU oracle (lbm_ib.c:1673-2538 semantics): the truth that the full-size
+    FP32 and 16-bit accuracy tests compare with is itself anchored at this
+    size.  Positions within 1e-12, velocity within 1e-11 and density within
+    1e-14 relative."""
+    kw, _ = configs.c2(precision="f64")
+    steps = 5
+    o = oracle_for(kw)
+    assert o.step(steps) == 0
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(steps)
+        assert e.status() == 0
+        got = e.fibers(0)
+        rho, vel = e.macroscopic()
+    ref = o.nodes(0)
+    dx = np.abs(got[..., :3] - ref[..., :3]).max()
+    eu, er = rel(vel, o.vel()), rel(rho, o.rho())
+    print(f"256^3 f64 vs oracle after {steps} steps: positions {dx:.2e}, velocity {eu:.2e}, density {er:.2e}")
+    assert dx < 1e-12
+    assert eu < 1e-11
+    assert er < 1e-14
+
+
+# instances of the C4 ensemble the small oracle tests do not reach: tau near
+# 0.55, stiffness near 1e-2, edges 32, 48 and 96
+C4_PICKS = {3: "32^3, tau 0.83", 18: "48^3, tau 0.57", 29: "smallest tau (32^3, 0.553)",
+            19: "largest K (96^3, 7.7e-3)"}
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_c4_instances_match_oracle(lbmib, configs, precision):
+    """Four instances of the seeded C4 ensemble stepped together by the
+    batched launches for 100 steps, each against its own CPU oracle run
+    (lbm_ib.c:3115-3563): F64 positions within 1e-11, velocity within 1e-10
+    and density within 1e-13 relative; F32 velocity within 1e-5 of max|u|
+    and positions within 1e-6."""
+    cfgs, _ = configs.c4(precision=precision)
+    sel = [cfgs[i] for i in C4_PICKS]
+    steps = 100
+    with lbmib.Batch(sel) as b:
+        b.init()
+        b.step(steps)
+        assert b.status() == 0
+        for e, (i, what), cfg in zip(b.instances, C4_PICKS.items(), sel):
+            o = oracle_for(cfg)
+            assert o.step(steps) == 0
+            got, ref = e.fibers(0), o.nodes(0)
+            rho, vel = e.macroscopic()
+            dx = np.abs(got[..., :3] - ref[..., :3]).max()
+            eu, er = rel(vel, o.vel()), rel(rho, o.rho())
+            print(f"c4 instance {i} ({what}) {precision}: positions {dx:.2e}, velocity {eu:.2e}, density {er:.2e}")
+            if precision == "f64":
+                assert dx < 1e-11 and eu < 1e-10 and er < 1e-13, (i, dx, eu, er)
+            else:
+                assert dx < 1e-6 and eu < 1e-5, (i, dx, eu)
+
+
 # ------------------------------------------------------- batched ensembles
 
 def _ensemble(lbmib, precision):
