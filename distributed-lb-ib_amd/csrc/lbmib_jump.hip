@@ -19,6 +19,8 @@
 #include <limits.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lbmib_ctx.h"
 #include "lbmib_device.h"
 
@@ -35,62 +37,25 @@ namespace {
     if (_r) return _r;  \
   } while (0)
 
+// Arithmetic of the coupling kernels: FP64 for FP64 storage; FP32 on the
+// deviations h = f - r for the deviation forms (their moments are rho_r +
+// sum h and rho_r u_r + sum c h, so nothing cancels against the O(0.3)
+// reference populations).
+template <int P>
+using JumpT = typename std::conditional<P == LBMIB_F64, double, float>::type;
+
 // ------------------------------------------------------------ coarse moments
 //
 // Block: 64 x 4 fine cells of one (x, y) tile; each thread walks the F cells
 // of its column in z, so a block covers (64/F) x (4/F) coarse cells of one
-// coarse z-plane.  Per fine cell the moments are those of
-// lbmib_get_macroscopic (post-stream populations, u = (sum c f + F/2) / rho,
-// the initial rho0/u0 outside the moment box); the block sums F^3 of them
-// in a fixed order (lane shuffles in x, LDS in y).
-template <int P, int F>
-__global__ void __launch_bounds__(256)
-    k_coarse_moments(Geometry g, const typename Traits<P>::S *__restrict__ f, int own,
-                     const typename Traits<P>::T *__restrict__ tab, const double *__restrict__ ref,
-                     typename Traits<P>::T b1, typename Traits<P>::T b2, double rho0, double u0x, double u0y,
-                     double u0z, double gz, int nsheets, const int *__restrict__ box,
-                     const typename Traits<P>::FT *__restrict__ Fc, float *__restrict__ out) {
-  using T = typename Traits<P>::T;
-  __shared__ double red[4][4][64];
+// coarse z-plane.  The block sums F^3 fine values per coarse cell in a fixed
+// order (lane shuffles in x, LDS in y) and writes their mean.
+
+// acc: this thread's sums over its F cells in z; out: 4 planes of nc cells
+template <int F>
+__device__ __forceinline__ void block_mean_store(double acc[4], double (*red)[4][64], const Geometry &g,
+                                                 float *__restrict__ out) {
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int x = blockIdx.x * 64 + tx, y = blockIdx.y * 4 + ty;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < F; ++k) {
-    const int z = blockIdx.z * F + k;
-    const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
-    const bool mom =
-        x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] && z <= g.mhi[2];
-    double r = rho0, ux = u0x, uy = u0y, uz = u0z;
-    if (act && mom && !own) {
-      T v[19];
-      const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
-      if (frozen) {
-        for (int q = 0; q < 19; ++q) v[q] = tab[q] * Traits<P>::inv_scale;
-      } else {
-        pull<P>(g, f, x, y, z, b1, b2, v);
-      }
-      const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
-      double fx = 0, fy = 0, fz = 0;
-      if (in_any_box(box, nsheets, x, y, z)) { fx = Fc[c0]; fy = Fc[g.n + c0]; fz = Fc[2 * g.n + c0]; }
-      double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        const double fq = Traits<P>::dev ? (double)v[q] + ref[q] : (double)v[q];
-        s1 += fq;
-        s2 += kCX[q] * fq;
-        s3 += kCY[q] * fq;
-        s4 += kCZ[q] * fq;
-      }
-      r = s1;
-      ux = (s2 + 0.5 * fx) / s1;
-      uy = (s3 + 0.5 * fy) / s1;
-      uz = (s4 + 0.5 * (fz + r * gz)) / s1;
-    }
-    acc[0] += r;
-    acc[1] += ux;
-    acc[2] += uy;
-    acc[3] += uz;
-  }
 #pragma unroll
   for (int m = 1; m < F; m <<= 1)
 #pragma unroll
@@ -99,17 +64,110 @@ __global__ void __launch_bounds__(256)
   for (int a = 0; a < 4; ++a) red[a][ty][tx] = acc[a];
   __syncthreads();
   if (ty % F == 0 && tx % F == 0) {
+    const int x = blockIdx.x * 64 + tx, y = blockIdx.y * 4 + ty;
     const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
     const long nc = (long)ncx * ncy * ncz;
     const long cc = (long)(x / F) + (long)ncx * ((long)(y / F) + (long)ncy * blockIdx.z);
     const double inv = 1.0 / (F * F * F);
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      double s = 0.0;
-      for (int k = 0; k < F; ++k) s += red[a][ty + k][tx];
-      out[a * nc + cc] = (float)(s * inv);
+      double t = 0.0;
+      for (int k = 0; k < F; ++k) t += red[a][ty + k][tx];
+      out[a * nc + cc] = (float)(t * inv);
     }
   }
+  __syncthreads();
+}
+
+// Per fine cell, the moments of lbmib_get_macroscopic (post-stream
+// populations, u = (sum c f + F/2) / rho, the initial rho0/u0 outside the
+// moment box) into `out`; with `stored`, also the moments of the stored
+// (post-collision) populations of the cell itself, u = sum c f / rho, on the
+// updated cells (the active box without the frozen inlet/outlet planes) and
+// rho0/u0 elsewhere -- the convention k_jump_apply reports the jumped state in.
+template <int P, int F>
+__global__ void __launch_bounds__(256)
+    k_coarse_moments(Geometry g, const typename Traits<P>::S *__restrict__ f, int own,
+                     const typename Traits<P>::T *__restrict__ tab, double rhor, double mrx,
+                     typename Traits<P>::T b1, typename Traits<P>::T b2, double rho0, double u0x, double u0y,
+                     double u0z, double gz, int nsheets, const int *__restrict__ box,
+                     const typename Traits<P>::FT *__restrict__ Fc, float *__restrict__ out,
+                     float *__restrict__ stored) {
+  using T = typename Traits<P>::T;
+  __shared__ double red[4][4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int x = blockIdx.x * 64 + tx, y = blockIdx.y * 4 + ty;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, sacc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < F; ++k) {
+    const int z = blockIdx.z * F + k;
+    const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
+    const bool mom =
+        x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] && z <= g.mhi[2];
+    const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+    using A = JumpT<P>;
+    A r = (A)rho0, ux = (A)u0x, uy = (A)u0y, uz = (A)u0z;
+    if (act && mom && !own) {
+      T v[19];
+      if (frozen) {
+        for (int q = 0; q < 19; ++q) v[q] = tab[q] * Traits<P>::inv_scale;
+      } else {
+        pull<P>(g, f, x, y, z, b1, b2, v);
+      }
+      const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
+      A fx = 0, fy = 0, fz = 0;
+      if (in_any_box(box, nsheets, x, y, z)) { fx = (A)Fc[c0]; fy = (A)Fc[g.n + c0]; fz = (A)Fc[2 * g.n + c0]; }
+      A s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const A fq = (A)v[q];  // f, or the deviation f - r
+        s1 += fq;
+        s2 += (A)kCX[q] * fq;
+        s3 += (A)kCY[q] * fq;
+        s4 += (A)kCZ[q] * fq;
+      }
+      if (Traits<P>::dev) {  // sum r = rho_r, sum c r = rho_r u_r e_x
+        s1 += (A)rhor;
+        s2 += (A)mrx;
+      }
+      r = s1;
+      ux = (s2 + (A)0.5 * fx) / s1;
+      uy = (s3 + (A)0.5 * fy) / s1;
+      uz = (s4 + (A)0.5 * (fz + r * (A)gz)) / s1;
+    }
+    acc[0] += (double)r;
+    acc[1] += (double)ux;
+    acc[2] += (double)uy;
+    acc[3] += (double)uz;
+    if (stored) {
+      A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
+      if (act && !frozen) {
+        const long p0 = pop_index(g, x, y, z);
+        A s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+          const A fq = (A)ld<P>(f + q * g.pq + p0);
+          s1 += fq;
+          s2 += (A)kCX[q] * fq;
+          s3 += (A)kCY[q] * fq;
+          s4 += (A)kCZ[q] * fq;
+        }
+        if (Traits<P>::dev) {
+          s1 += (A)rhor;
+          s2 += (A)mrx;
+        }
+        sr = s1;
+        sx = s2 / s1;
+        sy = s3 / s1;
+        sz = s4 / s1;
+      }
+      sacc[0] += (double)sr;
+      sacc[1] += (double)sx;
+      sacc[2] += (double)sy;
+      sacc[3] += (double)sz;
+    }
+  }
+  block_mean_store<F>(acc, red, g, out);
+  if (stored) block_mean_store<F>(sacc, red, g, stored);
 }
 
 // trilinear interpolation of coarse plane `a` at fine position (px, py, pz)
@@ -131,57 +189,109 @@ __device__ __forceinline__ double interp(const float *__restrict__ cf, int a, lo
   return s;
 }
 
-// feq_q(rho, u) of init_eqlbrmdistrfuncDF0 (lbm_ib.c:1561-1594)
+// feq_q(rho', u') - feq_q(rho, u) of init_eqlbrmdistrfuncDF0's expression
+// (lbm_ib.c:1561-1594) in difference form, u' = u + du, rho' = rho + drho:
+//   w [drho (1 + 3 cu + 4.5 cu^2 - 1.5 u^2)
+//      + rho' (3 c.du + 4.5 c.du (cu + cu') - 1.5 du.(u + u'))]
 template <class T>
-__device__ __forceinline__ T feq_q(int q, T rho, T ux, T uy, T uz) {
+__device__ __forceinline__ T dfeq_q(int q, T rho, T drho, T ux, T uy, T uz, T dx, T dy, T dz) {
+  const T cx = (T)kCX[q], cy = (T)kCY[q], cz = (T)kCZ[q];
+  const T cu = cx * ux + cy * uy + cz * uz;
+  const T cd = cx * dx + cy * dy + cz * dz;
   const T usq = ux * ux + uy * uy + uz * uz;
-  const T cu = (T)kCX[q] * ux + (T)kCY[q] * uy + (T)kCZ[q] * uz;
-  const T w = (T)1 / (T)kWDEN[q];
-  return w * rho * ((T)1 + (T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * usq);
+  const T ud = dx * (ux + ux + dx) + dy * (uy + uy + dy) + dz * (uz + uz + dz);  // du.(u + u')
+  const T a = (T)1 + (T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * usq;
+  const T b = (T)3 * cd + (T)4.5 * cd * (cu + cu + cd) - (T)1.5 * ud;
+  return (drho * a + (rho + drho) * b) / (T)kWDEN[q];
 }
 
-// One cell per thread (x across the block, y and z from the grid): the
-// updated cells -- the active box without the frozen inlet/outlet planes --
-// take the equilibrium shift; every other element is copied unchanged.
-template <int P>
-__global__ void k_jump_apply(Geometry g, const typename Traits<P>::S *__restrict__ fin,
-                             typename Traits<P>::S *__restrict__ fout, const double *__restrict__ ref,
-                             const float *__restrict__ delta, int F) {
-  using T = typename Traits<P>::T;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x >= g.nx) return;
-  const long p0 = pop_index(g, x, y, z);
-  const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
-  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
-  if (!act || frozen) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fout[q * g.pq + p0] = fin[q * g.pq + p0];
-    return;
-  }
-  T h[19];
-  T rho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    h[q] = ld_h<P>(fin + q * g.pq + p0);
-    const T fq = Traits<P>::dev ? h[q] * (T)Traits<P>::inv_scale + (T)ref[q] : h[q];
-    rho += fq;
-    jx += (T)kCX[q] * fq;
-    jy += (T)kCY[q] * fq;
-    jz += (T)kCZ[q] * fq;
-  }
-  const T ux = jx / rho, uy = jy / rho, uz = jz / rho;
+// The tile layout of k_coarse_moments (64 x 4 cells, F cells in z per
+// thread): the updated cells -- the active box without the frozen
+// inlet/outlet planes -- take the equilibrium shift; every other element is
+// copied unchanged.  With `after`, the block also writes the coarse means of
+// the jumped state's stored moments (rho + drho, u + du exactly, by the
+// moments of feq) in k_coarse_moments' stored convention.
+template <int P, int F>
+__global__ void __launch_bounds__(256)
+    k_jump_apply(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
+                 double rhor, double mrx, const float *__restrict__ delta, double rho0, double u0x, double u0y,
+                 double u0z, float *__restrict__ after) {
+  using T = JumpT<P>;
+  __shared__ double red[4][4][64];
+  // the coarse change around the block's tile: (64/F + 2) x (4/F + 2) x 3
+  // coarse cells (clamped at the grid's edges), 4 planes
+  constexpr int SX = 64 / F + 2, SY = 4 / F + 2, SZ = 3;
+  __shared__ float sd[4][SZ][SY][SX];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int x = blockIdx.x * 64 + tx, y = blockIdx.y * 4 + ty;
   const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
   const long nc = (long)ncx * ncy * ncz;
-  const T rn = rho + (T)interp(delta, 0, nc, ncx, ncy, ncz, F, x, y, z);
-  const T vx = ux + (T)interp(delta, 1, nc, ncx, ncy, ncz, F, x, y, z);
-  const T vy = uy + (T)interp(delta, 2, nc, ncx, ncy, ncz, F, x, y, z);
-  const T vz = uz + (T)interp(delta, 3, nc, ncx, ncy, ncz, F, x, y, z);
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const T d = feq_q<T>(q, rn, vx, vy, vz) - feq_q<T>(q, rho, ux, uy, uz);
-    st_h<P>(fout + q * g.pq + p0, h[q] + d * (T)Traits<P>::scale);
+  const int cx0 = blockIdx.x * (64 / F) - 1, cy0 = blockIdx.y * (4 / F) - 1, cz0 = (int)blockIdx.z - 1;
+  for (int e = threadIdx.x; e < 4 * SZ * SY * SX; e += 256) {
+    const int i = e % SX, j = (e / SX) % SY, k = (e / (SX * SY)) % SZ, a = e / (SX * SY * SZ);
+    const int jx = min(max(cx0 + i, 0), ncx - 1), jy = min(max(cy0 + j, 0), ncy - 1);
+    const int jz = min(max(cz0 + k, 0), ncz - 1);
+    sd[a][k][j][i] = delta[a * nc + (long)jx + (long)ncx * ((long)jy + (long)ncy * jz)];
   }
+  __syncthreads();
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < F; ++k) {
+    const int z = blockIdx.z * F + k;
+    const long p0 = pop_index(g, x, y, z);
+    const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
+    const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+    if (!act || frozen) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) fout[q * g.pq + p0] = fin[q * g.pq + p0];
+      acc[0] += rho0;
+      acc[1] += u0x;
+      acc[2] += u0y;
+      acc[3] += u0z;
+      continue;
+    }
+    typename Traits<P>::T h[19];
+    T rho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      h[q] = ld_h<P>(fin + q * g.pq + p0);
+      const T fq = (T)h[q] * (T)Traits<P>::inv_scale;  // f, or the deviation f - r
+      rho += fq;
+      jx += (T)kCX[q] * fq;
+      jy += (T)kCY[q] * fq;
+      jz += (T)kCZ[q] * fq;
+    }
+    if (Traits<P>::dev) {  // sum r = rho_r, sum c r = rho_r u_r e_x
+      rho += (T)rhor;
+      jx += (T)mrx;
+    }
+    const T ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    // trilinear weights once for the four planes (the clamping of interp():
+    // a cell left of the first coarse centre takes the edge value, which the
+    // clamped staging reproduces)
+    const double hc = 0.5 * (F - 1);
+    const double fx = floor((x - hc) / F), fy = floor((y - hc) / F), fz = floor((z - hc) / F);
+    const double wx = (x - hc) / F - fx, wy = (y - hc) / F - fy, wz = (z - hc) / F - fz;
+    const int ix = (int)fx - cx0, iy = (int)fy - cy0, iz = (int)fz - cz0;
+    double dd[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+      const double w = (dx ? wx : 1.0 - wx) * (dy ? wy : 1.0 - wy) * (dz ? wz : 1.0 - wz);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) dd[a] += w * (double)sd[a][iz + dz][iy + dy][ix + dx];
+    }
+    const T d0 = (T)dd[0], d1 = (T)dd[1], d2 = (T)dd[2], d3 = (T)dd[3];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      const T d = dfeq_q<T>(q, rho, d0, ux, uy, uz, d1, d2, d3);
+      st_h<P>(fout + q * g.pq + p0, h[q] + (typename Traits<P>::T)(d * (T)Traits<P>::scale));
+    }
+    acc[0] += (double)(rho + d0);
+    acc[1] += (double)(ux + d1);
+    acc[2] += (double)(uy + d2);
+    acc[3] += (double)(uz + d3);
+  }
+  if (after) block_mean_store<F>(acc, red, g, after);
 }
 
 // X += K (u(X) + u'(X)) / 2 with u the coarse velocity before the jump and
@@ -265,7 +375,7 @@ unsigned grid_of(long n, long cap = 16384) {
 }
 
 template <int P>
-int coarse_launch(lbmib_ctx *ctx, int F, float *out) {
+int coarse_launch(lbmib_ctx *ctx, int F, float *out, float *stored) {
   using T = typename Traits<P>::T;
   const Geometry &g = ctx->g;
   const double s = Traits<P>::scale;
@@ -281,21 +391,26 @@ int coarse_launch(lbmib_ctx *ctx, int F, float *out) {
   const T *tab = (const T *)ctx->c.dtab;
   const FluidConsts<double> &c = ctx->c;
   switch (F) {
-    case 1: k_coarse_moments<P, 1><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, ctx->d_ref, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out); break;
-    case 2: k_coarse_moments<P, 2><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, ctx->d_ref, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out); break;
-    default: k_coarse_moments<P, 4><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, ctx->d_ref, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out); break;
+    case 1: k_coarse_moments<P, 1><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out, stored); break;
+    case 2: k_coarse_moments<P, 2><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out, stored); break;
+    default: k_coarse_moments<P, 4><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out, stored); break;
   }
   JHIPCHK(hipGetLastError());
   return LBMIB_OK;
 }
 
 template <int P>
-int apply_launch(lbmib_ctx *ctx, const void *fin, void *fout, const float *delta, int F) {
+int apply_launch(lbmib_ctx *ctx, const void *fin, void *fout, const float *delta, int F, float *after) {
   const Geometry &g = ctx->g;
-  const int bx = g.nx >= 256 ? 256 : 64;
-  dim3 grid((g.nx + bx - 1) / bx, g.ny, g.nz);
-  k_jump_apply<P><<<grid, bx, 0, ctx->stream>>>(g, (const typename Traits<P>::S *)fin, (typename Traits<P>::S *)fout,
-                                                 ctx->d_ref, delta, F);
+  const FluidConsts<double> &c = ctx->c;
+  dim3 grid(g.nx / 64, g.ny / 4, g.nz / F);
+  const auto *in = (const typename Traits<P>::S *)fin;
+  auto *out = (typename Traits<P>::S *)fout;
+  switch (F) {
+    case 1: k_jump_apply<P, 1><<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z, after); break;
+    case 2: k_jump_apply<P, 2><<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z, after); break;
+    default: k_jump_apply<P, 4><<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z, after); break;
+  }
   JHIPCHK(hipGetLastError());
   return LBMIB_OK;
 }
@@ -318,16 +433,20 @@ int ready_for_jump(const lbmib_ctx *ctx) {
 
 using namespace lbmib;
 
-extern "C" int lbmib_coarse_moments(lbmib_ctx *ctx, int factor, float *d_out) {
-  if (!ctx || !d_out) return LBMIB_ERR_ARG;
+extern "C" int lbmib_coarse_state(lbmib_ctx *ctx, int factor, float *d_moments, float *d_stored) {
+  if (!ctx || !d_moments) return LBMIB_ERR_ARG;
   JRCHK(check_coarse(ctx, factor));
   if (ctx->init_bits != 7 || ctx->pending) return LBMIB_ERR_STATE;
   switch (ctx->fp.precision) {
-    case LBMIB_F64: return coarse_launch<LBMIB_F64>(ctx, factor, d_out);
-    case LBMIB_F32: return coarse_launch<LBMIB_F32>(ctx, factor, d_out);
-    case LBMIB_F16: return coarse_launch<LBMIB_F16>(ctx, factor, d_out);
-    default: return coarse_launch<LBMIB_BF16>(ctx, factor, d_out);
+    case LBMIB_F64: return coarse_launch<LBMIB_F64>(ctx, factor, d_moments, d_stored);
+    case LBMIB_F32: return coarse_launch<LBMIB_F32>(ctx, factor, d_moments, d_stored);
+    case LBMIB_F16: return coarse_launch<LBMIB_F16>(ctx, factor, d_moments, d_stored);
+    default: return coarse_launch<LBMIB_BF16>(ctx, factor, d_moments, d_stored);
   }
+}
+
+extern "C" int lbmib_coarse_moments(lbmib_ctx *ctx, int factor, float *d_out) {
+  return lbmib_coarse_state(ctx, factor, d_out, nullptr);
 }
 
 extern "C" int lbmib_coarse_sheets(lbmib_ctx *ctx, int factor, float *d_out) {
@@ -353,7 +472,8 @@ extern "C" int lbmib_coarse_sheets(lbmib_ctx *ctx, int factor, float *d_out) {
   return LBMIB_OK;
 }
 
-extern "C" int lbmib_jump(lbmib_ctx *ctx, int factor, const float *d_coarse, const float *d_delta, int ksteps) {
+extern "C" int lbmib_jump(lbmib_ctx *ctx, int factor, const float *d_coarse, const float *d_delta, int ksteps,
+                          float *d_after) {
   if (!ctx || !d_coarse || !d_delta || ksteps < 2 || (ksteps & 1)) return LBMIB_ERR_ARG;
   JRCHK(check_coarse(ctx, factor));
   JRCHK(ready_for_jump(ctx));
@@ -399,10 +519,10 @@ extern "C" int lbmib_jump(lbmib_ctx *ctx, int factor, const float *d_coarse, con
   ctx->f_spare = old;
   int rc;
   switch (ctx->fp.precision) {
-    case LBMIB_F64: rc = apply_launch<LBMIB_F64>(ctx, old, ctx->f[ctx->cur], d_delta, factor); break;
-    case LBMIB_F32: rc = apply_launch<LBMIB_F32>(ctx, old, ctx->f[ctx->cur], d_delta, factor); break;
-    case LBMIB_F16: rc = apply_launch<LBMIB_F16>(ctx, old, ctx->f[ctx->cur], d_delta, factor); break;
-    default: rc = apply_launch<LBMIB_BF16>(ctx, old, ctx->f[ctx->cur], d_delta, factor); break;
+    case LBMIB_F64: rc = apply_launch<LBMIB_F64>(ctx, old, ctx->f[ctx->cur], d_delta, factor, d_after); break;
+    case LBMIB_F32: rc = apply_launch<LBMIB_F32>(ctx, old, ctx->f[ctx->cur], d_delta, factor, d_after); break;
+    case LBMIB_F16: rc = apply_launch<LBMIB_F16>(ctx, old, ctx->f[ctx->cur], d_delta, factor, d_after); break;
+    default: rc = apply_launch<LBMIB_BF16>(ctx, old, ctx->f[ctx->cur], d_delta, factor, d_after); break;
   }
   if (rc) return rc;
   if (M > 0) {

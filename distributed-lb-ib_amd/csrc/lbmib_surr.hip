@@ -722,12 +722,18 @@ __global__ void __launch_bounds__(256) k_mse(const uint16_t *__restrict__ out, c
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_final_sum(const float *__restrict__ part, int n, float scale, float *__restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    float s = 0.f;
-    for (int k = 0; k < n; ++k) s += part[k];
-    *out = s * scale;
+__global__ void __launch_bounds__(256) k_final_sum(const float *__restrict__ part, int n, float scale,
+                                                   float *__restrict__ out) {
+  __shared__ float r[256];
+  float s = 0.f;
+  for (int k = threadIdx.x; k < n; k += 256) s += part[k];
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) *out = r[0] * scale;
 }
 
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
@@ -818,15 +824,28 @@ __global__ void __launch_bounds__(256) k_sqdiff(const float *__restrict__ a, con
   }
 }
 
-__global__ void k_sqdiff_final(const double *__restrict__ part, int nb, double *__restrict__ out2) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int k = 0; k < nb; ++k) {
-      s0 += part[2 * k];
-      s1 += part[2 * k + 1];
+// one block of 256: strided partial sums, then a fixed tree (deterministic)
+__global__ void __launch_bounds__(256) k_sqdiff_final(const double *__restrict__ part, int nb,
+                                                      double *__restrict__ out2) {
+  __shared__ double r0[256], r1[256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 256) {
+    s0 += part[2 * k];
+    s1 += part[2 * k + 1];
+  }
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r0[threadIdx.x] += r0[threadIdx.x + w];
+      r1[threadIdx.x] += r1[threadIdx.x + w];
     }
-    out2[0] = s0;
-    out2[1] = s1;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out2[0] = r0[0];
+    out2[1] = r1[0];
   }
 }
 
@@ -938,7 +957,7 @@ extern "C" int lbmib_sg_mse(const void *out, const float *tgt, void *dout, float
   const float denom = (float)((double)M * nl);
   k_mse<<<nb, 256, 0, s>>>((const uint16_t *)out, tgt, (uint16_t *)dout, (float *)ws, M, C, nl, 2.0f / denom);
   HIPCHK(hipGetLastError());
-  k_final_sum<<<1, 64, 0, s>>>((const float *)ws, (int)nb, 1.0f / denom, loss);
+  k_final_sum<<<1, 256, 0, s>>>((const float *)ws, (int)nb, 1.0f / denom, loss);
   HIPCHK(hipGetLastError());
   return LBMIB_OK;
 }
@@ -1002,7 +1021,7 @@ extern "C" int lbmib_sg_sqdiff(const float *a, const float *b, long n, double *o
   const unsigned nb = grid_for(n, 256, 1024);
   k_sqdiff<<<nb, 256, 0, s>>>(a, b, n, (double *)ws);
   HIPCHK(hipGetLastError());
-  k_sqdiff_final<<<1, 64, 0, s>>>((const double *)ws, (int)nb, out2);
+  k_sqdiff_final<<<1, 256, 0, s>>>((const double *)ws, (int)nb, out2);
   HIPCHK(hipGetLastError());
   return LBMIB_OK;
 }

@@ -41,7 +41,7 @@ ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lb
                "lbmib_batch_step", "lbmib_batch_set_tile", "lbmib_batch_set_overlap", "lbmib_batch_cells",
                "lbmib_batch_status",
                "lbmib_batch_time_steps", "lbmib_coarse_moments", "lbmib_coarse_sheets", "lbmib_jump",
-               "lbmib_jump_commit", "lbmib_jump_revert", "lbmib_get_stream") + tuple("lbmib_" + n for n in CTX_CALLS)
+               "lbmib_jump_commit", "lbmib_jump_revert", "lbmib_get_stream", "lbmib_coarse_state") + tuple("lbmib_" + n for n in CTX_CALLS)
 
 
 class LbmibError(RuntimeError):
@@ -116,7 +116,8 @@ def load_library(path=LIB_PATH):
     L.lbmib_batch_time_steps.argtypes = [vp, i, i, dp, dp]
     L.lbmib_coarse_moments.argtypes = [vp, i, vp]
     L.lbmib_coarse_sheets.argtypes = [vp, i, vp]
-    L.lbmib_jump.argtypes = [vp, i, vp, vp, i]
+    L.lbmib_jump.argtypes = [vp, i, vp, vp, i, vp]
+    L.lbmib_coarse_state.argtypes = [vp, i, vp, vp]
     L.lbmib_jump_commit.argtypes = [vp]
     L.lbmib_jump_revert.argtypes = [vp]
     L.lbmib_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
@@ -364,15 +365,16 @@ class Engine:
     # ------------------------------------------------ surrogate coupling
     # d_* arguments are device addresses (ints, e.g. a torch tensor's
     # data_ptr()) of FP32 planes [c][nz/f][ny/f][nx/f]
-    def coarse_moments(self, factor, d_out):
-        self._check(self.L.lbmib_coarse_moments(self.h, factor, ctypes.c_void_p(d_out)), "coarse_moments")
+    def coarse_moments(self, factor, d_out, d_stored=0):
+        self._check(self.L.lbmib_coarse_state(self.h, factor, ctypes.c_void_p(d_out), ctypes.c_void_p(d_stored)),
+                    "coarse_state")
 
     def coarse_sheets(self, factor, d_out):
         self._check(self.L.lbmib_coarse_sheets(self.h, factor, ctypes.c_void_p(d_out)), "coarse_sheets")
 
-    def jump(self, factor, d_coarse, d_delta, ksteps):
-        self._check(self.L.lbmib_jump(self.h, factor, ctypes.c_void_p(d_coarse), ctypes.c_void_p(d_delta), ksteps),
-                    "jump")
+    def jump(self, factor, d_coarse, d_delta, ksteps, d_after=0):
+        self._check(self.L.lbmib_jump(self.h, factor, ctypes.c_void_p(d_coarse), ctypes.c_void_p(d_delta), ksteps,
+                                      ctypes.c_void_p(d_after)), "jump")
 
     def stream_handle(self):
         """The context's HIP stream as an int (for torch.cuda.ExternalStream)."""

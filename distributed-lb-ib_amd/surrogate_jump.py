@@ -11,7 +11,9 @@ loop (lbm_ib.c:3115-3563).  One jump cycle here:
   3. lbmib_jump shifts every cell's equilibrium by the interpolated change,
      advects the sheet, keeps the pre-jump state aside and sets t += K;
   4. re-verification: R plain steps (they count as simulated steps); the
-     coarse velocity change over them, per step, divided by the jump's own
+     change of the coarse stored-state velocity over them (the jump reports
+     the jumped state's, the read after the steps the new one, both from
+     passes the cycle makes anyway), per step, divided by the jump's own
      mean change per step, is the verification ratio.  A state the plain
      dynamics accept changes at about the rate the prediction implies; a
      jump that leaves it inconsistent (pressure waves, a broken
@@ -95,10 +97,12 @@ class Coarse:
         self.nc = int(np.prod(self.dims))
         self.feat = torch.empty((NF, self.nc), dtype=torch.float32, device="cuda")
 
-    def read(self, out=None):
-        """Fills out (default: self.feat) with the engine's current state."""
+    def read(self, out=None, stored=None):
+        """Fills out (default: self.feat) with the engine's current state;
+        with `stored` ([4][nc]), also the stored-moment planes of the same
+        pass (lbmib_coarse_state)."""
         t = self.feat if out is None else out
-        self.e.coarse_moments(self.f, t[0].data_ptr())
+        self.e.coarse_moments(self.f, t[0].data_ptr(), 0 if stored is None else stored.data_ptr())
         self.e.coarse_sheets(self.f, t[4].data_ptr())
         return t
 
@@ -128,7 +132,10 @@ class JumpRunner:
         nz, ny, nx = self.co.dims
         self.x = torch.empty((1, nz, ny, nx, CIN), dtype=torch.bfloat16, device="cuda")
         self.delta = torch.empty((NOUT, self.co.nc), dtype=torch.float32, device="cuda")
-        self.after = torch.empty((NF, self.co.nc), dtype=torch.float32, device="cuda")
+        # stored-moment planes (lbmib_coarse_state's convention) of the jumped
+        # state and of the state after the verification steps
+        self.after = torch.empty((4, self.co.nc), dtype=torch.float32, device="cuda")
+        self.verified = torch.empty((4, self.co.nc), dtype=torch.float32, device="cuda")
         self.sums = torch.zeros(4, dtype=torch.float64, device="cuda")
         self.stats = {"jumps": 0, "accepted": 0, "rejected": 0, "plain_steps": 0, "ratios": []}
         self.have_feat = False
@@ -187,12 +194,14 @@ class JumpRunner:
             self.predict()
         else:
             predictor(self)
-        e.jump(self.co.f, self.co.feat.data_ptr(), self.delta.data_ptr(), self.K)
-        self.co.read(self.after)                      # coarse state just after the jump
+        # the jump also reports the jumped state's coarse stored moments
+        e.jump(self.co.f, self.co.feat.data_ptr(), self.delta.data_ptr(), self.K, self.after.data_ptr())
         e.step(self.R)
-        self.co.read()                                # after R plain steps: the next cycle's input
+        # after R plain steps: the next cycle's input and, from the same
+        # pass, the stored moments the verification compares
+        self.co.read(stored=self.verified)
         # verification ratio: |du over R steps| / R  against  |du of the jump| / K
-        self._sqdiff(self.co.feat[1:4], self.after[1:4], self.sums[0:2])
+        self._sqdiff(self.verified[1:4], self.after[1:4], self.sums[0:2])
         self._sqdiff(self.delta[1:4], None, self.sums[2:4])
         # a blocking copy on the engine's stream (no torch pinned buffer:
         # torch's pinned-memory allocator would record an event on this

@@ -236,6 +236,12 @@ int lbmib_batch_time_steps(lbmib_batch *b, int nwarm, int nsteps, double *ms_tot
 /* c = 0..3: rho, ux, uy, uz of lbmib_get_macroscopic (compute_rho_and_u,
  * lbm_ib.c:2440-2538) averaged over each f^3 block. */
 int lbmib_coarse_moments(lbmib_ctx *ctx, int factor, float *d_out);
+/* lbmib_coarse_moments into d_moments and, in the same pass over the
+ * populations (d_stored nullable), the means of the stored post-collision
+ * populations' own moments: rho and u = sum c f / rho on the updated cells
+ * (the active box without the frozen inlet/outlet planes), rho0/u0 on the
+ * other cells -- the convention of lbmib_jump's d_after. */
+int lbmib_coarse_state(lbmib_ctx *ctx, int factor, float *d_moments, float *d_stored);
 /* c = 0..3: number of fiber nodes in each coarse cell and the sums of their
  * elastic forces (compute_elasticforce, lbm_ib.c:1285-1318); deterministic. */
 int lbmib_coarse_sheets(lbmib_ctx *ctx, int factor, float *d_out);
@@ -245,10 +251,13 @@ int lbmib_coarse_sheets(lbmib_ctx *ctx, int factor, float *d_out);
  * keeps its non-equilibrium part and takes the equilibrium of its moments
  * plus the change interpolated trilinearly at the cell; fiber nodes move by
  * ksteps times the mean of the coarse velocity before and after the jump at
- * the node.  The pre-jump state is kept until lbmib_jump_commit or
+ * the node.  d_after (nullable) receives, from the same pass, the four
+ * coarse planes of the jumped state in lbmib_coarse_state's stored
+ * convention.  The pre-jump state is kept until lbmib_jump_commit or
  * lbmib_jump_revert; steps may run in between (the re-verification).  Needs
  * unsplit steps and at least one completed step. */
-int lbmib_jump(lbmib_ctx *ctx, int factor, const float *d_coarse, const float *d_delta, int ksteps);
+int lbmib_jump(lbmib_ctx *ctx, int factor, const float *d_coarse, const float *d_delta, int ksteps,
+               float *d_after);
 /* The context's HIP stream (hipStream_t as void *), on which every entry
  * point enqueues; work of other libraries placed on it is ordered with the
  * simulation's (e.g. the surrogate's kernels between lbmib_coarse_moments

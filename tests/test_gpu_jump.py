@@ -10,7 +10,8 @@ distributed-lb-ib_amd/surrogate_jump.py), through the C-ABI.
 - lbmib_jump: with a zero change the populations stay bit for bit and the
   nodes move by K times the interpolated coarse velocity (<= 1e-12); with a
   smooth random change the post-stream populations match the numpy
-  restatement (oracle/surrogate_oracle.jump_pulled) to 1e-13;
+  restatement (oracle/surrogate_oracle.jump_pulled) to 1e-13 (F64) and to
+  FP32 rounding (F32, whose shift is computed in FP32 on the deviations);
 - lbmib_jump_revert: after a jump and several steps, revert plus plain steps
   give the plain trajectory bit for bit (F64, F32, BF16, with a sheet);
 - the jump loop with a tolerance of 0 rejects every jump and reproduces
@@ -98,10 +99,15 @@ def test_zero_jump_keeps_populations_and_advects_sheet(lbmib):
         cm = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
         e.coarse_moments(4, cm.data_ptr())
         zero = torch.zeros((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        before = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        after = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.coarse_moments(4, cm.data_ptr(), before.data_ptr())
         t0 = e.time
-        e.jump(4, cm.data_ptr(), zero.data_ptr(), K)
+        e.jump(4, cm.data_ptr(), zero.data_ptr(), K, after.data_ptr())
         assert e.time == t0 + K
         np.testing.assert_array_equal(e.df(), df0)
+        # the jump's stored-moment planes of an unchanged state are the read's
+        np.testing.assert_array_equal(host(after), host(before))
         x1 = e.fibers(0)[..., :3].reshape(-1, 3)
         c = host(cm)
         want = x0 + K * np.stack([so.trilinear_at(c[1 + a], 4, x0) for a in range(3)], 1)
@@ -128,10 +134,70 @@ def test_jump_shift_matches_restatement(lbmib):
         delta = (rng.standard_normal((4, 8, 8, 16)) * np.array([1e-3, 1e-3, 5e-4, 5e-4])[:, None, None, None])
         delta = delta.astype(np.float32).astype(np.float64)
         d_dev = dev(delta)
-        e.jump(4, cm.data_ptr(), d_dev.data_ptr(), 4)
+        after = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.jump(4, cm.data_ptr(), d_dev.data_ptr(), 4, after.data_ptr())
         got = e.df()
+        reread = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.coarse_moments(4, cm.data_ptr(), reread.data_ptr())
+        e.sync()
     want = so.jump_pulled(f2, delta, 4)
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+    # the jumped state's stored moments (every cell is updated on this
+    # all-periodic grid): h'_q(y) = f2'_q(y + c_q), block means of rho, u
+    hq = np.array([np.roll(got[q], (-so.CZ[q], -so.CY[q], -so.CX[q]), axis=(0, 1, 2)) for q in range(19)])
+    rho = hq.sum(0)
+    mom = [rho] + [(c[:, None, None, None] * hq).sum(0) / rho for c in (so.CX, so.CY, so.CZ)]
+    ref = np.array([so.block_average(m, 4) for m in mom])
+    for a in range(4):
+        assert np.abs(host(after)[a] - ref[a]).max() <= 1e-6 * np.abs(ref[a]).max(), a
+        # the stored-moment read of the same state gives the same planes
+        assert np.abs(host(reread)[a] - ref[a]).max() <= 1e-6 * np.abs(ref[a]).max(), a
+
+
+def test_jump_shift_f32_matches_restatement(lbmib):
+    """The FP32 deviation form computes the shift in FP32 on h = f - r
+    (moments rho_r + sum h, rho_r u_r + sum c h; the change in difference
+    form).  A first random jump and six steps make the state non-uniform;
+    then, against the FP64 restatement applied to the FP32 state, each
+    population lies within half an FP32 ulp of its stored deviation plus
+    1e-6 of the largest shift (FP32 rounding of terms of that size), and the
+    coarse reads (pulled and stored moments) within 1e-6 of the block means
+    of the FP64 moments."""
+    kw = small_cfg(lbmib, "f32")
+    kw["sheets"] = []
+    kw["bc"] = (BC_PERIODIC, BC_PERIODIC, BC_PERIODIC)
+    rng = np.random.default_rng(12)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(3)
+        cm = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.coarse_moments(4, cm.data_ptr())
+        d1 = dev(rng.standard_normal((4, 8, 8, 16)) * np.array([2e-3, 2e-3, 1e-3, 1e-3])[:, None, None, None])
+        e.jump(4, cm.data_ptr(), d1.data_ptr(), 4)
+        e.jump_commit()
+        e.step(6)
+        f2 = e.df()
+        st = torch.empty((4, 8, 8, 16), dtype=torch.float32, device="cuda")
+        e.coarse_moments(4, cm.data_ptr(), st.data_ptr())
+        pulled, stored = host(cm), host(st)
+        delta = (rng.standard_normal((4, 8, 8, 16)) * np.array([1e-3, 1e-3, 5e-4, 5e-4])[:, None, None, None])
+        delta = delta.astype(np.float32).astype(np.float64)
+        d_dev = dev(delta)
+        e.jump(4, cm.data_ptr(), d_dev.data_ptr(), 4)
+        got = e.df()
+        e.sync()
+    h = np.array([np.roll(f2[q], (-so.CZ[q], -so.CY[q], -so.CX[q]), axis=(0, 1, 2)) for q in range(19)])
+    for got_planes, pops in ((pulled, f2), (stored, h)):
+        rho = pops.sum(0)
+        mom = [rho] + [(c[:, None, None, None] * pops).sum(0) / rho for c in (so.CX, so.CY, so.CZ)]
+        for a in range(4):
+            ref = so.block_average(mom[a], 4)
+            assert np.abs(got_planes[a] - ref).max() <= 1e-6 * np.abs(ref).max(), a
+    want = so.jump_pulled(f2, delta, 4)
+    r = so.feq(np.float64(kw["rho0"]), np.float64(kw["u0"][0]), 0.0, 0.0)[:, None, None, None]
+    bound = 2.0 ** -24 * np.abs(want - r) + 1e-6 * np.abs(want - f2).max() + 1e-15
+    err = np.abs(got - want)
+    assert np.all(err <= bound), float((err / bound).max())
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
