@@ -210,6 +210,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
 
   IBState &ib = ctx->ib;
   ib.nsheets = nsheets;
+  ctx->move_tiles = move_tile_count(ctx->sd, nsheets);
   ib.nnodes = ctx->nnodes;
   ib.quirks = p.quirks;
   ib.nby = (p.ny + 3) / 4;
@@ -407,7 +408,7 @@ extern "C" int lbmib_fiber_get_SpreadVelocity(lbmib_ctx *ctx) {
   if (!ctx->pending) return LBMIB_ERR_STATE;
   if (ctx->nnodes == 0 || ctx->spread_slot != slot_now(ctx)) return LBMIB_OK;
   const int s = slot_now(ctx);
-  RCHK(launch_move(ctx->fp.precision, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream));
+  RCHK(launch_move(ctx->fp.precision, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->move_tiles, ctx->stream));
   ctx->ib_clean = 1;  // k_move clears the bins and the next step's box slot
   return LBMIB_OK;
 }
@@ -446,7 +447,7 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
                              ctx->d_rowtab, ctx->kv, ctx->stream));
   if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
   if (ctx->nnodes > 0) {
-    RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream));
+    RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->move_tiles, ctx->stream));
     ctx->ib_clean = 1;
   }
   ctx->cur = 1 - ctx->cur;
@@ -531,7 +532,7 @@ static int split_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, bool more) {
                              ctx->d_rowtab, ctx->kv, ctx->stream_b, 1, band, band_blocks(ctx)));
   // band velocity and move, then the next step's fiber forces and spreading
   // (slot 1 - s, reset by the move)
-  RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->stream_b));
+  RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->move_tiles, ctx->stream_b));
   ctx->ib_clean = 1;
   ctx->ib_ready = 0;
   if (more) {

@@ -16,6 +16,7 @@ struct lbmib_ctx {
   lbmib_sheet_params sp[kMaxSheets];
   SheetDesc sd[kMaxSheets];
   int nnodes = 0;
+  int move_tiles = 0;  // workgroups of k_move_tiles (move_tile_count)
   Geometry g;
   FluidConsts<double> c;
   size_t esz = 0, fsz = 0;  // population / force element sizes

@@ -13,15 +13,18 @@
 //                  over the candidates in ascending (sheet, row, col) order --
 //                  the order of the sequential spread (lbm_ib.c:1420-1495).
 //                  No float atomics; the result is run-to-run identical.
-//   k_band_u       rho/u of the new populations on each support-box cell
-//                  (compute_rho_and_u, lbm_ib.c:2440-2538, evaluated only
-//                  where the interpolation reads), FP64.
-//   k_move         one wave per node: u(X) = Sum u delta over the node's 64
-//                  support cells summed in the reference's order
-//                  (lbm_ib.c:2670-2673), X += dt u(X) in FP64; also clears
-//                  the bins and the box slot that the next step will fill.
+//   k_move_tiles   one workgroup per 4x4-node tile of a sheet: rho/u of the
+//                  new populations (compute_rho_and_u, lbm_ib.c:2440-2538)
+//                  once per cell of the tile's support region into LDS, then
+//                  u(X) = Sum u delta over each node's 64 support cells in the
+//                  reference's order (lbm_ib.c:2670-2673) and X += dt u(X),
+//                  FP64; also clears the bins and the box slot that the next
+//                  step will fill.  The batched ensemble uses the two-kernel
+//                  form of the same arithmetic: k_band_u (rho/u on each
+//                  support-box cell) and k_move (one wave per node).
 #include <hip/hip_runtime.h>
 #include <limits.h>
+#include <stdlib.h>
 
 #include "lbmib_device.h"
 
@@ -308,9 +311,21 @@ __device__ __forceinline__ void gather_body(const IBState &ib, const Geometry &g
   __shared__ int s_tot;
   const int M = ib.nnodes;
   const double cf = 1.0 / (64.0 * 1.0 * 1.0 * 1.0);  // lbm_ib.c:1357
-  for (int w = bid;; w += nblk) {
+  // Workgroups are dealt to the 8 XCDs in turn: virtual index v = bid + k
+  // nblk (nblk a multiple of 8) runs on XCD v % 8 and takes tile (v % 8) per
+  // + v / 8, so that each XCD spreads a contiguous band of tiles and the
+  // candidates that neighbouring tiles share are read through its own L2.
+  int ntiles = 0;
+  for (int s = 0; s < ib.nsheets; ++s) {
+    const int *b = ib.box + (slot * kMaxSheets + s) * 6;
+    if (ldc(b) > ldc(b + 1)) continue;
+    ntiles += (ldc(b + 3) / kTile - ldc(b + 2) / kTile + 1) * (ldc(b + 5) / kTile - ldc(b + 4) / kTile + 1);
+  }
+  const int per = (ntiles + 7) / 8;
+  for (int v = bid; v < 8 * per; v += nblk) {
+    const int w = (v % 8) * per + v / 8;
     int s, ty, tz;
-    if (!tile_of(ib, slot, w, s, ty, tz)) break;
+    if (!tile_of(ib, slot, w, s, ty, tz)) continue;
     const int *b = ib.box + (slot * kMaxSheets + s) * 6;
     const int x0 = b[0], x1 = b[1], bybox0 = b[2], bybox1 = b[3], bzbox0 = b[4], bzbox1 = b[5];
     // a node with support start j0 reaches rows j0..j0+3, so tile rows
@@ -471,6 +486,39 @@ __global__ void __launch_bounds__(kGatherThreads)
 // products in the reference's (i2, j2, k2) order (lbm_ib.c:2614-2682) before
 // X += dt u(X).
 
+// compute_rho_and_u (lbm_ib.c:2498-2525) at cell (x, y, z) from the new
+// populations and the current force; cells outside the moment box keep
+// init_gv's velocity (u0x, u0y, u0z).
+template <int P>
+__device__ __forceinline__ void cell_u(const Geometry &g, const typename Traits<P>::S *__restrict__ f,
+                                       const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y,
+                                       double u0z, double gz, const double *__restrict__ ref, typename Traits<P>::T b1,
+                                       typename Traits<P>::T b2, int x, int y, int z, double u[3]) {
+  const bool mom =
+      x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] && z <= g.mhi[2];
+  u[0] = u0x;
+  u[1] = u0y;
+  u[2] = u0z;
+  if (mom) {
+    const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
+    typename Traits<P>::T v[19];
+    pull<P>(g, f, x, y, z, b1, b2, v);
+    double fv[19];
+    for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + ref[q] : (double)v[q];
+    double s1 = fv[0], s2 = 0, s3 = 0, s4 = 0;
+    for (int q = 1; q < 19; ++q) {
+      s1 += fv[q];
+      if (kCX[q]) s2 = kCX[q] > 0 ? s2 + fv[q] : s2 - fv[q];
+      if (kCY[q]) s3 = kCY[q] > 0 ? s3 + fv[q] : s3 - fv[q];
+      if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + fv[q] : s4 - fv[q];
+    }
+    const double fx = F[c0], fy = F[g.n + c0], fz = F[2 * g.n + c0];
+    u[0] = (s2 + 0.5 * fx) / s1;
+    u[1] = (s3 + 0.5 * fy) / s1;
+    u[2] = (s4 + 0.5 * (fz + s1 * gz)) / s1;
+  }
+}
+
 template <int P>
 __device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g, const typename Traits<P>::S *__restrict__ f,
                                             const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y,
@@ -501,29 +549,11 @@ __device__ __forceinline__ void band_u_body(const IBState &ib, const Geometry &g
       const int nxs = b[1] - x0 + 1, nys = b[3] - y0 + 1;
       const int x = x0 + (int)(e % nxs), y = y0 + (int)((e / nxs) % nys), z = z0 + (int)(e / ((long)nxs * nys));
       const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
-      const bool mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] &&
-                       z <= g.mhi[2];
-      double ux = u0x, uy = u0y, uz = u0z;
-      if (mom) {
-        typename Traits<P>::T v[19];
-        pull<P>(g, f, x, y, z, b1, b2, v);
-        double fv[19];
-        for (int q = 0; q < 19; ++q) fv[q] = Traits<P>::dev ? (double)v[q] + ref[q] : (double)v[q];
-        double s1 = fv[0], s2 = 0, s3 = 0, s4 = 0;
-        for (int q = 1; q < 19; ++q) {
-          s1 += fv[q];
-          if (kCX[q]) s2 = kCX[q] > 0 ? s2 + fv[q] : s2 - fv[q];
-          if (kCY[q]) s3 = kCY[q] > 0 ? s3 + fv[q] : s3 - fv[q];
-          if (kCZ[q]) s4 = kCZ[q] > 0 ? s4 + fv[q] : s4 - fv[q];
-        }
-        const double fx = F[c0], fy = F[g.n + c0], fz = F[2 * g.n + c0];
-        ux = (s2 + 0.5 * fx) / s1;
-        uy = (s3 + 0.5 * fy) / s1;
-        uz = (s4 + 0.5 * (fz + s1 * gz)) / s1;
-      }
-      ib.ub[c0] = ux;
-      ib.ub[g.n + c0] = uy;
-      ib.ub[2 * g.n + c0] = uz;
+      double u[3];
+      cell_u<P>(g, f, F, u0x, u0y, u0z, gz, ref, b1, b2, x, y, z, u);
+      ib.ub[c0] = u[0];
+      ib.ub[g.n + c0] = u[1];
+      ib.ub[2 * g.n + c0] = u[2];
     }
   }
 }
@@ -583,6 +613,132 @@ __device__ __forceinline__ void move_body(const IBState &ib, const Geometry &g, 
 
 __global__ void __launch_bounds__(kMoveThreads) k_move(IBState ib, Geometry g, int reset_slot) {
   move_body(ib, g, reset_slot, blockIdx.x, gridDim.x);
+}
+
+// ----------------------------------------------------------- k_move_tiles
+//
+// The band velocity and the move of the single-instance steps in one launch.
+// One workgroup per tile of 4x4 nodes of a sheet (rows x cols).  The tile's
+// 48 (node, component) lanes, all in wave 0, load their node's support
+// start, delta factors and position; the wave reduces the tile's support
+// bounding box; the workgroup evaluates cell_u once per cell of that box
+// into LDS (about 5x7x7 cells for a sheet across the flow), in x slabs when
+// the box is wider than LDS holds; each lane then sums its node's 64
+// products u wx wy wz / 64 in the reference's (i2, j2, k2) order
+// (lbm_ib.c:2614-2682) -- i2, the x offset, is the outer index, so slabs of
+// increasing x keep that order -- and sets X += dt u(X) (lbm_ib.c:2697-2699).
+// A box whose (y, z) face alone exceeds LDS (a sheet stretched far out of
+// shape) is evaluated per lane from global memory instead.  Same expressions
+// and summation order as k_band_u + k_move, so the same bits.  Every
+// workgroup also clears its share of the bins for the next spread, and
+// workgroup 0 the box slot the next step fills.
+
+constexpr int kMvThreads = 256;
+constexpr int kMvCells = 1024;  // cells of u per slab: 24 KB
+static_assert(3 * kMvNodes * kMvNodes <= 64, "the tile's lanes fit in one wave");
+
+template <int P>
+__global__ void __launch_bounds__(kMvThreads)
+    k_move_tiles(IBState ib, Geometry g, const typename Traits<P>::S *__restrict__ f,
+                 const typename Traits<P>::FT *__restrict__ F, double u0x, double u0y, double u0z, double gz,
+                 const double *__restrict__ ref, typename Traits<P>::T b1, typename Traits<P>::T b2, int reset_slot) {
+  __shared__ double su[3][kMvCells];
+  __shared__ int s_box[6];
+  const int tid = threadIdx.x;
+  const int M = ib.nnodes;
+  {
+    // clear the bins for the next spread and the box slot it will fill
+    const int nbins = ib.nby * ib.nbz;
+    for (int k = blockIdx.x * kMvThreads + tid; k < nbins; k += gridDim.x * kMvThreads) ib.bin_count[k] = 0;
+    if (blockIdx.x == 0 && tid < ib.nsheets * 6)
+      ib.box[(reset_slot * kMaxSheets + tid / 6) * 6 + tid % 6] = (tid & 1) ? INT_MIN : INT_MAX;
+  }
+  // this workgroup's tile: workgroups are dealt to the 8 XCDs in turn, so
+  // workgroup b takes tile (b % 8) * per + b / 8, a contiguous band of tiles
+  // per XCD, whose overlapping support regions then meet in that XCD's L2
+  const int per = (gridDim.x + 7) / 8;
+  int s = 0, w = (blockIdx.x % 8) * per + blockIdx.x / 8, tcols = 1;
+  for (;; ++s) {
+    if (s == ib.nsheets) return;
+    const SheetDesc &sd = ib.sheets[s];
+    tcols = (sd.cols + kMvNodes - 1) / kMvNodes;
+    const int nt = ((sd.rows + kMvNodes - 1) / kMvNodes) * tcols;
+    if (w < nt) break;
+    w -= nt;
+  }
+  const SheetDesc &sd = ib.sheets[s];
+  const int pr = tid / 3, comp = tid % 3;
+  const int i = (w / tcols) * kMvNodes + pr / kMvNodes, j = (w % tcols) * kMvNodes + pr % kMvNodes;
+  const int n = sd.off + i * sd.cols + j;
+  bool act = tid < 3 * kMvNodes * kMvNodes && i < sd.rows && j < sd.cols;
+  int st[3] = {0, 0, 0};
+  double wx[4], wy[4], wz[4], X = 0.0;
+  if (act) {
+    st[0] = ib.start[n];
+    st[1] = ib.start[M + n];
+    st[2] = ib.start[2 * M + n];
+    for (int a = 0; a < 4; ++a) {
+      wx[a] = ib.wgt[a * (long)M + n];
+      wy[a] = ib.wgt[(4 + a) * (long)M + n];
+      wz[a] = ib.wgt[(8 + a) * (long)M + n];
+    }
+    X = ib.pos[(long)comp * M + n];
+    act = st[0] >= 0;  // start -1: support off the grid (error latched), not moved
+  }
+  if (tid < 64) {
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = wave_min(act ? st[d] : INT_MAX);
+      hi[d] = wave_max(act ? st[d] + 3 : INT_MIN);
+    }
+    if (tid == 0)
+      for (int d = 0; d < 3; ++d) {
+        s_box[2 * d] = lo[d];
+        s_box[2 * d + 1] = hi[d];
+      }
+  }
+  __syncthreads();
+  const int xlo = s_box[0], xhi = s_box[1], ylo = s_box[2], zlo = s_box[4];
+  if (xlo > xhi) return;  // no node of the tile is moved
+  const int nyr = s_box[3] - ylo + 1, nzr = s_box[5] - zlo + 1;
+  const double c_64 = 1.0 / (6.4e1);
+  double acc = 0;
+  if (nyr * nzr <= kMvCells) {
+    const int W = min(xhi - xlo + 1, kMvCells / (nyr * nzr));
+    for (int xa = xlo; xa <= xhi; xa += W) {
+      const int nxw = min(W, xhi - xa + 1), ncell = nxw * nyr * nzr;
+      if (xa != xlo) __syncthreads();  // the previous slab's readers are done
+      for (int c = tid; c < ncell; c += kMvThreads) {
+        const int x = xa + c % nxw, y = ylo + (c / nxw) % nyr, z = zlo + c / (nxw * nyr);
+        double u[3];
+        cell_u<P>(g, f, F, u0x, u0y, u0z, gz, ref, b1, b2, x, y, z, u);
+        su[0][c] = u[0];
+        su[1][c] = u[1];
+        su[2][c] = u[2];
+      }
+      __syncthreads();
+      if (act) {
+        for (int i2 = 0; i2 < 4; ++i2) {
+          const int xr = st[0] + i2 - xa;
+          if (xr < 0 || xr >= nxw) continue;
+          for (int j2 = 0; j2 < 4; ++j2)
+            for (int k2 = 0; k2 < 4; ++k2) {
+              const double u = su[comp][xr + nxw * ((st[1] + j2 - ylo) + nyr * (st[2] + k2 - zlo))];
+              acc += u * wx[i2] * wy[j2] * wz[k2] * c_64;
+            }
+        }
+      }
+    }
+  } else if (act) {
+    for (int i2 = 0; i2 < 4; ++i2)
+      for (int j2 = 0; j2 < 4; ++j2)
+        for (int k2 = 0; k2 < 4; ++k2) {
+          double u[3];
+          cell_u<P>(g, f, F, u0x, u0y, u0z, gz, ref, b1, b2, st[0] + i2, st[1] + j2, st[2] + k2, u);
+          acc += u[comp] * wx[i2] * wy[j2] * wz[k2] * c_64;
+        }
+  }
+  if (act) ib.pos[(long)comp * M + n] = X + 1 * acc;  // X += dt * u(X)
 }
 
 __global__ void k_ib_reset(IBState ib, int reset_bins, int slot_mask) {
@@ -680,15 +836,49 @@ int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
+// LBMIB_MOVE_KERNELS=2 selects the two-kernel move (k_band_u + k_move, the
+// batched path's kernels) for the single-instance steps, to compare the two.
+static bool move_two_kernels() {
+  static const bool two = [] {
+    const char *v = getenv("LBMIB_MOVE_KERNELS");
+    return v && v[0] == '2';
+  }();
+  return two;
+}
+
 int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
-                const FluidConsts<double> &c, int reset_slot, hipStream_t s) {
+                const FluidConsts<double> &c, int reset_slot, int ntiles, hipStream_t s) {
   if (ib.nnodes <= 0) return LBMIB_OK;
   const int slot = 1 - reset_slot;  // the box of this step's spread
-  const int ublocks = band_u_blocks(ib.nnodes);
   // bounce-back correction of the deviation forms, scale 1 (F32) or 2^8
   const double s16 = kDevScale16;
   const double d1 = host_bounce_corr(c, 1, 1.0), d2 = host_bounce_corr(c, 2, 1.0);
   const float h1 = (float)host_bounce_corr(c, 1, s16), h2 = (float)host_bounce_corr(c, 2, s16);
+  if (!move_two_kernels()) {
+    const int mb = (ntiles + 7) / 8 * 8;  // whole rounds over the 8 XCDs
+    switch (prec) {
+      case LBMIB_F64:
+        k_move_tiles<LBMIB_F64><<<mb, kMvThreads, 0, s>>>(ib, g, (const double *)f, (const double *)F, c.u0x, c.u0y,
+                                                          c.u0z, c.g, c.dref, 0.0, 0.0, reset_slot);
+        break;
+      case LBMIB_F32:
+        k_move_tiles<LBMIB_F32><<<mb, kMvThreads, 0, s>>>(ib, g, (const float *)f, (const float *)F, c.u0x, c.u0y,
+                                                          c.u0z, c.g, c.dref, d1, d2, reset_slot);
+        break;
+      case LBMIB_F16:
+        k_move_tiles<LBMIB_F16><<<mb, kMvThreads, 0, s>>>(ib, g, (const half_t *)f, (const float *)F, c.u0x, c.u0y,
+                                                          c.u0z, c.g, c.dref, h1, h2, reset_slot);
+        break;
+      case LBMIB_BF16:
+        k_move_tiles<LBMIB_BF16><<<mb, kMvThreads, 0, s>>>(ib, g, (const uint16_t *)f, (const float *)F, c.u0x,
+                                                           c.u0y, c.u0z, c.g, c.dref, h1, h2, reset_slot);
+        break;
+      default:
+        return LBMIB_ERR_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+  }
+  const int ublocks = band_u_blocks(ib.nnodes);
   switch (prec) {
     case LBMIB_F64:
       k_band_u<LBMIB_F64><<<ublocks, 256, 0, s>>>(ib, g, (const double *)f, (const double *)F, c.u0x, c.u0y, c.u0z,

@@ -138,7 +138,9 @@ __host__ __device__ inline int fiber_blocks(int nnodes) { return (4 * nnodes + k
 // Grids of the other IB kernels for a sheet set of nnodes nodes (the kernels
 // stride over any further work).
 __host__ __device__ inline int gather_blocks(int nnodes) {
-  const int b = nnodes / 8;  // about one 4x4-bin tile per 16 nodes at unit spacing
+  // about one 4x4-bin tile per 16 nodes at unit spacing; a multiple of 8
+  // (the XCD count) for k_gather's tile mapping
+  const int b = (nnodes / 8 + 7) / 8 * 8;
   return b < 16 ? 16 : (b > 4096 ? 4096 : b);
 }
 __host__ __device__ inline int band_u_blocks(int nnodes) {
@@ -148,6 +150,15 @@ __host__ __device__ inline int band_u_blocks(int nnodes) {
   return b < 16 ? 16 : (b > 8192 ? 8192 : b);
 }
 __host__ __device__ inline int move_blocks(int nnodes) { return (nnodes + 3) / 4; }  // one wave per node
+// k_move_tiles: one workgroup per tile of kMvNodes x kMvNodes nodes, the
+// tiles of all sheets sheet after sheet, row-major within a sheet.
+constexpr int kMvNodes = 4;
+inline int move_tile_count(const SheetDesc *sd, int nsheets) {
+  int t = 0;
+  for (int s = 0; s < nsheets; ++s)
+    t += ((sd[s].rows + kMvNodes - 1) / kMvNodes) * ((sd[s].cols + kMvNodes - 1) / kMvNodes);
+  return t;
+}
 
 // One instance of a batched ensemble, as the batched kernels read it.
 struct BatchDesc {
@@ -216,7 +227,7 @@ int totals_blocks(const Geometry &g);
 int launch_fiber(const IBState &ib, const Geometry &g, int mask, int slot, hipStream_t s, bool band = false);
 int launch_gather(int prec, const IBState &ib, const Geometry &g, int slot, void *F, hipStream_t s);
 int launch_move(int prec, const IBState &ib, const Geometry &g, const void *f, const void *F,
-                const FluidConsts<double> &c, int reset_slot, hipStream_t s);
+                const FluidConsts<double> &c, int reset_slot, int ntiles, hipStream_t s);
 int launch_ib_reset(const IBState &ib, int reset_bins, int reset_slot_mask, hipStream_t s);
 
 }  // namespace lbmib
