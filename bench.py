@@ -209,7 +209,8 @@ def autotune(e, a, lbmib):
     return pick[1], pick[2]
 
 
-BATCH_TILES = [(2, 3), (4, 3), (8, 3), (2, 1), (4, 1)]  # (waves per block, nontemporal mask)
+# (waves per block, nontemporal mask, rows per lane)
+BATCH_TILES = [(2, 3, 1), (4, 3, 1), (8, 3, 1), (2, 1, 1), (2, 3, 2), (4, 3, 2), (8, 3, 2), (4, 1, 2)]
 
 
 def run_batch(a, lbmib, lbmib_configs):
@@ -228,13 +229,16 @@ def run_batch(a, lbmib, lbmib_configs):
         b.init()
         cells, nodes = b.cells()
         best = None
-        for wpb, nt in BATCH_TILES:
+        for wpb, nt, rows in BATCH_TILES:
             b.set_tile(wpb, nt)
+            b.set_rows_per_lane(rows)
             ms, _ = b.time_steps(3, 20, fused=False)
-            print(f"[autotune] tile {wpb} waves/block, nt {nt}: {ms / 20:.4f} ms/step", file=sys.stderr, flush=True)
+            print(f"[autotune] tile {wpb} waves/block, nt {nt}, {rows} row(s) per lane: {ms / 20:.4f} ms/step",
+                  file=sys.stderr, flush=True)
             if best is None or ms < best[0]:
-                best = (ms, wpb, nt)
+                best = (ms, wpb, nt, rows)
         b.set_tile(best[1], best[2])
+        b.set_rows_per_lane(best[3])
         ms_total, ms_fused = b.time_steps(a.warmup, a.steps)
         status = b.status()
         bpc = 2.0 * 19 * lbmib.STORAGE_BYTES[a.precision]
@@ -267,7 +271,7 @@ def run_batch(a, lbmib, lbmib_configs):
             "bytes_per_cell_update": bpc,
             "storage": storage_desc(a.precision, a.dev_ref),
             "layout": a.layout or lbmib.DEFAULT_LAYOUT,
-            "tile": {"waves_per_block": best[1], "nontemporal_mask": best[2]},
+            "tile": {"waves_per_block": best[1], "nontemporal_mask": best[2], "rows_per_lane": best[3]},
             "overlapping_groups": a.overlap != 0,
         },
         "roofline": {
@@ -276,7 +280,8 @@ def run_batch(a, lbmib, lbmib_configs):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            # HBM bytes of one step's batched fused launches (both groups)
+            "traffic": traffic_from_profiles("c4", a.precision),
         },
         "whole_step_hbm_frac": round(bpc * cells / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "fused_kernel_ms": round(ms_fused, 5),

@@ -55,6 +55,7 @@ struct lbmib_batch {
   int wpb = 0;        // waves per block of the fused launch (tile shape)
   int wpb_min = 0;    // widest instance row in waves
   int nt_mask = -1;   // nontemporal loads/stores of the fused launch
+  int rows = 1;       // rows per lane of the fused launch (2: row pairs, every ny even)
   int nnodes = 0;     // fiber nodes over all instances
   double cells = 0;   // grid cells over all instances
   hipStream_t stream = nullptr, stream2 = nullptr;
@@ -91,8 +92,8 @@ static int build_maps(lbmib_batch *b, BatchGroup &g) {
   std::vector<int> maps[kMapCount];
   for (int li = 0; li < (int)g.members.size(); ++li) {
     const lbmib_ctx *c = b->inst[g.members[li]];
-    const int rpb = b->wpb / waves_per_row(c);
-    const int nrows = c->g.ny * c->g.nz;
+    const int rpb = b->wpb / waves_per_row(c);  // rows (row pairs when b->rows == 2) per block
+    const int nrows = c->g.ny * c->g.nz / b->rows;
     const int M = c->nnodes;
     const int counts[kMapCount] = {(nrows + rpb - 1) / rpb, M ? fiber_blocks(M) : 0, M ? gather_blocks(M) : 0,
                                    M ? band_u_blocks(M) : 0, M ? move_blocks(M) : 0};
@@ -262,6 +263,18 @@ extern "C" int lbmib_batch_set_tile(lbmib_batch *b, int waves_per_block, int non
   return LBMIB_OK;
 }
 
+extern "C" int lbmib_batch_set_rows_per_lane(lbmib_batch *b, int rows) {
+  if (!b || (rows != 1 && rows != 2)) return LBMIB_ERR_ARG;
+  if (rows == 2)
+    for (lbmib_ctx *c : b->inst)
+      if (c->g.ny % 2) return LBMIB_ERR_ARG;  // row pairs within one z plane
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream2));
+  b->rows = rows;
+  for (int k = 0; k < b->ngroups; ++k) RCHK(build_maps(b, b->grp[k]));
+  return LBMIB_OK;
+}
+
 extern "C" int lbmib_batch_set_overlap(lbmib_batch *b, int on) {
   if (!b || on < 0 || on > 1) return LBMIB_ERR_ARG;
   b->overlap = on;
@@ -313,7 +326,7 @@ static int batch_steps(lbmib_batch *b, int nsteps, hipEvent_t e0, hipEvent_t e1,
       if (G > 1 && (gi > 0 || k > 0)) HIPCHK(hipStreamWaitEvent(st, b->grp[(gi + G - 1) % G].ev_fluid, 0));
       if (kev) HIPCHK(hipEventRecord(kev[2 * (k * G + gi)], st));
       RCHK(launch_collide_batch(b->prec, g.d_desc, g.d_map[kMapFused], g.nblk[kMapFused], b->wpb, cur, own, slot,
-                                b->nt_mask, st));
+                                b->nt_mask, b->rows, st));
       if (kev) HIPCHK(hipEventRecord(kev[2 * (k * G + gi) + 1], st));
       if (G > 1) HIPCHK(hipEventRecord(g.ev_fluid, st));
       RCHK(launch_band_u_batch(b->prec, g.d_desc, g.d_map[kMapBandU], g.nblk[kMapBandU], 1 - cur, slot, st));

@@ -623,11 +623,109 @@ __global__ void __launch_bounds__(512)
   }
 }
 
+// One lane of the two-row path: cells (x, row0) and (x, row0 + 1), rows
+// (y0, z) and (y0 + 1, z) of one z plane (row0 even, ny even), with two
+// independent sets of aligned buffer loads, which doubles the bytes each wave
+// keeps in flight.  Filler lanes (dead x, and the x edges of a wall or
+// periodic x axis, left to k_collide_xedges) store zeros so that the rows'
+// cache lines are written whole.
+template <int P, bool NT, bool NTL>
+__device__ __forceinline__ void row_pair_cell(const Geometry &g, __amdgpu_buffer_rsrc_t rin,
+                                              __amdgpu_buffer_rsrc_t rout, int own,
+                                              const KConsts<typename Traits<P>::T> &c, int nsheets,
+                                              const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                                              const typename Traits<P>::FT *__restrict__ Fcur,
+                                              const typename Traits<P>::FT *__restrict__ Fprev,
+                                              const uint32_t *__restrict__ rowtab, int row0, int x) {
+  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
+  constexpr uint32_t es = sizeof(S);
+  const uint32_t pitch_b = (uint32_t)g.pq * es;   // between the populations of a cell
+  const uint32_t sy_b = (uint32_t)g.prow * es;    // between rows (y, z) and (y + 1, z)
+  const bool edge = x == g.lo[0] || x == g.hi[0];
+  const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
+  const bool filler = x < g.lo[0] || x > g.hi[0] || (edge && !frozen);
+  const uint32_t vo = (uint32_t)x * es;
+  const int y0 = row0 % g.ny, z = row0 / g.ny;  // rows row0, row0 + 1 = (y0, z), (y0 + 1, z)
+  if (z < g.lo[2] || z > g.hi[2]) return;  // dead rows
+  bool live[2];
+  T v[2][19];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int y = y0 + k;
+    live[k] = y >= g.lo[1] && y <= g.hi[1];
+  }
+  if (filler) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) v[0][q] = v[1][q] = (T)0;
+  } else {
+    T d[2], rho[2], ux[2], uy[2], uz[2];
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        uint32_t qb = (uint32_t)(row0 + k) * sy_b;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+          v[k][q] = bld_h<P, NTL>(rin, vo, qb);
+          qb += pitch_b;
+        }
+        rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+      }
+    } else {
+      const uint32_t *rt = rowtab + (long)row0 * kRowTabStride;
+      uint32_t so[2][19];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) so[k][q] = ldc(rt + k * kRowTabStride + q);
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[k][q] = bld_h<P, NTL>(rin, vo, so[k][q]);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        bounce_fix<P>(ldc(rt + k * kRowTabStride + 19), c.br[1], c.br[2], v[k]);
+        const long ck = (long)x + (long)g.nx * (row0 + k);
+        T fpx = 0, fpy = 0, fpz = 0;
+        if (in_any_box(box_prev, nsheets, x, y0 + k, z)) {
+          fpx = (T)Fprev[ck]; fpy = (T)Fprev[g.n + ck]; fpz = (T)Fprev[2 * g.n + ck];
+        }
+        lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const long ck = (long)x + (long)g.nx * (row0 + k);
+      if (frozen) {
+        // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) v[k][q] = ldc(c.tab + q);
+        rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
+      }
+      T fx = 0, fy = 0, fz = 0;
+      if (in_any_box(box_cur, nsheets, x, y0 + k, z)) {
+        fx = (T)Fcur[ck]; fy = (T)Fcur[g.n + ck]; fz = (T)Fcur[2 * g.n + ck];
+      }
+      collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (!live[k]) continue;  // a dead row of a wall y axis
+    uint32_t sb = (uint32_t)(row0 + k) * sy_b;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      bst_h<P, NT>(rout, vo, sb, v[k][q]);
+      sb += pitch_b;
+    }
+  }
+}
+
 // Two rows per block (y and y + 1 of one z plane): lane x updates (x, y, z)
-// and (x, y + 1, z) with two independent sets of aligned buffer loads, which
-// doubles the bytes each wave keeps in flight without the unaligned pair
-// loads of k_collide_stream_pairs.  Requires ny even and blockDim.x >= nx;
-// x-edge cells of a wall or periodic x axis are left to k_collide_xedges.
+// and (x, y + 1, z) (row_pair_cell), which doubles the bytes each wave keeps
+// in flight without the unaligned pair loads of k_collide_stream_pairs.
+// Requires ny even and blockDim.x >= nx; x-edge cells of a wall or periodic x
+// axis are left to k_collide_xedges.
 template <int P, bool NT, bool NTL>
 __global__ void __launch_bounds__(512)
     k_collide_stream_rows2(Geometry g, const typename Traits<P>::S *__restrict__ fin,
@@ -635,96 +733,12 @@ __global__ void __launch_bounds__(512)
                            int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                            const typename Traits<P>::FT *__restrict__ Fcur,
                            const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab) {
-  using T = typename Traits<P>::T;
-  using S = typename Traits<P>::S;
   const int npairs = g.ny * g.nz / 2;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
-  constexpr uint32_t es = sizeof(S);
-  const uint32_t pitch_b = (uint32_t)g.pq * es;   // between the populations of a cell
-  const uint32_t sy_b = (uint32_t)g.prow * es;    // between rows (y, z) and (y + 1, z)
   const int x = threadIdx.x;
   if (x >= g.nx) return;
-  const bool edge = x == g.lo[0] || x == g.hi[0];
-  const bool frozen = edge && g.bc[0] == LBMIB_BC_INOUT;
-  // filler lanes store zeros so that the rows' cache lines are written whole
-  const bool filler = x < g.lo[0] || x > g.hi[0] || (edge && !frozen);
-  const uint32_t vo = (uint32_t)x * es;
-  for (int rp = blockIdx.x; rp < npairs; rp += gridDim.x) {
-    const int row0 = 2 * rp;
-    const int y0 = row0 % g.ny, z = row0 / g.ny;  // rows row0, row0 + 1 = (y0, z), (y0 + 1, z)
-    if (z < g.lo[2] || z > g.hi[2]) continue;  // dead rows
-    bool live[2];
-    T v[2][19];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int y = y0 + k;
-      live[k] = y >= g.lo[1] && y <= g.hi[1];
-    }
-    if (filler) {
-#pragma unroll
-      for (int q = 0; q < 19; ++q) v[0][q] = v[1][q] = (T)0;
-    } else {
-      T d[2], rho[2], ux[2], uy[2], uz[2];
-      if (own) {
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          uint32_t qb = (uint32_t)(row0 + k) * sy_b;
-#pragma unroll
-          for (int q = 0; q < 19; ++q) {
-            v[k][q] = bld_h<P, NTL>(rin, vo, qb);
-            qb += pitch_b;
-          }
-          rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
-        }
-      } else {
-        const uint32_t *rt = rowtab + (long)row0 * kRowTabStride;
-        uint32_t so[2][19];
-#pragma unroll
-        for (int k = 0; k < 2; ++k)
-#pragma unroll
-          for (int q = 0; q < 19; ++q) so[k][q] = ldc(rt + k * kRowTabStride + q);
-#pragma unroll
-        for (int k = 0; k < 2; ++k)
-#pragma unroll
-          for (int q = 0; q < 19; ++q) v[k][q] = bld_h<P, NTL>(rin, vo, so[k][q]);
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          bounce_fix<P>(ldc(rt + k * kRowTabStride + 19), c.br[1], c.br[2], v[k]);
-          const long ck = (long)x + (long)g.nx * (row0 + k);
-          T fpx = 0, fpy = 0, fpz = 0;
-          if (in_any_box(box_prev, nsheets, x, y0 + k, z)) {
-            fpx = (T)Fprev[ck]; fpy = (T)Fprev[g.n + ck]; fpz = (T)Fprev[2 * g.n + ck];
-          }
-          lagged_moments<P>(v[k], fpx, fpy, fpz, c, d[k], rho[k], ux[k], uy[k], uz[k]);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const long ck = (long)x + (long)g.nx * (row0 + k);
-        if (frozen) {
-          // inlet/outlet plane: the table, with init_gv's rho/u (lbm_ib.c:2482-2483)
-#pragma unroll
-          for (int q = 0; q < 19; ++q) v[k][q] = ldc(c.tab + q);
-          rho[k] = c.rho0; d[k] = c.d0; ux[k] = c.u0x; uy[k] = c.u0y; uz[k] = c.u0z;
-        }
-        T fx = 0, fy = 0, fz = 0;
-        if (in_any_box(box_cur, nsheets, x, y0 + k, z)) {
-          fx = (T)Fcur[ck]; fy = (T)Fcur[g.n + ck]; fz = (T)Fcur[2 * g.n + ck];
-        }
-        collide_cell<P>(v[k], d[k], rho[k], ux[k], uy[k], uz[k], fx, fy, fz, c);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      if (!live[k]) continue;  // a dead row of a wall y axis
-      uint32_t sb = (uint32_t)(row0 + k) * sy_b;
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        bst_h<P, NT>(rout, vo, sb, v[k][q]);
-        sb += pitch_b;
-      }
-    }
-  }
+  for (int rp = blockIdx.x; rp < npairs; rp += gridDim.x)
+    row_pair_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, 2 * rp, x);
 }
 
 // Batched ensemble: every instance's rows in one launch.  Block b serves
@@ -757,6 +771,35 @@ __global__ void __launch_bounds__(512)
   row_cell<P, NT, NTL>(g, make_rsrc(ldc(&dp->f[cur])), make_rsrc(ldc(&dp->f[1 - cur])), own, c, ldc(&dp->nsheets),
                        box + slot * kMaxSheets * 6, box + (1 - slot) * kMaxSheets * 6,
                        (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab), row, x);
+}
+
+// Two rows per lane: each wave serves one 64-lane x chunk of a row pair
+// (row0, row0 + 1) of one z plane (row_pair_cell), so that a wave keeps
+// twice the bytes in flight; rpb = wpb / wpr row pairs per block.  Every
+// instance's ny is even.
+template <int P, bool NT, bool NTL>
+__global__ void __launch_bounds__(512)
+    k_collide_batch2(const BatchDesc *__restrict__ descs, const int *__restrict__ map, int wpb, int cur, int own,
+                     int slot) {
+  using T = typename Traits<P>::T;
+  using FT = typename Traits<P>::FT;
+  const int m = map[blockIdx.x];
+  const int inst = m >> kBatchShift, lb = m & ((1 << kBatchShift) - 1);
+  const BatchDesc *dp = descs + inst;
+  const int wpr = ldc(&dp->wpr), nrows = ldc(&dp->nrows);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int rpb = wpb / wpr;
+  if (wave >= rpb * wpr) return;
+  const int rp = lb * rpb + wave / wpr;
+  const int x = (wave % wpr) * 64 + lane;
+  const Geometry g = ldc_obj(&dp->g);
+  if (2 * rp >= nrows || x >= g.nx) return;
+  const KConsts<T> c = ldc_obj(reinterpret_cast<const KConsts<T> *>(dp->kc));
+  const int *box = ldc(&dp->ib.box);
+  row_pair_cell<P, NT, NTL>(g, make_rsrc(ldc(&dp->f[cur])), make_rsrc(ldc(&dp->f[1 - cur])), own, c,
+                            ldc(&dp->nsheets), box + slot * kMaxSheets * 6, box + (1 - slot) * kMaxSheets * 6,
+                            (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab),
+                            2 * rp, x);
 }
 
 // The band of a split step: every cell of the band boxes (written by the
@@ -1323,12 +1366,21 @@ int upload_fluid_tables(int prec, const FluidConsts<double> &c, void *d_tab, dou
 }
 
 int launch_collide_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int wpb, int cur, int own,
-                         int slot, int nt_mask, hipStream_t s) {
+                         int slot, int nt_mask, int rows, hipStream_t s) {
   if (nblocks <= 0) return LBMIB_OK;
-  if (wpb < 1 || wpb > 8) return LBMIB_ERR_ARG;
+  if (wpb < 1 || wpb > 8 || (rows != 1 && rows != 2)) return LBMIB_ERR_ARG;
   const int nt = nt_mask < 0 ? 1 : (nt_mask & 1);
   const int ntl = nt_mask < 0 ? 1 : ((nt_mask >> 1) & 1);
   const dim3 b(64 * wpb);
+  if (rows == 2) {
+    LBMIB_DISPATCH(prec, ({
+      if (nt && ntl) k_collide_batch2<P, true, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+      else if (nt) k_collide_batch2<P, true, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+      else if (ntl) k_collide_batch2<P, false, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+      else k_collide_batch2<P, false, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+    }));
+    return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+  }
   LBMIB_DISPATCH(prec, ({
     if (nt && ntl) k_collide_batch<P, true, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
     else if (nt) k_collide_batch<P, true, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);

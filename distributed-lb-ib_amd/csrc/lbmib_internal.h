@@ -178,7 +178,7 @@ struct BatchDesc {
 // block index within the instance.
 constexpr int kBatchShift = 20;
 int launch_collide_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int wpb, int cur, int own,
-                         int slot, int nt_mask, hipStream_t s);
+                         int slot, int nt_mask, int rows, hipStream_t s);
 int launch_fiber_batch(const BatchDesc *d, const int *map, int nblocks, int slot, hipStream_t s);
 int launch_gather_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int slot, int mcap_max,
                         hipStream_t s);

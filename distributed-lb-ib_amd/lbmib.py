@@ -38,8 +38,8 @@ ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lb
                "lbmib_bytes_per_cell", "lbmib_probe_copy_bandwidth",
                "lbmib_set_kernel_variant", "lbmib_set_overlap", "lbmib_timed_kernel_cells",
                "lbmib_batch_create", "lbmib_batch_destroy", "lbmib_batch_instance", "lbmib_batch_init",
-               "lbmib_batch_step", "lbmib_batch_set_tile", "lbmib_batch_set_overlap", "lbmib_batch_cells",
-               "lbmib_batch_status",
+               "lbmib_batch_step", "lbmib_batch_set_tile", "lbmib_batch_set_rows_per_lane", "lbmib_batch_set_overlap",
+               "lbmib_batch_cells", "lbmib_batch_status",
                "lbmib_batch_time_steps", "lbmib_coarse_moments", "lbmib_coarse_sheets", "lbmib_jump",
                "lbmib_jump_commit", "lbmib_jump_revert", "lbmib_get_stream", "lbmib_coarse_state") + tuple("lbmib_" + n for n in CTX_CALLS)
 
@@ -110,6 +110,7 @@ def load_library(path=LIB_PATH):
     L.lbmib_batch_init.argtypes = [vp]
     L.lbmib_batch_step.argtypes = [vp, i]
     L.lbmib_batch_set_tile.argtypes = [vp, i, i]
+    L.lbmib_batch_set_rows_per_lane.argtypes = [vp, i]
     L.lbmib_batch_set_overlap.argtypes = [vp, i]
     L.lbmib_batch_cells.argtypes = [vp, dp, ip]
     L.lbmib_batch_status.argtypes = [vp]
@@ -475,6 +476,11 @@ class Batch:
 
     def set_tile(self, waves_per_block, nontemporal=-1):
         self._check(self.L.lbmib_batch_set_tile(self.h, waves_per_block, nontemporal), "batch_set_tile")
+
+    def set_rows_per_lane(self, rows):
+        """1 (default) or 2: each lane of the fused launch updates one cell of
+        two neighbouring rows; same results."""
+        self._check(self.L.lbmib_batch_set_rows_per_lane(self.h, int(rows)), "batch_set_rows_per_lane")
 
     def set_overlap(self, on):
         """Two instance groups whose IB kernels overlap each other's fluid

@@ -211,6 +211,12 @@ int lbmib_batch_step(lbmib_batch *b, int nsteps);
  * or 64-cell chunk of a row, per wave; a block holds whole rows) and the
  * nontemporal mask as in lbmib_set_kernel_variant.  bench.py autotunes it. */
 int lbmib_batch_set_tile(lbmib_batch *b, int waves_per_block, int nontemporal);
+/* Rows per lane of the batched fused launch: 1 (default) or 2, where each
+ * lane updates one cell of two neighbouring rows (y, y + 1) of one z plane,
+ * which doubles the bytes a wave keeps in flight; 2 needs every instance's
+ * ny even (else LBMIB_ERR_ARG).  Results are the same bits either way.
+ * bench.py autotunes it with the tile. */
+int lbmib_batch_set_rows_per_lane(lbmib_batch *b, int rows);
 /* on = 1 (default): two groups of instances of about equal cell count, each
  * on its own stream, whose fused launches alternate so that one group's IB
  * kernels run beside the other group's fluid update; on = 0: one group.

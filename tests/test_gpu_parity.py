@@ -678,15 +678,18 @@ def _ensemble(lbmib, precision):
     return out
 
 
+@pytest.mark.parametrize("rows", [1, 2])
 @pytest.mark.parametrize("overlap", [1, 0])
 @pytest.mark.parametrize("precision", ["f32", "bf16", "f64"])
-def test_batch_matches_single_instances(lbmib, precision, overlap):
+def test_batch_matches_single_instances(lbmib, precision, overlap, rows):
     """Each member of a batched ensemble ends with the bits of the same
     simulation stepped alone, with the instances in two overlapping groups
-    (the default) or in one."""
+    (the default) or in one, and one or two rows per lane in the fused
+    launch."""
     cfgs = _ensemble(lbmib, precision)
     with lbmib.Batch(cfgs) as b:
         b.set_overlap(overlap)
+        b.set_rows_per_lane(rows)
         b.init()
         b.step(5)
         b.step(7)
@@ -705,9 +708,10 @@ def test_batch_matches_single_instances(lbmib, precision, overlap):
 def test_batch_tile_shapes_are_bitwise_identical(lbmib):
     cfgs = _ensemble(lbmib, "f32")
     outs = []
-    for wpb, nt in ((2, 3), (4, 1), (8, 0)):
+    for wpb, nt, rows in ((2, 3, 1), (4, 1, 1), (8, 0, 1), (2, 3, 2), (4, 1, 2), (8, 3, 2)):
         with lbmib.Batch(cfgs) as b:
             b.set_tile(wpb, nt)
+            b.set_rows_per_lane(rows)
             b.init()
             b.step(10)
             assert b.status() == 0
@@ -715,6 +719,17 @@ def test_batch_tile_shapes_are_bitwise_identical(lbmib):
     for o in outs[1:]:
         for a, r in zip(o, outs[0]):
             assert np.array_equal(a, r)
+
+
+def test_batch_row_pairs_need_even_ny(lbmib):
+    cfgs = _ensemble(lbmib, "f32")[:2]
+    cfgs[1]["ny"] += 1
+    with lbmib.Batch(cfgs) as b:
+        with pytest.raises(RuntimeError):
+            b.set_rows_per_lane(2)
+        with pytest.raises(RuntimeError):
+            b.set_rows_per_lane(3)
+        b.set_rows_per_lane(1)
 
 
 def test_c4_workload_batch_matches_single_instances(lbmib, configs):
