@@ -71,6 +71,8 @@ def parse_args():
     p.add_argument("--overlap", type=int, default=-1, choices=[-1, 0, 1],
                    help="overlap the IB kernels with the fluid update (split steps; for c4, two instance "
                         "groups): -1 autotuned (c4: on), 0 off, 1 on")
+    p.add_argument("--tile", default="auto",
+                   help="c4: batched fused launch 'waves_per_block,ntmask,rows_per_lane' or 'auto' (autotune)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -229,7 +231,8 @@ def run_batch(a, lbmib, lbmib_configs):
         b.init()
         cells, nodes = b.cells()
         best = None
-        for wpb, nt, rows in BATCH_TILES:
+        tiles = BATCH_TILES if a.tile == "auto" else [tuple(int(v) for v in a.tile.split(","))]
+        for wpb, nt, rows in tiles:
             b.set_tile(wpb, nt)
             b.set_rows_per_lane(rows)
             ms, _ = b.time_steps(3, 20, fused=False)

@@ -132,6 +132,7 @@ class Net:
             self.wt[name] = torch.empty((ci, 27, co), dtype=torch.bfloat16, device=dev)
         self.t = 0
         self._bufs = {}
+        self.timing = None  # {(op, layer): [event pairs]} while bench() times the layers
 
     def _cast(self, src, dst):
         check(self.L.lbmib_sg_cast_bf16(ptr(src), ptr(dst), src.numel(), stream()), "cast_bf16")
@@ -148,11 +149,21 @@ class Net:
         return sum(p.numel() + self.b[k].numel() for k, p in self.p.items())
 
     # ------------------------------------------------------------ forward
+    def _timed(self, op, name, fn):
+        if self.timing is None:
+            return fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        self.timing.setdefault((op, name), []).append((e0, e1))
+
     def _conv(self, name, x, y, dims, relu):
         _, ci, co, _, _ = next(l for l in self.layers if l[0] == name)
         B, D, H, W = dims
-        check(self.L.lbmib_sg_conv3d_fwd(ptr(x), ptr(self.w16[name]), ptr(self.b[name]), None, ptr(y), B, D, H, W, ci,
-                                         co, int(relu), stream()), "conv3d_fwd " + name)
+        self._timed("fwd", name, lambda: check(
+            self.L.lbmib_sg_conv3d_fwd(ptr(x), ptr(self.w16[name]), ptr(self.b[name]), None, ptr(y), B, D, H, W, ci,
+                                       co, int(relu), stream()), "conv3d_fwd " + name))
 
     def forward(self, x):
         """x: BF16 [B, D, H, W, cin] on the GPU (D, H, W divisible by
@@ -200,13 +211,15 @@ class Net:
     # ----------------------------------------------------------- backward
     def _wgrad(self, name, x, dy, dims):
         _, ci, co, _, _ = next(l for l in self.layers if l[0] == name)
-        check(self.L.lbmib_sg_conv3d_bwd_weight(ptr(x), ptr(dy), ptr(self.g[name]), ptr(self.gb[name]), *dims, ci, co,
-                                                stream()), "conv3d_bwd_weight " + name)
+        self._timed("wgrad", name, lambda: check(
+            self.L.lbmib_sg_conv3d_bwd_weight(ptr(x), ptr(dy), ptr(self.g[name]), ptr(self.gb[name]), *dims, ci, co,
+                                              stream()), "conv3d_bwd_weight " + name))
 
     def _dgrad(self, name, dy, dx, mask, dims):
         _, ci, co, _, _ = next(l for l in self.layers if l[0] == name)
-        check(self.L.lbmib_sg_conv3d_bwd_data(ptr(dy), ptr(self.w16[name]), ptr(self.wt[name]), ptr(mask), ptr(dx),
-                                              *dims, ci, co, stream()), "conv3d_bwd_data " + name)
+        self._timed("dgrad", name, lambda: check(
+            self.L.lbmib_sg_conv3d_bwd_data(ptr(dy), ptr(self.w16[name]), ptr(self.wt[name]), ptr(mask), ptr(dx),
+                                            *dims, ci, co, stream()), "conv3d_bwd_data " + name))
 
     def backward(self, dout):
         """Parameter gradients (FP32, overwritten) for the output gradient
@@ -301,3 +314,63 @@ class Net:
             self.b[name].copy_(torch.from_numpy(np.asarray(st[name + ".b"], dtype=np.float32)))
             self._cast(self.p[name], self.w16[name])
         torch.cuda.synchronize()
+
+
+def bench(batch=4, edge=64, warm=3, iters=10, predict_batch=1):
+    """Times the network's kernels on synthetic data: the training step
+    (forward, MSE, data and weight gradients, Adam) at batch x edge^3 and the
+    forward pass at predict_batch x edge^3 (the coupling's prediction), with
+    HIP events on the library's stream.  FLOPs are the convolutions' 2 per
+    multiply-add: the forward pass F, a training step 3F (data and weight
+    gradients each cost one forward's MACs).  Returns a dict."""
+    net = Net()
+    rng = np.random.default_rng(1)
+    x = torch.tensor(rng.standard_normal((batch, edge, edge, edge, net.cin)), dtype=torch.bfloat16, device="cuda")
+    tgt = torch.tensor(rng.standard_normal((batch * edge ** 3, net.nout)) * 1e-3, dtype=torch.float32, device="cuda")
+    xp = x[:predict_batch].contiguous()
+    out = {"batch": batch, "edge": edge, "params": net.n_params()}
+    for name, fn, b, nflop in (("train_step", lambda: net.train_step(x, tgt, 1e-4), batch, 3),
+                               ("forward", lambda: net.forward(xp), predict_batch, 1)):
+        for _ in range(warm):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            fn()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        flops = nflop * conv_flops(net.layers, b, edge, edge, edge)
+        out[name] = {"batch": b, "ms": round(ms, 4), "conv_gflop": round(flops / 1e9, 2),
+                     "tflops": round(flops / (ms * 1e-3) / 1e12, 1)}
+    # per layer and operation: one more pass of each with events around every
+    # convolution call (the calls' own launches, split-K reductions included)
+    for tag, fn, b in (("train", lambda: net.train_step(x, tgt, 1e-4), batch),
+                       ("predict", lambda: net.forward(xp), predict_batch)):
+        net.timing = {}
+        for _ in range(iters):
+            fn()
+        torch.cuda.synchronize()
+        rows = []
+        for (op, lname), evs in net.timing.items():
+            _, ci, co, lvl, _ = next(l for l in net.layers if l[0] == lname)
+            ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / len(evs)
+            fl = 2 * b * (edge >> lvl) ** 3 * 27 * ci * co
+            rows.append({"op": op, "layer": lname, "cin": ci, "cout": co, "grid": edge >> lvl, "us": round(ms * 1e3, 1),
+                         "tflops": round(fl / (ms * 1e-3) / 1e12, 1)})
+        out[f"layers_{tag}"] = rows
+        net.timing = None
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="time the surrogate network's kernels on synthetic data")
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--edge", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--predict-batch", type=int, default=1)
+    args = ap.parse_args()
+    print(json.dumps(bench(args.batch, args.edge, iters=args.iters, predict_batch=args.predict_batch)), flush=True)

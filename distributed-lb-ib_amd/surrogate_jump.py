@@ -304,3 +304,27 @@ def evaluate(net, norm, kw, warm, horizon, factor=4, ksteps=50, verify_steps=1, 
         out[name] = leg
         log(f"[eval] {name}: {leg}")
     return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import os
+
+    import lbmib_configs
+
+    ap = argparse.ArgumentParser(description="the surrogate on the 256^3 one-sheet workload: plain steps against "
+                                             "surrogate jumps, with the jump method without the network beside")
+    ap.add_argument("--model", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
+                                                    "surrogate_c5.npz"))
+    ap.add_argument("--warm", type=int, default=100)
+    ap.add_argument("--horizon", type=int, default=1020)
+    ap.add_argument("--tol", type=float, default=5.0)
+    ap.add_argument("--verify-steps", type=int, default=1)
+    ap.add_argument("--no-baseline", action="store_true")
+    args = ap.parse_args()
+    net, norm, factor, ksteps, _ = load_model(args.model)
+    kw, _ = lbmib_configs.c2(precision="f32")
+    res = evaluate(net, norm, kw, args.warm, args.horizon, factor, ksteps, args.verify_steps, args.tol,
+                   baseline=not args.no_baseline)
+    print(json.dumps(res), flush=True)
