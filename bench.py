@@ -54,7 +54,9 @@ def storage_desc(precision, dev_ref):
 def parse_args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--steps", type=int, default=None,
+                   help="timed steps (default 1000; c4: 200, so that warm-up, timed steps and the per-launch "
+                        "event pass stay within the ensemble's 500-step workload)")
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--precision", default="f32", choices=["f32", "f64", "f16", "bf16"])
     p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
@@ -71,11 +73,17 @@ def parse_args():
     p.add_argument("--overlap", type=int, default=-1, choices=[-1, 0, 1],
                    help="overlap the IB kernels with the fluid update (split steps; for c4, two instance "
                         "groups): -1 autotuned (c4: on), 0 off, 1 on")
+    p.add_argument("--box-row-stores", type=int, default=-1, choices=[-1, 0, 1],
+                   help="1: the fused kernel stores the rows meeting a support box with the default cache "
+                        "policy, 0: every row nontemporally, -1: automatic (lbmib_set_box_row_stores)")
     p.add_argument("--tile", default="auto",
                    help="c4: batched fused launch 'waves_per_block,ntmask,rows_per_lane' or 'auto' (autotune)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 1 GiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    return p.parse_args()
+    a = p.parse_args()
+    if a.steps is None:
+        a.steps = 200 if a.workload == "c4" else 1000
+    return a
 
 
 def traffic_from_profiles(workload, precision):
@@ -90,6 +98,31 @@ def traffic_from_profiles(workload, precision):
         if m:
             return float(m.group(1))
     return None
+
+
+def cpu_port_baseline(kw, steps=2):
+    """The oracle's sequential FP64 restatement (oracle/lbmib_oracle.c, one
+    thread) on the same workload, when the reference binary is not built:
+    one warm step, then `steps` timed."""
+    sys.path.insert(0, REPO)
+    from oracle.oracle import Oracle
+    o = Oracle(kw["nx"], kw["ny"], kw["nz"], kw["bc"], kw["tau"], kw["rho0"], kw["u0"], kw.get("quirks", True))
+    try:
+        for sh in kw["sheets"]:
+            o.add_sheet(sh.rows, sh.cols, sh.width, sh.height, sh.ks, sh.kb, sh.xyz)
+        o.init()
+        o.preshift(all_sheets=True)
+        o.step(1)
+        t0 = time.time()
+        o.step(steps)
+        t_step = (time.time() - t0) / steps
+    finally:
+        o.close()
+    n = kw["nx"] * kw["ny"] * kw["nz"]
+    print(f"[cpu baseline] oracle port: {t_step:.2f} s per step", file=sys.stderr, flush=True)
+    return {"value": n / t_step / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"oracle/lbmib_oracle.c (FP64, sequential, gcc -O2), the same {kw['nx']}x{kw['ny']}x{kw['nz']} "
+                      f"workload, {steps} steps after one warm step"}
 
 
 def cpu_reference_baseline(n, rows, steps_pair):
@@ -225,21 +258,31 @@ def run_batch(a, lbmib, lbmib_configs):
         c["layout"] = a.layout or lbmib.DEFAULT_LAYOUT
     print(f"[bench] c4 ensemble: {len(cfgs)} instances, {a.precision}, warmup={a.warmup} steps={a.steps}",
           file=sys.stderr, flush=True)
+    tiles = BATCH_TILES if a.tile == "auto" else [tuple(int(v) for v in a.tile.split(","))]
+    best = None
+    if len(tiles) > 1:
+        with lbmib.Batch(cfgs) as b:
+            if a.overlap >= 0:
+                b.set_overlap(a.overlap)
+            b.init()
+            for wpb, nt, rows in tiles:
+                b.set_tile(wpb, nt)
+                b.set_rows_per_lane(rows)
+                ms, _ = b.time_steps(3, 20, fused=False)
+                print(f"[autotune] tile {wpb} waves/block, nt {nt}, {rows} row(s) per lane: {ms / 20:.4f} ms/step",
+                      file=sys.stderr, flush=True)
+                if best is None or ms < best[0]:
+                    best = (ms, wpb, nt, rows)
+    else:
+        best = (None,) + tiles[0]
+    # the timed steps on a fresh ensemble from the initial state, so that the
+    # sheets have not drifted through the tuning steps
     with lbmib.Batch(cfgs) as b:
         if a.overlap >= 0:
             b.set_overlap(a.overlap)
+        b.set_box_row_stores(a.box_row_stores)
         b.init()
         cells, nodes = b.cells()
-        best = None
-        tiles = BATCH_TILES if a.tile == "auto" else [tuple(int(v) for v in a.tile.split(","))]
-        for wpb, nt, rows in tiles:
-            b.set_tile(wpb, nt)
-            b.set_rows_per_lane(rows)
-            ms, _ = b.time_steps(3, 20, fused=False)
-            print(f"[autotune] tile {wpb} waves/block, nt {nt}, {rows} row(s) per lane: {ms / 20:.4f} ms/step",
-                  file=sys.stderr, flush=True)
-            if best is None or ms < best[0]:
-                best = (ms, wpb, nt, rows)
         b.set_tile(best[1], best[2])
         b.set_rows_per_lane(best[3])
         ms_total, ms_fused = b.time_steps(a.warmup, a.steps)
@@ -275,6 +318,7 @@ def run_batch(a, lbmib, lbmib_configs):
             "storage": storage_desc(a.precision, a.dev_ref),
             "layout": a.layout or lbmib.DEFAULT_LAYOUT,
             "tile": {"waves_per_block": best[1], "nontemporal_mask": best[2], "rows_per_lane": best[3]},
+            "box_row_stores": {-1: "automatic", 0: "nontemporal", 1: "cached"}[a.box_row_stores],
             "overlapping_groups": a.overlap != 0,
         },
         "roofline": {
@@ -409,11 +453,13 @@ def main():
     # start from the initial state on a fresh one, so that the sheet has not
     # drifted through the tuning steps (a small grid's sheet would leave it)
     with lbmib.Engine(**kw) as e:
+        e.set_box_row_stores(a.box_row_stores)
         e.init()
         variant, overlap = autotune(e, a, lbmib)
     with lbmib.Engine(**kw) as e:
         e.set_overlap(overlap)
         e.set_kernel_variant(*variant)
+        e.set_box_row_stores(a.box_row_stores)
         e.init()
         ms_total, _ = e.time_steps(a.warmup, a.steps)
         status = e.status()
@@ -457,6 +503,8 @@ def main():
         n1, n2 = (int(x) for x in a.cpu_steps.split(","))
         try:
             cpu = cpu_reference_baseline(kw["nx"], kw["sheets"][0].rows, (n1, n2))
+            if cpu is None:  # the reference binary is not built: time the oracle's restatement
+                cpu = cpu_port_baseline(kw)
         except Exception as ex:  # the baseline is a report, not the measurement
             print(f"[cpu baseline] failed: {ex}", file=sys.stderr)
             cpu = None
@@ -485,6 +533,7 @@ def main():
             "bytes_per_cell_update": bpc,
             "kernel_variant": {"cells_per_thread": variant[0], "nontemporal_mask": variant[1],
                                "blocks_per_cu": variant[2]},
+            "box_row_stores": {-1: "automatic", 0: "nontemporal", 1: "cached"}[a.box_row_stores],
             "split_steps": overlap != 0,
         },
         "roofline": {

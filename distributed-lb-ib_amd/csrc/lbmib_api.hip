@@ -80,6 +80,21 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   return LBMIB_OK;
 }
 
+namespace lbmib {
+// Bytes of one step's stores in the rows the sheets' support boxes meet
+// (estimated from the sheets' extents).
+double box_row_bytes(const lbmib_ctx *ctx) {
+  double rows = 0;
+  for (int s = 0; s < ctx->nsheets; ++s) rows += box_rows_estimate(ctx->sd[s]);
+  const double nrows = (double)ctx->g.ny * ctx->g.nz;
+  return (rows < nrows ? rows : nrows) * 19.0 * ctx->g.nx * (double)ctx->esz;
+}
+
+int box_rows_auto(const lbmib_ctx *ctx) {
+  return ctx->nsheets > 0 && box_row_bytes(ctx) <= kBoxRowCacheBytes ? 1 : 0;
+}
+}  // namespace lbmib
+
 extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const lbmib_sheet_params *sheets,
                             lbmib_ctx **out) {
   if (!fluid || !out || nsheets < 0 || nsheets > kMaxSheets || (nsheets > 0 && !sheets)) return LBMIB_ERR_ARG;
@@ -210,6 +225,7 @@ extern "C" int lbmib_create(const lbmib_fluid_params *fluid, int nsheets, const 
 
   IBState &ib = ctx->ib;
   ib.nsheets = nsheets;
+  ctx->kv.box_rows_cached = box_rows_auto(ctx);
   ctx->move_tiles = move_tile_count(ctx->sd, nsheets);
   ib.nnodes = ctx->nnodes;
   ib.quirks = p.quirks;
@@ -774,6 +790,12 @@ extern "C" int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *m
     *ms_main_mean = acc / nsteps;
   }
   return poll_status(ctx);
+}
+
+extern "C" int lbmib_set_box_row_stores(lbmib_ctx *ctx, int cached) {
+  if (!ctx || cached < -1 || cached > 1) return LBMIB_ERR_ARG;
+  ctx->kv.box_rows_cached = cached < 0 ? box_rows_auto(ctx) : cached;
+  return LBMIB_OK;
 }
 
 extern "C" int lbmib_set_kernel_variant(lbmib_ctx *ctx, int cells_per_thread, int nontemporal, int blocks_per_cu) {

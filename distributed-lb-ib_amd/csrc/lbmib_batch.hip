@@ -56,6 +56,7 @@ struct lbmib_batch {
   int wpb_min = 0;    // widest instance row in waves
   int nt_mask = -1;   // nontemporal loads/stores of the fused launch
   int rows = 1;       // rows per lane of the fused launch (2: row pairs, every ny even)
+  int box_rows = 0;   // rows meeting a support box stored with the default cache policy (set at create)
   int nnodes = 0;     // fiber nodes over all instances
   double cells = 0;   // grid cells over all instances
   hipStream_t stream = nullptr, stream2 = nullptr;
@@ -72,6 +73,13 @@ struct lbmib_batch {
     int _r = (x);      \
     if (_r) return _r; \
   } while (0)
+
+// The batch's box rows are cached when all instances' together fit.
+static int batch_box_rows_auto(const lbmib_batch *b) {
+  double bytes = 0;
+  for (const lbmib_ctx *c : b->inst) bytes += box_row_bytes(c);
+  return bytes > 0 && bytes <= kBoxRowCacheBytes ? 1 : 0;
+}
 
 static int waves_per_row(const lbmib_ctx *c) { return (c->g.nx + 63) / 64; }
 
@@ -224,7 +232,10 @@ extern "C" int lbmib_batch_create(int n, const lbmib_fluid_params *fluids, const
   b->ev_inst.assign(n, nullptr);
   for (int i = 0; i < n && !rc; ++i)
     if (hipEventCreateWithFlags(&b->ev_inst[i], hipEventDisableTiming) != hipSuccess) rc = LBMIB_ERR_HIP;
-  if (!rc) rc = build_groups(b);
+  if (!rc) {
+    b->box_rows = batch_box_rows_auto(b);
+    rc = build_groups(b);
+  }
   if (rc) {
     lbmib_batch_destroy(b);
     return rc;
@@ -272,6 +283,12 @@ extern "C" int lbmib_batch_set_rows_per_lane(lbmib_batch *b, int rows) {
   HIPCHK(hipStreamSynchronize(b->stream2));
   b->rows = rows;
   for (int k = 0; k < b->ngroups; ++k) RCHK(build_maps(b, b->grp[k]));
+  return LBMIB_OK;
+}
+
+extern "C" int lbmib_batch_set_box_row_stores(lbmib_batch *b, int cached) {
+  if (!b || cached < -1 || cached > 1) return LBMIB_ERR_ARG;
+  b->box_rows = cached < 0 ? batch_box_rows_auto(b) : cached;
   return LBMIB_OK;
 }
 
@@ -326,7 +343,7 @@ static int batch_steps(lbmib_batch *b, int nsteps, hipEvent_t e0, hipEvent_t e1,
       if (G > 1 && (gi > 0 || k > 0)) HIPCHK(hipStreamWaitEvent(st, b->grp[(gi + G - 1) % G].ev_fluid, 0));
       if (kev) HIPCHK(hipEventRecord(kev[2 * (k * G + gi)], st));
       RCHK(launch_collide_batch(b->prec, g.d_desc, g.d_map[kMapFused], g.nblk[kMapFused], b->wpb, cur, own, slot,
-                                b->nt_mask, b->rows, st));
+                                b->nt_mask, b->rows, b->box_rows, st));
       if (kev) HIPCHK(hipEventRecord(kev[2 * (k * G + gi) + 1], st));
       if (G > 1) HIPCHK(hipEventRecord(g.ev_fluid, st));
       RCHK(launch_band_u_batch(b->prec, g.d_desc, g.d_map[kMapBandU], g.nblk[kMapBandU], 1 - cur, slot, st));

@@ -43,7 +43,7 @@ struct lbmib_ctx {
   int time = 1;
   int init_bits = 0;  // 1: DF0, 2: df1, 4: inout
   int latched = LBMIB_OK;
-  KernelVariant kv = {0, -1, 0};  // fused-kernel launch shape
+  KernelVariant kv = {0, -1, 0, 0};  // fused-kernel launch shape (box_rows_cached set at create)
   int ncu = 0;                    // compute units of the device
   // split steps (lbmib_set_overlap): the bulk on `stream`; the band, band
   // velocity, move and the next step's fiber forces and spreading on

@@ -193,6 +193,17 @@ __device__ __forceinline__ bool band_box_of(const Geometry &g, const int b1[6], 
   return r[0] <= r[1] && r[2] <= r[3] && r[4] <= r[5];
 }
 
+// Whether row (y, z) lies in any support box widened by one cell in y and z
+// (wave-uniform).
+__device__ __forceinline__ bool row_near_box(const int *boxes, int nsheets, int y, int z) {
+  bool in = false;
+  for (int s = 0; s < nsheets; ++s) {
+    const int *b = boxes + 6 * s;
+    in |= y >= ldc(b + 2) - 1 && y <= ldc(b + 3) + 1 && z >= ldc(b + 4) - 1 && z <= ldc(b + 5) + 1;
+  }
+  return in;
+}
+
 __device__ __forceinline__ bool in_box3(const int *r, int x, int y, int z) {
   return x >= r[0] && x <= r[1] && y >= r[2] && y <= r[3] && z >= r[4] && z <= r[5];
 }

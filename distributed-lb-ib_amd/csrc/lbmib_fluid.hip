@@ -373,7 +373,7 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
                                          const typename Traits<P>::FT *__restrict__ Fcur,
                                          const typename Traits<P>::FT *__restrict__ Fprev,
                                          const uint32_t *__restrict__ rowtab, int row, int x,
-                                         const int *__restrict__ band = nullptr) {
+                                         const int *__restrict__ band = nullptr, int box_rows_cached = 0) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   constexpr uint32_t es = sizeof(S);
@@ -440,6 +440,16 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
     collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
   }
   uint32_t sb = row_b;
+  if (NT && box_rows_cached && row_near_box(box_cur, nsheets, y, z)) {
+    // the rows meeting a support box: the default cache policy (as in
+    // k_collide_stream_pairs)
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      bst_h<P, false>(rout, vo, sb, v[q]);
+      sb += pitch_b;
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     bst_h<P, NT>(rout, vo, sb, v[q]);
@@ -466,7 +476,7 @@ __global__ void __launch_bounds__(1024)
                           int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                           const typename Traits<P>::FT *__restrict__ Fcur,
                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
-                          const int *__restrict__ band) {
+                          const int *__restrict__ band, int box_rows_cached) {
   if constexpr (!BUF) {
     int row;
     for (long it = blockIdx.x; next_row(g, it, row); it += gridDim.x) {
@@ -482,7 +492,8 @@ __global__ void __launch_bounds__(1024)
     if (x >= g.nx) return;
     int row;
     for (long it = blockIdx.x; next_row(g, it, row); it += gridDim.x)
-      row_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, row, x, band);
+      row_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, row, x, band,
+                           box_rows_cached);
   }
 }
 
@@ -541,7 +552,7 @@ __global__ void __launch_bounds__(512)
                            int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                            const typename Traits<P>::FT *__restrict__ Fcur,
                            const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
-                           const int *__restrict__ band) {
+                           const int *__restrict__ band, int box_rows_cached) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
@@ -615,10 +626,21 @@ __global__ void __launch_bounds__(512)
       }
     }
     uint32_t sb = row_b;
+    if (NT && box_rows_cached && row_near_box(box_cur, nsheets, y, z)) {
+      // rows meeting a support box (widened by the pulls' one cell) are
+      // stored with the default cache policy: measured 6% faster for the
+      // whole F32 launch than nontemporal stores on every row
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      bst_pair<P, NT>(rout, vo, sb, v[0][q], v[1][q]);
-      sb += pitch_b;
+      for (int q = 0; q < 19; ++q) {
+        bst_pair<P, false>(rout, vo, sb, v[0][q], v[1][q]);
+        sb += pitch_b;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        bst_pair<P, NT>(rout, vo, sb, v[0][q], v[1][q]);
+        sb += pitch_b;
+      }
     }
   }
 }
@@ -636,7 +658,8 @@ __device__ __forceinline__ void row_pair_cell(const Geometry &g, __amdgpu_buffer
                                               const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                                               const typename Traits<P>::FT *__restrict__ Fcur,
                                               const typename Traits<P>::FT *__restrict__ Fprev,
-                                              const uint32_t *__restrict__ rowtab, int row0, int x) {
+                                              const uint32_t *__restrict__ rowtab, int row0, int x,
+                                              int box_rows_cached = 0) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   constexpr uint32_t es = sizeof(S);
@@ -713,6 +736,16 @@ __device__ __forceinline__ void row_pair_cell(const Geometry &g, __amdgpu_buffer
   for (int k = 0; k < 2; ++k) {
     if (!live[k]) continue;  // a dead row of a wall y axis
     uint32_t sb = (uint32_t)(row0 + k) * sy_b;
+    if (NT && box_rows_cached && row_near_box(box_cur, nsheets, y0 + k, z)) {
+      // the rows meeting a support box: the default cache policy (as in
+      // k_collide_stream_pairs)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        bst_h<P, false>(rout, vo, sb, v[k][q]);
+        sb += pitch_b;
+      }
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       bst_h<P, NT>(rout, vo, sb, v[k][q]);
@@ -732,13 +765,15 @@ __global__ void __launch_bounds__(512)
                            typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
                            int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                            const typename Traits<P>::FT *__restrict__ Fcur,
-                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab) {
+                           const typename Traits<P>::FT *__restrict__ Fprev, const uint32_t *__restrict__ rowtab,
+                           int box_rows_cached) {
   const int npairs = g.ny * g.nz / 2;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
   const int x = threadIdx.x;
   if (x >= g.nx) return;
   for (int rp = blockIdx.x; rp < npairs; rp += gridDim.x)
-    row_pair_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, 2 * rp, x);
+    row_pair_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, 2 * rp, x,
+                              box_rows_cached);
 }
 
 // Batched ensemble: every instance's rows in one launch.  Block b serves
@@ -750,7 +785,7 @@ __global__ void __launch_bounds__(512)
 template <int P, bool NT, bool NTL>
 __global__ void __launch_bounds__(512)
     k_collide_batch(const BatchDesc *__restrict__ descs, const int *__restrict__ map, int wpb, int cur, int own,
-                    int slot) {
+                    int slot, int box_rows_cached) {
   using T = typename Traits<P>::T;
   using FT = typename Traits<P>::FT;
   const int m = map[blockIdx.x];
@@ -770,7 +805,8 @@ __global__ void __launch_bounds__(512)
   const int *box = ldc(&dp->ib.box);
   row_cell<P, NT, NTL>(g, make_rsrc(ldc(&dp->f[cur])), make_rsrc(ldc(&dp->f[1 - cur])), own, c, ldc(&dp->nsheets),
                        box + slot * kMaxSheets * 6, box + (1 - slot) * kMaxSheets * 6,
-                       (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab), row, x);
+                       (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab), row, x,
+                       nullptr, box_rows_cached);
 }
 
 // Two rows per lane: each wave serves one 64-lane x chunk of a row pair
@@ -780,7 +816,7 @@ __global__ void __launch_bounds__(512)
 template <int P, bool NT, bool NTL>
 __global__ void __launch_bounds__(512)
     k_collide_batch2(const BatchDesc *__restrict__ descs, const int *__restrict__ map, int wpb, int cur, int own,
-                     int slot) {
+                     int slot, int box_rows_cached) {
   using T = typename Traits<P>::T;
   using FT = typename Traits<P>::FT;
   const int m = map[blockIdx.x];
@@ -799,7 +835,7 @@ __global__ void __launch_bounds__(512)
   row_pair_cell<P, NT, NTL>(g, make_rsrc(ldc(&dp->f[cur])), make_rsrc(ldc(&dp->f[1 - cur])), own, c,
                             ldc(&dp->nsheets), box + slot * kMaxSheets * 6, box + (1 - slot) * kMaxSheets * 6,
                             (const FT *)ldc(&dp->F[slot]), (const FT *)ldc(&dp->F[1 - slot]), ldc(&dp->rowtab),
-                            2 * rp, x);
+                            2 * rp, x, box_rows_cached);
 }
 
 // The band of a split step: every cell of the band boxes (written by the
@@ -1159,14 +1195,16 @@ bool row_path_buffered(const Geometry &g, size_t es) {
 template <int P, bool NT, bool NTL>
 static void launch_rows2(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
-                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, hipStream_t s) {
+                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, int box_rows_cached,
+                         hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const int npairs = g.ny * g.nz / 2;
   const dim3 b((unsigned)((g.nx + 63) / 64 * 64));
   const dim3 gr(nblocks < npairs ? nblocks : npairs);
   k_collide_stream_rows2<P, NT, NTL><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
-                                                      box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab);
+                                                      box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab,
+                                                      box_rows_cached);
   if (g.bc[0] != LBMIB_BC_INOUT) {  // whole-grid steps only
     const long items = 2L * g.ny * g.nz;
     k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
@@ -1187,7 +1225,7 @@ template <int P, bool NT, bool NTL>
 static void launch_pairs(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                          const void *Fcur, const void *Fprev, const uint32_t *rowtab, const int *band,
-                         hipStream_t s) {
+                         int box_rows_cached, hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const int nrows = g.ny * g.nz;
@@ -1195,7 +1233,7 @@ static void launch_pairs(int nblocks, const Geometry &g, const void *fin, void *
   const dim3 gr(nblocks < nrows ? nblocks : nrows);
   k_collide_stream_pairs<P, NT, NTL><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
                                                       box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev, rowtab,
-                                                      band);
+                                                      band, box_rows_cached);
   if (g.bc[0] != LBMIB_BC_INOUT) {
     const long items = 2L * g.ny * g.nz;
     k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
@@ -1208,7 +1246,7 @@ template <int P, bool NT, bool NTL>
 static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *fout, int own,
                         const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                         const void *Fcur, const void *Fprev, const uint32_t *rowtab, const int *band,
-                        hipStream_t s) {
+                        int box_rows_cached, hipStream_t s) {
   using FT = typename Traits<P>::FT;
   using S = typename Traits<P>::S;
   const dim3 b = row_block(g);
@@ -1217,7 +1255,7 @@ static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *f
   if (rowtab && row_path_buffered(g, sizeof(S))) {
     k_collide_stream_rows<P, NT, NTL, true><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c), nsheets,
                                                              box_cur, box_prev, (const FT *)Fcur, (const FT *)Fprev,
-                                                             rowtab, band);
+                                                             rowtab, band, box_rows_cached);
     if (g.bc[0] != LBMIB_BC_INOUT) {
       const long items = 2L * g.ny * g.nz;
       k_collide_xedges<P, NT, NTL><<<(unsigned)((items + 255) / 256), 256, 0, s>>>(
@@ -1227,7 +1265,7 @@ static void launch_rows(int nblocks, const Geometry &g, const void *fin, void *f
   } else
     k_collide_stream_rows<P, NT, NTL, false><<<gr, b, 0, s>>>(g, (const S *)fin, (S *)fout, own, kconsts<P>(c),
                                                               nsheets, box_cur, box_prev, (const FT *)Fcur,
-                                                              (const FT *)Fprev, nullptr, band);
+                                                              (const FT *)Fprev, nullptr, band, box_rows_cached);
 }
 
 template <int P, bool NT>
@@ -1244,7 +1282,7 @@ static void launch_any(int v, const Geometry &g, const void *fin, void *fout, in
     if (v == 8) return launch_vec<P, 8, NT>(g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   }
   launch_rows<P, NT, false>(g.ny * g.nz, g, fin, fout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, nullptr,
-                            nullptr, s);
+                            nullptr, 0, s);
 }
 
 int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
@@ -1279,20 +1317,20 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
   if (cpt == 3 && rowtab && rows2_path_ok(g)) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz / 2;
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_rows2<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else if (nt) launch_rows2<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else if (ntl) launch_rows2<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
-      else launch_rows2<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, s);
+      if (nt && ntl) launch_rows2<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, kv.box_rows_cached, s);
+      else if (nt) launch_rows2<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, kv.box_rows_cached, s);
+      else if (ntl) launch_rows2<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, kv.box_rows_cached, s);
+      else launch_rows2<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, kv.box_rows_cached, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
   if (cpt == 2 && rowtab && pair_path_ok(g)) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
-      else if (nt) launch_pairs<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
-      else if (ntl) launch_pairs<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
-      else launch_pairs<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      if (nt && ntl) launch_pairs<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
+      else if (nt) launch_pairs<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
+      else if (ntl) launch_pairs<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
+      else launch_pairs<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
@@ -1300,10 +1338,10 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
   if (v == 1) {
     const int nb = kv.blocks > 0 ? kv.blocks : g.ny * g.nz;
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
-      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
-      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
-      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, s);
+      if (nt && ntl) launch_rows<P, true, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
+      else if (nt) launch_rows<P, true, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
+      else if (ntl) launch_rows<P, false, true>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
+      else launch_rows<P, false, false>(nb, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, band, kv.box_rows_cached, s);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
@@ -1366,7 +1404,7 @@ int upload_fluid_tables(int prec, const FluidConsts<double> &c, void *d_tab, dou
 }
 
 int launch_collide_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int wpb, int cur, int own,
-                         int slot, int nt_mask, int rows, hipStream_t s) {
+                         int slot, int nt_mask, int rows, int box_rows_cached, hipStream_t s) {
   if (nblocks <= 0) return LBMIB_OK;
   if (wpb < 1 || wpb > 8 || (rows != 1 && rows != 2)) return LBMIB_ERR_ARG;
   const int nt = nt_mask < 0 ? 1 : (nt_mask & 1);
@@ -1374,18 +1412,18 @@ int launch_collide_batch(int prec, const BatchDesc *d, const int *map, int nbloc
   const dim3 b(64 * wpb);
   if (rows == 2) {
     LBMIB_DISPATCH(prec, ({
-      if (nt && ntl) k_collide_batch2<P, true, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
-      else if (nt) k_collide_batch2<P, true, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
-      else if (ntl) k_collide_batch2<P, false, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
-      else k_collide_batch2<P, false, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+      if (nt && ntl) k_collide_batch2<P, true, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
+      else if (nt) k_collide_batch2<P, true, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
+      else if (ntl) k_collide_batch2<P, false, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
+      else k_collide_batch2<P, false, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
     }));
     return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
   }
   LBMIB_DISPATCH(prec, ({
-    if (nt && ntl) k_collide_batch<P, true, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
-    else if (nt) k_collide_batch<P, true, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
-    else if (ntl) k_collide_batch<P, false, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
-    else k_collide_batch<P, false, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot);
+    if (nt && ntl) k_collide_batch<P, true, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
+    else if (nt) k_collide_batch<P, true, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
+    else if (ntl) k_collide_batch<P, false, true><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
+    else k_collide_batch<P, false, false><<<nblocks, b, 0, s>>>(d, map, wpb, cur, own, slot, box_rows_cached);
   }));
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }

@@ -177,8 +177,11 @@ struct BatchDesc {
 // Block map of a batched launch: one int per block, instance << kBatchShift |
 // block index within the instance.
 constexpr int kBatchShift = 20;
+double box_row_bytes(const ::lbmib_ctx *ctx);  // lbmib_api.hip
+int box_rows_auto(const ::lbmib_ctx *ctx);
+
 int launch_collide_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int wpb, int cur, int own,
-                         int slot, int nt_mask, int rows, hipStream_t s);
+                         int slot, int nt_mask, int rows, int box_rows_cached, hipStream_t s);
 int launch_fiber_batch(const BatchDesc *d, const int *map, int nblocks, int slot, hipStream_t s);
 int launch_gather_batch(int prec, const BatchDesc *d, const int *map, int nblocks, int slot, int mcap_max,
                         hipStream_t s);
@@ -194,10 +197,28 @@ int upload_fluid_tables(int prec, const FluidConsts<double> &c, void *d_tab, dou
 
 int launch_fill_init(int prec, const Geometry &g, void *f, const FluidConsts<double> &c, hipStream_t s);
 // Launch shape of the fused kernel (all forms compute identical bits).
+// Bytes of one step's population stores that the fused kernel may keep in
+// the default cache policy (the rows meeting the support boxes): about
+// MI355X's 256 MB memory-side Infinity Cache.  Measured at 256^3: caching
+// 216 MB (F32) or 108 MB (BF16) of box rows made the step 4.6% / 1.4%
+// faster, 433 MB (F64) or more (two 128^2 sheets, the 64-instance
+// ensemble) 6-8% slower.
+constexpr double kBoxRowCacheBytes = 256.0 * (1 << 20);
+
+// Rows of a grid that one sheet's support boxes meet, estimated from the
+// sheet's extent (rows along y, columns along z, as the reference lays
+// sheets out): the extent plus the 4-point support and the pulls' halo.
+inline double box_rows_estimate(const SheetDesc &sd) {
+  const double h = sd.ds2 * (sd.rows - 1), w = sd.ds1 * (sd.cols - 1);
+  return (h + 7.0) * (w + 7.0);
+}
+
 struct KernelVariant {
   int cells_per_thread;  // 0 automatic, 1, 2 (x pairs), 3 (two rows per block), 4, 8 (x vectors)
   int nt_mask;           // -1 automatic; bit 0 nontemporal stores, bit 1 nontemporal loads
   int blocks;            // > 0: persistent grid of this many blocks walking rows (one cell per thread)
+  int box_rows_cached;   // with nontemporal stores: rows meeting a support box stored with the default policy
+                         // (1), or not (0); -1 automatic (box_rows_auto)
 };
 
 // rowtab: the row source table (row_source_table) on the device, or null for

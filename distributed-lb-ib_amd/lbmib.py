@@ -36,9 +36,10 @@ ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lb
                "lbmib_get_time", "lbmib_get_fibers", "lbmib_get_macroscopic", "lbmib_get_df",
                "lbmib_get_force", "lbmib_get_totals", "lbmib_get_band_box", "lbmib_time_steps",
                "lbmib_bytes_per_cell", "lbmib_probe_copy_bandwidth",
-               "lbmib_set_kernel_variant", "lbmib_set_overlap", "lbmib_timed_kernel_cells",
+               "lbmib_set_kernel_variant", "lbmib_set_box_row_stores", "lbmib_set_overlap", "lbmib_timed_kernel_cells",
                "lbmib_batch_create", "lbmib_batch_destroy", "lbmib_batch_instance", "lbmib_batch_init",
-               "lbmib_batch_step", "lbmib_batch_set_tile", "lbmib_batch_set_rows_per_lane", "lbmib_batch_set_overlap",
+               "lbmib_batch_step", "lbmib_batch_set_tile", "lbmib_batch_set_rows_per_lane", "lbmib_batch_set_box_row_stores",
+               "lbmib_batch_set_overlap",
                "lbmib_batch_cells", "lbmib_batch_status",
                "lbmib_batch_time_steps", "lbmib_coarse_moments", "lbmib_coarse_sheets", "lbmib_jump",
                "lbmib_jump_commit", "lbmib_jump_revert", "lbmib_get_stream", "lbmib_coarse_state") + tuple("lbmib_" + n for n in CTX_CALLS)
@@ -101,6 +102,7 @@ def load_library(path=LIB_PATH):
     L.lbmib_bytes_per_cell.argtypes = [vp, dp]
     L.lbmib_probe_copy_bandwidth.argtypes = [d, i, dp]
     L.lbmib_set_kernel_variant.argtypes = [vp, i, i, i]
+    L.lbmib_set_box_row_stores.argtypes = [vp, i]
     L.lbmib_set_overlap.argtypes = [vp, i]
     L.lbmib_timed_kernel_cells.argtypes = [vp, dp]
     L.lbmib_batch_create.argtypes = [i, ctypes.POINTER(FluidParams), ip, ctypes.POINTER(SheetParams),
@@ -111,6 +113,7 @@ def load_library(path=LIB_PATH):
     L.lbmib_batch_step.argtypes = [vp, i]
     L.lbmib_batch_set_tile.argtypes = [vp, i, i]
     L.lbmib_batch_set_rows_per_lane.argtypes = [vp, i]
+    L.lbmib_batch_set_box_row_stores.argtypes = [vp, i]
     L.lbmib_batch_set_overlap.argtypes = [vp, i]
     L.lbmib_batch_cells.argtypes = [vp, dp, ip]
     L.lbmib_batch_status.argtypes = [vp]
@@ -347,6 +350,12 @@ class Engine:
                                             ctypes.byref(main) if main_kernel else None), "time_steps")
         return tot.value, (main.value if main_kernel else None)
 
+    def set_box_row_stores(self, cached):
+        """1 (default): the rows meeting a support box are stored with the
+        default cache policy, 0: nontemporally like the others (see
+        lbmib_set_box_row_stores); same results."""
+        self._check(self.L.lbmib_set_box_row_stores(self.h, int(cached)), "set_box_row_stores")
+
     def set_kernel_variant(self, cells_per_thread=0, nontemporal=-1, blocks_per_cu=0):
         """Fused-kernel launch shape (see lbmib_set_kernel_variant); all
         variants give identical results."""
@@ -481,6 +490,10 @@ class Batch:
         """1 (default) or 2: each lane of the fused launch updates one cell of
         two neighbouring rows; same results."""
         self._check(self.L.lbmib_batch_set_rows_per_lane(self.h, int(rows)), "batch_set_rows_per_lane")
+
+    def set_box_row_stores(self, cached):
+        """See Engine.set_box_row_stores; same results."""
+        self._check(self.L.lbmib_batch_set_box_row_stores(self.h, int(cached)), "batch_set_box_row_stores")
 
     def set_overlap(self, on):
         """Two instance groups whose IB kernels overlap each other's fluid

@@ -180,6 +180,15 @@ int lbmib_time_steps(lbmib_ctx *ctx, int nwarm, int nsteps, double *ms_total, do
  * reference: the knobs bench.py's autotuning sweep sets.  Every variant
  * computes bit-identical results. */
 int lbmib_set_kernel_variant(lbmib_ctx *ctx, int cells_per_thread, int nontemporal, int blocks_per_cu);
+/* With nontemporal stores (the default), the fused kernel can store the rows
+ * that meet a sheet's support box (widened by one cell in y and z) with the
+ * default cache policy (cached = 1) instead (0).  -1, the default at
+ * create: 1 when those rows' stores, estimated from the sheets' extents,
+ * fit MI355X's 256 MB memory-side cache, else 0 -- measured at 256^3, F32
+ * 4.6% and BF16 1.4% faster with them cached (216 / 108 MB), F64 and the
+ * larger workloads 6-8% slower.  Not in the reference; results are the same
+ * bits. */
+int lbmib_set_box_row_stores(lbmib_ctx *ctx, int cached);
 /* Bytes per cell update of the population storage (2 * 19 * sizeof). */
 int lbmib_bytes_per_cell(lbmib_ctx *ctx, double *bytes);
 /* Device copy bandwidth on the current device: a 16-byte-per-lane streaming
@@ -217,6 +226,9 @@ int lbmib_batch_set_tile(lbmib_batch *b, int waves_per_block, int nontemporal);
  * ny even (else LBMIB_ERR_ARG).  Results are the same bits either way.
  * bench.py autotunes it with the tile. */
 int lbmib_batch_set_rows_per_lane(lbmib_batch *b, int rows);
+/* lbmib_set_box_row_stores for the batched fused launch; -1 (the default
+ * at create) caches the box rows when all instances' together fit. */
+int lbmib_batch_set_box_row_stores(lbmib_batch *b, int cached);
 /* on = 1 (default): two groups of instances of about equal cell count, each
  * on its own stream, whose fused launches alternate so that one group's IB
  * kernels run beside the other group's fluid update; on = 0: one group.
