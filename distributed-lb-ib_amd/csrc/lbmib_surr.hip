@@ -105,17 +105,33 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x
     c_begin = blockIdx.z * chunks_per_split;
     c_end = min(c_end, c_begin + chunks_per_split);
   }
-  bf16x8 a[WN], b[WM];
-  auto load = [&](int c) {
+  // The weight tile of a chunk (WN x 16 output channels x 32 k) is the same
+  // for the block's four waves: it is loaded once per block, 16 bytes per
+  // thread, into LDS (double-buffered, rows padded to 40 elements so that the
+  // fragment reads of a wave spread over the banks); each wave reads its A
+  // fragments from there.  The input fragments differ per wave and come
+  // straight from L1/L2.
+  constexpr int kPitch = 40;
+  constexpr int kPieces = WN * 16 * 4;  // 16-byte pieces of a weight tile
+  __shared__ __attribute__((aligned(16))) uint16_t sw[2][WN * 16 * kPitch];
+  const int tid = threadIdx.x;
+  const int prow = tid >> 2, pk = (tid & 3) << 3;  // this thread's piece: row, k offset
+  u32x4 wpiece = {0u, 0u, 0u, 0u};
+  auto load_w = [&](int c) {
+    if (tid < kPieces) {
+      const int k = (c << 5) + pk, n = n0 + prow;
+      wpiece = (k < K && n < Cout) ? *(const u32x4 *)(w + (long)n * K + k) : u32x4{0u, 0u, 0u, 0u};
+    }
+  };
+  auto store_w = [&](int buf) {
+    if (tid < kPieces) *(u32x4 *)(&sw[buf][prow * kPitch + pk]) = wpiece;
+  };
+  bf16x8 b[WM];
+  auto load_x = [&](int c) {
     const int k = (c << 5) + (kq << 3);
     const bool kin = k < K;
     const int tap = k >> lgCin, ci = k & (Cin - 1);
     const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-#pragma unroll
-    for (int j = 0; j < WN; ++j) {
-      const int n = n0 + j * 16 + col;
-      a[j] = (kin && n < Cout) ? ld8(w + (long)n * K + k) : zero8();
-    }
 #pragma unroll
     for (int i = 0; i < WM; ++i) {
       const int zz = vz[i] + dz, yy = vy[i] + dy, xx = vx[i] + dx;
@@ -125,18 +141,29 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x
       b[i] = ok ? ld8(x + (m + dz * HW + (long)dy * W + dx) * Cin + ci) : zero8();
     }
   };
-  if (c_begin < c_end) load(c_begin);
+  if (c_begin < c_end) {
+    load_w(c_begin);
+    load_x(c_begin);
+    store_w(0);
+  }
+  __syncthreads();
   for (int c = c_begin; c < c_end; ++c) {
-    bf16x8 ca[WN], cb[WM];
-#pragma unroll
-    for (int j = 0; j < WN; ++j) ca[j] = a[j];
+    const int buf = (c - c_begin) & 1;
+    bf16x8 cb[WM];
 #pragma unroll
     for (int i = 0; i < WM; ++i) cb[i] = b[i];
-    if (c + 1 < c_end) load(c + 1);
+    if (c + 1 < c_end) {
+      load_w(c + 1);
+      load_x(c + 1);
+    }
 #pragma unroll
-    for (int j = 0; j < WN; ++j)
+    for (int j = 0; j < WN; ++j) {
+      const bf16x8 ca = __builtin_bit_cast(bf16x8, *(const u32x4 *)(&sw[buf][(j * 16 + col) * kPitch + (kq << 3)]));
 #pragma unroll
-      for (int i = 0; i < WM; ++i) acc[j][i] = mfma16(ca[j], cb[i], acc[j][i]);
+      for (int i = 0; i < WM; ++i) acc[j][i] = mfma16(ca, cb[i], acc[j][i]);
+    }
+    if (c + 1 < c_end) store_w(buf ^ 1);
+    __syncthreads();
   }
   // epilogue: lane holds channels n0 + 16 j + 4 kq + r of voxel mw + 16 i + col
   if (SPLIT) {
