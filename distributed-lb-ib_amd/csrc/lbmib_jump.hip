@@ -235,25 +235,40 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // the F cells of the thread's column in z; the next cell's populations are
+  // loaded before this cell's arithmetic and stores, so that two cells'
+  // loads are in flight
+  typename Traits<P>::T h[19], hn[19];
+  {
+    const long p = pop_index(g, x, y, (int)blockIdx.z * F);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) h[q] = ld_h<P>(fin + q * g.pq + p);
+  }
   for (int k = 0; k < F; ++k) {
     const int z = blockIdx.z * F + k;
     const long p0 = pop_index(g, x, y, z);
+    if (k + 1 < F) {
+      const long p1 = pop_index(g, x, y, z + 1);
+#pragma unroll
+      for (int q = 0; q < 19; ++q) hn[q] = ld_h<P>(fin + q * g.pq + p1);
+    }
     const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
     const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
     if (!act || frozen) {
 #pragma unroll
-      for (int q = 0; q < 19; ++q) fout[q * g.pq + p0] = fin[q * g.pq + p0];
+      for (int q = 0; q < 19; ++q) st_h<P>(fout + q * g.pq + p0, h[q]);  // unchanged
+      if (k + 1 < F)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) h[q] = hn[q];
       acc[0] += rho0;
       acc[1] += u0x;
       acc[2] += u0y;
       acc[3] += u0z;
       continue;
     }
-    typename Traits<P>::T h[19];
     T rho = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
-      h[q] = ld_h<P>(fin + q * g.pq + p0);
       const T fq = (T)h[q] * (T)Traits<P>::inv_scale;  // f, or the deviation f - r
       rho += fq;
       jx += (T)kCX[q] * fq;
@@ -290,6 +305,9 @@ __global__ void __launch_bounds__(256)
     acc[1] += (double)(ux + d1);
     acc[2] += (double)(uy + d2);
     acc[3] += (double)(uz + d3);
+    if (k + 1 < F)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) h[q] = hn[q];
   }
   if (after) block_mean_store<F>(acc, red, g, after);
 }
