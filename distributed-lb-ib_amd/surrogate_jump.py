@@ -282,7 +282,10 @@ def evaluate(net, norm, kw, warm, horizon, factor=4, ksteps=50, verify_steps=1, 
             e.step(warm)
             r = JumpRunner(e, net, norm, kw["tau"], u_in, factor, ksteps, verify_steps, leg_tol)
             if pred is None:
-                r.predict()   # first use allocates the network's buffers outside the timed leg
+                # outside the timed leg: the first prediction sizes the network's
+                # buffers, the second records the graph that every cycle replays
+                r.predict()
+                r.predict()
             e.sync()
             t0 = time.time()
             st = r.run(horizon, pred)
