@@ -213,7 +213,9 @@ class JumpRunner:
         ratio = float(step_rate / max(jump_rate, 1e-30))
         self.stats["jumps"] += 1
         self.stats["ratios"].append(ratio)
-        if (ratio <= self.tol or self.tol == float("inf")) and e.status() == 0:
+        # (a non-finite prediction gives a NaN ratio and is rejected; device
+        # errors are latched for good, so run() checks them once)
+        if ratio <= self.tol or self.tol == float("inf"):
             e.jump_commit()
             self.stats["accepted"] += 1
             return True
@@ -235,6 +237,7 @@ class JumpRunner:
             self.e.step(nsteps - done)
             self.stats["plain_steps"] += nsteps - done
             self.have_feat = False
+        self.stats["status"] = self.e.status()
         return self.stats
 
 

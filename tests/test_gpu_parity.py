@@ -270,6 +270,29 @@ def test_kernel_variants_are_bitwise_identical(lbmib, precision):
                 assert np.array_equal(a, b), (precision, v, nt, bpc)
 
 
+@pytest.mark.parametrize("precision", ["f32", "bf16", "f64"])
+def test_box_row_store_policy_is_bitwise_neutral(lbmib, precision):
+    """The cache policy of the box rows' stores (lbmib_set_box_row_stores)
+    changes no value: every row kernel (one cell per lane, x pairs, two rows
+    per block) with the box rows cached or nontemporal gives the same bits."""
+    kw = small_ib_config(lbmib, precision, n=(64, 32, 36))
+    ref = None
+    for v in ((1, 3, 0), (2, 3, 0), (3, 3, 0)):
+        for cached in (0, 1):
+            with lbmib.Engine(**kw) as e:
+                e.set_kernel_variant(*v)
+                e.set_box_row_stores(cached)
+                e.init()
+                e.step(12)
+                assert e.status() == 0
+                out = (e.df(), e.fibers(0))
+            if ref is None:
+                ref = out
+            else:
+                for a, b in zip(out, ref):
+                    assert np.array_equal(a, b), (precision, v, cached)
+
+
 @pytest.mark.parametrize("bc", [(BC_WALL, BC_PERIODIC, BC_WALL), (BC_PERIODIC, BC_WALL, BC_PERIODIC),
                                 (BC_INOUT, BC_WALL, BC_WALL)])
 @pytest.mark.parametrize("precision", ["f32", "bf16"])
@@ -708,10 +731,12 @@ def test_batch_matches_single_instances(lbmib, precision, overlap, rows):
 def test_batch_tile_shapes_are_bitwise_identical(lbmib):
     cfgs = _ensemble(lbmib, "f32")
     outs = []
-    for wpb, nt, rows in ((2, 3, 1), (4, 1, 1), (8, 0, 1), (2, 3, 2), (4, 1, 2), (8, 3, 2)):
+    for wpb, nt, rows, cached in ((2, 3, 1, 0), (4, 1, 1, 1), (8, 0, 1, 1), (2, 3, 2, 1), (4, 1, 2, 0),
+                                  (8, 3, 2, 1)):
         with lbmib.Batch(cfgs) as b:
             b.set_tile(wpb, nt)
             b.set_rows_per_lane(rows)
+            b.set_box_row_stores(cached)
             b.init()
             b.step(10)
             assert b.status() == 0
