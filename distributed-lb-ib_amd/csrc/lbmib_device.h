@@ -214,6 +214,32 @@ __device__ __forceinline__ bool in_any_box(const int *boxes, int nsheets, int x,
   return in;
 }
 
+// Population q arriving at (x,y,z) (pop_index c) through the boundary rules
+// of pull() below.
+template <int P, bool NTL = false, bool RAW = false>
+__device__ __forceinline__ typename Traits<P>::T pull_one(const Geometry &g, const typename Traits<P>::S *__restrict__ f,
+                                                          int x, int y, int z, long c, int q,
+                                                          typename Traits<P>::T b1, typename Traits<P>::T b2) {
+  const long sy = g.prow, sz = g.prow * g.ny;
+  int px = x - kCX[q], py = y - kCY[q], pz = z - kCZ[q];
+  bool bb = false;
+  if (g.bc[0] == LBMIB_BC_PERIODIC) px = px < 0 ? px + g.nx : (px >= g.nx ? px - g.nx : px);
+  else bb |= (px < g.lo[0] || px > g.hi[0]);
+  if (g.bc[1] == LBMIB_BC_PERIODIC) py = py < 0 ? py + g.ny : (py >= g.ny ? py - g.ny : py);
+  else bb |= (py < g.lo[1] || py > g.hi[1]);
+  if (g.bc[2] == LBMIB_BC_PERIODIC) pz = pz < 0 ? pz + g.nz : (pz >= g.nz ? pz - g.nz : pz);
+  else bb |= (pz < g.lo[2] || pz > g.hi[2]);
+  const long src = bb ? c + (long)kOPP[q] * g.pq - (long)q * g.pq : (long)px + sy * py + sz * pz;
+  typename Traits<P>::T v = RAW ? ld_h<P, NTL>(f + q * g.pq + src) : ld<P, NTL>(f + q * g.pq + src);
+  if constexpr (Traits<P>::dev) {
+    if (kCX[q] != 0 && bb) {
+      const typename Traits<P>::T corr = -kCX[q] * (kCLS[q] == 1 ? b1 : b2);
+      v += RAW ? corr : corr * Traits<P>::inv_scale;
+    }
+  }
+  return v;
+}
+
 // Pulls the 19 populations arriving at (x,y,z) from the post-collision buffer.
 // Wall axes: a source outside the active box is replaced by the opposite
 // population of the same node (half-way bounce-back, lbm_ib.c:2310-2423).
@@ -237,25 +263,7 @@ __device__ __forceinline__ void pull(const Geometry &g, const typename Traits<P>
     return;
   }
 #pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    int px = x - kCX[q], py = y - kCY[q], pz = z - kCZ[q];
-    bool bb = false;
-    if (g.bc[0] == LBMIB_BC_PERIODIC) px = px < 0 ? px + g.nx : (px >= g.nx ? px - g.nx : px);
-    else bb |= (px < g.lo[0] || px > g.hi[0]);
-    if (g.bc[1] == LBMIB_BC_PERIODIC) py = py < 0 ? py + g.ny : (py >= g.ny ? py - g.ny : py);
-    else bb |= (py < g.lo[1] || py > g.hi[1]);
-    if (g.bc[2] == LBMIB_BC_PERIODIC) pz = pz < 0 ? pz + g.nz : (pz >= g.nz ? pz - g.nz : pz);
-    else bb |= (pz < g.lo[2] || pz > g.hi[2]);
-    const long src = bb ? c + (long)kOPP[q] * g.pq - (long)q * g.pq
-                        : (long)px + sy * py + sz * pz;
-    v[q] = RAW ? ld_h<P, NTL>(f + q * g.pq + src) : ld<P, NTL>(f + q * g.pq + src);
-    if constexpr (Traits<P>::dev) {
-      if (kCX[q] != 0 && bb) {
-        const typename Traits<P>::T corr = -kCX[q] * (kCLS[q] == 1 ? b1 : b2);
-        v[q] += RAW ? corr : corr * Traits<P>::inv_scale;
-      }
-    }
-  }
+  for (int q = 0; q < 19; ++q) v[q] = pull_one<P, NTL, RAW>(g, f, x, y, z, c, q, b1, b2);
 }
 
 }  // namespace lbmib

@@ -44,16 +44,39 @@ namespace {
 template <int P>
 using JumpT = typename std::conditional<P == LBMIB_F64, double, float>::type;
 
+// s + c v for a lattice component c in {-1, 0, 1}.  c v is exact and adding
+// a zero term changes no finite sum, so this keeps the bits of s + (T)c * v
+// while sparing the multiply and add that IEEE semantics keep for 0 * v.
+template <class T>
+__device__ __forceinline__ T add_c(T s, int c, T v) {
+  return c > 0 ? s + v : (c < 0 ? s - v : s);
+}
+
 // ------------------------------------------------------------ coarse moments
 //
 // Block: 64 x 4 fine cells of one (x, y) tile; each thread walks the F cells
 // of its column in z, so a block covers (64/F) x (4/F) coarse cells of one
 // coarse z-plane.  The block sums F^3 fine values per coarse cell in a fixed
 // order (lane shuffles in x, LDS in y) and writes their mean.
+//
+// The launch is 1-D over the (nx/64) x (ny/4) x (nz/F) tiles.  Workgroup b
+// runs on XCD b % 8, so when the tile count is a multiple of 8 workgroup b
+// takes tile (b % 8) * (count / 8) + b / 8: each XCD sweeps one contiguous
+// slab of z-planes, and the rows that a tile pulls from its y and z
+// neighbours are in that XCD's own L2.
+struct Tile {
+  int x, y, z;
+};
+__device__ __forceinline__ Tile tile_of(const Geometry &g, int F) {
+  const int gx = g.nx / 64, gy = g.ny / 4, n = gx * gy * (g.nz / F);
+  const int b = blockIdx.x;
+  const int t = (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+  return {t % gx, (t / gx) % gy, t / (gx * gy)};
+}
 
 // acc: this thread's sums over its F cells in z; out: 4 planes of nc cells
 template <int F>
-__device__ __forceinline__ void block_mean_store(double acc[4], double (*red)[4][64], const Geometry &g,
+__device__ __forceinline__ void block_mean_store(double acc[4], double (*red)[4][64], const Geometry &g, Tile tl,
                                                  float *__restrict__ out) {
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
@@ -64,10 +87,10 @@ __device__ __forceinline__ void block_mean_store(double acc[4], double (*red)[4]
   for (int a = 0; a < 4; ++a) red[a][ty][tx] = acc[a];
   __syncthreads();
   if (ty % F == 0 && tx % F == 0) {
-    const int x = blockIdx.x * 64 + tx, y = blockIdx.y * 4 + ty;
+    const int x = tl.x * 64 + tx, y = tl.y * 4 + ty;
     const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
     const long nc = (long)ncx * ncy * ncz;
-    const long cc = (long)(x / F) + (long)ncx * ((long)(y / F) + (long)ncy * blockIdx.z);
+    const long cc = (long)(x / F) + (long)ncx * ((long)(y / F) + (long)ncy * tl.z);
     const double inv = 1.0 / (F * F * F);
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -79,13 +102,35 @@ __device__ __forceinline__ void block_mean_store(double acc[4], double (*red)[4]
   __syncthreads();
 }
 
+// A stored population as stored (no conversion), buffer-addressed: the
+// per-lane byte offset voff and the wave-uniform soff, both below 2^32.
+template <int P>
+__device__ __forceinline__ typename Traits<P>::S bld_raw(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  using S = typename Traits<P>::S;
+  if constexpr (P == LBMIB_F64) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else if constexpr (P == LBMIB_F32) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+template <int P>
+__device__ __forceinline__ void bst_raw(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                        typename Traits<P>::S s) {
+  if constexpr (P == LBMIB_F64) {
+    typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, s), r, voff, soff, 0);
+  } else if constexpr (P == LBMIB_F32) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, s), r, voff, soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, s), r, voff, soff, 0);
+  }
+}
+
 // Per fine cell, the moments of lbmib_get_macroscopic (post-stream
 // populations, u = (sum c f + F/2) / rho, the initial rho0/u0 outside the
 // moment box) into `out`; with `stored`, also the moments of the stored
 // (post-collision) populations of the cell itself, u = sum c f / rho, on the
 // updated cells (the active box without the frozen inlet/outlet planes) and
 // rho0/u0 elsewhere -- the convention k_jump_apply reports the jumped state in.
-template <int P, int F>
+template <int P, int F, bool BUF>
 __global__ void __launch_bounds__(256)
     k_coarse_moments(Geometry g, const typename Traits<P>::S *__restrict__ f, int own,
                      const typename Traits<P>::T *__restrict__ tab, double rhor, double mrx,
@@ -93,37 +138,77 @@ __global__ void __launch_bounds__(256)
                      double u0z, double gz, int nsheets, const int *__restrict__ box,
                      const typename Traits<P>::FT *__restrict__ Fc, float *__restrict__ out,
                      float *__restrict__ stored) {
-  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
+  using A = JumpT<P>;
   __shared__ double red[4][4][64];
+  const Tile tl = tile_of(g, F);
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int x = blockIdx.x * 64 + tx, y = blockIdx.y * 4 + ty;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0}, sacc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < F; ++k) {
-    const int z = blockIdx.z * F + k;
-    const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
-    const bool mom =
-        x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1] && z >= g.mlo[2] && z <= g.mhi[2];
-    const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
-    using A = JumpT<P>;
-    A r = (A)rho0, ux = (A)u0x, uy = (A)u0y, uz = (A)u0z;
-    if (act && mom && !own) {
-      T v[19];
-      if (frozen) {
-        for (int q = 0; q < 19; ++q) v[q] = tab[q] * Traits<P>::inv_scale;
-      } else {
-        pull<P>(g, f, x, y, z, b1, b2, v);
+  const int x = tl.x * 64 + tx, y = tl.y * 4 + ty;
+  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+  const bool row_act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1];
+  const bool row_mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1];
+  const bool row_int = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1];
+  // which loads cell z needs: the pulled populations (interior cells read
+  // them here; the others pull at their turn through the boundary rules) and
+  // the cell's own stored ones
+  auto act_at = [&](int z) { return row_act && z >= g.lo[2] && z <= g.hi[2]; };
+  auto pulls = [&](int z) { return act_at(z) && row_mom && z >= g.mlo[2] && z <= g.mhi[2] && !own && !frozen; };
+  auto fast = [&](int z) { return row_int && z > g.lo[2] && z < g.hi[2]; };
+  // a cell's pulled and stored values are loaded together, before either
+  // sum, so that both sets are in flight at once
+  S pv[19], sv[19];
+  // (BUF: the whole population array lies within 2^32 bytes, so the loads
+  // take a 32-bit lane offset and a uniform one instead of 64-bit addresses)
+  const __amdgpu_buffer_rsrc_t rf = make_rsrc(f);
+  auto fetch = [&](int z, S(&pp)[19], S(&ss)[19]) {
+    const long c = pop_index(g, x, y, z);
+    const long sy = g.prow, sz = g.prow * g.ny;
+    if (pulls(z) && fast(z)) {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        // the source element: the neighbour's population q, inside the array
+        // for an interior cell (the lane part of the offset carries the
+        // neighbour's position, which may precede this cell's)
+        const long src = c - (kCX[q] + kCY[q] * sy + kCZ[q] * sz);
+        if constexpr (BUF) pp[q] = bld_raw<P>(rf, (uint32_t)(src * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
+        else pp[q] = f[q * g.pq + src];
       }
+    }
+    if (stored && act_at(z) && !frozen && fast(z)) {  // boundary cells load theirs after the pulled sums
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        if constexpr (BUF) ss[q] = bld_raw<P>(rf, (uint32_t)(c * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
+        else ss[q] = f[q * g.pq + c];
+      }
+    }
+  };
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, sacc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int k = 0; k < F; ++k) {
+    const int z = tl.z * F + k;
+    fetch(z, pv, sv);
+    A r = (A)rho0, ux = (A)u0x, uy = (A)u0y, uz = (A)u0z;
+    if (act_at(z) && row_mom && z >= g.mlo[2] && z <= g.mhi[2] && !own) {
       const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
       A fx = 0, fy = 0, fz = 0;
       if (in_any_box(box, nsheets, x, y, z)) { fx = (A)Fc[c0]; fy = (A)Fc[g.n + c0]; fz = (A)Fc[2 * g.n + c0]; }
       A s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        const A fq = (A)v[q];  // f, or the deviation f - r
+      auto add = [&](int q, A fq) {  // fq: f, or the deviation f - r
         s1 += fq;
-        s2 += (A)kCX[q] * fq;
-        s3 += (A)kCY[q] * fq;
-        s4 += (A)kCZ[q] * fq;
+        s2 = add_c(s2, kCX[q], fq);
+        s3 = add_c(s3, kCY[q], fq);
+        s4 = add_c(s4, kCZ[q], fq);
+      };
+      if (frozen) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) add(q, (A)(tab[q] * Traits<P>::inv_scale));
+      } else if (fast(z)) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) add(q, (A)(from_bits<P>(pv[q]) * Traits<P>::inv_scale));
+      } else {  // boundary cells: one population at a time, in the same order
+        const long c = pop_index(g, x, y, z);
+#pragma unroll 1
+        for (int q = 0; q < 19; ++q) add(q, (A)pull_one<P>(g, f, x, y, z, c, q, b1, b2));
       }
       if (Traits<P>::dev) {  // sum r = rho_r, sum c r = rho_r u_r e_x
         s1 += (A)rhor;
@@ -140,16 +225,20 @@ __global__ void __launch_bounds__(256)
     acc[3] += (double)uz;
     if (stored) {
       A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
-      if (act && !frozen) {
-        const long p0 = pop_index(g, x, y, z);
+      if (act_at(z) && !frozen) {
         A s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+        if (!fast(z)) {
+          const long c = pop_index(g, x, y, z);
+#pragma unroll
+          for (int q = 0; q < 19; ++q) sv[q] = f[q * g.pq + c];
+        }
 #pragma unroll
         for (int q = 0; q < 19; ++q) {
-          const A fq = (A)ld<P>(f + q * g.pq + p0);
+          const A fq = (A)(from_bits<P>(sv[q]) * Traits<P>::inv_scale);
           s1 += fq;
-          s2 += (A)kCX[q] * fq;
-          s3 += (A)kCY[q] * fq;
-          s4 += (A)kCZ[q] * fq;
+          s2 = add_c(s2, kCX[q], fq);
+          s3 = add_c(s3, kCY[q], fq);
+          s4 = add_c(s4, kCZ[q], fq);
         }
         if (Traits<P>::dev) {
           s1 += (A)rhor;
@@ -166,8 +255,72 @@ __global__ void __launch_bounds__(256)
       sacc[3] += (double)sz;
     }
   }
-  block_mean_store<F>(acc, red, g, out);
-  if (stored) block_mean_store<F>(sacc, red, g, stored);
+  block_mean_store<F>(acc, red, g, tl, out);
+  if (stored) block_mean_store<F>(sacc, red, g, tl, stored);
+}
+
+// The stored-moment planes alone (lbmib_coarse_state without d_moments): the
+// `stored` half of k_coarse_moments -- the same sums in the same order, so
+// the same bits -- with the next cell's populations loaded before this
+// cell's sums, so that two cells' loads are in flight per thread.
+template <int P, int F, bool BUF>
+__global__ void __launch_bounds__(256)
+    k_coarse_stored(Geometry g, const typename Traits<P>::S *__restrict__ f, double rhor, double mrx, double rho0,
+                    double u0x, double u0y, double u0z, float *__restrict__ stored) {
+  using S = typename Traits<P>::S;
+  using A = JumpT<P>;
+  __shared__ double red[4][4][64];
+  const Tile tl = tile_of(g, F);
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int x = tl.x * 64 + tx, y = tl.y * 4 + ty;
+  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+  const bool row_upd = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && !frozen;
+  auto upd = [&](int z) { return row_upd && z >= g.lo[2] && z <= g.hi[2]; };
+  const __amdgpu_buffer_rsrc_t rf = make_rsrc(f);
+  auto fetch = [&](int z, S(&d)[19]) {
+    if (!upd(z)) return;
+    const long c = pop_index(g, x, y, z);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      if constexpr (BUF) d[q] = bld_raw<P>(rf, (uint32_t)(c * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
+      else d[q] = f[q * g.pq + c];
+    }
+  };
+  S h[19], hn[19];
+  double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+  fetch(tl.z * F, h);
+  for (int k = 0; k < F; ++k) {
+    const int z = tl.z * F + k;
+    if (k + 1 < F) fetch(z + 1, hn);
+    A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
+    if (upd(z)) {
+      A s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const A fq = (A)(from_bits<P>(h[q]) * Traits<P>::inv_scale);
+        s1 += fq;
+        s2 = add_c(s2, kCX[q], fq);
+        s3 = add_c(s3, kCY[q], fq);
+        s4 = add_c(s4, kCZ[q], fq);
+      }
+      if (Traits<P>::dev) {
+        s1 += (A)rhor;
+        s2 += (A)mrx;
+      }
+      sr = s1;
+      sx = s2 / s1;
+      sy = s3 / s1;
+      sz = s4 / s1;
+    }
+    sacc[0] += (double)sr;
+    sacc[1] += (double)sx;
+    sacc[2] += (double)sy;
+    sacc[3] += (double)sz;
+    if (k + 1 < F)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) h[q] = hn[q];
+  }
+  block_mean_store<F>(sacc, red, g, tl, stored);
 }
 
 // trilinear interpolation of coarse plane `a` at fine position (px, py, pz)
@@ -195,9 +348,8 @@ __device__ __forceinline__ double interp(const float *__restrict__ cf, int a, lo
 //      + rho' (3 c.du + 4.5 c.du (cu + cu') - 1.5 du.(u + u'))]
 template <class T>
 __device__ __forceinline__ T dfeq_q(int q, T rho, T drho, T ux, T uy, T uz, T dx, T dy, T dz) {
-  const T cx = (T)kCX[q], cy = (T)kCY[q], cz = (T)kCZ[q];
-  const T cu = cx * ux + cy * uy + cz * uz;
-  const T cd = cx * dx + cy * dy + cz * dz;
+  const T cu = add_c(add_c(add_c((T)0, kCX[q], ux), kCY[q], uy), kCZ[q], uz);  // c.u
+  const T cd = add_c(add_c(add_c((T)0, kCX[q], dx), kCY[q], dy), kCZ[q], dz);  // c.du
   const T usq = ux * ux + uy * uy + uz * uz;
   const T ud = dx * (ux + ux + dx) + dy * (uy + uy + dy) + dz * (uz + uz + dz);  // du.(u + u')
   const T a = (T)1 + (T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * usq;
@@ -211,8 +363,10 @@ __device__ __forceinline__ T dfeq_q(int q, T rho, T drho, T ux, T uy, T uz, T dx
 // copied unchanged.  With `after`, the block also writes the coarse means of
 // the jumped state's stored moments (rho + drho, u + du exactly, by the
 // moments of feq) in k_coarse_moments' stored convention.
-template <int P, int F>
-__global__ void __launch_bounds__(256)
+// (four blocks per CU for the deviation forms: two cells' populations and
+// the trilinear weights fit in 128 VGPRs; FP64 keeps its registers)
+template <int P, int F, bool BUF>
+__global__ void __launch_bounds__(256, P == LBMIB_F64 ? 1 : 4)
     k_jump_apply(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
                  double rhor, double mrx, const float *__restrict__ delta, double rho0, double u0x, double u0y,
                  double u0z, float *__restrict__ after) {
@@ -222,11 +376,12 @@ __global__ void __launch_bounds__(256)
   // coarse cells (clamped at the grid's edges), 4 planes
   constexpr int SX = 64 / F + 2, SY = 4 / F + 2, SZ = 3;
   __shared__ float sd[4][SZ][SY][SX];
+  const Tile tl = tile_of(g, F);
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int x = blockIdx.x * 64 + tx, y = blockIdx.y * 4 + ty;
+  const int x = tl.x * 64 + tx, y = tl.y * 4 + ty;
   const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
   const long nc = (long)ncx * ncy * ncz;
-  const int cx0 = blockIdx.x * (64 / F) - 1, cy0 = blockIdx.y * (4 / F) - 1, cz0 = (int)blockIdx.z - 1;
+  const int cx0 = tl.x * (64 / F) - 1, cy0 = tl.y * (4 / F) - 1, cz0 = tl.z - 1;
   for (int e = threadIdx.x; e < 4 * SZ * SY * SX; e += 256) {
     const int i = e % SX, j = (e / SX) % SY, k = (e / (SX * SY)) % SZ, a = e / (SX * SY * SZ);
     const int jx = min(max(cx0 + i, 0), ncx - 1), jy = min(max(cy0 + j, 0), ncy - 1);
@@ -236,27 +391,39 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   // the F cells of the thread's column in z; the next cell's populations are
-  // loaded before this cell's arithmetic and stores, so that two cells'
-  // loads are in flight
-  typename Traits<P>::T h[19], hn[19];
+  // loaded (as stored, in registers of the storage type) before this cell's
+  // arithmetic and stores, so that two cells' loads are in flight
+  // (BUF: both population arrays lie within 2^32 bytes; loads and stores
+  // take a 32-bit lane offset and a uniform one)
+  using S = typename Traits<P>::S;
+  const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
+  auto load = [&](long p, int q) -> S {
+    if constexpr (BUF) return bld_raw<P>(rin, (uint32_t)(p * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
+    else return fin[q * g.pq + p];
+  };
+  auto store = [&](long p, int q, S v) {
+    if constexpr (BUF) bst_raw<P>(rout, (uint32_t)(p * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)), v);
+    else fout[q * g.pq + p] = v;
+  };
+  S h[19], hn[19];
   {
-    const long p = pop_index(g, x, y, (int)blockIdx.z * F);
+    const long p = pop_index(g, x, y, tl.z * F);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) h[q] = ld_h<P>(fin + q * g.pq + p);
+    for (int q = 0; q < 19; ++q) h[q] = load(p, q);
   }
   for (int k = 0; k < F; ++k) {
-    const int z = blockIdx.z * F + k;
+    const int z = tl.z * F + k;
     const long p0 = pop_index(g, x, y, z);
     if (k + 1 < F) {
       const long p1 = pop_index(g, x, y, z + 1);
 #pragma unroll
-      for (int q = 0; q < 19; ++q) hn[q] = ld_h<P>(fin + q * g.pq + p1);
+      for (int q = 0; q < 19; ++q) hn[q] = load(p1, q);
     }
     const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
     const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
     if (!act || frozen) {
 #pragma unroll
-      for (int q = 0; q < 19; ++q) st_h<P>(fout + q * g.pq + p0, h[q]);  // unchanged
+      for (int q = 0; q < 19; ++q) store(p0, q, h[q]);  // unchanged
       if (k + 1 < F)
 #pragma unroll
         for (int q = 0; q < 19; ++q) h[q] = hn[q];
@@ -269,11 +436,11 @@ __global__ void __launch_bounds__(256)
     T rho = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
-      const T fq = (T)h[q] * (T)Traits<P>::inv_scale;  // f, or the deviation f - r
+      const T fq = (T)from_bits<P>(h[q]) * (T)Traits<P>::inv_scale;  // f, or the deviation f - r
       rho += fq;
-      jx += (T)kCX[q] * fq;
-      jy += (T)kCY[q] * fq;
-      jz += (T)kCZ[q] * fq;
+      jx = add_c(jx, kCX[q], fq);
+      jy = add_c(jy, kCY[q], fq);
+      jz = add_c(jz, kCZ[q], fq);
     }
     if (Traits<P>::dev) {  // sum r = rho_r, sum c r = rho_r u_r e_x
       rho += (T)rhor;
@@ -299,7 +466,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       const T d = dfeq_q<T>(q, rho, d0, ux, uy, uz, d1, d2, d3);
-      st_h<P>(fout + q * g.pq + p0, h[q] + (typename Traits<P>::T)(d * (T)Traits<P>::scale));
+      store(p0, q, to_bits<P>(from_bits<P>(h[q]) + (typename Traits<P>::T)(d * (T)Traits<P>::scale)));
     }
     acc[0] += (double)(rho + d0);
     acc[1] += (double)(ux + d1);
@@ -309,7 +476,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int q = 0; q < 19; ++q) h[q] = hn[q];
   }
-  if (after) block_mean_store<F>(acc, red, g, after);
+  if (after) block_mean_store<F>(acc, red, g, tl, after);
 }
 
 // X += K (u(X) + u'(X)) / 2 with u the coarse velocity before the jump and
@@ -404,14 +571,31 @@ int coarse_launch(lbmib_ctx *ctx, int F, float *out, float *stored) {
   const int *box = ctx->ib.box + (force ? sl : 0) * kMaxSheets * 6;
   const typename Traits<P>::FT *Fc = force ? (const typename Traits<P>::FT *)ctx->F[sl] : nullptr;
   const int nsh = force ? ctx->nsheets : 0;
-  dim3 grid(g.nx / 64, g.ny / 4, g.nz / F);
+  const int grid = (g.nx / 64) * (g.ny / 4) * (g.nz / F);  // tiles, dealt by tile_of
   const auto *fp = (const typename Traits<P>::S *)ctx->f[ctx->cur];
   const T *tab = (const T *)ctx->c.dtab;
   const FluidConsts<double> &c = ctx->c;
+  const bool buf = (unsigned long long)g.pelems * sizeof(typename Traits<P>::S) <= 0xFFFFFFFFull;
+  if (!out) {  // the stored-moment planes alone
+    auto go = [&](auto kern) {
+      kern<<<grid, 256, 0, ctx->stream>>>(g, fp, c.rhor, c.rhor * c.urx, c.rho0, c.u0x, c.u0y, c.u0z, stored);
+    };
+    switch (F) {
+      case 1: buf ? go(k_coarse_stored<P, 1, true>) : go(k_coarse_stored<P, 1, false>); break;
+      case 2: buf ? go(k_coarse_stored<P, 2, true>) : go(k_coarse_stored<P, 2, false>); break;
+      default: buf ? go(k_coarse_stored<P, 4, true>) : go(k_coarse_stored<P, 4, false>); break;
+    }
+    JHIPCHK(hipGetLastError());
+    return LBMIB_OK;
+  }
+  auto go = [&](auto kern) {
+    kern<<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y,
+                                        c.u0z, c.g, nsh, box, Fc, out, stored);
+  };
   switch (F) {
-    case 1: k_coarse_moments<P, 1><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out, stored); break;
-    case 2: k_coarse_moments<P, 2><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out, stored); break;
-    default: k_coarse_moments<P, 4><<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y, c.u0z, c.g, nsh, box, Fc, out, stored); break;
+    case 1: buf ? go(k_coarse_moments<P, 1, true>) : go(k_coarse_moments<P, 1, false>); break;
+    case 2: buf ? go(k_coarse_moments<P, 2, true>) : go(k_coarse_moments<P, 2, false>); break;
+    default: buf ? go(k_coarse_moments<P, 4, true>) : go(k_coarse_moments<P, 4, false>); break;
   }
   JHIPCHK(hipGetLastError());
   return LBMIB_OK;
@@ -421,13 +605,18 @@ template <int P>
 int apply_launch(lbmib_ctx *ctx, const void *fin, void *fout, const float *delta, int F, float *after) {
   const Geometry &g = ctx->g;
   const FluidConsts<double> &c = ctx->c;
-  dim3 grid(g.nx / 64, g.ny / 4, g.nz / F);
+  const int grid = (g.nx / 64) * (g.ny / 4) * (g.nz / F);  // tiles, dealt by tile_of
   const auto *in = (const typename Traits<P>::S *)fin;
   auto *out = (typename Traits<P>::S *)fout;
+  const bool buf = (unsigned long long)g.pelems * sizeof(typename Traits<P>::S) <= 0xFFFFFFFFull;
+  auto go = [&](auto kern) {
+    kern<<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z,
+                                        after);
+  };
   switch (F) {
-    case 1: k_jump_apply<P, 1><<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z, after); break;
-    case 2: k_jump_apply<P, 2><<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z, after); break;
-    default: k_jump_apply<P, 4><<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z, after); break;
+    case 1: buf ? go(k_jump_apply<P, 1, true>) : go(k_jump_apply<P, 1, false>); break;
+    case 2: buf ? go(k_jump_apply<P, 2, true>) : go(k_jump_apply<P, 2, false>); break;
+    default: buf ? go(k_jump_apply<P, 4, true>) : go(k_jump_apply<P, 4, false>); break;
   }
   JHIPCHK(hipGetLastError());
   return LBMIB_OK;
@@ -452,7 +641,7 @@ int ready_for_jump(const lbmib_ctx *ctx) {
 using namespace lbmib;
 
 extern "C" int lbmib_coarse_state(lbmib_ctx *ctx, int factor, float *d_moments, float *d_stored) {
-  if (!ctx || !d_moments) return LBMIB_ERR_ARG;
+  if (!ctx || (!d_moments && !d_stored)) return LBMIB_ERR_ARG;
   JRCHK(check_coarse(ctx, factor));
   if (ctx->init_bits != 7 || ctx->pending) return LBMIB_ERR_STATE;
   switch (ctx->fp.precision) {

@@ -4,20 +4,21 @@ plain solver, with fallback to plain steps when the verification fails.
 
 Not in the reference: cube_pthread/lbm_ib.c runs every step of its do_thread
 loop (lbm_ib.c:3115-3563).  One jump cycle here:
-  1. the coarse state at t: rho/u of compute_rho_and_u (lbm_ib.c:2440-2538)
+  1. the coarse state at t: rho/u of the stored populations (the state
+     compute_rho_and_u, lbm_ib.c:2440-2538, reads one streaming later)
      averaged over f^3 blocks and the sheet raster (node count, elastic
-     force sums) -- lbmib_coarse_moments / lbmib_coarse_sheets;
+     force sums) -- lbmib_coarse_state's stored planes / lbmib_coarse_sheets;
   2. the network predicts the change of the four fluid planes over K steps;
   3. lbmib_jump shifts every cell's equilibrium by the interpolated change,
      advects the sheet, keeps the pre-jump state aside and sets t += K;
   4. re-verification: R plain steps (they count as simulated steps); the
-     change of the coarse stored-state velocity over them (the jump reports
-     the jumped state's, the read after the steps the new one, both from
-     passes the cycle makes anyway), per step, divided by the jump's own
-     mean change per step, is the verification ratio.  A state the plain
-     dynamics accept changes at about the rate the prediction implies; a
-     jump that leaves it inconsistent (pressure waves, a broken
-     non-equilibrium part) changes much faster.  Ratio <= tol: commit;
+     change of the coarse velocity over them (the jump reports the jumped
+     state's, and the read after the steps -- the next cycle's input -- the
+     new one, both from passes the cycle makes anyway), per step, divided by
+     the jump's own mean change per step, is the verification ratio.  A
+     state the plain dynamics accept changes at about the rate the
+     prediction implies; a jump that leaves it inconsistent (pressure waves,
+     a broken non-equilibrium part) changes much faster.  Ratio <= tol: commit;
      otherwise lbmib_jump_revert and K + R plain steps from the checkpoint,
      which reproduce the plain trajectory bit for bit.
 
@@ -97,12 +98,14 @@ class Coarse:
         self.nc = int(np.prod(self.dims))
         self.feat = torch.empty((NF, self.nc), dtype=torch.float32, device="cuda")
 
-    def read(self, out=None, stored=None):
-        """Fills out (default: self.feat) with the engine's current state;
-        with `stored` ([4][nc]), also the stored-moment planes of the same
-        pass (lbmib_coarse_state)."""
+    def read(self, out=None):
+        """Fills out (default: self.feat) with the engine's current state:
+        planes 0-3 the coarse means of the stored moments (rho, and
+        u = sum c f / rho on the updated cells: lbmib_coarse_state's stored
+        planes, the convention lbmib_jump reports a jumped state in, read
+        without the neighbours' populations), planes 4-7 the sheet raster."""
         t = self.feat if out is None else out
-        self.e.coarse_moments(self.f, t[0].data_ptr(), 0 if stored is None else stored.data_ptr())
+        self.e.coarse_moments(self.f, 0, t[0].data_ptr())
         self.e.coarse_sheets(self.f, t[4].data_ptr())
         return t
 
@@ -132,13 +135,20 @@ class JumpRunner:
         nz, ny, nx = self.co.dims
         self.x = torch.empty((1, nz, ny, nx, CIN), dtype=torch.bfloat16, device="cuda")
         self.delta = torch.empty((NOUT, self.co.nc), dtype=torch.float32, device="cuda")
-        # stored-moment planes (lbmib_coarse_state's convention) of the jumped
-        # state and of the state after the verification steps
+        # the jumped state's coarse planes (lbmib_jump's d_after, the
+        # convention of the feature planes 0-3)
         self.after = torch.empty((4, self.co.nc), dtype=torch.float32, device="cuda")
-        self.verified = torch.empty((4, self.co.nc), dtype=torch.float32, device="cuda")
         self.sums = torch.zeros(4, dtype=torch.float64, device="cuda")
+        # the verification sums reach the host through a side stream, so that
+        # the engine's stream goes on with the next cycle's prediction while
+        # the host waits for them (the page-locked buffer is only ever used
+        # on that torch-owned stream)
+        self.side = torch.cuda.Stream()
+        self.host_sums = torch.empty(4, dtype=torch.float64, pin_memory=True)
+        self.sums_ready = torch.cuda.Event()
         self.stats = {"jumps": 0, "accepted": 0, "rejected": 0, "plain_steps": 0, "ratios": []}
         self.have_feat = False
+        self.pred_ready = False  # self.delta holds the network's prediction from self.co.feat
         self.use_graph, self.graph = graph, None
 
     def _predict_kernels(self):
@@ -172,42 +182,84 @@ class JumpRunner:
                 g.replay()
         return self.delta
 
+    def prepare(self, predictor=None):
+        """One-time set-up, outside any timed region: with the network
+        (predictor None), two predictions -- the first sizes its buffers, the
+        second records the graph -- a jump of the current state, reverted at
+        once, which allocates the engine's checkpoint buffers, and one pass
+        of the verification sums to the host.  The engine's state is left
+        bit for bit as it was; the first cycle reads and predicts afresh."""
+        with torch.cuda.stream(self.ext):
+            self.co.read()
+            if predictor is None:
+                self.predict()
+                self.predict()
+            else:
+                predictor(self)
+            self.e.jump(self.co.f, self.co.feat.data_ptr(), self.delta.data_ptr(), self.K, self.after.data_ptr())
+            self.e.jump_revert()
+            # the verification sums and their way to the host, once
+            self._sqdiff(self.co.feat[1:4], self.after[1:4], self.sums[0:2])
+            self._sqdiff(self.delta[1:4], None, self.sums[2:4])
+            self._sums_to_host()
+        self.sums_ready.synchronize()
+        self.have_feat = self.pred_ready = False
+
+    def _sums_to_host(self):
+        """Copies self.sums to the page-locked buffer on the side stream
+        after the engine's stream has produced them (a pinned buffer used on
+        the engine's stream would have torch's allocator record an event on
+        that stream when the buffer is freed, which may be after the engine
+        has destroyed it); self.sums_ready marks the copy's end."""
+        done = torch.cuda.Event()
+        done.record(self.ext)
+        self.side.wait_event(done)
+        with torch.cuda.stream(self.side):
+            self.host_sums.copy_(self.sums, non_blocking=True)
+            self.sums_ready.record(self.side)
+
     def _sqdiff(self, a, b, out):
         L = sg._bind()
         sg.check(L.lbmib_sg_sqdiff(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(0 if b is None else b.data_ptr()),
                                    a.numel(), ctypes.c_void_p(out.data_ptr()), self.stream), "sqdiff")
 
-    def cycle(self, predictor=None):
+    def cycle(self, predictor=None, lookahead=False):
         """One jump of K steps plus R verification steps (K + R simulated
         steps), or K + R plain steps after a rejected jump.  predictor: a
         function filling self.delta instead of the network (None: the
-        network).  Returns True if the jump was accepted."""
+        network).  lookahead (network only): another cycle follows, so its
+        prediction from the verified state is enqueued before the host
+        waits for the verification sums; a rejected jump discards it.
+        Returns True if the jump was accepted."""
         with torch.cuda.stream(self.ext):
-            return self._cycle(predictor)
+            return self._cycle(predictor, lookahead and predictor is None)
 
-    def _cycle(self, predictor):
+    def _cycle(self, predictor, lookahead):
         e = self.e
         if not self.have_feat:
             self.co.read()
             self.have_feat = True
-        if predictor is None:
-            self.predict()
-        else:
+            self.pred_ready = False
+        if predictor is not None:
             predictor(self)
+        elif not self.pred_ready:
+            self.predict()
+        self.pred_ready = False
         # the jump also reports the jumped state's coarse stored moments
         e.jump(self.co.f, self.co.feat.data_ptr(), self.delta.data_ptr(), self.K, self.after.data_ptr())
         e.step(self.R)
-        # after R plain steps: the next cycle's input and, from the same
-        # pass, the stored moments the verification compares
-        self.co.read(stored=self.verified)
+        # after R plain steps: the next cycle's input, which the
+        # verification also compares with the jumped state
+        self.co.read()
         # verification ratio: |du over R steps| / R  against  |du of the jump| / K
-        self._sqdiff(self.verified[1:4], self.after[1:4], self.sums[0:2])
+        self._sqdiff(self.co.feat[1:4], self.after[1:4], self.sums[0:2])
         self._sqdiff(self.delta[1:4], None, self.sums[2:4])
-        # a blocking copy on the engine's stream (no torch pinned buffer:
-        # torch's pinned-memory allocator would record an event on this
-        # stream when the buffer is freed, which may be after the engine has
-        # destroyed the stream)
-        s = self.sums.cpu().numpy()
+        self._sums_to_host()
+        if lookahead:
+            self.predict()  # the next cycle's change, from the verified state
+            self.pred_ready = True
+        self.sums_ready.synchronize()
+        s = self.host_sums.numpy()
         step_rate = np.sqrt(s[0]) / self.R
         jump_rate = np.sqrt(s[2]) / self.K
         ratio = float(step_rate / max(jump_rate, 1e-30))
@@ -224,15 +276,16 @@ class JumpRunner:
         self.stats["rejected"] += 1
         self.stats["plain_steps"] += self.K + self.R
         self.co.read()
+        self.pred_ready = False
         return False
 
     def run(self, nsteps, predictor=None):
         """Advances nsteps simulated steps: whole jump cycles, then plain steps
         for the remainder."""
-        done = 0
-        while done + self.K + self.R <= nsteps:
-            self.cycle(predictor)
-            done += self.K + self.R
+        done, span = 0, self.K + self.R
+        while done + span <= nsteps:
+            self.cycle(predictor, lookahead=done + 2 * span <= nsteps)
+            done += span
         if done < nsteps:
             self.e.step(nsteps - done)
             self.stats["plain_steps"] += nsteps - done
@@ -284,11 +337,7 @@ def evaluate(net, norm, kw, warm, horizon, factor=4, ksteps=50, verify_steps=1, 
             e.init()
             e.step(warm)
             r = JumpRunner(e, net, norm, kw["tau"], u_in, factor, ksteps, verify_steps, leg_tol)
-            if pred is None:
-                # outside the timed leg: the first prediction sizes the network's
-                # buffers, the second records the graph that every cycle replays
-                r.predict()
-                r.predict()
+            r.prepare(pred)  # buffers and the prediction graph, outside the timed leg
             e.sync()
             t0 = time.time()
             st = r.run(horizon, pred)

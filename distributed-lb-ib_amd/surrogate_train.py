@@ -5,8 +5,9 @@ Trajectories come from the batched solver (lbmib.Batch): instances of the
 256^3 workload's geometry -- inlet/outlet x, periodic y, walls z, one
 104 x 104 sheet centred at (64, 128, 128) -- with tau, the sheet stiffness
 Ks = Kb and the inflow speed drawn from SplitMix64(seed).  Every `every`
-steps the coarse state of each instance (surrogate_jump.Coarse: rho, u, node
-count, elastic force, on 64^3 at f = 4) is kept on the GPU; a training pair
+steps the coarse state of each instance (surrogate_jump.Coarse: rho, u of
+the stored populations, node count, elastic force, on 64^3 at f = 4) is kept
+on the GPU; a training pair
 is (state at t, change of rho/u over K steps).  Training runs the library's
 own forward, backward and Adam kernels.
 

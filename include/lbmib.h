@@ -258,13 +258,16 @@ int lbmib_coarse_moments(lbmib_ctx *ctx, int factor, float *d_out);
  * populations (d_stored nullable), the means of the stored post-collision
  * populations' own moments: rho and u = sum c f / rho on the updated cells
  * (the active box without the frozen inlet/outlet planes), rho0/u0 on the
- * other cells -- the convention of lbmib_jump's d_after. */
+ * other cells -- the convention of lbmib_jump's d_after.  With d_moments
+ * null only the stored planes are computed, in a cheaper pass that reads
+ * each cell's own populations and no neighbour's (same bits). */
 int lbmib_coarse_state(lbmib_ctx *ctx, int factor, float *d_moments, float *d_stored);
 /* c = 0..3: number of fiber nodes in each coarse cell and the sums of their
  * elastic forces (compute_elasticforce, lbm_ib.c:1285-1318); deterministic. */
 int lbmib_coarse_sheets(lbmib_ctx *ctx, int factor, float *d_out);
 /* Advances the simulation by ksteps (even, >= 2) steps without stepping:
- * d_coarse is lbmib_coarse_moments of the current state, d_delta the
+ * d_coarse holds coarse moments of the current state (either plane set of
+ * lbmib_coarse_state; its velocity planes move the fiber nodes), d_delta the
  * predicted change of those four planes over ksteps.  Every updated cell
  * keeps its non-equilibrium part and takes the equilibrium of its moments
  * plus the change interpolated trilinearly at the cell; fiber nodes move by

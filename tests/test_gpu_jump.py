@@ -4,6 +4,7 @@ distributed-lb-ib_amd/surrogate_jump.py), through the C-ABI.
 - lbmib_coarse_moments: FP64 engine against the block average of the FP64
   CPU oracle's rho/u (lbm_ib.c:2440-2538), <= 1e-6 relative (FP32 output of
   FP64 sums), and bit-consistent with lbmib_get_macroscopic's fields;
+- lbmib_coarse_state's stored planes alone: the combined read's bit for bit;
 - lbmib_coarse_sheets: node counts exact, force sums within FP32 rounding and
   the 2^-40 fixed-point quantum per node, against a numpy raster of the
   downloaded nodes;
@@ -69,6 +70,25 @@ def test_coarse_moments_match_oracle_block_average(lbmib, factor):
         assert np.abs(c[a] - ref[a]).max() <= 1e-6 * np.abs(ref[a]).max(), a
     # the same fields lbmib_get_macroscopic returns, averaged
     np.testing.assert_allclose(c[0], so.block_average(rho, factor), rtol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
+def test_stored_planes_alone_match_the_combined_read(lbmib, precision):
+    """lbmib_coarse_state with d_moments null (each cell's own populations
+    only) gives the stored planes of the combined read bit for bit."""
+    kw = small_cfg(lbmib, precision)
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(7)
+        for f in (2, 4):
+            shp = (4, e.nz // f, e.ny // f, e.nx // f)
+            cm, both, alone = (torch.empty(shp, dtype=torch.float32, device="cuda") for _ in range(3))
+            e.coarse_moments(f, cm.data_ptr(), both.data_ptr())
+            e.coarse_moments(f, 0, alone.data_ptr())
+            e.sync()
+            np.testing.assert_array_equal(host(alone), host(both))
+        with pytest.raises(lbmib.LbmibError):  # neither plane set
+            e.coarse_moments(4, 0, 0)
 
 
 def test_coarse_sheets_raster(lbmib):
@@ -242,6 +262,35 @@ def test_rejecting_every_jump_reproduces_plain_steps(lbmib):
         assert a.time == b.time
         np.testing.assert_array_equal(a.df(), b.df())
         np.testing.assert_array_equal(a.fibers(0), b.fibers(0))
+
+
+def test_lookahead_prediction_gives_the_same_run(lbmib):
+    """Cycles whose next prediction is enqueued before the host reads the
+    verification sums (run()) reach the same bits as cycles that predict at
+    their start, accepted and rejected jumps alike."""
+    import surrogate as sg
+    import surrogate_jump as sj
+    kw = small_cfg(lbmib, "f32")
+    net = sg.Net(cin=sj.CIN, cout=16, nout=sj.NOUT, seed=4, out_scale=0.05)
+    norm = sj.Norm([1.0, 0.02, 0, 0, 0, 0, 0, 0], [1e-3, 3e-3, 3e-3, 3e-3, 1.0, 1e-4, 1e-4, 1e-4], [0.0] * 4,
+                   [1e-4] * 4)
+    for tol in (float("inf"), 0.0):
+        with lbmib.Engine(**kw) as a, lbmib.Engine(**kw) as b:
+            a.init()
+            b.init()
+            a.step(2)
+            b.step(2)
+            t0 = a.time
+            ra = sj.JumpRunner(a, net, norm, kw["tau"], kw["u0"][0], factor=4, ksteps=10, verify_steps=2, tol=tol)
+            rb = sj.JumpRunner(b, net, norm, kw["tau"], kw["u0"][0], factor=4, ksteps=10, verify_steps=2, tol=tol)
+            sa = ra.run(36)
+            for _ in range(3):
+                rb.cycle(lookahead=False)
+            assert sa["jumps"] == 3 and rb.stats["jumps"] == 3
+            assert sa["accepted"] == rb.stats["accepted"] == (3 if tol > 0 else 0)
+            assert a.time == b.time == t0 + 36
+            np.testing.assert_array_equal(a.df(), b.df())
+            np.testing.assert_array_equal(a.fibers(0), b.fibers(0))
 
 
 def test_coupling_rejects_bad_shapes_and_states(lbmib):
