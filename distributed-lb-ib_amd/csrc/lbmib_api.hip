@@ -72,7 +72,7 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
   void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F[0], ctx->F[1], ctx->ib.pos, ctx->ib.frc, ctx->ib.start,
                   ctx->ib.wgt, ctx->ib.bin_count, ctx->ib.bin_list, ctx->ib.box, ctx->ib.done, ctx->ib.ub, ctx->d_sheets,
                   ctx->d_err, ctx->d_partials, ctx->d_tot, ctx->d_mat, ctx->d_tab, ctx->d_rowtab, ctx->d_rect, ctx->d_ref,
-                  ctx->f_spare, ctx->pos_save, ctx->F_save, ctx->box_save, ctx->d_raster};
+                  ctx->f_spare, ctx->pos_save, ctx->F_save, ctx->box_save, ctx->d_raster, ctx->d_cpart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -447,7 +447,9 @@ extern "C" int lbmib_replace_old_DF(lbmib_ctx *ctx) {
   return LBMIB_OK;
 }
 
-static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
+// cstored (lbmib_step_coarse): the fluid launch also writes the coarse
+// stored-moment planes of the new state at factor cf.
+static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1, int cf = 0, float *cstored = nullptr) {
   const int s = slot_now(ctx);
   const int prec = ctx->fp.precision;
   if (ctx->nnodes > 0) {
@@ -458,9 +460,14 @@ static int one_step(lbmib_ctx *ctx, hipEvent_t e0, hipEvent_t e1) {
   }
   const int *box = ctx->ib.box;
   if (e0) HIPCHK(hipEventRecord(e0, ctx->stream));
-  RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
-                             box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
-                             ctx->d_rowtab, ctx->kv, ctx->stream));
+  if (cstored)
+    RCHK(launch_collide_coarse(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c, ctx->nsheets,
+                               box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s], ctx->F[1 - s],
+                               ctx->d_rowtab, ctx->kv, cf, ctx->d_cpart, cstored, ctx->stream));
+  else
+    RCHK(launch_collide_stream(prec, ctx->g, ctx->f[ctx->cur], ctx->f[1 - ctx->cur], ctx->own, ctx->c,
+                               ctx->nsheets, box + s * kMaxSheets * 6, box + (1 - s) * kMaxSheets * 6, ctx->F[s],
+                               ctx->F[1 - s], ctx->d_rowtab, ctx->kv, ctx->stream));
   if (e1) HIPCHK(hipEventRecord(e1, ctx->stream));
   if (ctx->nnodes > 0) {
     RCHK(launch_move(prec, ctx->ib, ctx->g, ctx->f[1 - ctx->cur], ctx->F[s], ctx->c, 1 - s, ctx->move_tiles, ctx->stream));
@@ -597,6 +604,29 @@ extern "C" int lbmib_step(lbmib_ctx *ctx, int nsteps) {
   }
   for (int k = 0; k < nsteps; ++k) RCHK(one_step(ctx, nullptr, nullptr));
   return LBMIB_OK;
+}
+
+extern "C" int lbmib_step_coarse(lbmib_ctx *ctx, int factor, float *d_stored) {
+  RCHK(ready(ctx));
+  if (!d_stored || (factor != 1 && factor != 2 && factor != 4)) return LBMIB_ERR_ARG;
+  const Geometry &g = ctx->g;
+  if (g.nx % 64 || g.ny % 4 || g.nz % 4) return LBMIB_ERR_ARG;
+  if (ctx->pending) return LBMIB_ERR_STATE;
+  if (split_enabled(ctx) || !collide_coarse_ok(g, ctx->esz, ctx->d_rowtab != nullptr, factor)) {
+    // the same result in two passes: a plain step, then the read
+    RCHK(lbmib_step(ctx, 1));
+    return lbmib_coarse_state(ctx, factor, nullptr, d_stored);
+  }
+  // per-row partial sums: 4 doubles per coarse x cell of every (y, z) row
+  const size_t need = sizeof(double) * 4 * (size_t)(g.nx / factor) * g.ny * g.nz;
+  if (ctx->cpart_bytes < need) {
+    if (ctx->d_cpart) (void)hipFree(ctx->d_cpart);
+    ctx->d_cpart = nullptr;
+    ctx->cpart_bytes = 0;
+    if (hipMalloc((void **)&ctx->d_cpart, need) != hipSuccess) return LBMIB_ERR_ALLOC;
+    ctx->cpart_bytes = need;
+  }
+  return one_step(ctx, nullptr, nullptr, factor, d_stored);
 }
 
 // --------------------------------------------------------------- queries

@@ -33,6 +33,7 @@
 
 #include <vector>
 
+#include "lbmib_coarse.h"
 #include "lbmib_device.h"
 
 namespace lbmib {
@@ -362,18 +363,27 @@ __device__ __forceinline__ void bounce_fix(uint32_t mask, typename Traits<P>::T 
   }
 }
 
+// A sink that takes nothing (row_cell's default).
+struct NoSink {
+  template <class V>
+  __device__ __forceinline__ void operator()(const V *) const {}
+};
+
 // One lane of the buffer-addressed row path: the update of cell (x, row) by
 // a wave whose lanes hold consecutive x of the same row (SGPR row offsets).
 // Dead lanes of an active row store zeros (whole cache lines); x edges of a
-// wall or periodic x axis are left to k_collide_xedges.
-template <int P, bool NT, bool NTL>
+// wall or periodic x axis are left to k_collide_xedges.  sink(v) sees the
+// post-collision values (kernel units, before the stores round them) of an
+// updated cell: not on dead rows or lanes, x edges or frozen planes.
+template <int P, bool NT, bool NTL, class Sink = NoSink>
 __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc_t rin, __amdgpu_buffer_rsrc_t rout,
                                          int own, const KConsts<typename Traits<P>::T> &c, int nsheets,
                                          const int *__restrict__ box_cur, const int *__restrict__ box_prev,
                                          const typename Traits<P>::FT *__restrict__ Fcur,
                                          const typename Traits<P>::FT *__restrict__ Fprev,
                                          const uint32_t *__restrict__ rowtab, int row, int x,
-                                         const int *__restrict__ band = nullptr, int box_rows_cached = 0) {
+                                         const int *__restrict__ band = nullptr, int box_rows_cached = 0,
+                                         Sink sink = Sink()) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   constexpr uint32_t es = sizeof(S);
@@ -438,6 +448,7 @@ __device__ __forceinline__ void row_cell(const Geometry &g, __amdgpu_buffer_rsrc
       fx = (T)Fcur[c0]; fy = (T)Fcur[g.n + c0]; fz = (T)Fcur[2 * g.n + c0];
     }
     collide_cell<P>(v, d, rho, ux, uy, uz, fx, fy, fz, c);
+    if (!frozen) sink(v);
   }
   uint32_t sb = row_b;
   if (NT && box_rows_cached && row_near_box(box_cur, nsheets, y, z)) {
@@ -1349,6 +1360,127 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
     if (nt) launch_any<P, true>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
     else launch_any<P, false>(v, g, fin, fout, own_mode, c, nsheets, box_cur, box_prev, Fcur, Fprev, s);
   }));
+  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+}
+
+// ------------------------------------------------ step with coarse moments
+//
+// One step's fluid update in the row form (one block per (y, z) row, one
+// lane per x, every cell updated as row_cell updates it) that also reports
+// the stored moments of the state it writes, taken from the post-collision
+// values the stores round: each group of F lanes sums its cells' moments
+// (lane shuffles) into part[row][x / F][4] (rows without updated cells
+// report rho0/u0 like k_coarse_stored), and k_coarse_rows then sums F x F
+// rows per coarse cell.  The planes equal lbmib_coarse_state's stored
+// planes of the new state up to the order of the double sums.  Needs the
+// buffer-addressed row path with a block as wide as the row, and an
+// inlet/outlet x axis (the x edges of other axes are updated by a second
+// launch, after the sums).
+template <int P, bool NT, bool NTL, int F>
+__global__ void __launch_bounds__(1024)
+    k_collide_stream_rows_coarse(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                                 typename Traits<P>::S *__restrict__ fout, int own, KConsts<typename Traits<P>::T> c,
+                                 int nsheets, const int *__restrict__ box_cur, const int *__restrict__ box_prev,
+                                 const typename Traits<P>::FT *__restrict__ Fcur,
+                                 const typename Traits<P>::FT *__restrict__ Fprev,
+                                 const uint32_t *__restrict__ rowtab, int box_rows_cached, double rhor, double mrx,
+                                 double rho0, double u0x, double u0y, double u0z, double *__restrict__ part) {
+  using T = typename Traits<P>::T;
+  using S = typename Traits<P>::S;
+  using A = JumpT<P>;
+  const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
+  const int x = threadIdx.x;  // blockDim.x == nx, a multiple of 64
+  const long ncx = g.nx / F;
+  int row;
+  for (long it = blockIdx.x; next_row(g, it, row); it += gridDim.x) {
+    A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
+    row_cell<P, NT, NTL>(g, rin, rout, own, c, nsheets, box_cur, box_prev, Fcur, Fprev, rowtab, row, x, nullptr,
+                         box_rows_cached, [&](const T *v) {
+                           S h[19];
+#pragma unroll
+                           for (int q = 0; q < 19; ++q) h[q] = to_bits<P>(v[q]);
+                           stored_moments<P>(h, rhor, mrx, sr, sx, sy, sz);
+                         });
+    double m[4] = {(double)sr, (double)sx, (double)sy, (double)sz};
+#pragma unroll
+    for (int d = 1; d < F; d <<= 1)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) m[a] += __shfl_xor(m[a], d);
+    if (x % F == 0) {
+      double *o = part + ((long)row * ncx + x / F) * 4;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) o[a] = m[a];
+    }
+  }
+}
+
+// out[a][cz][cy][cx] = mean over the coarse cell of part: for each of its F
+// rows in y, the sum over its F rows in z, summed in y order
+template <int F>
+__global__ void k_coarse_rows(const Geometry g, const double *__restrict__ part, float *__restrict__ out) {
+  const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
+  const long nc = (long)ncx * ncy * ncz;
+  const long cc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= nc) return;
+  const int cx = (int)(cc % ncx), cy = (int)((cc / ncx) % ncy), cz = (int)(cc / ((long)ncx * ncy));
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < F; ++j) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < F; ++k) {
+      const long row = (long)(cy * F + j) + (long)g.ny * (cz * F + k);
+      const double *p = part + (row * ncx + cx) * 4;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] += p[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) t[a] += acc[a];
+  }
+  const double inv = 1.0 / (F * F * F);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) out[a * nc + cc] = (float)(t[a] * inv);
+}
+
+bool collide_coarse_ok(const Geometry &g, size_t es, bool rowtab, int F) {
+  return rowtab && row_path_buffered(g, es) && (int)row_block(g).x == g.nx && g.bc[0] == LBMIB_BC_INOUT &&
+         (F == 1 || F == 2 || F == 4) && g.nx % 64 == 0 && g.ny % 4 == 0 && g.nz % 4 == 0;
+}
+
+int launch_collide_coarse(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
+                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                          const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
+                          int F, double *part, float *stored, hipStream_t s) {
+  if (!part || !stored) return LBMIB_ERR_ARG;
+  const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
+  const int ntl = kv.nt_mask < 0 ? 1 : ((kv.nt_mask >> 1) & 1);
+  const int nrows = g.ny * g.nz;
+  const int nb = kv.blocks > 0 && kv.blocks < nrows ? kv.blocks : nrows;
+  const double mrx = c.rhor * c.urx;
+  LBMIB_DISPATCH(prec, ({
+    using S = typename Traits<P>::S;
+    using FT = typename Traits<P>::FT;
+    if (!collide_coarse_ok(g, sizeof(S), rowtab != nullptr, F)) return LBMIB_ERR_ARG;
+    auto go = [&](auto kern) {
+      kern<<<nb, g.nx, 0, s>>>(g, (const S *)fin, (S *)fout, own_mode, kconsts<P>(c), nsheets, box_cur, box_prev,
+                               (const FT *)Fcur, (const FT *)Fprev, rowtab, kv.box_rows_cached, c.rhor, mrx, c.rho0,
+                               c.u0x, c.u0y, c.u0z, part);
+    };
+    auto pick = [&](auto nt_c, auto ntl_c) {
+      constexpr bool NTc = decltype(nt_c)::value, NTLc = decltype(ntl_c)::value;
+      if (F == 1) go(k_collide_stream_rows_coarse<P, NTc, NTLc, 1>);
+      else if (F == 2) go(k_collide_stream_rows_coarse<P, NTc, NTLc, 2>);
+      else go(k_collide_stream_rows_coarse<P, NTc, NTLc, 4>);
+    };
+    if (nt && ntl) pick(std::true_type{}, std::true_type{});
+    else if (nt) pick(std::true_type{}, std::false_type{});
+    else if (ntl) pick(std::false_type{}, std::true_type{});
+    else pick(std::false_type{}, std::false_type{});
+  }));
+  if (hipGetLastError() != hipSuccess) return LBMIB_ERR_HIP;
+  const long nc = (long)(g.nx / F) * (g.ny / F) * (g.nz / F);
+  const unsigned blocks = (unsigned)((nc + 255) / 256);
+  if (F == 1) k_coarse_rows<1><<<blocks, 256, 0, s>>>(g, part, stored);
+  else if (F == 2) k_coarse_rows<2><<<blocks, 256, 0, s>>>(g, part, stored);
+  else k_coarse_rows<4><<<blocks, 256, 0, s>>>(g, part, stored);
   return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 

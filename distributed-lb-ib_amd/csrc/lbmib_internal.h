@@ -232,6 +232,14 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
                           hipStream_t s, int sel = 0, int *rect_buf = nullptr, int band_blocks = 0);
 constexpr int kRectBufInts = 1 + 6 * kMaxSheets;
 bool row_path_buffered(const Geometry &g, size_t es);
+// A fused step that also writes the coarse stored-moment planes of the state
+// it writes (lbmib_step_coarse): whether it applies, and its launch
+// (LBMIB_ERR_ARG where it does not apply).
+bool collide_coarse_ok(const Geometry &g, size_t es, bool rowtab, int F);
+int launch_collide_coarse(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
+                          const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
+                          const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
+                          int F, double *part, float *stored, hipStream_t s);
 bool pair_path_ok(const Geometry &g);
 bool rows2_path_ok(const Geometry &g);
 int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab);

@@ -21,6 +21,7 @@
 
 #include <type_traits>
 
+#include "lbmib_coarse.h"
 #include "lbmib_ctx.h"
 #include "lbmib_device.h"
 
@@ -36,71 +37,6 @@ namespace {
     int _r = (x);       \
     if (_r) return _r;  \
   } while (0)
-
-// Arithmetic of the coupling kernels: FP64 for FP64 storage; FP32 on the
-// deviations h = f - r for the deviation forms (their moments are rho_r +
-// sum h and rho_r u_r + sum c h, so nothing cancels against the O(0.3)
-// reference populations).
-template <int P>
-using JumpT = typename std::conditional<P == LBMIB_F64, double, float>::type;
-
-// s + c v for a lattice component c in {-1, 0, 1}.  c v is exact and adding
-// a zero term changes no finite sum, so this keeps the bits of s + (T)c * v
-// while sparing the multiply and add that IEEE semantics keep for 0 * v.
-template <class T>
-__device__ __forceinline__ T add_c(T s, int c, T v) {
-  return c > 0 ? s + v : (c < 0 ? s - v : s);
-}
-
-// ------------------------------------------------------------ coarse moments
-//
-// Block: 64 x 4 fine cells of one (x, y) tile; each thread walks the F cells
-// of its column in z, so a block covers (64/F) x (4/F) coarse cells of one
-// coarse z-plane.  The block sums F^3 fine values per coarse cell in a fixed
-// order (lane shuffles in x, LDS in y) and writes their mean.
-//
-// The launch is 1-D over the (nx/64) x (ny/4) x (nz/F) tiles.  Workgroup b
-// runs on XCD b % 8, so when the tile count is a multiple of 8 workgroup b
-// takes tile (b % 8) * (count / 8) + b / 8: each XCD sweeps one contiguous
-// slab of z-planes, and the rows that a tile pulls from its y and z
-// neighbours are in that XCD's own L2.
-struct Tile {
-  int x, y, z;
-};
-__device__ __forceinline__ Tile tile_of(const Geometry &g, int F) {
-  const int gx = g.nx / 64, gy = g.ny / 4, n = gx * gy * (g.nz / F);
-  const int b = blockIdx.x;
-  const int t = (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
-  return {t % gx, (t / gx) % gy, t / (gx * gy)};
-}
-
-// acc: this thread's sums over its F cells in z; out: 4 planes of nc cells
-template <int F>
-__device__ __forceinline__ void block_mean_store(double acc[4], double (*red)[4][64], const Geometry &g, Tile tl,
-                                                 float *__restrict__ out) {
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-#pragma unroll
-  for (int m = 1; m < F; m <<= 1)
-#pragma unroll
-    for (int a = 0; a < 4; ++a) acc[a] += __shfl_xor(acc[a], m);
-#pragma unroll
-  for (int a = 0; a < 4; ++a) red[a][ty][tx] = acc[a];
-  __syncthreads();
-  if (ty % F == 0 && tx % F == 0) {
-    const int x = tl.x * 64 + tx, y = tl.y * 4 + ty;
-    const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
-    const long nc = (long)ncx * ncy * ncz;
-    const long cc = (long)(x / F) + (long)ncx * ((long)(y / F) + (long)ncy * tl.z);
-    const double inv = 1.0 / (F * F * F);
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      double t = 0.0;
-      for (int k = 0; k < F; ++k) t += red[a][ty + k][tx];
-      out[a * nc + cc] = (float)(t * inv);
-    }
-  }
-  __syncthreads();
-}
 
 // A stored population as stored (no conversion), buffer-addressed: the
 // per-lane byte offset voff and the wave-uniform soff, both below 2^32.
@@ -226,28 +162,12 @@ __global__ void __launch_bounds__(256)
     if (stored) {
       A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
       if (act_at(z) && !frozen) {
-        A s1 = 0, s2 = 0, s3 = 0, s4 = 0;
         if (!fast(z)) {
           const long c = pop_index(g, x, y, z);
 #pragma unroll
           for (int q = 0; q < 19; ++q) sv[q] = f[q * g.pq + c];
         }
-#pragma unroll
-        for (int q = 0; q < 19; ++q) {
-          const A fq = (A)(from_bits<P>(sv[q]) * Traits<P>::inv_scale);
-          s1 += fq;
-          s2 = add_c(s2, kCX[q], fq);
-          s3 = add_c(s3, kCY[q], fq);
-          s4 = add_c(s4, kCZ[q], fq);
-        }
-        if (Traits<P>::dev) {
-          s1 += (A)rhor;
-          s2 += (A)mrx;
-        }
-        sr = s1;
-        sx = s2 / s1;
-        sy = s3 / s1;
-        sz = s4 / s1;
+        stored_moments<P>(sv, rhor, mrx, sr, sx, sy, sz);
       }
       sacc[0] += (double)sr;
       sacc[1] += (double)sx;
@@ -293,25 +213,7 @@ __global__ void __launch_bounds__(256)
     const int z = tl.z * F + k;
     if (k + 1 < F) fetch(z + 1, hn);
     A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
-    if (upd(z)) {
-      A s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        const A fq = (A)(from_bits<P>(h[q]) * Traits<P>::inv_scale);
-        s1 += fq;
-        s2 = add_c(s2, kCX[q], fq);
-        s3 = add_c(s3, kCY[q], fq);
-        s4 = add_c(s4, kCZ[q], fq);
-      }
-      if (Traits<P>::dev) {
-        s1 += (A)rhor;
-        s2 += (A)mrx;
-      }
-      sr = s1;
-      sx = s2 / s1;
-      sy = s3 / s1;
-      sz = s4 / s1;
-    }
+    if (upd(z)) stored_moments<P>(h, rhor, mrx, sr, sx, sy, sz);
     sacc[0] += (double)sr;
     sacc[1] += (double)sx;
     sacc[2] += (double)sy;

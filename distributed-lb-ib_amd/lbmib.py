@@ -42,7 +42,8 @@ ALL_SYMBOLS = ("lbmib_create", "lbmib_destroy", "lbmib_set_fiber_positions", "lb
                "lbmib_batch_set_overlap",
                "lbmib_batch_cells", "lbmib_batch_status",
                "lbmib_batch_time_steps", "lbmib_coarse_moments", "lbmib_coarse_sheets", "lbmib_jump",
-               "lbmib_jump_commit", "lbmib_jump_revert", "lbmib_get_stream", "lbmib_coarse_state") + tuple("lbmib_" + n for n in CTX_CALLS)
+               "lbmib_jump_commit", "lbmib_jump_revert", "lbmib_get_stream", "lbmib_coarse_state",
+               "lbmib_step_coarse") + tuple("lbmib_" + n for n in CTX_CALLS)
 
 
 class LbmibError(RuntimeError):
@@ -122,6 +123,7 @@ def load_library(path=LIB_PATH):
     L.lbmib_coarse_sheets.argtypes = [vp, i, vp]
     L.lbmib_jump.argtypes = [vp, i, vp, vp, i, vp]
     L.lbmib_coarse_state.argtypes = [vp, i, vp, vp]
+    L.lbmib_step_coarse.argtypes = [vp, i, vp]
     L.lbmib_jump_commit.argtypes = [vp]
     L.lbmib_jump_revert.argtypes = [vp]
     L.lbmib_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
@@ -378,6 +380,11 @@ class Engine:
     def coarse_moments(self, factor, d_out, d_stored=0):
         self._check(self.L.lbmib_coarse_state(self.h, factor, ctypes.c_void_p(d_out), ctypes.c_void_p(d_stored)),
                     "coarse_state")
+
+    def step_coarse(self, factor, d_stored):
+        """One step that also writes the coarse stored-moment planes of the
+        new state to d_stored (lbmib_step_coarse)."""
+        self._check(self.L.lbmib_step_coarse(self.h, factor, ctypes.c_void_p(d_stored)), "step_coarse")
 
     def coarse_sheets(self, factor, d_out):
         self._check(self.L.lbmib_coarse_sheets(self.h, factor, ctypes.c_void_p(d_out)), "coarse_sheets")

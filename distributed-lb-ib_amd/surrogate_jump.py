@@ -109,6 +109,15 @@ class Coarse:
         self.e.coarse_sheets(self.f, t[4].data_ptr())
         return t
 
+    def step_read(self):
+        """One plain step whose fluid launch also writes planes 0-3 of the
+        new state (lbmib_step_coarse: the same bits as a step and then
+        read(), without the read's pass over the populations), then the
+        sheet raster."""
+        self.e.step_coarse(self.f, self.feat[0].data_ptr())
+        self.e.coarse_sheets(self.f, self.feat[4].data_ptr())
+        return self.feat
+
 
 def pack_input(feat, norm, consts, x, nc, stream):
     """Network input [.., CIN] (BF16) of feature planes feat [NF][nc]."""
@@ -247,10 +256,11 @@ class JumpRunner:
         self.pred_ready = False
         # the jump also reports the jumped state's coarse stored moments
         e.jump(self.co.f, self.co.feat.data_ptr(), self.delta.data_ptr(), self.K, self.after.data_ptr())
-        e.step(self.R)
-        # after R plain steps: the next cycle's input, which the
-        # verification also compares with the jumped state
-        self.co.read()
+        # R plain steps, the last of which also reads the next cycle's input,
+        # which the verification compares with the jumped state
+        if self.R > 1:
+            e.step(self.R - 1)
+        self.co.step_read()
         # verification ratio: |du over R steps| / R  against  |du of the jump| / K
         self._sqdiff(self.co.feat[1:4], self.after[1:4], self.sums[0:2])
         self._sqdiff(self.delta[1:4], None, self.sums[2:4])
@@ -272,10 +282,10 @@ class JumpRunner:
             self.stats["accepted"] += 1
             return True
         e.jump_revert()
-        e.step(self.K + self.R)
+        e.step(self.K + self.R - 1)
+        self.co.step_read()
         self.stats["rejected"] += 1
         self.stats["plain_steps"] += self.K + self.R
-        self.co.read()
         self.pred_ready = False
         return False
 
