@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "lbmib_coarse.h"
 #include "lbmib_ctx.h"
 #include "lbmib_internal.h"
 
@@ -81,6 +82,17 @@ extern "C" int lbmib_destroy(lbmib_ctx *ctx) {
 }
 
 namespace lbmib {
+int ensure_cpart(::lbmib_ctx *ctx, int F) {
+  const size_t need = coarse_part_bytes(ctx->g, F);
+  if (ctx->cpart_bytes >= need) return LBMIB_OK;
+  if (ctx->d_cpart) (void)hipFree(ctx->d_cpart);
+  ctx->d_cpart = nullptr;
+  ctx->cpart_bytes = 0;
+  if (hipMalloc((void **)&ctx->d_cpart, need) != hipSuccess) return LBMIB_ERR_ALLOC;
+  ctx->cpart_bytes = need;
+  return LBMIB_OK;
+}
+
 // Bytes of one step's stores in the rows the sheets' support boxes meet
 // (estimated from the sheets' extents).
 double box_row_bytes(const lbmib_ctx *ctx) {
@@ -617,15 +629,7 @@ extern "C" int lbmib_step_coarse(lbmib_ctx *ctx, int factor, float *d_stored) {
     RCHK(lbmib_step(ctx, 1));
     return lbmib_coarse_state(ctx, factor, nullptr, d_stored);
   }
-  // per-row partial sums: 4 doubles per coarse x cell of every (y, z) row
-  const size_t need = sizeof(double) * 4 * (size_t)(g.nx / factor) * g.ny * g.nz;
-  if (ctx->cpart_bytes < need) {
-    if (ctx->d_cpart) (void)hipFree(ctx->d_cpart);
-    ctx->d_cpart = nullptr;
-    ctx->cpart_bytes = 0;
-    if (hipMalloc((void **)&ctx->d_cpart, need) != hipSuccess) return LBMIB_ERR_ALLOC;
-    ctx->cpart_bytes = need;
-  }
+  RCHK(ensure_cpart(ctx, factor));
   return one_step(ctx, nullptr, nullptr, factor, d_stored);
 }
 

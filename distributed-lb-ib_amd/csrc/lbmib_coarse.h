@@ -104,4 +104,64 @@ __device__ __forceinline__ void stored_moments(const typename Traits<P>::S h[19]
   uz = s4 / s1;
 }
 
+// ------------------------------------------------- stored planes by rows
+//
+// Every stored-moment plane set of the library -- lbmib_coarse_state's
+// d_stored, lbmib_step_coarse's planes and lbmib_jump's d_after -- comes
+// from per-row partial sums in one fixed order, so that equal cell values
+// give equal planes: a row pass leaves, for each (y, z) row and each group
+// of F cells along x, the FP32 sums of the group's cell values (rho, ux, uy,
+// uz; lane shuffles) in part[row][x / F][4]; k_coarse_rows then adds, per
+// coarse cell, the F x F rows of its block (over z within each y, then over
+// y, in FP64) and writes the mean.  The F lanes of a group must all be
+// active (F divides 64, and a block holds whole waves of one row).
+template <int F>
+__device__ __forceinline__ void row_partial(float m[4], int x, long row, long ncx, float *__restrict__ part) {
+#pragma unroll
+  for (int d = 1; d < F; d <<= 1)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) m[a] += __shfl_xor(m[a], d);
+  if (x % F == 0) *(float4 *)(part + (row * ncx + x / F) * 4) = make_float4(m[0], m[1], m[2], m[3]);
+}
+
+template <int F>
+__global__ void __launch_bounds__(256) k_coarse_rows(const Geometry g, const float *__restrict__ part,
+                                                     float *__restrict__ out) {
+  const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
+  const long nc = (long)ncx * ncy * ncz;
+  const long cc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= nc) return;
+  const int cx = (int)(cc % ncx), cy = (int)((cc / ncx) % ncy), cz = (int)(cc / ((long)ncx * ncy));
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < F; ++j) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < F; ++k) {
+      const long row = (long)(cy * F + j) + (long)g.ny * (cz * F + k);
+      const float4 p = *(const float4 *)(part + (row * ncx + cx) * 4);
+      acc[0] += (double)p.x;
+      acc[1] += (double)p.y;
+      acc[2] += (double)p.z;
+      acc[3] += (double)p.w;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) t[a] += acc[a];
+  }
+  const double inv = 1.0 / (F * F * F);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) out[a * nc + cc] = (float)(t[a] * inv);
+}
+
+// bytes of the partial sums at factor F; the reduction's launch
+inline size_t coarse_part_bytes(const Geometry &g, int F) {
+  return sizeof(float) * 4 * (size_t)(g.nx / F) * g.ny * g.nz;
+}
+inline hipError_t launch_coarse_rows(const Geometry &g, int F, const float *part, float *out, hipStream_t s) {
+  const long nc = (long)(g.nx / F) * (g.ny / F) * (g.nz / F);
+  const unsigned blocks = (unsigned)((nc + 255) / 256);
+  if (F == 1) k_coarse_rows<1><<<blocks, 256, 0, s>>>(g, part, out);
+  else if (F == 2) k_coarse_rows<2><<<blocks, 256, 0, s>>>(g, part, out);
+  else k_coarse_rows<4><<<blocks, 256, 0, s>>>(g, part, out);
+  return hipGetLastError();
+}
+
 }  // namespace lbmib

@@ -66,10 +66,15 @@ struct lbmib_ctx {
   int *box_save = nullptr;
   unsigned long long *d_raster = nullptr;
   long raster_cells = 0;
-  double *d_cpart = nullptr;  // lbmib_step_coarse's per-row partial sums
+  float *d_cpart = nullptr;  // per-row partial sums of the stored planes (lbmib_coarse.h)
   size_t cpart_bytes = 0;
   int jump_active = 0;      // a jump awaits lbmib_jump_commit / lbmib_jump_revert
   int jump_time = 0;        // time and spread slot before the jump
   int jump_spread_slot = -1;
 };
 
+namespace lbmib {
+// The context's partial-sum buffer for stored planes at factor F (lbmib_coarse.h),
+// grown on demand.
+int ensure_cpart(::lbmib_ctx *ctx, int F);
+}  // namespace lbmib

@@ -1368,12 +1368,10 @@ int launch_collide_stream(int prec, const Geometry &g, const void *fin, void *fo
 // One step's fluid update in the row form (one block per (y, z) row, one
 // lane per x, every cell updated as row_cell updates it) that also reports
 // the stored moments of the state it writes, taken from the post-collision
-// values the stores round: each group of F lanes sums its cells' moments
-// (lane shuffles) into part[row][x / F][4] (rows without updated cells
-// report rho0/u0 like k_coarse_stored), and k_coarse_rows then sums F x F
-// rows per coarse cell.  The planes equal lbmib_coarse_state's stored
-// planes of the new state up to the order of the double sums.  Needs the
-// buffer-addressed row path with a block as wide as the row, and an
+// values the stores round, as per-row partial sums (lbmib_coarse.h; cells
+// that are not updated report rho0/u0 like the stored read).  The planes are
+// lbmib_coarse_state's stored planes of the new state, bit for bit.  Needs
+// the buffer-addressed row path with a block as wide as the row, and an
 // inlet/outlet x axis (the x edges of other axes are updated by a second
 // launch, after the sums).
 template <int P, bool NT, bool NTL, int F>
@@ -1384,7 +1382,7 @@ __global__ void __launch_bounds__(1024)
                                  const typename Traits<P>::FT *__restrict__ Fcur,
                                  const typename Traits<P>::FT *__restrict__ Fprev,
                                  const uint32_t *__restrict__ rowtab, int box_rows_cached, double rhor, double mrx,
-                                 double rho0, double u0x, double u0y, double u0z, double *__restrict__ part) {
+                                 double rho0, double u0x, double u0y, double u0z, float *__restrict__ part) {
   using T = typename Traits<P>::T;
   using S = typename Traits<P>::S;
   using A = JumpT<P>;
@@ -1401,43 +1399,9 @@ __global__ void __launch_bounds__(1024)
                            for (int q = 0; q < 19; ++q) h[q] = to_bits<P>(v[q]);
                            stored_moments<P>(h, rhor, mrx, sr, sx, sy, sz);
                          });
-    double m[4] = {(double)sr, (double)sx, (double)sy, (double)sz};
-#pragma unroll
-    for (int d = 1; d < F; d <<= 1)
-#pragma unroll
-      for (int a = 0; a < 4; ++a) m[a] += __shfl_xor(m[a], d);
-    if (x % F == 0) {
-      double *o = part + ((long)row * ncx + x / F) * 4;
-#pragma unroll
-      for (int a = 0; a < 4; ++a) o[a] = m[a];
-    }
+    float m[4] = {(float)sr, (float)sx, (float)sy, (float)sz};
+    row_partial<F>(m, x, row, ncx, part);
   }
-}
-
-// out[a][cz][cy][cx] = mean over the coarse cell of part: for each of its F
-// rows in y, the sum over its F rows in z, summed in y order
-template <int F>
-__global__ void k_coarse_rows(const Geometry g, const double *__restrict__ part, float *__restrict__ out) {
-  const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
-  const long nc = (long)ncx * ncy * ncz;
-  const long cc = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cc >= nc) return;
-  const int cx = (int)(cc % ncx), cy = (int)((cc / ncx) % ncy), cz = (int)(cc / ((long)ncx * ncy));
-  double t[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = 0; j < F; ++j) {
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < F; ++k) {
-      const long row = (long)(cy * F + j) + (long)g.ny * (cz * F + k);
-      const double *p = part + (row * ncx + cx) * 4;
-#pragma unroll
-      for (int a = 0; a < 4; ++a) acc[a] += p[a];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) t[a] += acc[a];
-  }
-  const double inv = 1.0 / (F * F * F);
-#pragma unroll
-  for (int a = 0; a < 4; ++a) out[a * nc + cc] = (float)(t[a] * inv);
 }
 
 bool collide_coarse_ok(const Geometry &g, size_t es, bool rowtab, int F) {
@@ -1448,7 +1412,7 @@ bool collide_coarse_ok(const Geometry &g, size_t es, bool rowtab, int F) {
 int launch_collide_coarse(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
-                          int F, double *part, float *stored, hipStream_t s) {
+                          int F, float *part, float *stored, hipStream_t s) {
   if (!part || !stored) return LBMIB_ERR_ARG;
   const int nt = kv.nt_mask < 0 ? 1 : (kv.nt_mask & 1);
   const int ntl = kv.nt_mask < 0 ? 1 : ((kv.nt_mask >> 1) & 1);
@@ -1476,12 +1440,7 @@ int launch_collide_coarse(int prec, const Geometry &g, const void *fin, void *fo
     else pick(std::false_type{}, std::false_type{});
   }));
   if (hipGetLastError() != hipSuccess) return LBMIB_ERR_HIP;
-  const long nc = (long)(g.nx / F) * (g.ny / F) * (g.nz / F);
-  const unsigned blocks = (unsigned)((nc + 255) / 256);
-  if (F == 1) k_coarse_rows<1><<<blocks, 256, 0, s>>>(g, part, stored);
-  else if (F == 2) k_coarse_rows<2><<<blocks, 256, 0, s>>>(g, part, stored);
-  else k_coarse_rows<4><<<blocks, 256, 0, s>>>(g, part, stored);
-  return hipGetLastError() == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
+  return launch_coarse_rows(g, F, part, stored, s) == hipSuccess ? LBMIB_OK : LBMIB_ERR_HIP;
 }
 
 int launch_materialize(int prec, const Geometry &g, const void *f, int own_mode, const FluidConsts<double> &c,

@@ -239,7 +239,7 @@ bool collide_coarse_ok(const Geometry &g, size_t es, bool rowtab, int F);
 int launch_collide_coarse(int prec, const Geometry &g, const void *fin, void *fout, int own_mode,
                           const FluidConsts<double> &c, int nsheets, const int *box_cur, const int *box_prev,
                           const void *Fcur, const void *Fprev, const uint32_t *rowtab, const KernelVariant &kv,
-                          int F, double *part, float *stored, hipStream_t s);
+                          int F, float *part, float *stored, hipStream_t s);
 bool pair_path_ok(const Geometry &g);
 bool rows2_path_ok(const Geometry &g);
 int row_source_table(const Geometry &g, size_t es, std::vector<uint32_t> &tab);

@@ -62,18 +62,14 @@ __device__ __forceinline__ void bst_raw(__amdgpu_buffer_rsrc_t r, uint32_t voff,
 
 // Per fine cell, the moments of lbmib_get_macroscopic (post-stream
 // populations, u = (sum c f + F/2) / rho, the initial rho0/u0 outside the
-// moment box) into `out`; with `stored`, also the moments of the stored
-// (post-collision) populations of the cell itself, u = sum c f / rho, on the
-// updated cells (the active box without the frozen inlet/outlet planes) and
-// rho0/u0 elsewhere -- the convention k_jump_apply reports the jumped state in.
+// moment box), block-averaged into `out`.
 template <int P, int F, bool BUF>
 __global__ void __launch_bounds__(256)
     k_coarse_moments(Geometry g, const typename Traits<P>::S *__restrict__ f, int own,
                      const typename Traits<P>::T *__restrict__ tab, double rhor, double mrx,
                      typename Traits<P>::T b1, typename Traits<P>::T b2, double rho0, double u0x, double u0y,
                      double u0z, double gz, int nsheets, const int *__restrict__ box,
-                     const typename Traits<P>::FT *__restrict__ Fc, float *__restrict__ out,
-                     float *__restrict__ stored) {
+                     const typename Traits<P>::FT *__restrict__ Fc, float *__restrict__ out) {
   using S = typename Traits<P>::S;
   using A = JumpT<P>;
   __shared__ double red[4][4][64];
@@ -84,19 +80,16 @@ __global__ void __launch_bounds__(256)
   const bool row_act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1];
   const bool row_mom = x >= g.mlo[0] && x <= g.mhi[0] && y >= g.mlo[1] && y <= g.mhi[1];
   const bool row_int = x > g.lo[0] && x < g.hi[0] && y > g.lo[1] && y < g.hi[1];
-  // which loads cell z needs: the pulled populations (interior cells read
-  // them here; the others pull at their turn through the boundary rules) and
-  // the cell's own stored ones
+  // which cells pull here: interior cells load their pulled populations up
+  // front; the others pull at their turn through the boundary rules
   auto act_at = [&](int z) { return row_act && z >= g.lo[2] && z <= g.hi[2]; };
   auto pulls = [&](int z) { return act_at(z) && row_mom && z >= g.mlo[2] && z <= g.mhi[2] && !own && !frozen; };
   auto fast = [&](int z) { return row_int && z > g.lo[2] && z < g.hi[2]; };
-  // a cell's pulled and stored values are loaded together, before either
-  // sum, so that both sets are in flight at once
-  S pv[19], sv[19];
+  S pv[19];
   // (BUF: the whole population array lies within 2^32 bytes, so the loads
   // take a 32-bit lane offset and a uniform one instead of 64-bit addresses)
   const __amdgpu_buffer_rsrc_t rf = make_rsrc(f);
-  auto fetch = [&](int z, S(&pp)[19], S(&ss)[19]) {
+  auto fetch = [&](int z, S(&pp)[19]) {
     const long c = pop_index(g, x, y, z);
     const long sy = g.prow, sz = g.prow * g.ny;
     if (pulls(z) && fast(z)) {
@@ -110,19 +103,12 @@ __global__ void __launch_bounds__(256)
         else pp[q] = f[q * g.pq + src];
       }
     }
-    if (stored && act_at(z) && !frozen && fast(z)) {  // boundary cells load theirs after the pulled sums
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        if constexpr (BUF) ss[q] = bld_raw<P>(rf, (uint32_t)(c * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
-        else ss[q] = f[q * g.pq + c];
-      }
-    }
   };
-  double acc[4] = {0.0, 0.0, 0.0, 0.0}, sacc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
   for (int k = 0; k < F; ++k) {
     const int z = tl.z * F + k;
-    fetch(z, pv, sv);
+    fetch(z, pv);
     A r = (A)rho0, ux = (A)u0x, uy = (A)u0y, uz = (A)u0z;
     if (act_at(z) && row_mom && z >= g.mlo[2] && z <= g.mhi[2] && !own) {
       const long c0 = (long)x + (long)g.nx * ((long)y + (long)g.ny * z);
@@ -159,70 +145,43 @@ __global__ void __launch_bounds__(256)
     acc[1] += (double)ux;
     acc[2] += (double)uy;
     acc[3] += (double)uz;
-    if (stored) {
-      A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
-      if (act_at(z) && !frozen) {
-        if (!fast(z)) {
-          const long c = pop_index(g, x, y, z);
-#pragma unroll
-          for (int q = 0; q < 19; ++q) sv[q] = f[q * g.pq + c];
-        }
-        stored_moments<P>(sv, rhor, mrx, sr, sx, sy, sz);
-      }
-      sacc[0] += (double)sr;
-      sacc[1] += (double)sx;
-      sacc[2] += (double)sy;
-      sacc[3] += (double)sz;
-    }
   }
   block_mean_store<F>(acc, red, g, tl, out);
-  if (stored) block_mean_store<F>(sacc, red, g, tl, stored);
 }
 
-// The stored-moment planes alone (lbmib_coarse_state without d_moments): the
-// `stored` half of k_coarse_moments -- the same sums in the same order, so
-// the same bits -- with the next cell's populations loaded before this
-// cell's sums, so that two cells' loads are in flight per thread.
+// The stored moments of every cell -- rho and u = sum c f / rho of its own
+// populations on the updated cells (the active box without the frozen
+// inlet/outlet planes), rho0/u0 elsewhere; the convention lbmib_jump reports
+// a jumped state in -- as per-row partial sums (lbmib_coarse.h): one block
+// per (y, z) row, blockDim.x a multiple of 64 dividing into nx in x steps.
 template <int P, int F, bool BUF>
-__global__ void __launch_bounds__(256)
-    k_coarse_stored(Geometry g, const typename Traits<P>::S *__restrict__ f, double rhor, double mrx, double rho0,
-                    double u0x, double u0y, double u0z, float *__restrict__ stored) {
+__global__ void __launch_bounds__(1024)
+    k_coarse_stored_rows(Geometry g, const typename Traits<P>::S *__restrict__ f, double rhor, double mrx,
+                         double rho0, double u0x, double u0y, double u0z, float *__restrict__ part) {
   using S = typename Traits<P>::S;
   using A = JumpT<P>;
-  __shared__ double red[4][4][64];
-  const Tile tl = tile_of(g, F);
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int x = tl.x * 64 + tx, y = tl.y * 4 + ty;
-  const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
-  const bool row_upd = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && !frozen;
-  auto upd = [&](int z) { return row_upd && z >= g.lo[2] && z <= g.hi[2]; };
   const __amdgpu_buffer_rsrc_t rf = make_rsrc(f);
-  auto fetch = [&](int z, S(&d)[19]) {
-    if (!upd(z)) return;
-    const long c = pop_index(g, x, y, z);
+  const long nrows = (long)g.ny * g.nz, ncx = g.nx / F;
+  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+    const bool live = y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
+    for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
+      const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+      A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
+      if (live && x >= g.lo[0] && x <= g.hi[0] && !frozen) {
+        const long c = pop_index(g, x, y, z);
+        S h[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      if constexpr (BUF) d[q] = bld_raw<P>(rf, (uint32_t)(c * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
-      else d[q] = f[q * g.pq + c];
+        for (int q = 0; q < 19; ++q) {
+          if constexpr (BUF) h[q] = bld_raw<P>(rf, (uint32_t)(c * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
+          else h[q] = f[q * g.pq + c];
+        }
+        stored_moments<P>(h, rhor, mrx, sr, sx, sy, sz);
+      }
+      float m[4] = {(float)sr, (float)sx, (float)sy, (float)sz};
+      row_partial<F>(m, x, row, ncx, part);
     }
-  };
-  S h[19], hn[19];
-  double sacc[4] = {0.0, 0.0, 0.0, 0.0};
-  fetch(tl.z * F, h);
-  for (int k = 0; k < F; ++k) {
-    const int z = tl.z * F + k;
-    if (k + 1 < F) fetch(z + 1, hn);
-    A sr = (A)rho0, sx = (A)u0x, sy = (A)u0y, sz = (A)u0z;
-    if (upd(z)) stored_moments<P>(h, rhor, mrx, sr, sx, sy, sz);
-    sacc[0] += (double)sr;
-    sacc[1] += (double)sx;
-    sacc[2] += (double)sy;
-    sacc[3] += (double)sz;
-    if (k + 1 < F)
-#pragma unroll
-      for (int q = 0; q < 19; ++q) h[q] = hn[q];
   }
-  block_mean_store<F>(sacc, red, g, tl, stored);
 }
 
 // trilinear interpolation of coarse plane `a` at fine position (px, py, pz)
@@ -259,45 +218,31 @@ __device__ __forceinline__ T dfeq_q(int q, T rho, T drho, T ux, T uy, T uz, T dx
   return (drho * a + (rho + drho) * b) / (T)kWDEN[q];
 }
 
-// The tile layout of k_coarse_moments (64 x 4 cells, F cells in z per
-// thread): the updated cells -- the active box without the frozen
-// inlet/outlet planes -- take the equilibrium shift; every other element is
-// copied unchanged.  With `after`, the block also writes the coarse means of
-// the jumped state's stored moments (rho + drho, u + du exactly, by the
-// moments of feq) in k_coarse_moments' stored convention.
-// (four blocks per CU for the deviation forms: two cells' populations and
-// the trilinear weights fit in 128 VGPRs; FP64 keeps its registers)
+// One block per (y, z) row, in x chunks of blockDim.x cells (a multiple of
+// 64 that divides nx, at most 512): the updated cells -- the active box
+// without the frozen inlet/outlet planes -- take the equilibrium shift;
+// every other element is copied unchanged.  With `part`, the jumped state's
+// stored moments (rho + drho, u + du exactly, by the moments of feq; rho0/u0
+// on the other cells) go to per-row partial sums (lbmib_coarse.h), the
+// stored planes' convention.
 template <int P, int F, bool BUF>
-__global__ void __launch_bounds__(256, P == LBMIB_F64 ? 1 : 4)
-    k_jump_apply(Geometry g, const typename Traits<P>::S *__restrict__ fin, typename Traits<P>::S *__restrict__ fout,
-                 double rhor, double mrx, const float *__restrict__ delta, double rho0, double u0x, double u0y,
-                 double u0z, float *__restrict__ after) {
+__global__ void __launch_bounds__(512)
+    k_jump_apply_rows(Geometry g, const typename Traits<P>::S *__restrict__ fin,
+                      typename Traits<P>::S *__restrict__ fout, double rhor, double mrx,
+                      const float *__restrict__ delta, double rho0, double u0x, double u0y, double u0z,
+                      float *__restrict__ part) {
   using T = JumpT<P>;
-  __shared__ double red[4][4][64];
-  // the coarse change around the block's tile: (64/F + 2) x (4/F + 2) x 3
-  // coarse cells (clamped at the grid's edges), 4 planes
-  constexpr int SX = 64 / F + 2, SY = 4 / F + 2, SZ = 3;
-  __shared__ float sd[4][SZ][SY][SX];
-  const Tile tl = tile_of(g, F);
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int x = tl.x * 64 + tx, y = tl.y * 4 + ty;
+  using S = typename Traits<P>::S;
+  // the coarse change along the chunk: its coarse cells and one more on each
+  // side in x, the two coarse rows around the row in y and in z (clamped at
+  // the grid's edges), 4 planes
+  constexpr int kSX = 512 / F + 2;
+  __shared__ float sd[4][2][2][kSX];
+  const int SX = (int)blockDim.x / F + 2;
   const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
-  const long nc = (long)ncx * ncy * ncz;
-  const int cx0 = tl.x * (64 / F) - 1, cy0 = tl.y * (4 / F) - 1, cz0 = tl.z - 1;
-  for (int e = threadIdx.x; e < 4 * SZ * SY * SX; e += 256) {
-    const int i = e % SX, j = (e / SX) % SY, k = (e / (SX * SY)) % SZ, a = e / (SX * SY * SZ);
-    const int jx = min(max(cx0 + i, 0), ncx - 1), jy = min(max(cy0 + j, 0), ncy - 1);
-    const int jz = min(max(cz0 + k, 0), ncz - 1);
-    sd[a][k][j][i] = delta[a * nc + (long)jx + (long)ncx * ((long)jy + (long)ncy * jz)];
-  }
-  __syncthreads();
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  // the F cells of the thread's column in z; the next cell's populations are
-  // loaded (as stored, in registers of the storage type) before this cell's
-  // arithmetic and stores, so that two cells' loads are in flight
+  const long nc = (long)ncx * ncy * ncz, nrows = (long)g.ny * g.nz;
   // (BUF: both population arrays lie within 2^32 bytes; loads and stores
   // take a 32-bit lane offset and a uniform one)
-  using S = typename Traits<P>::S;
   const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin), rout = make_rsrc(fout);
   auto load = [&](long p, int q) -> S {
     if constexpr (BUF) return bld_raw<P>(rin, (uint32_t)(p * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)));
@@ -307,78 +252,75 @@ __global__ void __launch_bounds__(256, P == LBMIB_F64 ? 1 : 4)
     if constexpr (BUF) bst_raw<P>(rout, (uint32_t)(p * sizeof(S)), (uint32_t)(q * g.pq * sizeof(S)), v);
     else fout[q * g.pq + p] = v;
   };
-  S h[19], hn[19];
-  {
-    const long p = pop_index(g, x, y, tl.z * F);
+  const double hc = 0.5 * (F - 1);
+  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+    const bool live = y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
+    const double fy = floor((y - hc) / F), fz = floor((z - hc) / F);
+    const double wy = (y - hc) / F - fy, wz = (z - hc) / F - fz;
+    for (int xb = 0; xb < g.nx; xb += blockDim.x) {
+      const int cx0 = xb / F - 1;
+      __syncthreads();  // the previous chunk's readers are done
+      for (int e = threadIdx.x; e < 16 * SX; e += blockDim.x) {
+        const int i = e % SX, dy = (e / SX) & 1, dz = (e / (2 * SX)) & 1, a = e / (4 * SX);
+        const int jx = min(max(cx0 + i, 0), ncx - 1), jy = min(max((int)fy + dy, 0), ncy - 1);
+        const int jz = min(max((int)fz + dz, 0), ncz - 1);
+        sd[a][dz][dy][i] = delta[a * nc + (long)jx + (long)ncx * ((long)jy + (long)ncy * jz)];
+      }
+      __syncthreads();
+      const int x = xb + (int)threadIdx.x;
+      const long p0 = pop_index(g, x, y, z);
+      S h[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) h[q] = load(p, q);
+      for (int q = 0; q < 19; ++q) h[q] = load(p0, q);
+      const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
+      float m[4] = {(float)(T)rho0, (float)(T)u0x, (float)(T)u0y, (float)(T)u0z};
+      if (!live || x < g.lo[0] || x > g.hi[0] || frozen) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) store(p0, q, h[q]);  // unchanged
+      } else {
+        T rho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+          const T fq = (T)from_bits<P>(h[q]) * (T)Traits<P>::inv_scale;  // f, or the deviation f - r
+          rho += fq;
+          jx = add_c(jx, kCX[q], fq);
+          jy = add_c(jy, kCY[q], fq);
+          jz = add_c(jz, kCZ[q], fq);
+        }
+        if (Traits<P>::dev) {  // sum r = rho_r, sum c r = rho_r u_r e_x
+          rho += (T)rhor;
+          jx += (T)mrx;
+        }
+        const T ux = jx / rho, uy = jy / rho, uz = jz / rho;
+        // trilinear weights once for the four planes (the clamping of
+        // interp(): a cell left of the first coarse centre takes the edge
+        // value, which the clamped staging reproduces)
+        const double fx = floor((x - hc) / F);
+        const double wx = (x - hc) / F - fx;
+        const int ix = (int)fx - cx0;
+        double dd[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+          const double w = (dx ? wx : 1.0 - wx) * (dy ? wy : 1.0 - wy) * (dz ? wz : 1.0 - wz);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) dd[a] += w * (double)sd[a][dz][dy][ix + dx];
+        }
+        const T d0 = (T)dd[0], d1 = (T)dd[1], d2 = (T)dd[2], d3 = (T)dd[3];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+          const T d = dfeq_q<T>(q, rho, d0, ux, uy, uz, d1, d2, d3);
+          store(p0, q, to_bits<P>(from_bits<P>(h[q]) + (typename Traits<P>::T)(d * (T)Traits<P>::scale)));
+        }
+        m[0] = (float)(rho + d0);
+        m[1] = (float)(ux + d1);
+        m[2] = (float)(uy + d2);
+        m[3] = (float)(uz + d3);
+      }
+      if (part) row_partial<F>(m, x, row, ncx, part);
+    }
   }
-  for (int k = 0; k < F; ++k) {
-    const int z = tl.z * F + k;
-    const long p0 = pop_index(g, x, y, z);
-    if (k + 1 < F) {
-      const long p1 = pop_index(g, x, y, z + 1);
-#pragma unroll
-      for (int q = 0; q < 19; ++q) hn[q] = load(p1, q);
-    }
-    const bool act = x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
-    const bool frozen = g.bc[0] == LBMIB_BC_INOUT && (x == g.lo[0] || x == g.hi[0]);
-    if (!act || frozen) {
-#pragma unroll
-      for (int q = 0; q < 19; ++q) store(p0, q, h[q]);  // unchanged
-      if (k + 1 < F)
-#pragma unroll
-        for (int q = 0; q < 19; ++q) h[q] = hn[q];
-      acc[0] += rho0;
-      acc[1] += u0x;
-      acc[2] += u0y;
-      acc[3] += u0z;
-      continue;
-    }
-    T rho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const T fq = (T)from_bits<P>(h[q]) * (T)Traits<P>::inv_scale;  // f, or the deviation f - r
-      rho += fq;
-      jx = add_c(jx, kCX[q], fq);
-      jy = add_c(jy, kCY[q], fq);
-      jz = add_c(jz, kCZ[q], fq);
-    }
-    if (Traits<P>::dev) {  // sum r = rho_r, sum c r = rho_r u_r e_x
-      rho += (T)rhor;
-      jx += (T)mrx;
-    }
-    const T ux = jx / rho, uy = jy / rho, uz = jz / rho;
-    // trilinear weights once for the four planes (the clamping of interp():
-    // a cell left of the first coarse centre takes the edge value, which the
-    // clamped staging reproduces)
-    const double hc = 0.5 * (F - 1);
-    const double fx = floor((x - hc) / F), fy = floor((y - hc) / F), fz = floor((z - hc) / F);
-    const double wx = (x - hc) / F - fx, wy = (y - hc) / F - fy, wz = (z - hc) / F - fz;
-    const int ix = (int)fx - cx0, iy = (int)fy - cy0, iz = (int)fz - cz0;
-    double dd[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-      const double w = (dx ? wx : 1.0 - wx) * (dy ? wy : 1.0 - wy) * (dz ? wz : 1.0 - wz);
-#pragma unroll
-      for (int a = 0; a < 4; ++a) dd[a] += w * (double)sd[a][iz + dz][iy + dy][ix + dx];
-    }
-    const T d0 = (T)dd[0], d1 = (T)dd[1], d2 = (T)dd[2], d3 = (T)dd[3];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const T d = dfeq_q<T>(q, rho, d0, ux, uy, uz, d1, d2, d3);
-      store(p0, q, to_bits<P>(from_bits<P>(h[q]) + (typename Traits<P>::T)(d * (T)Traits<P>::scale)));
-    }
-    acc[0] += (double)(rho + d0);
-    acc[1] += (double)(ux + d1);
-    acc[2] += (double)(uy + d2);
-    acc[3] += (double)(uz + d3);
-    if (k + 1 < F)
-#pragma unroll
-      for (int q = 0; q < 19; ++q) h[q] = hn[q];
-  }
-  if (after) block_mean_store<F>(acc, red, g, tl, after);
 }
 
 // X += K (u(X) + u'(X)) / 2 with u the coarse velocity before the jump and
@@ -461,6 +403,14 @@ unsigned grid_of(long n, long cap = 16384) {
   return (unsigned)b;
 }
 
+// Threads of a row kernel's block: the largest multiple of 64 up to cap that
+// divides nx (nx is a multiple of 64), so that every chunk of a row is whole.
+inline int row_threads(const Geometry &g, int cap) {
+  int t = cap;
+  while (g.nx % t) t -= 64;
+  return t;
+}
+
 template <int P>
 int coarse_launch(lbmib_ctx *ctx, int F, float *out, float *stored) {
   using T = typename Traits<P>::T;
@@ -478,28 +428,33 @@ int coarse_launch(lbmib_ctx *ctx, int F, float *out, float *stored) {
   const T *tab = (const T *)ctx->c.dtab;
   const FluidConsts<double> &c = ctx->c;
   const bool buf = (unsigned long long)g.pelems * sizeof(typename Traits<P>::S) <= 0xFFFFFFFFull;
-  if (!out) {  // the stored-moment planes alone
+  if (out) {  // the pulled moments, on the coarse tiles
     auto go = [&](auto kern) {
-      kern<<<grid, 256, 0, ctx->stream>>>(g, fp, c.rhor, c.rhor * c.urx, c.rho0, c.u0x, c.u0y, c.u0z, stored);
+      kern<<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x,
+                                          c.u0y, c.u0z, c.g, nsh, box, Fc, out);
     };
     switch (F) {
-      case 1: buf ? go(k_coarse_stored<P, 1, true>) : go(k_coarse_stored<P, 1, false>); break;
-      case 2: buf ? go(k_coarse_stored<P, 2, true>) : go(k_coarse_stored<P, 2, false>); break;
-      default: buf ? go(k_coarse_stored<P, 4, true>) : go(k_coarse_stored<P, 4, false>); break;
+      case 1: buf ? go(k_coarse_moments<P, 1, true>) : go(k_coarse_moments<P, 1, false>); break;
+      case 2: buf ? go(k_coarse_moments<P, 2, true>) : go(k_coarse_moments<P, 2, false>); break;
+      default: buf ? go(k_coarse_moments<P, 4, true>) : go(k_coarse_moments<P, 4, false>); break;
     }
     JHIPCHK(hipGetLastError());
-    return LBMIB_OK;
   }
-  auto go = [&](auto kern) {
-    kern<<<grid, 256, 0, ctx->stream>>>(g, fp, ctx->own, tab, c.rhor, c.rhor * c.urx, b1, b2, c.rho0, c.u0x, c.u0y,
-                                        c.u0z, c.g, nsh, box, Fc, out, stored);
-  };
-  switch (F) {
-    case 1: buf ? go(k_coarse_moments<P, 1, true>) : go(k_coarse_moments<P, 1, false>); break;
-    case 2: buf ? go(k_coarse_moments<P, 2, true>) : go(k_coarse_moments<P, 2, false>); break;
-    default: buf ? go(k_coarse_moments<P, 4, true>) : go(k_coarse_moments<P, 4, false>); break;
+  if (stored) {  // the stored moments, by rows
+    JRCHK(ensure_cpart(ctx, F));
+    const int rows = g.ny * g.nz;
+    auto go = [&](auto kern) {
+      kern<<<rows, row_threads(g, 1024), 0, ctx->stream>>>(g, fp, c.rhor, c.rhor * c.urx, c.rho0, c.u0x, c.u0y,
+                                                            c.u0z, ctx->d_cpart);
+    };
+    switch (F) {
+      case 1: buf ? go(k_coarse_stored_rows<P, 1, true>) : go(k_coarse_stored_rows<P, 1, false>); break;
+      case 2: buf ? go(k_coarse_stored_rows<P, 2, true>) : go(k_coarse_stored_rows<P, 2, false>); break;
+      default: buf ? go(k_coarse_stored_rows<P, 4, true>) : go(k_coarse_stored_rows<P, 4, false>); break;
+    }
+    JHIPCHK(hipGetLastError());
+    JHIPCHK(launch_coarse_rows(g, F, ctx->d_cpart, stored, ctx->stream));
   }
-  JHIPCHK(hipGetLastError());
   return LBMIB_OK;
 }
 
@@ -507,20 +462,23 @@ template <int P>
 int apply_launch(lbmib_ctx *ctx, const void *fin, void *fout, const float *delta, int F, float *after) {
   const Geometry &g = ctx->g;
   const FluidConsts<double> &c = ctx->c;
-  const int grid = (g.nx / 64) * (g.ny / 4) * (g.nz / F);  // tiles, dealt by tile_of
+  if (after) JRCHK(ensure_cpart(ctx, F));
+  float *part = after ? ctx->d_cpart : nullptr;
   const auto *in = (const typename Traits<P>::S *)fin;
   auto *out = (typename Traits<P>::S *)fout;
   const bool buf = (unsigned long long)g.pelems * sizeof(typename Traits<P>::S) <= 0xFFFFFFFFull;
+  const int rows = g.ny * g.nz;
   auto go = [&](auto kern) {
-    kern<<<grid, 256, 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x, c.u0y, c.u0z,
-                                        after);
+    kern<<<rows, row_threads(g, 512), 0, ctx->stream>>>(g, in, out, c.rhor, c.rhor * c.urx, delta, c.rho0, c.u0x,
+                                                         c.u0y, c.u0z, part);
   };
   switch (F) {
-    case 1: buf ? go(k_jump_apply<P, 1, true>) : go(k_jump_apply<P, 1, false>); break;
-    case 2: buf ? go(k_jump_apply<P, 2, true>) : go(k_jump_apply<P, 2, false>); break;
-    default: buf ? go(k_jump_apply<P, 4, true>) : go(k_jump_apply<P, 4, false>); break;
+    case 1: buf ? go(k_jump_apply_rows<P, 1, true>) : go(k_jump_apply_rows<P, 1, false>); break;
+    case 2: buf ? go(k_jump_apply_rows<P, 2, true>) : go(k_jump_apply_rows<P, 2, false>); break;
+    default: buf ? go(k_jump_apply_rows<P, 4, true>) : go(k_jump_apply_rows<P, 4, false>); break;
   }
   JHIPCHK(hipGetLastError());
+  if (after) JHIPCHK(launch_coarse_rows(g, F, part, after, ctx->stream));
   return LBMIB_OK;
 }
 
