@@ -1,6 +1,7 @@
 // lbmib_coarse.h -- coarse-grained moments shared by the coupling kernels
 // (lbmib_jump.hip) and the fused step that reports them (lbmib_fluid.hip):
-// the tile of a coarse read, its block means, and one cell's stored moments.
+// the tile of the pulled read and its block means, one cell's stored
+// moments, and the per-row partial sums every stored plane set comes from.
 #pragma once
 
 #include <type_traits>

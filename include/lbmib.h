@@ -264,13 +264,13 @@ int lbmib_coarse_moments(lbmib_ctx *ctx, int factor, float *d_out);
 int lbmib_coarse_state(lbmib_ctx *ctx, int factor, float *d_moments, float *d_stored);
 /* One step (lbmib_step(ctx, 1)) whose fluid launch also reports the stored
  * moments of the state it writes, from the values it stores, into d_stored:
- * the populations of lbmib_step(ctx, 1) bit for bit, and the planes of a
- * following lbmib_coarse_state(ctx, factor, NULL, d_stored) up to the order
- * of their double sums (within one FP32 rounding), without that read's pass
- * over the populations.  Where that launch does not apply (an x axis other
- * than inlet/outlet, rows wider than 1024 cells, a population array over
- * 4 GiB) it makes exactly those two calls.  Allocates its partial sums
- * (32 bytes per coarse x cell of each row) on first use. */
+ * the populations of lbmib_step(ctx, 1) and the planes of a following
+ * lbmib_coarse_state(ctx, factor, NULL, d_stored), bit for bit, without that
+ * read's pass over the populations.  Where that launch does not apply (an x
+ * axis other than inlet/outlet, rows wider than 1024 cells, a population
+ * array over 4 GiB) it makes exactly those two calls.  The stored planes of
+ * every call come from per-row partial sums (16 bytes per coarse x cell of
+ * each row), which the context allocates on first use. */
 int lbmib_step_coarse(lbmib_ctx *ctx, int factor, float *d_stored);
 /* c = 0..3: number of fiber nodes in each coarse cell and the sums of their
  * elastic forces (compute_elasticforce, lbm_ib.c:1285-1318); deterministic. */
