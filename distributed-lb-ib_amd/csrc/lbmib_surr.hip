@@ -214,6 +214,162 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x
   }
 }
 
+// Large-grid forward convolution with the input staged in LDS.  A block owns
+// 256 consecutive voxels in linear order -- R = 256 / W whole x-rows of one
+// z-plane of one sample (W divides 256 and R divides H) -- and WN x 16 output
+// channels; its 4 waves own 64 voxels each (4 fragments of 16).  The input
+// box around the tile, (W + 2) x (R + 2) x 3 voxels with zeros outside the
+// volume, is staged CB channels at a time, and all 27 taps read it there, so
+// each input voxel is fetched from memory once per block and channel block
+// instead of once per tap.  CB = 8: rows of 8 elements (16 bytes), 19 KB for
+// W = 64, so that 5 blocks fit a CU; the reduction runs over chunks of four
+// taps x 8 channels, slot 8 kq + e of a chunk (kq = lane >> 4) being tap
+// 4p + kq, channel c0 + e.  CB = 16, for wide inputs: rows of 24 elements
+// (48-byte steps put the 16-byte reads of 8 lanes on distinct banks), chunks
+// of two taps x 16 channels, slot 8 kq + e being tap 2p + (kq >> 1), channel
+// c0 + 8 (kq & 1) + e.  The slot order is the same in both operands; taps
+// past the 27th are zero.  Each chunk's weight tile is staged in
+// double-buffered LDS as in k_conv_fwd, the next chunk's loaded while the
+// current one multiplies.
+__host__ __device__ constexpr int box_pitch(int CB) { return CB == 8 ? 8 : 24; }
+__host__ __device__ inline int box_voxels(int W) { return (W + 2) * (256 / W + 2) * 3; }
+inline bool box_conv_ok(int H, int W, long M) {
+  return W >= 16 && 256 % W == 0 && H % (256 / W) == 0 && M % 256 == 0;
+}
+
+template <int WN, int CB>
+__global__ void __launch_bounds__(256) k_conv_fwd_box(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
+                                                      const float *__restrict__ bias,
+                                                      const uint16_t *__restrict__ mask, uint16_t *__restrict__ y,
+                                                      int D, int H, int W, long M, int Cin, int Cout, int relu) {
+  constexpr int WM = 4, kBoxPitch = box_pitch(CB), TPC = 32 / CB, NCH = (27 + TPC - 1) / TPC;
+  extern __shared__ __attribute__((aligned(16))) uint16_t xs[];  // [box voxel][kBoxPitch]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const long m0 = (long)blockIdx.x * 256;
+  const int n0 = blockIdx.y * (WN * 16);
+  const int R = 256 / W, BX = W + 2, BY = R + 2, NB = BX * BY * 3;
+  const long HW = (long)H * W, plane = m0 / HW;  // the tile's (sample, z) plane
+  const int zt = (int)(plane % D), y0 = (int)((m0 % HW) / W);
+  const int K = 27 * Cin;
+  int vb[WM];  // box index of each fragment's voxel (box coordinates x + 1, row + 1, 1)
+#pragma unroll
+  for (int i = 0; i < WM; ++i) {
+    const int lm = wave * 64 + i * 16 + col;
+    vb[i] = (BY + lm / W + 1) * BX + lm % W + 1;
+  }
+  f32x4 acc[WN][WM];
+#pragma unroll
+  for (int j = 0; j < WN; ++j)
+#pragma unroll
+    for (int i = 0; i < WM; ++i) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // the weight tile of a chunk (WN x 16 output channels x 32 slots), loaded
+  // once per block into double-buffered LDS, 16 bytes per thread: row
+  // tid >> 2, slot group tid & 3 (the same slot assignment as the fragments)
+  constexpr int kPitch = 40, kPieces = WN * 16 * 4;
+  __shared__ __attribute__((aligned(16))) uint16_t sw[2][WN * 16 * kPitch];
+  const int tid = threadIdx.x, prow = tid >> 2, pq = tid & 3;
+  u32x4 wpiece = {0u, 0u, 0u, 0u};
+  auto load_w = [&](int p, int c0) {
+    if (tid < kPieces) {
+      const int tap = TPC * p + pq / (4 / TPC), ci = c0 + 8 * (pq % (CB / 8)), n = n0 + prow;
+      wpiece = (tap < 27 && n < Cout) ? *(const u32x4 *)(w + (long)n * K + tap * Cin + ci) : u32x4{0u, 0u, 0u, 0u};
+    }
+  };
+  auto store_w = [&](int buf) {
+    if (tid < kPieces) *(u32x4 *)(&sw[buf][prow * kPitch + pq * 8]) = wpiece;
+  };
+  for (int c0 = 0; c0 < Cin; c0 += CB) {
+    __syncthreads();  // the previous channel block's readers are done
+    for (int e = threadIdx.x; e < (CB / 8) * NB; e += 256) {
+      const int vi = e / (CB / 8), half = e % (CB / 8);
+      const int bx = vi % BX, by = (vi / BX) % BY, bz = vi / (BX * BY);
+      const int gx = bx - 1, gy = y0 + by - 1, gz = zt + bz - 1;
+      u32x4 v = {0u, 0u, 0u, 0u};
+      if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
+        v = *(const u32x4 *)(x + ((plane - zt + gz) * HW + (long)gy * W + gx) * Cin + c0 + 8 * half);
+      *(u32x4 *)(xs + vi * kBoxPitch + 8 * half) = v;
+    }
+    load_w(0, c0);
+    store_w(0);
+    __syncthreads();
+#pragma unroll 2
+    for (int p = 0; p < NCH; ++p) {
+      const int buf = p & 1;
+      if (p + 1 < NCH) load_w(p + 1, c0);
+      const int tap = TPC * p + kq / (4 / TPC);
+      bf16x8 b[WM];
+      if (tap < 27) {
+        const int off = ((tap / 9 - 1) * BY + (tap / 3) % 3 - 1) * BX + tap % 3 - 1;
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+          b[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(xs + (vb[i] + off) * kBoxPitch + 8 * (kq % (CB / 8))));
+      } else {
+#pragma unroll
+        for (int i = 0; i < WM; ++i) b[i] = zero8();
+      }
+#pragma unroll
+      for (int j = 0; j < WN; ++j) {
+        const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4 *)(&sw[buf][(j * 16 + col) * kPitch + (kq << 3)]));
+#pragma unroll
+        for (int i = 0; i < WM; ++i) acc[j][i] = mfma16(a, b[i], acc[j][i]);
+      }
+      if (p + 1 < NCH) store_w(buf ^ 1);
+      __syncthreads();
+    }
+  }
+  // epilogue: lane holds channels n0 + 16 j + 4 kq + r of voxel m0 + 64 wave + 16 i + col
+  const long mw = m0 + wave * 64;
+#pragma unroll
+  for (int j = 0; j < WN; ++j) {
+    const int n = n0 + j * 16 + 4 * kq;
+    if (n >= Cout) continue;
+    float bs[4] = {0.f, 0.f, 0.f, 0.f};
+    if (bias)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) bs[r] = bias[n + r];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      const long m = mw + i * 16 + col;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        v[r] = acc[j][i][r] + bs[r];
+        if (relu) v[r] = fmaxf(v[r], 0.f);
+      }
+      if (mask) {
+        const u32x2 mk = *(const u32x2 *)(mask + m * Cout + n);
+        const uint16_t mb[4] = {(uint16_t)(mk[0] & 0xffff), (uint16_t)(mk[0] >> 16), (uint16_t)(mk[1] & 0xffff),
+                                (uint16_t)(mk[1] >> 16)};
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (!(bf2f(mb[r]) > 0.f)) v[r] = 0.f;
+      }
+      u32x2 o;
+      o[0] = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+      o[1] = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+      *(u32x2 *)(y + m * Cout + n) = o;
+    }
+  }
+}
+
+template <int WN>
+int launch_fwd_box(const void *x, const void *w, const float *bias, const void *mask, void *y, int D, int H, int W,
+                   long M, int Cin, int Cout, int relu, hipStream_t s) {
+  const dim3 grid((unsigned)(M / 256), (unsigned)((Cout + WN * 16 - 1) / (WN * 16)));
+  if (Cin >= 64) {  // fewer, longer channel blocks pay once the input is wide
+    const size_t shm = sizeof(uint16_t) * box_pitch(16) * box_voxels(W);
+    k_conv_fwd_box<WN, 16><<<grid, 256, shm, s>>>((const uint16_t *)x, (const uint16_t *)w, bias,
+                                                  (const uint16_t *)mask, (uint16_t *)y, D, H, W, M, Cin, Cout, relu);
+  } else {
+    const size_t shm = sizeof(uint16_t) * box_pitch(8) * box_voxels(W);
+    k_conv_fwd_box<WN, 8><<<grid, 256, shm, s>>>((const uint16_t *)x, (const uint16_t *)w, bias,
+                                                 (const uint16_t *)mask, (uint16_t *)y, D, H, W, M, Cin, Cout, relu);
+  }
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
 // y[v][n] = act(sum over z < ns of part[z][v][n] + bias[n]) [* (mask > 0)]:
 // one thread per (voxel, 4 channels)
 __global__ void k_conv_reduce(const float *__restrict__ part, int ns, const float *__restrict__ bias,
@@ -297,6 +453,11 @@ int conv_fwd(const void *x, const void *w, const float *bias, const void *mask, 
   const int wn = Cout % 64 == 0 ? 4 : (Cout % 32 == 0 ? 2 : 1);
   const long blocks64 = (M + 255) / 256 * (Cout / (16 * wn));
   if (blocks64 >= 512) {
+    if (box_conv_ok(H, W, M)) {  // the input box staged in LDS
+      if (wn == 4) return launch_fwd_box<4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+      if (wn == 2) return launch_fwd_box<2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+      return launch_fwd_box<1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+    }
     if (wn == 4) return launch_fwd<4, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
     if (wn == 2) return launch_fwd<4, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
     return launch_fwd<4, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
