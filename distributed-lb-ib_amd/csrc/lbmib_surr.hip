@@ -370,6 +370,143 @@ int launch_fwd_box(const void *x, const void *w, const float *bias, const void *
   return LBMIB_OK;
 }
 
+// Small-grid forward convolution: the KW waves of a block all own the same
+// WM x 16 voxels x WN x 16 output channels and split the reduction between
+// them -- wave w multiplies chunks w, w + KW, w + 2 KW, ... of 32 k.  Both
+// operands come straight from global memory (L1/L2) into a ring of P
+// register stages, so that P chunks' loads are in flight while one
+// multiplies, and the loop has no barrier.  The loads are raw buffer loads
+// whose resources end at the tensors' ends: a lane with nothing to load
+// (a neighbour outside the volume, k past 27 Cin) points its offset past
+// the end and the hardware returns zeros, so the loads carry no branch and
+// the waits stay per stage.  At the end the KW partial sums
+// meet in LDS and are added in wave order (a fixed order: the result does
+// not depend on timing), then bias, activation and mask as in k_conv_fwd.
+// One launch per layer, where a split over blocks needs a second pass over
+// FP32 partials in memory.
+template <int WM, int WN, int KW, int P>
+__global__ void __launch_bounds__(KW * 64) k_conv_fwd_kw(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
+                                                         const float *__restrict__ bias,
+                                                         const uint16_t *__restrict__ mask, uint16_t *__restrict__ y,
+                                                         int D, int H, int W, long M, int Cin, int lgCin, int Cout,
+                                                         int K, int relu) {
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col = lane & 15, kq = lane >> 4;
+  const long mb = (long)blockIdx.x * (WM * 16);
+  const int n0 = blockIdx.y * (WN * 16);
+  const long HW = (long)H * W;
+  int vz[WM], vy[WM], vx[WM];
+  bool vin[WM];
+#pragma unroll
+  for (int i = 0; i < WM; ++i) {
+    const long m = mb + i * 16 + col;
+    vin[i] = m < M;
+    long r = vin[i] ? m : 0;
+    vx[i] = (int)(r % W);
+    r /= W;
+    vy[i] = (int)(r % H);
+    r /= H;
+    vz[i] = (int)(r % D);
+  }
+  f32x4 acc[WN][WM];
+#pragma unroll
+  for (int j = 0; j < WN; ++j)
+#pragma unroll
+    for (int i = 0; i < WM; ++i) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nch = (K + 31) >> 5;
+  const int nw = nch > wave ? (nch - wave + KW - 1) / KW : 0;  // this wave's chunks
+  const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(w), (short)0,
+                                                                      Cout * K * 2, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(x), (short)0,
+                                                                      (int)(M * Cin * 2), 0x00020000);
+  constexpr uint32_t kNone = 0x80000000u;  // past the end of both resources: loads zeros
+  bf16x8 a[P][WN], b[P][WM];
+  // stage s <- the wave's t-th chunk; slot 8 kq + e of a fragment is k = 32 c + 8 kq + e
+  auto load = [&](int s, int t) {
+    const int k = ((wave + t * KW) << 5) + (kq << 3);
+    const bool kin = k < K;
+    // k past 27 Cin needs no test on this side: the input slot is zero
+    // there (the weights are finite), and past the end of w the load is zero
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const uint32_t off = (uint32_t)(((n0 + j * 16 + col) * K + k) * 2);
+      a[s][j] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rw, off, 0, 0));
+    }
+    const int tap = k >> lgCin, ci = k & (Cin - 1);
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      const int zz = vz[i] + dz, yy = vy[i] + dy, xx = vx[i] + dx;
+      const bool ok = kin && vin[i] && (unsigned)zz < (unsigned)D && (unsigned)yy < (unsigned)H &&
+                      (unsigned)xx < (unsigned)W;
+      const int m = (int)mb + i * 16 + col;
+      const uint32_t off = ok ? (uint32_t)(((m + dz * (int)HW + dy * W + dx) * Cin + ci) * 2) : kNone;
+      b[s][i] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rx, off, 0, 0));
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < P; ++s) load(s, s);
+  for (int t0 = 0; t0 < nw; t0 += P) {
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      if (t0 + s < nw) {
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+#pragma unroll
+          for (int i = 0; i < WM; ++i) acc[j][i] = mfma16(a[s][j], b[s][i], acc[j][i]);
+      }
+      load(s, t0 + s + P);
+    }
+  }
+  // the waves' partial sums, fragment f = j WM + i, lane l
+  __shared__ f32x4 red[KW][WN * WM][64];
+#pragma unroll
+  for (int j = 0; j < WN; ++j)
+#pragma unroll
+    for (int i = 0; i < WM; ++i) red[wave][j * WM + i][lane] = acc[j][i];
+  __syncthreads();
+  for (int e = threadIdx.x; e < WN * WM * 64; e += KW * 64) {
+    const int f = e >> 6, l = e & 63;
+    const int j = f / WM, i = f % WM;
+    f32x4 sum = red[0][f][l];
+#pragma unroll
+    for (int v = 1; v < KW; ++v) sum += red[v][f][l];
+    // lane l of fragment (j, i): channels n0 + 16 j + 4 (l >> 4) + r of voxel mb + 16 i + (l & 15)
+    const int n = n0 + j * 16 + 4 * (l >> 4);
+    const long m = mb + i * 16 + (l & 15);
+    if (n >= Cout || m >= M) continue;
+    float v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = sum[r] + (bias ? bias[n + r] : 0.f);
+      if (relu) v[r] = fmaxf(v[r], 0.f);
+    }
+    if (mask) {
+      const u32x2 mk = *(const u32x2 *)(mask + m * Cout + n);
+      const uint16_t mbits[4] = {(uint16_t)(mk[0] & 0xffff), (uint16_t)(mk[0] >> 16), (uint16_t)(mk[1] & 0xffff),
+                                 (uint16_t)(mk[1] >> 16)};
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (!(bf2f(mbits[r]) > 0.f)) v[r] = 0.f;
+    }
+    u32x2 o;
+    o[0] = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+    o[1] = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+    *(u32x2 *)(y + m * Cout + n) = o;
+  }
+}
+
+template <int WM, int WN, int KW, int P>
+int launch_fwd_kw(const void *x, const void *w, const float *bias, const void *mask, void *y, int D, int H, int W,
+                  long M, int Cin, int Cout, int relu, hipStream_t s) {
+  const dim3 grid((unsigned)((M + WM * 16 - 1) / (WM * 16)), (unsigned)((Cout + WN * 16 - 1) / (WN * 16)));
+  k_conv_fwd_kw<WM, WN, KW, P><<<grid, KW * 64, 0, s>>>((const uint16_t *)x, (const uint16_t *)w, bias,
+                                                        (const uint16_t *)mask, (uint16_t *)y, D, H, W, M, Cin,
+                                                        ilog2(Cin), Cout, 27 * Cin, relu);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
 // y[v][n] = act(sum over z < ns of part[z][v][n] + bias[n]) [* (mask > 0)]:
 // one thread per (voxel, 4 channels)
 __global__ void k_conv_reduce(const float *__restrict__ part, int ns, const float *__restrict__ bias,
@@ -461,6 +598,15 @@ int conv_fwd(const void *x, const void *w, const float *bias, const void *mask, 
     if (wn == 4) return launch_fwd<4, 4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
     if (wn == 2) return launch_fwd<4, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
     return launch_fwd<4, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
+  }
+  // small grids whose weights stay modest: the two waves of a block split
+  // the reduction of one 32-voxel tile, one launch and no partial sums in
+  // memory; the weights are read once per tile, at most 128 MB in all
+  const double kw_bytes = (double)((M + 31) / 32) * Cout * 27 * Cin * 2;
+  if (M >= 4096 && kw_bytes <= 128e6 && M * Cin * 2 < (1L << 31)) {
+    if (wn == 4) return launch_fwd_kw<2, 4, 2, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+    if (wn == 2) return launch_fwd_kw<2, 2, 2, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
+    return launch_fwd_kw<2, 1, 2, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
   }
   // small grids: split the reduction so that about 1024 blocks run, each
   // with at least 4 chunks of 32, within the workspace

@@ -78,11 +78,15 @@ CONV_CASES = [
     (1, 4, 8, 8, 64, 128), (2, 4, 4, 4, 128, 128), (1, 4, 4, 8, 128, 256), (1, 2, 4, 4, 256, 256),
     (1, 4, 4, 4, 256, 128), (1, 8, 8, 8, 128, 64), (1, 8, 16, 8, 64, 32), (1, 8, 8, 8, 32, 16),
     (3, 5, 7, 3, 16, 48), (1, 64, 16, 16, 32, 32),
-    (1, 32, 64, 64, 32, 32),  # 131072 voxels: the unsplit four-fragment path
+    (1, 16, 16, 16, 16, 32),  # two waves per tile splitting k, the last chunk half past 27 Cin
+    (1, 32, 64, 64, 32, 32),  # 131072 voxels: the input box in LDS, 8-channel blocks
 ]
+# 65536 voxels x 64 -> 128 channels: the input box with 16-channel blocks
+# (forward only: its oracle takes seconds per convolution)
+WIDE_BOX_CASE = (1, 32, 32, 64, 64, 128)
 
 
-@pytest.mark.parametrize("case", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
+@pytest.mark.parametrize("case", CONV_CASES + [WIDE_BOX_CASE], ids=lambda c: "x".join(map(str, c)))
 def test_conv3d_fwd_matches_oracle(sg, case):
     B, D, H, W, ci, co = case
     print(f"conv fwd {case}", flush=True)
@@ -102,8 +106,10 @@ def test_conv3d_fwd_matches_oracle(sg, case):
         assert_bf16_close(host_bf16(y), ref, absum, f"fwd relu={relu}")
 
 
-def test_conv3d_fwd_mask(sg):
-    B, D, H, W, ci, co = 1, 4, 8, 8, 32, 64
+@pytest.mark.parametrize("case", [(1, 4, 8, 8, 32, 64), (1, 16, 16, 16, 32, 64), (1, 32, 64, 64, 32, 32)],
+                         ids=["split", "wave-split", "box"])
+def test_conv3d_fwd_mask(sg, case):
+    B, D, H, W, ci, co = case
     rng = np.random.default_rng(1)
     x = so.bf16_round(rng.standard_normal((B, D, H, W, ci)))
     w = so.bf16_round(rng.standard_normal((co, 27, ci)) / 30)
