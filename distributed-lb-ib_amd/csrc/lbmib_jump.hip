@@ -258,6 +258,9 @@ __global__ void __launch_bounds__(512)
     const bool live = y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2];
     const double fy = floor((y - hc) / F), fz = floor((z - hc) / F);
     const double wy = (y - hc) / F - fy, wz = (z - hc) / F - fz;
+    // the y-z factors of the trilinear weights (exact in FP32)
+    const float wyz[2][2] = {{(float)((1.0 - wy) * (1.0 - wz)), (float)(wy * (1.0 - wz))},
+                             {(float)((1.0 - wy) * wz), (float)(wy * wz)}};
     for (int xb = 0; xb < g.nx; xb += blockDim.x) {
       const int cx0 = xb / F - 1;
       __syncthreads();  // the previous chunk's readers are done
@@ -296,22 +299,66 @@ __global__ void __launch_bounds__(512)
         // trilinear weights once for the four planes (the clamping of
         // interp(): a cell left of the first coarse centre takes the edge
         // value, which the clamped staging reproduces)
-        const double fx = floor((x - hc) / F);
-        const double wx = (x - hc) / F - fx;
-        const int ix = (int)fx - cx0;
-        double dd[4] = {0.0, 0.0, 0.0, 0.0};
+        T d0, d1, d2, d3;
+        if constexpr (P == LBMIB_F64) {
+          const double fx = floor((x - hc) / F);
+          const double wx = (x - hc) / F - fx;
+          const int ix = (int)fx - cx0;
+          double dd[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-          const double w = (dx ? wx : 1.0 - wx) * (dy ? wy : 1.0 - wy) * (dz ? wz : 1.0 - wz);
+          for (int c = 0; c < 8; ++c) {
+            const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+            const double w = (dx ? wx : 1.0 - wx) * (dy ? wy : 1.0 - wy) * (dz ? wz : 1.0 - wz);
 #pragma unroll
-          for (int a = 0; a < 4; ++a) dd[a] += w * (double)sd[a][dz][dy][ix + dx];
-        }
-        const T d0 = (T)dd[0], d1 = (T)dd[1], d2 = (T)dd[2], d3 = (T)dd[3];
+            for (int a = 0; a < 4; ++a) dd[a] += w * (double)sd[a][dz][dy][ix + dx];
+          }
+          d0 = dd[0], d1 = dd[1], d2 = dd[2], d3 = dd[3];
 #pragma unroll
-        for (int q = 0; q < 19; ++q) {
-          const T d = dfeq_q<T>(q, rho, d0, ux, uy, uz, d1, d2, d3);
-          store(p0, q, to_bits<P>(from_bits<P>(h[q]) + (typename Traits<P>::T)(d * (T)Traits<P>::scale)));
+          for (int q = 0; q < 19; ++q) {
+            const T d = dfeq_q<T>(q, rho, d0, ux, uy, uz, d1, d2, d3);
+            store(p0, q, to_bits<P>(from_bits<P>(h[q]) + (typename Traits<P>::T)(d * (T)Traits<P>::scale)));
+          }
+        } else {
+          // FP32 throughout: the weights are multiples of 1 / (2F) with a
+          // few significant bits, exact in FP32, as are their products
+          const float v = ((float)x - (float)hc) * (1.0f / F), fx = floorf(v), wx = v - fx;
+          const int ix = (int)fx - cx0;
+          float dd[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+            const float w = (dx ? wx : 1.0f - wx) * wyz[dz][dy];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) dd[a] = fmaf(w, sd[a][dz][dy][ix + dx], dd[a]);
+          }
+          d0 = dd[0], d1 = dd[1], d2 = dd[2], d3 = dd[3];
+          // the shift s w_q [drho a_q + rho' b_q] with the per-cell terms
+          // taken out:
+          //   drho a_q + rho' b_q = A0 + 3 p_q + 4.5 r_q,
+          //   A0 = drho (1 - 1.5 u^2) - 1.5 rho' du.(u + u'),
+          //   p_q = drho cu + rho' cd,  r_q = drho cu^2 + rho' cd (2 cu + cd);
+          // the opposite direction (c -> -c) has the same r and -p, so each
+          // pair (2k - 1, 2k) is evaluated once
+          const float rp = rho + d0;
+          const float usq = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+          const float ud = fmaf(d1, ux + ux + d1, fmaf(d2, uy + uy + d2, d3 * (uz + uz + d3)));
+          const float A0 = fmaf(-1.5f, rp * ud, d0 * fmaf(-1.5f, usq, 1.0f));
+          constexpr float sc = Traits<P>::scale;
+          const float wA[3] = {A0 * (sc / 3.0f), A0 * (sc / 18.0f), A0 * (sc / 36.0f)};
+          store(p0, 0, to_bits<P>((typename Traits<P>::T)((float)from_bits<P>(h[0]) + wA[0])));
+#pragma unroll
+          for (int q = 1; q < 19; q += 2) {
+            const float cu = add_c(add_c(add_c(0.0f, kCX[q], ux), kCY[q], uy), kCZ[q], uz);
+            const float cd = add_c(add_c(add_c(0.0f, kCX[q], d1), kCY[q], d2), kCZ[q], d3);
+            const float dcu = d0 * cu;
+            const float pq = fmaf(rp, cd, dcu);
+            const float r = fmaf(rp * cd, fmaf(2.0f, cu, cd), dcu * cu);
+            const float wq = sc / (float)kWDEN[q];  // folded: q is unrolled
+            const float base = fmaf(4.5f * wq, r, wA[kCLS[q]]);
+            const float s1 = fmaf(3.0f * wq, pq, base), s2 = fmaf(-3.0f * wq, pq, base);
+            store(p0, q, to_bits<P>((typename Traits<P>::T)((float)from_bits<P>(h[q]) + s1)));
+            store(p0, q + 1, to_bits<P>((typename Traits<P>::T)((float)from_bits<P>(h[q + 1]) + s2)));
+          }
         }
         m[0] = (float)(rho + d0);
         m[1] = (float)(ux + d1);
