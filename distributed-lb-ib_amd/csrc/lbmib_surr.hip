@@ -216,35 +216,36 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t *__restrict__ x
 
 // Large-grid forward convolution with the input staged in LDS.  A block owns
 // 256 consecutive voxels in linear order -- R = 256 / W whole x-rows of one
-// z-plane of one sample (W divides 256 and R divides H) -- and WN x 16 output
-// channels; its 4 waves own 64 voxels each (4 fragments of 16).  The input
-// box around the tile, (W + 2) x (R + 2) x 3 voxels with zeros outside the
-// volume, is staged CB channels at a time, and all 27 taps read it there, so
-// each input voxel is fetched from memory once per block and channel block
-// instead of once per tap.  CB = 8: rows of 8 elements (16 bytes), 19 KB for
-// W = 64, so that 5 blocks fit a CU; the reduction runs over chunks of four
-// taps x 8 channels, slot 8 kq + e of a chunk (kq = lane >> 4) being tap
-// 4p + kq, channel c0 + e.  CB = 16, for wide inputs: rows of 24 elements
-// (48-byte steps put the 16-byte reads of 8 lanes on distinct banks), chunks
-// of two taps x 16 channels, slot 8 kq + e being tap 2p + (kq >> 1), channel
-// c0 + 8 (kq & 1) + e.  The slot order is the same in both operands; taps
-// past the 27th are zero.  Each chunk's weight tile is staged in
-// double-buffered LDS as in k_conv_fwd, the next chunk's loaded while the
-// current one multiplies.
-__host__ __device__ constexpr int box_pitch(int CB) { return CB == 8 ? 8 : 24; }
+// z-plane of one sample (W divides 256, R divides H, W <= 64) -- and WN x 16
+// output channels; its 4 waves own 64 voxels each (4 fragments of 16).  Per
+// block of 8 input channels it stages in LDS the input box around the tile,
+// (W + 2) x (R + 2) x 3 voxels with zeros outside the volume (16 bytes a
+// voxel), and the weights of its output channels for all 27 taps, so each
+// input voxel is fetched from memory once per block and channel block
+// instead of once per tap, and the 7 chunks of the channel block (four taps
+// x 8 channels each: slot 8 kq + e of chunk p, kq = lane >> 4, is tap
+// 4p + kq, channel c0 + e, in both operands; a 28th tap is zero) read both
+// operands from LDS without a barrier.  The next channel block's box and
+// weights are loaded into registers while the current one multiplies
+// (buffer loads: a piece with nothing to load points past the end of its
+// tensor and reads zeros, so the loads carry no branch and no early wait).
+constexpr int kBoxPieces = 5;  // ceil(box voxels / 256) for W = 16, 32, 64
+constexpr int kWRow = 28 * 8 + 8;  // weight row pitch: 8 rows' 16-byte reads on distinct banks
 __host__ __device__ inline int box_voxels(int W) { return (W + 2) * (256 / W + 2) * 3; }
-inline bool box_conv_ok(int H, int W, long M) {
-  return W >= 16 && 256 % W == 0 && H % (256 / W) == 0 && M % 256 == 0;
+inline bool box_conv_ok(int H, int W, long M, int Cin, int Cout) {
+  return W >= 16 && W <= 64 && 256 % W == 0 && H % (256 / W) == 0 && M % 256 == 0 &&
+         M * Cin * 2 < (1L << 31) && (long)Cout * 27 * Cin * 2 < (1L << 30);
 }
 
-template <int WN, int CB>
+template <int WN>
 __global__ void __launch_bounds__(256) k_conv_fwd_box(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
                                                       const float *__restrict__ bias,
                                                       const uint16_t *__restrict__ mask, uint16_t *__restrict__ y,
                                                       int D, int H, int W, long M, int Cin, int Cout, int relu) {
-  constexpr int WM = 4, kBoxPitch = box_pitch(CB), TPC = 32 / CB, NCH = (27 + TPC - 1) / TPC;
-  extern __shared__ __attribute__((aligned(16))) uint16_t xs[];  // [box voxel][kBoxPitch]
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int WM = 4, kWPieces = (WN * 16 * 28 + 255) / 256;
+  // [box voxel][8 channels], then the weights [WN x 16 rows][kWRow]
+  extern __shared__ __attribute__((aligned(16))) uint16_t xs[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int col = lane & 15, kq = lane >> 4;
   const long m0 = (long)blockIdx.x * 256;
   const int n0 = blockIdx.y * (WN * 16);
@@ -252,6 +253,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd_box(const uint16_t *__restrict
   const long HW = (long)H * W, plane = m0 / HW;  // the tile's (sample, z) plane
   const int zt = (int)(plane % D), y0 = (int)((m0 % HW) / W);
   const int K = 27 * Cin;
+  uint16_t *ws = xs + NB * 8;
   int vb[WM];  // box index of each fragment's voxel (box coordinates x + 1, row + 1, 1)
 #pragma unroll
   for (int i = 0; i < WM; ++i) {
@@ -263,59 +265,72 @@ __global__ void __launch_bounds__(256) k_conv_fwd_box(const uint16_t *__restrict
   for (int j = 0; j < WN; ++j)
 #pragma unroll
     for (int i = 0; i < WM; ++i) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // the weight tile of a chunk (WN x 16 output channels x 32 slots), loaded
-  // once per block into double-buffered LDS, 16 bytes per thread: row
-  // tid >> 2, slot group tid & 3 (the same slot assignment as the fragments)
-  constexpr int kPitch = 40, kPieces = WN * 16 * 4;
-  __shared__ __attribute__((aligned(16))) uint16_t sw[2][WN * 16 * kPitch];
-  const int tid = threadIdx.x, prow = tid >> 2, pq = tid & 3;
-  u32x4 wpiece = {0u, 0u, 0u, 0u};
-  auto load_w = [&](int p, int c0) {
-    if (tid < kPieces) {
-      const int tap = TPC * p + pq / (4 / TPC), ci = c0 + 8 * (pq % (CB / 8)), n = n0 + prow;
-      wpiece = (tap < 27 && n < Cout) ? *(const u32x4 *)(w + (long)n * K + tap * Cin + ci) : u32x4{0u, 0u, 0u, 0u};
+  const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(x), (short)0,
+                                                                      (int)(M * Cin * 2), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(w), (short)0,
+                                                                      Cout * K * 2, 0x00020000);
+  constexpr uint32_t kNone = 0x80000000u;  // past the end of both resources: loads zeros
+  // this thread's pieces: byte offsets relative to the channel block (or kNone)
+  uint32_t xo[kBoxPieces], wo[kWPieces];
+#pragma unroll
+  for (int r = 0; r < kBoxPieces; ++r) {
+    const int vi = tid + 256 * r;
+    const int bx = vi % BX, by = (vi / BX) % BY, bz = vi / (BX * BY);
+    const int gx = bx - 1, gy = y0 + by - 1, gz = zt + bz - 1;
+    const bool in = vi < NB && (unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D;
+    xo[r] = in ? (uint32_t)((((plane - zt + gz) * HW + (long)gy * W + gx) * Cin) * 2) : kNone;
+  }
+#pragma unroll
+  for (int r = 0; r < kWPieces; ++r) {
+    const int e = tid + 256 * r, row = e / 28, tap = e % 28, n = n0 + row;
+    wo[r] = (row < WN * 16 && tap < 27 && n < Cout) ? (uint32_t)((n * K + tap * Cin) * 2) : kNone;
+  }
+  u32x4 px[kBoxPieces], pw[kWPieces];
+  auto fetch = [&](int c0) {  // the channel block's pieces into registers
+    const uint32_t cb = (uint32_t)(c0 * 2);
+#pragma unroll
+    for (int r = 0; r < kBoxPieces; ++r)
+      px[r] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rx, xo[r] | 0u, cb, 0));
+#pragma unroll
+    for (int r = 0; r < kWPieces; ++r)
+      pw[r] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rw, wo[r], cb, 0));
+  };
+  auto put = [&]() {  // registers into LDS
+#pragma unroll
+    for (int r = 0; r < kBoxPieces; ++r) {
+      const int vi = tid + 256 * r;
+      if (vi < NB) *(u32x4 *)(xs + vi * 8) = px[r];
+    }
+#pragma unroll
+    for (int r = 0; r < kWPieces; ++r) {
+      const int e = tid + 256 * r;
+      if (e < WN * 16 * 28) *(u32x4 *)(ws + (e / 28) * kWRow + (e % 28) * 8) = pw[r];
     }
   };
-  auto store_w = [&](int buf) {
-    if (tid < kPieces) *(u32x4 *)(&sw[buf][prow * kPitch + pq * 8]) = wpiece;
-  };
-  for (int c0 = 0; c0 < Cin; c0 += CB) {
+  fetch(0);
+  for (int c0 = 0; c0 < Cin; c0 += 8) {
     __syncthreads();  // the previous channel block's readers are done
-    for (int e = threadIdx.x; e < (CB / 8) * NB; e += 256) {
-      const int vi = e / (CB / 8), half = e % (CB / 8);
-      const int bx = vi % BX, by = (vi / BX) % BY, bz = vi / (BX * BY);
-      const int gx = bx - 1, gy = y0 + by - 1, gz = zt + bz - 1;
-      u32x4 v = {0u, 0u, 0u, 0u};
-      if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
-        v = *(const u32x4 *)(x + ((plane - zt + gz) * HW + (long)gy * W + gx) * Cin + c0 + 8 * half);
-      *(u32x4 *)(xs + vi * kBoxPitch + 8 * half) = v;
-    }
-    load_w(0, c0);
-    store_w(0);
+    put();
     __syncthreads();
+    if (c0 + 8 < Cin) fetch(c0 + 8);  // in flight while this block multiplies
 #pragma unroll 2
-    for (int p = 0; p < NCH; ++p) {
-      const int buf = p & 1;
-      if (p + 1 < NCH) load_w(p + 1, c0);
-      const int tap = TPC * p + kq / (4 / TPC);
+    for (int p = 0; p < 7; ++p) {
+      const int tap = 4 * p + kq;
       bf16x8 b[WM];
       if (tap < 27) {
         const int off = ((tap / 9 - 1) * BY + (tap / 3) % 3 - 1) * BX + tap % 3 - 1;
 #pragma unroll
-        for (int i = 0; i < WM; ++i)
-          b[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(xs + (vb[i] + off) * kBoxPitch + 8 * (kq % (CB / 8))));
+        for (int i = 0; i < WM; ++i) b[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(xs + (vb[i] + off) * 8));
       } else {
 #pragma unroll
         for (int i = 0; i < WM; ++i) b[i] = zero8();
       }
 #pragma unroll
       for (int j = 0; j < WN; ++j) {
-        const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4 *)(&sw[buf][(j * 16 + col) * kPitch + (kq << 3)]));
+        const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4 *)(ws + (j * 16 + col) * kWRow + 32 * p + 8 * kq));
 #pragma unroll
         for (int i = 0; i < WM; ++i) acc[j][i] = mfma16(a, b[i], acc[j][i]);
       }
-      if (p + 1 < NCH) store_w(buf ^ 1);
-      __syncthreads();
     }
   }
   // epilogue: lane holds channels n0 + 16 j + 4 kq + r of voxel m0 + 64 wave + 16 i + col
@@ -357,15 +372,9 @@ template <int WN>
 int launch_fwd_box(const void *x, const void *w, const float *bias, const void *mask, void *y, int D, int H, int W,
                    long M, int Cin, int Cout, int relu, hipStream_t s) {
   const dim3 grid((unsigned)(M / 256), (unsigned)((Cout + WN * 16 - 1) / (WN * 16)));
-  if (Cin >= 64) {  // fewer, longer channel blocks pay once the input is wide
-    const size_t shm = sizeof(uint16_t) * box_pitch(16) * box_voxels(W);
-    k_conv_fwd_box<WN, 16><<<grid, 256, shm, s>>>((const uint16_t *)x, (const uint16_t *)w, bias,
-                                                  (const uint16_t *)mask, (uint16_t *)y, D, H, W, M, Cin, Cout, relu);
-  } else {
-    const size_t shm = sizeof(uint16_t) * box_pitch(8) * box_voxels(W);
-    k_conv_fwd_box<WN, 8><<<grid, 256, shm, s>>>((const uint16_t *)x, (const uint16_t *)w, bias,
-                                                 (const uint16_t *)mask, (uint16_t *)y, D, H, W, M, Cin, Cout, relu);
-  }
+  const size_t shm = sizeof(uint16_t) * (8 * box_voxels(W) + WN * 16 * kWRow);
+  k_conv_fwd_box<WN><<<grid, 256, shm, s>>>((const uint16_t *)x, (const uint16_t *)w, bias, (const uint16_t *)mask,
+                                            (uint16_t *)y, D, H, W, M, Cin, Cout, relu);
   HIPCHK(hipGetLastError());
   return LBMIB_OK;
 }
@@ -590,7 +599,7 @@ int conv_fwd(const void *x, const void *w, const float *bias, const void *mask, 
   const int wn = Cout % 64 == 0 ? 4 : (Cout % 32 == 0 ? 2 : 1);
   const long blocks64 = (M + 255) / 256 * (Cout / (16 * wn));
   if (blocks64 >= 512) {
-    if (box_conv_ok(H, W, M)) {  // the input box staged in LDS
+    if (box_conv_ok(H, W, M, Cin, Cout)) {  // the input box staged in LDS
       if (wn == 4) return launch_fwd_box<4>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
       if (wn == 2) return launch_fwd_box<2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
       return launch_fwd_box<1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);

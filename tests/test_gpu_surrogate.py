@@ -79,10 +79,10 @@ CONV_CASES = [
     (1, 4, 4, 4, 256, 128), (1, 8, 8, 8, 128, 64), (1, 8, 16, 8, 64, 32), (1, 8, 8, 8, 32, 16),
     (3, 5, 7, 3, 16, 48), (1, 64, 16, 16, 32, 32),
     (1, 16, 16, 16, 16, 32),  # two waves per tile splitting k, the last chunk half past 27 Cin
-    (1, 32, 64, 64, 32, 32),  # 131072 voxels: the input box in LDS, 8-channel blocks
+    (1, 32, 64, 64, 32, 32),  # 131072 voxels: the input box in LDS
 ]
-# 65536 voxels x 64 -> 128 channels: the input box with 16-channel blocks
-# (forward only: its oracle takes seconds per convolution)
+# 65536 voxels x 64 -> 128 channels: the input box with eight channel blocks
+# and 64-channel tiles (forward only: its oracle takes seconds per convolution)
 WIDE_BOX_CASE = (1, 32, 32, 64, 64, 128)
 
 
