@@ -160,8 +160,11 @@ __device__ __forceinline__ void fiber_body(const IBState &ib, const Geometry &g,
     if (live && role > 0) {
       Xd = pos[(long)d * M + n];
       const int dim = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
-      sdv = (int)(floor(Xd / 1.0 - 2) + 1);
-      okd = sdv >= 0 && sdv + 3 < dim;
+      // tested in FP64 before the conversion: a NaN or a value beyond the
+      // int range (which the conversion would turn into 0 or INT_MAX) fails
+      const double sdd = floor(Xd / 1.0 - 2) + 1;
+      okd = sdd >= 0.0 && sdd + 3.0 < (double)dim;
+      if (okd) sdv = (int)sdd;
     }
     const int q0 = (threadIdx.x & 63) & ~3;  // the quad's lane 0 in the wave
     for (int a = 0; a < 3; ++a) st[a] = __shfl(sdv, q0 + 1 + a, 64);

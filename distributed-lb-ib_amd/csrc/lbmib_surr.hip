@@ -902,6 +902,27 @@ int workspace(void **out) {
   return LBMIB_OK;
 }
 
+// lbmib_sg_sqdiff's block partials (two doubles for each of at most
+// kSqBlocks blocks): a buffer of their own, one per device, so that the
+// sums a caller takes on one stream never share memory with a network
+// running on another stream through the workspace.
+constexpr int kSqBlocks = 1024;
+void *g_sq[kMaxDevices] = {};
+
+int sq_partials(void **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return LBMIB_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  if (!g_sq[dev]) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * 2 * kSqBlocks) != hipSuccess) return LBMIB_ERR_ALLOC;
+    g_sq[dev] = p;
+  }
+  *out = g_sq[dev];
+  return LBMIB_OK;
+}
+
 // ------------------------------------------------------------ elementwise
 //
 // One thread per (voxel, 8-channel piece); C is a multiple of 16.
@@ -1358,10 +1379,10 @@ extern "C" int lbmib_sg_unpack(const void *out, int C, int nl, const double *mea
 extern "C" int lbmib_sg_sqdiff(const float *a, const float *b, long n, double *out2, void *stream) {
   if (!a || !out2 || n < 1) return LBMIB_ERR_ARG;
   void *ws = nullptr;
-  const int rc = workspace(&ws);
+  const int rc = sq_partials(&ws);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  const unsigned nb = grid_for(n, 256, 1024);
+  const unsigned nb = grid_for(n, 256, kSqBlocks);
   k_sqdiff<<<nb, 256, 0, s>>>(a, b, n, (double *)ws);
   HIPCHK(hipGetLastError());
   k_sqdiff_final<<<1, 256, 0, s>>>((const double *)ws, (int)nb, out2);

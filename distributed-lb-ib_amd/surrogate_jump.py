@@ -22,7 +22,9 @@ loop (lbm_ib.c:3115-3563).  One jump cycle here:
      otherwise lbmib_jump_revert and K + R plain steps from the checkpoint,
      which reproduce the plain trajectory bit for bit.
 
-Every kernel runs on the engine's stream; PyTorch only allocates memory.
+Every kernel runs on the engine's stream (with early=True the network's
+next prediction runs on a stream of its own, ordered by events, beside the
+verification steps); PyTorch only allocates memory.
 """
 import ctypes
 
@@ -133,7 +135,8 @@ class JumpRunner:
     tol: acceptance bound of the verification ratio; 0 rejects every jump
     (all plain steps, bit-identical to lbmib_step)."""
 
-    def __init__(self, engine, net, norm, tau, u_in, factor=4, ksteps=50, verify_steps=1, tol=4.0, graph=True):
+    def __init__(self, engine, net, norm, tau, u_in, factor=4, ksteps=50, verify_steps=1, tol=4.0, graph=True,
+                 early=False):
         self.e, self.net, self.norm = engine, net, norm
         self.K, self.R, self.tol = ksteps, verify_steps, tol
         self.consts = const_channels(tau, u_in)
@@ -144,9 +147,23 @@ class JumpRunner:
         nz, ny, nx = self.co.dims
         self.x = torch.empty((1, nz, ny, nx, CIN), dtype=torch.bfloat16, device="cuda")
         self.delta = torch.empty((NOUT, self.co.nc), dtype=torch.float32, device="cuda")
+        # early: the next cycle's prediction reads the jumped state (its
+        # planes 0-3 as lbmib_jump reports them, the raster of the moved
+        # sheet) and runs on a stream of its own beside the verification
+        # steps; the feature planes the network reads are then a buffer of
+        # their own, into which the jump writes its d_after
+        self.early = early
+        if early:
+            self.pfeat = torch.empty((NF, self.co.nc), dtype=torch.float32, device="cuda")
+            self.pred_stream = torch.cuda.Stream()
+            self.pred_done = torch.cuda.Event()
+            self.pred_pending = False
+        else:
+            self.pfeat = self.co.feat
         # the jumped state's coarse planes (lbmib_jump's d_after, the
         # convention of the feature planes 0-3)
-        self.after = torch.empty((4, self.co.nc), dtype=torch.float32, device="cuda")
+        self.after = self.pfeat[0:4] if early else torch.empty((4, self.co.nc), dtype=torch.float32,
+                                                               device="cuda")
         self.sums = torch.zeros(4, dtype=torch.float64, device="cuda")
         # the verification sums reach the host through a side stream, so that
         # the engine's stream goes on with the next cycle's prediction while
@@ -162,7 +179,7 @@ class JumpRunner:
 
     def _predict_kernels(self):
         L = sg._bind()
-        pack_input(self.co.feat, self.norm, self.consts, self.x, self.co.nc, self.stream)
+        pack_input(self.pfeat, self.norm, self.consts, self.x, self.co.nc, self.stream)
         out = self.net.forward(self.x)
         sg.check(L.lbmib_sg_unpack(ctypes.c_void_p(out.data_ptr()), 16, NOUT, sg.dvec(self.norm.out_mean),
                                    sg.dvec(self.norm.out_std), ctypes.c_void_p(self.delta.data_ptr()), self.co.nc,
@@ -178,6 +195,8 @@ class JumpRunner:
         buffers that stay in place -- into a HIP graph, and every call from
         then on replays it."""
         with torch.cuda.stream(self.ext):
+            if self.early and self.pfeat is not self.co.feat:
+                self.pfeat.copy_(self.co.feat)
             if self.graph is not None:
                 self.graph.replay()
             elif not self.use_graph or not getattr(self, "_eager_done", False):
@@ -245,6 +264,9 @@ class JumpRunner:
 
     def _cycle(self, predictor, lookahead):
         e = self.e
+        if self.early and self.pred_pending:  # the side prediction has read pfeat and written delta
+            self.ext.wait_event(self.pred_done)
+            self.pred_pending = False
         if not self.have_feat:
             self.co.read()
             self.have_feat = True
@@ -256,16 +278,29 @@ class JumpRunner:
         self.pred_ready = False
         # the jump also reports the jumped state's coarse stored moments
         e.jump(self.co.f, self.co.feat.data_ptr(), self.delta.data_ptr(), self.K, self.after.data_ptr())
+        # verification ratio: |du over R steps| / R  against  |du of the jump| / K
+        self._sqdiff(self.delta[1:4], None, self.sums[2:4])
+        early = lookahead and self.early and self.graph is not None
+        if early:
+            # the next change, from the jumped state, beside the verification
+            e.coarse_sheets(self.co.f, self.pfeat[4].data_ptr())
+            ready = torch.cuda.Event()
+            ready.record(self.ext)
+            self.pred_stream.wait_event(ready)
+            with torch.cuda.stream(self.pred_stream):
+                self.graph.replay()
+                self.pred_done.record(self.pred_stream)
+            self.pred_pending = True
         # R plain steps, the last of which also reads the next cycle's input,
         # which the verification compares with the jumped state
         if self.R > 1:
             e.step(self.R - 1)
         self.co.step_read()
-        # verification ratio: |du over R steps| / R  against  |du of the jump| / K
         self._sqdiff(self.co.feat[1:4], self.after[1:4], self.sums[0:2])
-        self._sqdiff(self.delta[1:4], None, self.sums[2:4])
         self._sums_to_host()
-        if lookahead:
+        if early:
+            self.pred_ready = True
+        elif lookahead:
             self.predict()  # the next cycle's change, from the verified state
             self.pred_ready = True
         self.sums_ready.synchronize()
@@ -296,6 +331,9 @@ class JumpRunner:
         while done + span <= nsteps:
             self.cycle(predictor, lookahead=done + 2 * span <= nsteps)
             done += span
+        if self.early and self.pred_pending:  # no side work outlives the call
+            self.ext.wait_event(self.pred_done)
+            self.pred_pending = False
         if done < nsteps:
             self.e.step(nsteps - done)
             self.stats["plain_steps"] += nsteps - done
@@ -312,7 +350,7 @@ def zero_predictor(runner):
 
 
 def evaluate(net, norm, kw, warm, horizon, factor=4, ksteps=50, verify_steps=1, tol=4.0, baseline=True,
-             log=print):
+             log=print, early=False):
     """The 256^3 evaluation: from the same state after `warm` plain steps,
     `horizon` more steps (a) plain, (b) with surrogate jumps, (c) with jumps
     of the method without the network (zero change).  Returns timings
@@ -346,7 +384,7 @@ def evaluate(net, norm, kw, warm, horizon, factor=4, ksteps=50, verify_steps=1, 
         with lbmib.Engine(**kw) as e:
             e.init()
             e.step(warm)
-            r = JumpRunner(e, net, norm, kw["tau"], u_in, factor, ksteps, verify_steps, leg_tol)
+            r = JumpRunner(e, net, norm, kw["tau"], u_in, factor, ksteps, verify_steps, leg_tol, early=early)
             r.prepare(pred)  # buffers and the prediction graph, outside the timed leg
             e.sync()
             t0 = time.time()
@@ -387,9 +425,11 @@ if __name__ == "__main__":
     ap.add_argument("--tol", type=float, default=5.0)
     ap.add_argument("--verify-steps", type=int, default=1)
     ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--early", action="store_true",
+                    help="predict the next change from the jumped state, beside the verification step")
     args = ap.parse_args()
     net, norm, factor, ksteps, _ = load_model(args.model)
     kw, _ = lbmib_configs.c2(precision="f32")
     res = evaluate(net, norm, kw, args.warm, args.horizon, factor, ksteps, args.verify_steps, args.tol,
-                   baseline=not args.no_baseline)
+                   baseline=not args.no_baseline, early=args.early)
     print(json.dumps(res), flush=True)

@@ -33,7 +33,11 @@ extern "C" {
 
 /* Forward convolution: y = act(conv(x, w) + bias) [* (mask > 0)].
  * relu: 1 applies max(0, .); mask (nullable) has the shape of y.  bias is
- * FP32 [Cout] or NULL.  Cin in {16, 32, 64, 128, 256}, Cout a multiple of 16. */
+ * FP32 [Cout] or NULL.  Cin in {16, 32, 64, 128, 256}, Cout a multiple of 16.
+ * On some small grids the reduction is split into FP32 partials in the
+ * library workspace (below): the calls that use it -- this one,
+ * lbmib_sg_conv3d_bwd_data, lbmib_sg_conv3d_bwd_weight, lbmib_sg_mse -- must
+ * not overlap on one device (one stream, or ordered). */
 int lbmib_sg_conv3d_fwd(const void *x, const void *w, const float *bias, const void *mask, void *y, int B, int D,
                         int H, int W, int Cin, int Cout, int relu, void *stream);
 
@@ -111,7 +115,10 @@ int lbmib_sg_target(const float *f0, const float *f1, int nl, const double *mean
 int lbmib_sg_unpack(const void *out, int C, int nl, const double *mean, const double *std, float *d, long M,
                     void *stream);
 /* FP64 sums {sum (a - b)^2, sum b^2} over n FP32 values (b nullable: then
- * {sum a^2, 0}), written to out2 (device, 2 doubles); deterministic. */
+ * {sum a^2, 0}), written to out2 (device, 2 doubles); deterministic.  Its
+ * partial sums have a buffer of their own, not the workspace, so it may run
+ * beside the convolutions on another stream (calls of it must not overlap
+ * each other). */
 int lbmib_sg_sqdiff(const float *a, const float *b, long n, double *out2, void *stream);
 
 #ifdef __cplusplus

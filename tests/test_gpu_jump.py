@@ -327,6 +327,43 @@ def test_lookahead_prediction_gives_the_same_run(lbmib):
             np.testing.assert_array_equal(a.fibers(0), b.fibers(0))
 
 
+def test_early_prediction_beside_the_verification(lbmib):
+    """early=True: the next prediction reads the jumped state and runs on a
+    stream of its own while the engine's stream verifies.  Two such runs
+    give the same bits (the streams are ordered by events, not by timing),
+    and with every jump rejected the discarded side predictions leave the
+    plain trajectory bit for bit."""
+    import surrogate as sg
+    import surrogate_jump as sj
+    kw = small_cfg(lbmib, "f32")
+    net = sg.Net(cin=sj.CIN, cout=16, nout=sj.NOUT, seed=4, out_scale=0.05)
+    norm = sj.Norm([1.0, 0.02, 0, 0, 0, 0, 0, 0], [1e-3, 3e-3, 3e-3, 3e-3, 1.0, 1e-4, 1e-4, 1e-4], [0.0] * 4,
+                   [1e-4] * 4)
+    for tol in (float("inf"), 0.0):
+        print(f"early prediction, tol {tol}", flush=True)
+        with lbmib.Engine(**kw) as a, lbmib.Engine(**kw) as b, lbmib.Engine(**kw) as c:
+            for e in (a, b, c):
+                e.init()
+                e.step(2)
+            t0 = a.time
+            runs = []
+            for e in (a, b):
+                r = sj.JumpRunner(e, net, norm, kw["tau"], kw["u0"][0], factor=4, ksteps=10, verify_steps=2, tol=tol,
+                                  early=True)
+                r.prepare()
+                runs.append(r.run(48))
+            assert runs[0]["jumps"] == runs[1]["jumps"] == 4
+            assert runs[0]["accepted"] == runs[1]["accepted"] == (4 if tol > 0 else 0)
+            assert runs[0]["status"] == 0
+            assert a.time == b.time == t0 + 48
+            np.testing.assert_array_equal(a.df(), b.df())
+            np.testing.assert_array_equal(a.fibers(0), b.fibers(0))
+            if tol == 0.0:
+                c.step(48)
+                np.testing.assert_array_equal(a.df(), c.df())
+                np.testing.assert_array_equal(a.fibers(0), c.fibers(0))
+
+
 def test_coupling_rejects_bad_shapes_and_states(lbmib):
     kw = small_cfg(lbmib, "f32", n=(40, 32, 32))
     buf = torch.zeros((4, 1024), dtype=torch.float32, device="cuda")

@@ -240,6 +240,19 @@ def test_fiber_leaving_grid_reports_error(lbmib):
         assert e.status() == lbmib.ERR_FIBER_OOB
 
 
+def test_fiber_far_off_grid_reports_error(lbmib):
+    """A node far beyond the grid: its support start is beyond the int range,
+    so the test on it has to be made before the conversion (which saturates);
+    the node is left out of spreading and moving and the error is latched."""
+    kw = small_ib_config(lbmib, "f32")
+    s = kw["sheets"][0]
+    s.xyz[0, 0, 0] = 1e30
+    with lbmib.Engine(**kw) as e:
+        e.init()
+        e.step(2)
+        assert e.status() == lbmib.ERR_FIBER_OOB
+
+
 VARIANTS = {  # (cells per thread, nontemporal mask, blocks per CU)
     "f64": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 3, 4), (3, 3, 0), (1, 3, 0), (1, 3, 4), (1, 1, 8)],
     "f32": [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 3, 0), (2, 3, 4), (3, 3, 0), (3, 1, 2), (4, 1, 0), (1, 3, 0),
