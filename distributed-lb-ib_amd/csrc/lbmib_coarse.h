@@ -128,17 +128,21 @@ __device__ __forceinline__ void row_partial(float m[4], int x, long row, long nc
 template <int F>
 __global__ void __launch_bounds__(256) k_coarse_rows(const Geometry g, const float *__restrict__ part,
                                                      float *__restrict__ out) {
+  // (the host checks that the coarse cells and the partials' float4 count
+  // fit 32-bit indices; the F x F loads are all issued before the sums)
   const int ncx = g.nx / F, ncy = g.ny / F, ncz = g.nz / F;
-  const long nc = (long)ncx * ncy * ncz;
-  const long cc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = ncx * ncy * ncz;
+  const int cc = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (cc >= nc) return;
-  const int cx = (int)(cc % ncx), cy = (int)((cc / ncx) % ncy), cz = (int)(cc / ((long)ncx * ncy));
+  const int cx = cc % ncx, cy = (cc / ncx) % ncy, cz = cc / (ncx * ncy);
   double t[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
   for (int j = 0; j < F; ++j) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
     for (int k = 0; k < F; ++k) {
-      const long row = (long)(cy * F + j) + (long)g.ny * (cz * F + k);
-      const float4 p = *(const float4 *)(part + (row * ncx + cx) * 4);
+      const int row = (cy * F + j) + g.ny * (cz * F + k);
+      const float4 p = *(const float4 *)(part + ((long)row * ncx + cx) * 4);
       acc[0] += (double)p.x;
       acc[1] += (double)p.y;
       acc[2] += (double)p.z;
@@ -149,7 +153,7 @@ __global__ void __launch_bounds__(256) k_coarse_rows(const Geometry g, const flo
   }
   const double inv = 1.0 / (F * F * F);
 #pragma unroll
-  for (int a = 0; a < 4; ++a) out[a * nc + cc] = (float)(t[a] * inv);
+  for (int a = 0; a < 4; ++a) out[(long)a * nc + cc] = (float)(t[a] * inv);
 }
 
 // bytes of the partial sums at factor F; the reduction's launch
@@ -158,6 +162,7 @@ inline size_t coarse_part_bytes(const Geometry &g, int F) {
 }
 inline hipError_t launch_coarse_rows(const Geometry &g, int F, const float *part, float *out, hipStream_t s) {
   const long nc = (long)(g.nx / F) * (g.ny / F) * (g.nz / F);
+  if ((long)g.ny * g.nz >= (1L << 31) || nc >= (1L << 31)) return hipErrorInvalidValue;
   const unsigned blocks = (unsigned)((nc + 255) / 256);
   if (F == 1) k_coarse_rows<1><<<blocks, 256, 0, s>>>(g, part, out);
   else if (F == 2) k_coarse_rows<2><<<blocks, 256, 0, s>>>(g, part, out);
