@@ -516,6 +516,85 @@ int launch_fwd_kw(const void *x, const void *w, const float *bias, const void *m
   return LBMIB_OK;
 }
 
+// Tiny grids (a sample of at most 512 voxels whose padded volume, one cell
+// more on every side, has at most kVolMax voxels): a block owns up to 256
+// voxels of one sample (4 waves x 4 fragments of 16), 16 output channels,
+// and one slice of 16 input channels of the reduction (gridDim.z slices).
+// It stages the sample's whole padded volume of that slice in LDS (zeros on
+// the rim; rows of 24 elements, so that 8 lanes' 16-byte reads fall on
+// distinct banks) and the slice's weights for all 27 taps (rows of
+// 28 x 16 + 8 elements), then multiplies its 14 chunks from LDS with no
+// barrier: slot 8 kq + e of chunk p is tap 2p + (kq >> 1), channel
+// c0 + 8 (kq & 1) + e, in both operands (a 28th tap is zero).  The FP32
+// partial of the slice goes to part[slice][v][n]; k_conv_reduce adds the
+// slices in order with bias, activation and mask.  Each input and weight
+// element is then read once per block, where the split over chunks reads
+// the weights once per 64 voxels and the inputs once per tap.
+constexpr int kVolMax = 1024, kVolPitch = 24, kVolWRow = 28 * 16 + 8;
+
+__global__ void __launch_bounds__(256) k_conv_fwd_vol(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
+                                                      float *__restrict__ part, int D, int H, int W, long M, int Cin,
+                                                      int Cout) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t xs[];  // [padded voxel][24], then [16 rows][kVolWRow]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int S = D * H * W, parts = (S + 255) / 256;
+  const int smp = blockIdx.x / parts, v0 = (blockIdx.x % parts) * 256;
+  const int n0 = blockIdx.y * 16, c0 = blockIdx.z * 16, K = 27 * Cin;
+  const int PX = W + 2, PY = H + 2, NP = PX * PY * (D + 2);
+  uint16_t *ws = xs + NP * kVolPitch;
+  const long base = (long)smp * S;
+  for (int e = tid; e < 2 * NP; e += 256) {
+    const int vi = e >> 1, half = e & 1;
+    const int gx = vi % PX - 1, gy = (vi / PX) % PY - 1, gz = vi / (PX * PY) - 1;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
+      v = *(const u32x4 *)(x + (base + ((long)gz * H + gy) * W + gx) * Cin + c0 + 8 * half);
+    *(u32x4 *)(xs + vi * kVolPitch + 8 * half) = v;
+  }
+  for (int e = tid; e < 16 * 56; e += 256) {
+    const int row = e / 56, pc = e % 56, tap = pc >> 1, half = pc & 1, n = n0 + row;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (tap < 27 && n < Cout) v = *(const u32x4 *)(w + (long)n * K + tap * Cin + c0 + 8 * half);
+    *(u32x4 *)(ws + row * kVolWRow + tap * 16 + 8 * half) = v;
+  }
+  __syncthreads();
+  int vb[4], lv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    lv[i] = v0 + wave * 64 + i * 16 + col;
+    const int l = lv[i] < S ? lv[i] : 0;
+    const int vx = l % W, vy = (l / W) % H, vz = l / (W * H);
+    vb[i] = ((vz + 1) * PY + vy + 1) * PX + vx + 1;
+  }
+  f32x4 acc[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int p = 0; p < 14; ++p) {
+    const int tap = 2 * p + (kq >> 1);
+    bf16x8 b[4];
+    if (tap < 27) {
+      const int off = ((tap / 9 - 1) * PY + (tap / 3) % 3 - 1) * PX + tap % 3 - 1;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        b[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(xs + (vb[i] + off) * kVolPitch + 8 * (kq & 1)));
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) b[i] = zero8();
+    }
+    const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4 *)(ws + col * kVolWRow + 32 * p + 8 * kq));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = mfma16(a, b[i], acc[i]);
+  }
+  // lane holds channels n0 + 4 kq + r of voxel lv[i]
+  float *dst = part + (long)blockIdx.z * M * Cout;
+  const int n = n0 + 4 * kq;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (lv[i] < S && n < Cout) *(f32x4 *)(dst + (base + lv[i]) * Cout + n) = acc[i];
+}
+
 // y[v][n] = act(sum over z < ns of part[z][v][n] + bias[n]) [* (mask > 0)]:
 // one thread per (voxel, 4 channels)
 __global__ void k_conv_reduce(const float *__restrict__ part, int ns, const float *__restrict__ bias,
@@ -581,6 +660,33 @@ int launch_fwd(const void *x, const void *w, const float *bias, const void *mask
   return LBMIB_OK;
 }
 
+// (the slices' partials at most 8 MB: beyond that, at batch 4 on 8^3, the
+// reduction's pass over them cost more than the form saves -- 49.5 against
+// 32.5 us at 256 -> 256 channels)
+bool vol_conv_ok(int D, int H, int W, long M, int Cin, int Cout) {
+  return (long)D * H * W <= 512 && (long)(D + 2) * (H + 2) * (W + 2) <= kVolMax && Cin % 16 == 0 &&
+         (size_t)(Cin / 16) * M * Cout * sizeof(float) <= (8u << 20);
+}
+
+int launch_fwd_vol(const void *x, const void *w, const float *bias, const void *mask, void *y, int D, int H, int W,
+                   long M, int Cin, int Cout, int relu, hipStream_t s) {
+  void *ws = nullptr;
+  const int rc = workspace(&ws);
+  if (rc) return rc;
+  const int S = D * H * W;
+  const dim3 grid((unsigned)(M / S * ((S + 255) / 256)), (unsigned)(Cout / 16), (unsigned)(Cin / 16));
+  const size_t shm = sizeof(uint16_t) * ((size_t)(D + 2) * (H + 2) * (W + 2) * kVolPitch + 16 * kVolWRow);
+  k_conv_fwd_vol<<<grid, 256, shm, s>>>((const uint16_t *)x, (const uint16_t *)w, (float *)ws, D, H, W, M, Cin, Cout);
+  HIPCHK(hipGetLastError());
+  const long nq = M * (Cout / 4);
+  long nb = (nq + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  k_conv_reduce<<<(unsigned)nb, 256, 0, s>>>((const float *)ws, (int)grid.z, bias, (const uint16_t *)mask,
+                                              (uint16_t *)y, M, Cout, relu);
+  HIPCHK(hipGetLastError());
+  return LBMIB_OK;
+}
+
 bool shape_ok(int B, int D, int H, int W, int Cin, int Cout) {
   if (B < 1 || D < 1 || H < 1 || W < 1) return false;
   if (Cin < 16 || Cin > 256 || (Cin & (Cin - 1))) return false;
@@ -608,6 +714,9 @@ int conv_fwd(const void *x, const void *w, const float *bias, const void *mask, 
     if (wn == 2) return launch_fwd<4, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
     return launch_fwd<4, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
   }
+  // tiny grids: the whole padded sample in LDS, the reduction split by input slices
+  if (vol_conv_ok(D, H, W, M, Cin, Cout))
+    return launch_fwd_vol(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
   // small grids whose weights stay modest: the two waves of a block split
   // the reduction of one 32-voxel tile, one launch and no partial sums in
   // memory; the weights are read once per tile, at most 128 MB in all

@@ -73,7 +73,8 @@ def assert_bf16_close(gpu, ref, absum, what):
 
 CONV_CASES = [
     # (B, D, H, W, Cin, Cout): every pair of the network, grids that fill and
-    # that do not fill the block tiles
+    # that do not fill the block tiles; samples of at most 512 voxels take
+    # the whole-volume form, (1, 8, 16, 8, ...) the split over chunks
     (1, 8, 8, 8, 16, 32), (2, 4, 6, 8, 32, 32), (1, 16, 16, 16, 32, 64), (1, 8, 8, 8, 64, 64),
     (1, 4, 8, 8, 64, 128), (2, 4, 4, 4, 128, 128), (1, 4, 4, 8, 128, 256), (1, 2, 4, 4, 256, 256),
     (1, 4, 4, 4, 256, 128), (1, 8, 8, 8, 128, 64), (1, 8, 16, 8, 64, 32), (1, 8, 8, 8, 32, 16),
@@ -106,8 +107,9 @@ def test_conv3d_fwd_matches_oracle(sg, case):
         assert_bf16_close(host_bf16(y), ref, absum, f"fwd relu={relu}")
 
 
-@pytest.mark.parametrize("case", [(1, 4, 8, 8, 32, 64), (1, 16, 16, 16, 32, 64), (1, 32, 64, 64, 32, 32)],
-                         ids=["split", "wave-split", "box"])
+@pytest.mark.parametrize("case", [(1, 4, 8, 8, 32, 64), (1, 8, 16, 8, 64, 32), (1, 16, 16, 16, 32, 64),
+                                  (1, 32, 64, 64, 32, 32)],
+                         ids=["volume", "split", "wave-split", "box"])
 def test_conv3d_fwd_mask(sg, case):
     B, D, H, W, ci, co = case
     rng = np.random.default_rng(1)
