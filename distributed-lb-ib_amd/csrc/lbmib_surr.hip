@@ -788,40 +788,76 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
   const long c_begin = (long)blockIdx.y * chunks_per_split;
   long c_end = c_begin + chunks_per_split;
   if (c_end > nchunks) c_end = nchunks;
-  const int xpieces = nci * 2;  // 8-channel pieces per voxel in the ci tile
-  const int npx = 27 * kWgChunk * xpieces;
+  const int lgxp = nci == 2 ? 2 : 1;  // log2 of the 8-channel pieces per voxel in the ci tile
+  const int npx = 27 * kWgChunk << lgxp;
   const int npy = kWgChunk * nco * 2;
+  // the chunk's voxels: x, y, z within the sample, and 1 if the voxel exists
+  __shared__ int svc[kWgChunk][4];
+  constexpr int kXRounds = (27 * kWgChunk * 4 + 255) / 256;  // x pieces per thread at most
   for (long c = c_begin; c < c_end; ++c) {
     const long v0 = c * kWgChunk;
-    for (int e = tid; e < npx; e += 256) {
-      const int q = e % xpieces;
-      const int r = e / xpieces;
-      const int p = r % kWgChunk, t = r / kWgChunk;
-      const long v = v0 + p;
-      if (v >= M) {  // the last chunk's voxels past the end add nothing
-        uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
-        *(u32x2 *)dst = u32x2{0u, 0u};
-        *(u32x2 *)(dst + 4) = u32x2{0u, 0u};
-        continue;
+    if (tid < kWgChunk) {
+      const long v = v0 + tid;
+      int cv[4] = {0, 0, 0, 0};
+      if (v < M) {
+        long rr = v;
+        cv[0] = (int)(rr % W);
+        rr /= W;
+        cv[1] = (int)(rr % H);
+        rr /= H;
+        cv[2] = (int)(rr % D);
+        cv[3] = 1;
       }
-      long rr = v;
-      const int vx = (int)(rr % W);
-      rr /= W;
-      const int vy = (int)(rr % H);
-      rr /= H;
-      const int vz = (int)(rr % D);
-      const int dz = t / 9 - 1, dyy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      u32x4 val = {0u, 0u, 0u, 0u};
-      if ((unsigned)(vz + dz) < (unsigned)D && (unsigned)(vy + dyy) < (unsigned)H && (unsigned)(vx + dx) < (unsigned)W)
-        val = *(const u32x4 *)(x + (v + dz * HW + (long)dyy * W + dx) * Cin + ci0 + q * 8);
-      uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
-      *(u32x2 *)dst = u32x2{val[0], val[1]};
-      *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
+#pragma unroll
+      for (int a = 0; a < 4; ++a) svc[tid][a] = cv[a];
     }
-    for (int e = tid; e < npy; e += 256) {
-      const int q = e % (nco * 2), p = e / (nco * 2);
-      u32x4 val = {0u, 0u, 0u, 0u};
-      if (v0 + p < M) val = *(const u32x4 *)(dy + (v0 + p) * Cout + co0 + q * 8);
+    __syncthreads();
+    // every piece's load is issued before any is stored: an unconditional
+    // load from a valid address (the tensor's start where there is nothing
+    // to load -- a neighbour outside the volume, a voxel past the end), the
+    // value zeroed after it
+    u32x4 px[kXRounds];
+    bool pok[kXRounds];
+#pragma unroll
+    for (int r = 0; r < kXRounds; ++r) {
+      const int e = tid + 256 * r;
+      const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
+      const uint16_t *src = x;
+      bool ok = false;
+      if (e < npx) {
+        const int dz = t / 9 - 1, dyy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        ok = svc[p][3] && (unsigned)(svc[p][2] + dz) < (unsigned)D && (unsigned)(svc[p][1] + dyy) < (unsigned)H &&
+             (unsigned)(svc[p][0] + dx) < (unsigned)W;
+        if (ok) src = x + (v0 + p + dz * HW + (long)dyy * W + dx) * Cin + ci0 + q * 8;
+      }
+      px[r] = *(const u32x4 *)src;
+      pok[r] = ok;
+    }
+    u32x4 py = {0u, 0u, 0u, 0u};
+    bool yok = false;
+    {
+      const int q = tid % (nco * 2), p = tid / (nco * 2);
+      const uint16_t *src = dy;
+      if (tid < npy && v0 + p < M) {
+        src = dy + (v0 + p) * Cout + co0 + q * 8;
+        yok = true;
+      }
+      py = *(const u32x4 *)src;
+    }
+#pragma unroll
+    for (int r = 0; r < kXRounds; ++r) {
+      const int e = tid + 256 * r;
+      if (e < npx) {
+        const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
+        const u32x4 val = pok[r] ? px[r] : u32x4{0u, 0u, 0u, 0u};
+        uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
+        *(u32x2 *)dst = u32x2{val[0], val[1]};
+        *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
+      }
+    }
+    if (tid < npy) {
+      const int q = tid % (nco * 2), p = tid / (nco * 2);
+      const u32x4 val = yok ? py : u32x4{0u, 0u, 0u, 0u};
       uint16_t *dst = sdy + p * kWgPitch + q * 8;
       *(u32x2 *)dst = u32x2{val[0], val[1]};
       *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
