@@ -794,6 +794,7 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
   // the chunk's voxels: x, y, z within the sample, and 1 if the voxel exists
   __shared__ int svc[kWgChunk][4];
   constexpr int kXRounds = (27 * kWgChunk * 4 + 255) / 256;  // x pieces per thread at most
+  constexpr int kXHalf = (kXRounds + 1) / 2;                // staged in two halves (register pressure)
   for (long c = c_begin; c < c_end; ++c) {
     const long v0 = c * kWgChunk;
     if (tid < kWgChunk) {
@@ -812,26 +813,40 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
       for (int a = 0; a < 4; ++a) svc[tid][a] = cv[a];
     }
     __syncthreads();
-    // every piece's load is issued before any is stored: an unconditional
-    // load from a valid address (the tensor's start where there is nothing
-    // to load -- a neighbour outside the volume, a voxel past the end), the
-    // value zeroed after it
-    u32x4 px[kXRounds];
-    bool pok[kXRounds];
+    // each half's piece loads are issued before any of them is stored: an
+    // unconditional load from a valid address (the tensor's start where
+    // there is nothing to load -- a neighbour outside the volume, a voxel
+    // past the end), the value zeroed after it
+#pragma unroll 1
+    for (int hf = 0; hf < 2; ++hf) {
+      u32x4 px[kXHalf];
+      bool pok[kXHalf];
 #pragma unroll
-    for (int r = 0; r < kXRounds; ++r) {
-      const int e = tid + 256 * r;
-      const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
-      const uint16_t *src = x;
-      bool ok = false;
-      if (e < npx) {
-        const int dz = t / 9 - 1, dyy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        ok = svc[p][3] && (unsigned)(svc[p][2] + dz) < (unsigned)D && (unsigned)(svc[p][1] + dyy) < (unsigned)H &&
-             (unsigned)(svc[p][0] + dx) < (unsigned)W;
-        if (ok) src = x + (v0 + p + dz * HW + (long)dyy * W + dx) * Cin + ci0 + q * 8;
+      for (int r = 0; r < kXHalf; ++r) {
+        const int e = tid + 256 * (hf * kXHalf + r);
+        const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
+        const uint16_t *src = x;
+        bool ok = false;
+        if (e < npx) {
+          const int dz = t / 9 - 1, dyy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+          ok = svc[p][3] && (unsigned)(svc[p][2] + dz) < (unsigned)D && (unsigned)(svc[p][1] + dyy) < (unsigned)H &&
+               (unsigned)(svc[p][0] + dx) < (unsigned)W;
+          if (ok) src = x + (v0 + p + dz * HW + (long)dyy * W + dx) * Cin + ci0 + q * 8;
+        }
+        px[r] = *(const u32x4 *)src;
+        pok[r] = ok;
       }
-      px[r] = *(const u32x4 *)src;
-      pok[r] = ok;
+#pragma unroll
+      for (int r = 0; r < kXHalf; ++r) {
+        const int e = tid + 256 * (hf * kXHalf + r);
+        if (e < npx) {
+          const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
+          const u32x4 val = pok[r] ? px[r] : u32x4{0u, 0u, 0u, 0u};
+          uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
+          *(u32x2 *)dst = u32x2{val[0], val[1]};
+          *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
+        }
+      }
     }
     u32x4 py = {0u, 0u, 0u, 0u};
     bool yok = false;
@@ -843,17 +858,6 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
         yok = true;
       }
       py = *(const u32x4 *)src;
-    }
-#pragma unroll
-    for (int r = 0; r < kXRounds; ++r) {
-      const int e = tid + 256 * r;
-      if (e < npx) {
-        const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
-        const u32x4 val = pok[r] ? px[r] : u32x4{0u, 0u, 0u, 0u};
-        uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
-        *(u32x2 *)dst = u32x2{val[0], val[1]};
-        *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
-      }
     }
     if (tid < npy) {
       const int q = tid % (nco * 2), p = tid / (nco * 2);
