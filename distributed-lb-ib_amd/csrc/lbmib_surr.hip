@@ -925,32 +925,58 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
 }
 
 // out[e] = sum over s < ns of part[s][e], in order of s
+// out[e] = sum over k < ns of part[k][e], in a fixed order: four running
+// sums over k = 4j, 4j + 1, 4j + 2, 4j + 3 (so four loads are in flight per
+// step), then (s0 + s1) + (s2 + s3)
 __global__ void k_sum_parts(const float *__restrict__ part, float *__restrict__ out, long n, int ns) {
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (int k = 0; k < ns; ++k) s += part[(long)k * n + e];
-    out[e] = s;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int k = 0;
+    for (; k + 4 <= ns; k += 4) {
+      s0 += part[(long)k * n + e];
+      s1 += part[(long)(k + 1) * n + e];
+      s2 += part[(long)(k + 2) * n + e];
+      s3 += part[(long)(k + 3) * n + e];
+    }
+    if (k < ns) s0 += part[(long)k * n + e];
+    if (k + 1 < ns) s1 += part[(long)(k + 1) * n + e];
+    if (k + 2 < ns) s2 += part[(long)(k + 2) * n + e];
+    out[e] = (s0 + s1) + (s2 + s3);
   }
 }
 
 // bias gradient partials: part[blk][co] = sum of dy[v][co] over the block's
-// voxel range (256 threads: 16 channel lanes x 16 voxel lanes)
+// voxel range.  A thread sums 8 channels of every G-th voxel (16-byte
+// loads; G = 256 / (C / 8) voxel lanes), then the G lanes of each channel
+// group are added through LDS in lane order.  C is a multiple of 16 and at
+// most 512.
 __global__ void __launch_bounds__(256) k_colsum(const uint16_t *__restrict__ dy, float *__restrict__ part, long M,
                                                 int C, long per_block) {
-  __shared__ float red[16][17];
-  const int cl = threadIdx.x & 15, vl = threadIdx.x >> 4;
+  __shared__ float red[256][9];
+  const int ng = C / 8, G = 256 / ng;  // channel groups, voxel lanes
+  const int tid = threadIdx.x, cg = tid % ng, vl = tid / ng;
   const long v0 = (long)blockIdx.x * per_block;
   long v1 = v0 + per_block;
   if (v1 > M) v1 = M;
-  for (int c0 = 0; c0 < C; c0 += 16) {
-    float s = 0.f;
-    for (long v = v0 + vl; v < v1; v += 16) s += bf2f(dy[v * C + c0 + cl]);
-    red[vl][cl] = s;
+  for (int c0 = 0; c0 < C; c0 += 8 * ng) {  // one pass: 8 ng = C
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (vl < G)
+      for (long v = v0 + vl; v < v1; v += G) {
+        const u32x4 q = *(const u32x4 *)(dy + v * C + c0 + 8 * cg);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          s[2 * r] += bf2f((uint16_t)(q[r] & 0xffff));
+          s[2 * r + 1] += bf2f((uint16_t)(q[r] >> 16));
+        }
+      }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) red[tid][r] = s[r];
     __syncthreads();
-    if (vl == 0) {
+    for (int ch = tid; ch < 8 * ng; ch += 256) {
+      const int g = ch / 8, r = ch % 8;
       float t = 0.f;
-      for (int k = 0; k < 16; ++k) t += red[k][cl];
-      part[(long)blockIdx.x * C + c0 + cl] = t;
+      for (int k = 0; k < G; ++k) t += red[k * ng + g][r];
+      part[(long)blockIdx.x * C + c0 + ch] = t;
     }
     __syncthreads();
   }
@@ -972,7 +998,7 @@ WgradPlan wgrad_plan(long M, int Cin, int Cout, size_t cap_bytes) {
   p.nchunks = (M + kWgChunk - 1) / kWgChunk;
   const int tiles = p.ci_tiles * p.co_tiles;
   const size_t per_split = (size_t)Cout * 27 * Cin * sizeof(float);
-  long want = (1024 + tiles - 1) / tiles;  // about 1024 blocks
+  long want = (512 + tiles - 1) / tiles;  // about 512 blocks: one round at 2 per CU
   if (want > p.nchunks) want = p.nchunks;
   long cap = (long)(cap_bytes / per_split);
   if (want > cap) want = cap;
@@ -983,7 +1009,7 @@ WgradPlan wgrad_plan(long M, int Cin, int Cout, size_t cap_bytes) {
   return p;
 }
 
-constexpr int kColsumBlocks = 256;
+constexpr int kColsumBlocks = 1024;
 
 int wgrad_with(const void *x, const void *dy, float *dw, float *db, int B, int D, int H, int W, int Cin, int Cout,
                void *ws, size_t ws_bytes, hipStream_t s) {
