@@ -764,11 +764,16 @@ __global__ void k_flip(const uint16_t *__restrict__ w, uint16_t *__restrict__ wt
 // (72 bytes) puts the four 8-voxel groups of a fragment on distinct banks.
 constexpr int kWgPitch = 36;
 constexpr int kWgChunk = 32;
+constexpr int kWgBox = kWgChunk + 2;  // a chunk's voxels and one more on each side
 
 __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy,
                                                     float *__restrict__ part, int D, int H, int W, long M, int Cin,
                                                     int Cout, int ci_tiles, int chunks_per_split, long nchunks) {
-  __shared__ uint16_t sx[27 * kWgChunk * kWgPitch];
+  // the chunk's neighbourhood: for each (dz, dy), interval j = 3 (dz + 1) +
+  // dy + 1 of kWgBox linear voxel indices starting one before the chunk's
+  // first voxel shifted by dz HW + dy W; tap (dz, dy, dx) of voxel p is slot
+  // p + 1 + dx of interval j wherever that neighbour lies in the volume
+  __shared__ uint16_t sx[9 * kWgBox * kWgPitch];
   __shared__ uint16_t sdy[kWgChunk * kWgPitch];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int col = lane & 15, kq = lane >> 4;
@@ -789,12 +794,11 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
   long c_end = c_begin + chunks_per_split;
   if (c_end > nchunks) c_end = nchunks;
   const int lgxp = nci == 2 ? 2 : 1;  // log2 of the 8-channel pieces per voxel in the ci tile
-  const int npx = 27 * kWgChunk << lgxp;
+  const int npx = 9 * kWgBox << lgxp;
   const int npy = kWgChunk * nco * 2;
   // the chunk's voxels: x, y, z within the sample, and 1 if the voxel exists
   __shared__ int svc[kWgChunk][4];
-  constexpr int kXRounds = (27 * kWgChunk * 4 + 255) / 256;  // x pieces per thread at most
-  constexpr int kXHalf = (kXRounds + 1) / 2;                // staged in two halves (register pressure)
+  constexpr int kXRounds = (9 * kWgBox * 4 + 255) / 256;  // x pieces per thread at most
   for (long c = c_begin; c < c_end; ++c) {
     const long v0 = c * kWgChunk;
     if (tid < kWgChunk) {
@@ -812,40 +816,34 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
 #pragma unroll
       for (int a = 0; a < 4; ++a) svc[tid][a] = cv[a];
     }
-    __syncthreads();
-    // each half's piece loads are issued before any of them is stored: an
-    // unconditional load from a valid address (the tensor's start where
-    // there is nothing to load -- a neighbour outside the volume, a voxel
-    // past the end), the value zeroed after it
-#pragma unroll 1
-    for (int hf = 0; hf < 2; ++hf) {
-      u32x4 px[kXHalf];
-      bool pok[kXHalf];
+    // every piece's load is issued before any is stored: an unconditional
+    // load from a valid address (the tensor's start for an index outside
+    // [0, M)), the value zeroed after it
+    u32x4 px[kXRounds];
+    bool pok[kXRounds];
 #pragma unroll
-      for (int r = 0; r < kXHalf; ++r) {
-        const int e = tid + 256 * (hf * kXHalf + r);
-        const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
-        const uint16_t *src = x;
-        bool ok = false;
-        if (e < npx) {
-          const int dz = t / 9 - 1, dyy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-          ok = svc[p][3] && (unsigned)(svc[p][2] + dz) < (unsigned)D && (unsigned)(svc[p][1] + dyy) < (unsigned)H &&
-               (unsigned)(svc[p][0] + dx) < (unsigned)W;
-          if (ok) src = x + (v0 + p + dz * HW + (long)dyy * W + dx) * Cin + ci0 + q * 8;
-        }
-        px[r] = *(const u32x4 *)src;
-        pok[r] = ok;
+    for (int r = 0; r < kXRounds; ++r) {
+      const int e = tid + 256 * r;
+      const uint16_t *src = x;
+      bool ok = false;
+      if (e < npx) {
+        const int q = e & ((1 << lgxp) - 1), rest = e >> lgxp, sl = rest % kWgBox, j = rest / kWgBox;
+        const long li = v0 - 1 + sl + (long)(j / 3 - 1) * HW + (long)(j % 3 - 1) * W;
+        ok = li >= 0 && li < M;
+        if (ok) src = x + li * Cin + ci0 + q * 8;
       }
+      px[r] = *(const u32x4 *)src;
+      pok[r] = ok;
+    }
 #pragma unroll
-      for (int r = 0; r < kXHalf; ++r) {
-        const int e = tid + 256 * (hf * kXHalf + r);
-        if (e < npx) {
-          const int q = e & ((1 << lgxp) - 1), rr = e >> lgxp, p = rr & (kWgChunk - 1), t = rr / kWgChunk;
-          const u32x4 val = pok[r] ? px[r] : u32x4{0u, 0u, 0u, 0u};
-          uint16_t *dst = sx + (t * kWgChunk + p) * kWgPitch + q * 8;
-          *(u32x2 *)dst = u32x2{val[0], val[1]};
-          *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
-        }
+    for (int r = 0; r < kXRounds; ++r) {
+      const int e = tid + 256 * r;
+      if (e < npx) {
+        const int q = e & ((1 << lgxp) - 1), rest = e >> lgxp;
+        const u32x4 val = pok[r] ? px[r] : u32x4{0u, 0u, 0u, 0u};
+        uint16_t *dst = sx + rest * kWgPitch + q * 8;
+        *(u32x2 *)dst = u32x2{val[0], val[1]};
+        *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
       }
     }
     u32x4 py = {0u, 0u, 0u, 0u};
@@ -867,6 +865,21 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
       *(u32x2 *)(dst + 4) = u32x2{val[2], val[3]};
     }
     __syncthreads();
+    // which of this lane's voxels p = 8 kq + e have their neighbour at
+    // offset d - 1 along x, y, z inside the volume (bit e); mz also carries
+    // whether the voxel exists
+    uint32_t mx[3] = {0u, 0u, 0u}, my[3] = {0u, 0u, 0u}, mz[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int *cv = svc[kq * 8 + e];
+      const int vx = cv[0], vy = cv[1], vz = cv[2], ex = cv[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        mx[d] |= (uint32_t)((unsigned)(vx + d - 1) < (unsigned)W) << e;
+        my[d] |= (uint32_t)((unsigned)(vy + d - 1) < (unsigned)H) << e;
+        mz[d] |= (uint32_t)(ex && (unsigned)(vz + d - 1) < (unsigned)D) << e;
+      }
+    }
     // A: dy^T fragments (row co = 16 a + col, voxels 8 kq .. + 8)
     bf16x8 fa[2];
 #pragma unroll
@@ -883,12 +896,18 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
     for (int tt = 0; tt < 7; ++tt) {
       const int t = wave + 4 * tt;
       if (t >= 27) break;
+      const int tz = t / 9, ty = (t / 3) % 3, tx = t % 3;
+      const uint32_t vm = mx[tx] & my[ty] & mz[tz];
+      const uint16_t *row = sx + ((tz * 3 + ty) * kWgBox + kq * 8 + tx) * kWgPitch;
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         if (b >= nci) break;
         uint16_t h[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) h[e] = sx[(t * kWgChunk + kq * 8 + e) * kWgPitch + b * 16 + col];
+        for (int e = 0; e < 8; ++e) {
+          const uint16_t v = row[e * kWgPitch + b * 16 + col];
+          h[e] = (vm >> e) & 1u ? v : (uint16_t)0;
+        }
         u32x4 pk;
 #pragma unroll
         for (int e = 0; e < 4; ++e) pk[e] = (uint32_t)h[2 * e] | ((uint32_t)h[2 * e + 1] << 16);
