@@ -944,24 +944,30 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t *__restrict__
 }
 
 // out[e] = sum over s < ns of part[s][e], in order of s
-// out[e] = sum over k < ns of part[k][e], in a fixed order: four running
-// sums over k = 4j, 4j + 1, 4j + 2, 4j + 3 (so four loads are in flight per
-// step), then (s0 + s1) + (s2 + s3)
-__global__ void k_sum_parts(const float *__restrict__ part, float *__restrict__ out, long n, int ns) {
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int k = 0;
-    for (; k + 4 <= ns; k += 4) {
+// out[e] = sum over k < ns of part[k][e], in a fixed order.  A block covers
+// 64 consecutive e (one per lane, so every wave reads 256 contiguous bytes
+// per k) and its 4 waves split k: wave g sums k = g, g + 8, g + 16, ... and
+// k = g + 4, g + 12, ... in two running sums; the four waves' sums are then
+// added through LDS as (w0 + w1) + (w2 + w3).  Launch with kSumBlock
+// threads and ceil(n / 64) blocks.
+constexpr int kSumBlock = 256;
+__global__ void __launch_bounds__(kSumBlock) k_sum_parts(const float *__restrict__ part, float *__restrict__ out,
+                                                         long n, int ns) {
+  __shared__ float red[4][64];
+  const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const long e = (long)blockIdx.x * 64 + el;
+  float s0 = 0.f, s1 = 0.f;
+  if (e < n) {
+    int k = g;
+    for (; k + 4 < ns; k += 8) {
       s0 += part[(long)k * n + e];
-      s1 += part[(long)(k + 1) * n + e];
-      s2 += part[(long)(k + 2) * n + e];
-      s3 += part[(long)(k + 3) * n + e];
+      s1 += part[(long)(k + 4) * n + e];
     }
     if (k < ns) s0 += part[(long)k * n + e];
-    if (k + 1 < ns) s1 += part[(long)(k + 1) * n + e];
-    if (k + 2 < ns) s2 += part[(long)(k + 2) * n + e];
-    out[e] = (s0 + s1) + (s2 + s3);
   }
+  red[g][el] = s0 + s1;
+  __syncthreads();
+  if (g == 0 && e < n) out[e] = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
 }
 
 // bias gradient partials: part[blk][co] = sum of dy[v][co] over the block's
@@ -1040,7 +1046,7 @@ int wgrad_with(const void *x, const void *dy, float *dw, float *db, int B, int D
       (const uint16_t *)x, (const uint16_t *)dy, part, D, H, W, M, Cin, Cout, p.ci_tiles, p.cps, p.nchunks);
   HIPCHK(hipGetLastError());
   const long n = (long)Cout * 27 * Cin;
-  k_sum_parts<<<(unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096), 256, 0, s>>>(part, dw, n, p.splits);
+  k_sum_parts<<<(unsigned)((n + 63) / 64), kSumBlock, 0, s>>>(part, dw, n, p.splits);
   HIPCHK(hipGetLastError());
   if (db) {
     if ((size_t)kColsumBlocks * Cout * sizeof(float) > ws_bytes) return LBMIB_ERR_ARG;
@@ -1048,7 +1054,7 @@ int wgrad_with(const void *x, const void *dy, float *dw, float *db, int B, int D
     const int nb = (int)((M + per - 1) / per);
     k_colsum<<<nb, 256, 0, s>>>((const uint16_t *)dy, part, M, Cout, per);
     HIPCHK(hipGetLastError());
-    k_sum_parts<<<1, 256, 0, s>>>(part, db, Cout, nb);
+    k_sum_parts<<<(unsigned)((Cout + 63) / 64), kSumBlock, 0, s>>>(part, db, Cout, nb);
     HIPCHK(hipGetLastError());
   }
   return LBMIB_OK;
