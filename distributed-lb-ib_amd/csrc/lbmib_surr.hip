@@ -714,6 +714,10 @@ int conv_fwd(const void *x, const void *w, const float *bias, const void *mask, 
     if (wn == 2) return launch_fwd<4, 2>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
     return launch_fwd<4, 1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, 1, s);
   }
+  // mid-size grids with wide inputs: the input box with 16-channel tiles
+  // (32^3 at batch 1, 64 -> 64 channels: 29.1 against 35.4 us split)
+  if (M >= 32768 && Cin >= 64 && box_conv_ok(H, W, M, Cin, Cout))
+    return launch_fwd_box<1>(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);
   // tiny grids: the whole padded sample in LDS, the reduction split by input slices
   if (vol_conv_ok(D, H, W, M, Cin, Cout))
     return launch_fwd_vol(x, w, bias, mask, y, D, H, W, M, Cin, Cout, relu, s);

@@ -80,6 +80,7 @@ CONV_CASES = [
     (1, 4, 4, 4, 256, 128), (1, 8, 8, 8, 128, 64), (1, 8, 16, 8, 64, 32), (1, 8, 8, 8, 32, 16),
     (3, 5, 7, 3, 16, 48), (1, 64, 16, 16, 32, 32),
     (1, 16, 16, 16, 16, 32),  # two waves per tile splitting k, the last chunk half past 27 Cin
+    (1, 16, 32, 64, 64, 32),  # 32768 voxels, wide input: the input box with 16-channel tiles
     (1, 32, 64, 64, 32, 32),  # 131072 voxels: the input box in LDS
 ]
 # 65536 voxels x 64 -> 128 channels: the input box with eight channel blocks
