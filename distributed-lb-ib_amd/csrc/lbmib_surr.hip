@@ -516,14 +516,14 @@ int launch_fwd_kw(const void *x, const void *w, const float *bias, const void *m
   return LBMIB_OK;
 }
 
-// Tiny grids (a sample of at most 512 voxels whose padded volume, one cell
-// more on every side, has at most kVolMax voxels): a block owns up to 256
-// voxels of one sample (4 waves x 4 fragments of 16), 16 output channels,
-// and one slice of 16 input channels of the reduction (gridDim.z slices).
-// It stages the sample's whole padded volume of that slice in LDS (zeros on
-// the rim; rows of 24 elements, so that 8 lanes' 16-byte reads fall on
-// distinct banks) and the slice's weights for all 27 taps (rows of
-// 28 x 16 + 8 elements), then multiplies its 14 chunks from LDS with no
+// Small samples (8^3, 16^3: the z-planes a block of 256 voxels spans, one
+// more on each side, padded by one cell in x and y, hold at most kVolMax
+// voxels): a block owns up to 256 voxels of one sample (4 waves x 4
+// fragments of 16), 16 output channels, and one slice of 16 input channels
+// of the reduction (gridDim.z slices).  It stages that padded slab of the
+// slice in LDS (zeros outside the volume; rows of 24 elements, so that 8
+// lanes' 16-byte reads fall on distinct banks) and the slice's weights for
+// all 27 taps (rows of 28 x 16 + 8 elements), then multiplies its 14 chunks from LDS with no
 // barrier: slot 8 kq + e of chunk p is tap 2p + (kq >> 1), channel
 // c0 + 8 (kq & 1) + e, in both operands (a 28th tap is zero).  The FP32
 // partial of the slice goes to part[slice][v][n]; k_conv_reduce adds the
@@ -535,18 +535,20 @@ constexpr int kVolMax = 1024, kVolPitch = 24, kVolWRow = 28 * 16 + 8;
 __global__ void __launch_bounds__(256) k_conv_fwd_vol(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
                                                       float *__restrict__ part, int D, int H, int W, long M, int Cin,
                                                       int Cout) {
-  extern __shared__ __attribute__((aligned(16))) uint16_t xs[];  // [padded voxel][24], then [16 rows][kVolWRow]
+  extern __shared__ __attribute__((aligned(16))) uint16_t ws[];  // [16 rows][kVolWRow], then [slab voxel][24]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int col = lane & 15, kq = lane >> 4;
   const int S = D * H * W, parts = (S + 255) / 256;
   const int smp = blockIdx.x / parts, v0 = (blockIdx.x % parts) * 256;
   const int n0 = blockIdx.y * 16, c0 = blockIdx.z * 16, K = 27 * Cin;
-  const int PX = W + 2, PY = H + 2, NP = PX * PY * (D + 2);
-  uint16_t *ws = xs + NP * kVolPitch;
+  // the z-planes of the block's voxels, and one more on each side
+  const int zlo = v0 / (H * W), zhi = min(v0 + 255, S - 1) / (H * W);
+  const int PX = W + 2, PY = H + 2, NP = PX * PY * (zhi - zlo + 3);
+  uint16_t *xs = ws + 16 * kVolWRow;
   const long base = (long)smp * S;
   for (int e = tid; e < 2 * NP; e += 256) {
     const int vi = e >> 1, half = e & 1;
-    const int gx = vi % PX - 1, gy = (vi / PX) % PY - 1, gz = vi / (PX * PY) - 1;
+    const int gx = vi % PX - 1, gy = (vi / PX) % PY - 1, gz = zlo - 1 + vi / (PX * PY);
     u32x4 v = {0u, 0u, 0u, 0u};
     if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
       v = *(const u32x4 *)(x + (base + ((long)gz * H + gy) * W + gx) * Cin + c0 + 8 * half);
@@ -563,9 +565,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd_vol(const uint16_t *__restrict
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     lv[i] = v0 + wave * 64 + i * 16 + col;
-    const int l = lv[i] < S ? lv[i] : 0;
+    const int l = lv[i] < S ? lv[i] : v0;  // past the end: any voxel of the slab
     const int vx = l % W, vy = (l / W) % H, vz = l / (W * H);
-    vb[i] = ((vz + 1) * PY + vy + 1) * PX + vx + 1;
+    vb[i] = ((vz - zlo + 1) * PY + vy + 1) * PX + vx + 1;
   }
   f32x4 acc[4];
 #pragma unroll
@@ -660,11 +662,29 @@ int launch_fwd(const void *x, const void *w, const float *bias, const void *mask
   return LBMIB_OK;
 }
 
-// (the slices' partials at most 8 MB: beyond that, at batch 4 on 8^3, the
-// reduction's pass over them cost more than the form saves -- 49.5 against
-// 32.5 us at 256 -> 256 channels)
+// the largest padded slab over a sample's blocks of 256 voxels
+long vol_slab_voxels(int D, int H, int W) {
+  const long S = (long)D * H * W, HW = (long)H * W;
+  long span = 0;
+  for (long v0 = 0; v0 < S; v0 += 256) {
+    const long v1 = v0 + 255 < S - 1 ? v0 + 255 : S - 1;
+    const long sp = v1 / HW - v0 / HW + 1;
+    if (sp > span) span = sp;
+  }
+  return (long)(W + 2) * (H + 2) * (span + 2);
+}
+
+// Taken where it measured faster than the other small-grid forms: samples
+// of at most 512 voxels, or larger ones with at least 128 input channels
+// (16^3, 128 -> 64: 17.7 against 22.2 us; with 64 input channels the
+// two-wave form is faster: 13.9 against 16.6 us at 64 -> 128), and the
+// slices' partials at most 8 MB (beyond, the reduction's pass over them
+// cost more than the form saves: 16 MB at batch 4 on 8^3, 128 -> 256,
+// 25.5 against 23.6 us split over chunks; 32 MB at 256 -> 256, 49.5 against
+// 32.5 us).
 bool vol_conv_ok(int D, int H, int W, long M, int Cin, int Cout) {
-  return (long)D * H * W <= 512 && (long)(D + 2) * (H + 2) * (W + 2) <= kVolMax && Cin % 16 == 0 &&
+  const long S = (long)D * H * W;
+  return (S <= 512 || (S <= (1L << 20) && Cin >= 128)) && vol_slab_voxels(D, H, W) <= kVolMax && Cin % 16 == 0 &&
          (size_t)(Cin / 16) * M * Cout * sizeof(float) <= (8u << 20);
 }
 
@@ -675,7 +695,7 @@ int launch_fwd_vol(const void *x, const void *w, const float *bias, const void *
   if (rc) return rc;
   const int S = D * H * W;
   const dim3 grid((unsigned)(M / S * ((S + 255) / 256)), (unsigned)(Cout / 16), (unsigned)(Cin / 16));
-  const size_t shm = sizeof(uint16_t) * ((size_t)(D + 2) * (H + 2) * (W + 2) * kVolPitch + 16 * kVolWRow);
+  const size_t shm = sizeof(uint16_t) * ((size_t)vol_slab_voxels(D, H, W) * kVolPitch + 16 * kVolWRow);
   k_conv_fwd_vol<<<grid, 256, shm, s>>>((const uint16_t *)x, (const uint16_t *)w, (float *)ws, D, H, W, M, Cin, Cout);
   HIPCHK(hipGetLastError());
   const long nq = M * (Cout / 4);

@@ -74,13 +74,14 @@ def assert_bf16_close(gpu, ref, absum, what):
 CONV_CASES = [
     # (B, D, H, W, Cin, Cout): every pair of the network, grids that fill and
     # that do not fill the block tiles; samples of at most 512 voxels take
-    # the whole-volume form, (1, 8, 16, 8, ...) the split over chunks
+    # the padded-slab form, (1, 8, 16, 8, ...) the split over chunks
     (1, 8, 8, 8, 16, 32), (2, 4, 6, 8, 32, 32), (1, 16, 16, 16, 32, 64), (1, 8, 8, 8, 64, 64),
     (1, 4, 8, 8, 64, 128), (2, 4, 4, 4, 128, 128), (1, 4, 4, 8, 128, 256), (1, 2, 4, 4, 256, 256),
     (1, 4, 4, 4, 256, 128), (1, 8, 8, 8, 128, 64), (1, 8, 16, 8, 64, 32), (1, 8, 8, 8, 32, 16),
     (3, 5, 7, 3, 16, 48), (1, 64, 16, 16, 32, 32),
     (1, 16, 16, 16, 16, 32),  # two waves per tile splitting k, the last chunk half past 27 Cin
     (1, 16, 32, 64, 64, 32),  # 32768 voxels, wide input: the input box with 16-channel tiles
+    (1, 16, 16, 16, 128, 64),  # 16^3 with 128 input channels: the padded slab of each z-plane
     (1, 32, 64, 64, 32, 32),  # 131072 voxels: the input box in LDS
 ]
 # 65536 voxels x 64 -> 128 channels: the input box with eight channel blocks
